@@ -1,0 +1,1487 @@
+// Coarsening on gfx950: stable-receiver aggregation (seed matching, frozen
+// roles, exact top-two cluster affinity, regret-ordered weight-prefix
+// admission, isolated packing, two-hop pairing, compaction) and exact
+// coarse-CSR contraction.  Restates src/coarsen.cu and src/coarsening/*.cuh
+// of the reference with wave64 kernels and a canonical (vertex-id) order.
+#include "gp_coarsen.hpp"
+#include "gp_segsort.hpp"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace gp {
+namespace {
+
+constexpr int kLowDegree = 16;     // group of 16 lanes
+constexpr int kWaveDegree = 512;   // one wave, up to 8 items per lane
+constexpr int kBlockDegree = 4096; // one block, LDS bitonic
+
+__device__ inline bool sclp_better(std::uint64_t a, std::uint64_t t, std::uint64_t oa, std::uint64_t ot) {
+    return a > oa || (a == oa && a != 0 && t > ot);  // aggregation.cuh:79-84
+}
+
+// ------------------------------------------------------------ metadata
+// record_bound_kernel (src/coarsen.cu:51-99): saturating incident weight
+// sums, unsorted/zero/non-unit flags and the maximum raw degree.  Each lane
+// owns a vertex; rows longer than 64 are scanned cooperatively by the wave.
+struct MetaOut {
+    unsigned long long max_row;   // 0
+    unsigned long long unsorted;  // 1
+    unsigned long long flags;     // 2: bit0 zero weight, bit1 non-unit
+    unsigned long long max_deg;   // 3
+    unsigned long long heavy;     // 4: any degree >= 4096
+    unsigned long long total_vw;  // 5
+    unsigned long long max_vw;    // 6
+};
+
+__device__ inline std::uint64_t sat_add(std::uint64_t a, std::uint64_t b) {
+    return ~0ULL - a < b ? ~0ULL : a + b;
+}
+
+__global__ void k_level0_meta(std::int64_t n, const std::int32_t* off, const std::int32_t* nbr,
+                              const std::uint32_t* ew, const std::uint32_t* vw, std::uint64_t* row_bound,
+                              MetaOut* out) {
+    const int lane = lane_id();
+    unsigned long long maxrow = 0, maxdeg = 0, totalvw = 0, maxvw = 0;
+    unsigned flags = 0, unsorted = 0;
+    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane; base < n;
+         base += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t v = base + lane;
+        std::int64_t b = 0, e = 0;
+        if (v < n) {
+            b = off[v];
+            e = off[v + 1];
+            totalvw += vw[v];
+            maxvw = maxvw > vw[v] ? maxvw : vw[v];
+            maxdeg = maxdeg > (unsigned long long)(e - b) ? maxdeg : (unsigned long long)(e - b);
+        }
+        const bool heavy = v < n && e - b > 64;
+        std::uint64_t d = 0;
+        if (v < n && !heavy) {
+            for (std::int64_t x = b; x < e; ++x) {
+                const std::uint64_t w = ew_at(ew, x);
+                d = sat_add(d, w);
+                if (x > b && nbr[x] <= nbr[x - 1]) unsorted = 1;
+                if (!w) flags |= 1u;
+                if (w != 1) flags |= 2u;
+            }
+            row_bound[v] = d;
+            maxrow = maxrow > d ? maxrow : d;
+        }
+        unsigned long long hv = __ballot(heavy);
+        while (hv) {
+            const int src = __ffsll(hv) - 1;
+            hv &= hv - 1;
+            const std::int64_t hvx = __shfl(v, src);
+            const std::int64_t hb = off[hvx], he = off[hvx + 1];
+            std::uint64_t hd = 0;
+            for (std::int64_t x = hb + lane; x < he; x += kWave) {
+                const std::uint64_t w = ew_at(ew, x);
+                hd = sat_add(hd, w);
+                if (x > hb && nbr[x] <= nbr[x - 1]) unsorted = 1;
+                if (!w) flags |= 1u;
+                if (w != 1) flags |= 2u;
+            }
+            for (int dd = kWave / 2; dd; dd >>= 1) hd = sat_add(hd, __shfl_xor(hd, dd));
+            if (lane == src) {
+                row_bound[hvx] = hd;
+                maxrow = maxrow > hd ? maxrow : hd;
+            }
+        }
+    }
+    maxrow = wave_max(maxrow);
+    maxdeg = wave_max(maxdeg);
+    maxvw = wave_max(maxvw);
+    totalvw = wave_sum(totalvw);
+    const unsigned long long anyuns = __ballot(unsorted != 0);
+    const unsigned long long anyzero = __ballot((flags & 1u) != 0);
+    const unsigned long long anynon = __ballot((flags & 2u) != 0);
+    if (lane == 0) {
+        if (maxrow) atomicMax(&out->max_row, maxrow);
+        if (maxdeg) atomicMax(&out->max_deg, maxdeg);
+        if (maxvw) atomicMax(&out->max_vw, maxvw);
+        if (totalvw) atomicAdd(&out->total_vw, totalvw);
+        if (anyuns) atomicOr(&out->unsorted, 1ULL);
+        if (anyzero || anynon) atomicOr(&out->flags, (anyzero ? 1ULL : 0ULL) | (anynon ? 2ULL : 0ULL));
+    }
+}
+
+// Degree classes for this level (src/coarsening/aggregation.cuh:5-71).
+// Queue order is irrelevant: every later write is per vertex.
+struct ClassCounts {
+    unsigned long long c[8];  // 0: <=16, 1: <=64, 2: <=128, 3: <=256, 4: <=512, 5: <=4096, 6: >4096, 7: max degree
+};
+
+__device__ inline int degree_class(std::int64_t d) {
+    return d <= kLowDegree ? 0 : d <= 64 ? 1 : d <= 128 ? 2 : d <= 256 ? 3 : d <= kWaveDegree ? 4 : d <= kBlockDegree ? 5 : 6;
+}
+
+__global__ void k_degree_classes(std::int64_t n, const std::int32_t* off, std::int32_t* lists, std::int64_t stride,
+                                 ClassCounts* counts) {
+    const int lane = lane_id();
+    unsigned long long maxdeg = 0;
+    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane; base < n;
+         base += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t v = base + lane;
+        const std::int64_t d = v < n ? off[v + 1] - off[v] : 0;
+        maxdeg = maxdeg > (unsigned long long)d ? maxdeg : (unsigned long long)d;
+        const int cls = v < n ? degree_class(d) : -1;
+        for (int c = 0; c < 7; ++c) {
+            const unsigned long long mask = __ballot(cls == c);
+            if (!mask) continue;
+            unsigned long long start = 0;
+            const int leader = __ffsll(mask) - 1;
+            if (lane == leader) start = atomicAdd(&counts->c[c], (unsigned long long)__popcll(mask));
+            start = __shfl(start, leader);
+            if (cls == c) lists[c * stride + start + __popcll(mask & ((1ULL << lane) - 1))] = (std::int32_t)v;
+        }
+    }
+    maxdeg = wave_max(maxdeg);
+    if (lane == 0 && maxdeg) atomicMax(&counts->c[7], maxdeg);
+}
+
+// -------------------------------------------------------- seed matching
+// seed_matching.cuh:12-91: mutual heaviest admissible edge; order
+// (weight, symmetric hash, id); rows with degree > 1024 are skipped.
+__global__ void k_match_choose(std::int64_t n, const std::int32_t* off, const std::int32_t* nbr,
+                               const std::uint32_t* ew, const std::uint32_t* vw, std::uint64_t cap, std::uint32_t salt,
+                               std::int32_t* choices) {
+    const int lane = lane_id();
+    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane; base < n;
+         base += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t v = base + lane;
+        std::int64_t b = 0, e = 0;
+        if (v < n) {
+            b = off[v];
+            e = off[v + 1];
+        }
+        const bool heavy = v < n && e - b > 32 && e - b <= 1024;
+        if (v < n && !heavy) {
+            std::int32_t best = -1;
+            std::uint64_t bw = 0;
+            std::uint32_t bh = 0;
+            const std::uint64_t wv = vw[v];
+            if (e - b <= 1024) {
+                for (std::int64_t x = b; x < e; ++x) {
+                    const std::int32_t u = nbr[x];
+                    if (u == v || wv > cap || (std::uint64_t)vw[u] > cap - wv) continue;
+                    const std::uint32_t lo = (std::uint32_t)(u < v ? u : v), hi = (std::uint32_t)(u < v ? v : u);
+                    const std::uint32_t h = mix32(lo ^ mix32(hi ^ salt));
+                    const std::uint64_t w = ew_at(ew, x);
+                    if (best < 0 || w > bw || (w == bw && (h > bh || (h == bh && u > best)))) {
+                        best = u;
+                        bw = w;
+                        bh = h;
+                    }
+                }
+            }
+            choices[v] = best;
+        }
+        unsigned long long hv = __ballot(heavy);
+        while (hv) {
+            const int src = __ffsll(hv) - 1;
+            hv &= hv - 1;
+            const std::int64_t hvx = __shfl(v, src);
+            const std::int64_t hb = off[hvx], he = off[hvx + 1];
+            const std::uint64_t wv = vw[hvx];
+            std::int32_t best = -1;
+            std::uint64_t bw = 0;
+            std::uint32_t bh = 0;
+            for (std::int64_t x = hb + lane; x < he; x += kWave) {
+                const std::int32_t u = nbr[x];
+                if (u == hvx || wv > cap || (std::uint64_t)vw[u] > cap - wv) continue;
+                const std::uint32_t lo = (std::uint32_t)(u < hvx ? u : hvx), hi = (std::uint32_t)(u < hvx ? hvx : u);
+                const std::uint32_t h = mix32(lo ^ mix32(hi ^ salt));
+                const std::uint64_t w = ew_at(ew, x);
+                if (best < 0 || w > bw || (w == bw && (h > bh || (h == bh && u > best)))) {
+                    best = u;
+                    bw = w;
+                    bh = h;
+                }
+            }
+            for (int d = kWave / 2; d; d >>= 1) {
+                const std::int32_t ob = __shfl_xor(best, d);
+                const std::uint64_t ow = __shfl_xor(bw, d);
+                const std::uint32_t oh = __shfl_xor(bh, d);
+                if (ob >= 0 && (best < 0 || ow > bw || (ow == bw && (oh > bh || (oh == bh && ob > best))))) {
+                    best = ob;
+                    bw = ow;
+                    bh = oh;
+                }
+            }
+            if (lane == src) choices[hvx] = best;
+        }
+    }
+}
+
+__global__ void k_match_roots(std::int64_t n, const std::int32_t* choices, const std::uint32_t* vw,
+                              std::int32_t* clusters, unsigned long long* cw, unsigned long long* roots) {
+    unsigned long long count = 0;
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t u = choices[v];
+        const bool mutual = u >= 0 && choices[u] == v;
+        const std::int32_t root = mutual ? (u < v ? u : (std::int32_t)v) : (std::int32_t)v;
+        clusters[v] = root;
+        count += root == v;
+        atomicAdd(&cw[root], (unsigned long long)vw[v]);
+    }
+    count = wave_sum(count);
+    if (lane_id() == 0 && count) atomicAdd(roots, count);
+}
+
+__global__ void k_init_singletons(std::int64_t n, const std::uint32_t* vw, std::int32_t* clusters,
+                                  unsigned long long* cw) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x) {
+        clusters[v] = (std::int32_t)v;
+        cw[v] = vw[v];
+    }
+}
+
+// -------------------------------------------------------------- roles
+// residual_freeze_roles (singleton_growth.cuh:27-40); also resets proposals.
+__global__ void k_roles(std::int64_t n, const std::int32_t* clusters, const std::uint32_t* vw,
+                        const unsigned long long* cw, std::uint32_t salt, std::uint32_t threshold, bool all,
+                        std::uint8_t* roles, std::int32_t* proposals) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x) {
+        roles[v] = clusters[v] == v && cw[v] == (unsigned long long)vw[v] &&
+                   (all || mix32((std::uint32_t)v ^ salt) < threshold);
+        proposals[v] = kInvalid;
+    }
+}
+
+struct AffinityArgs {
+    const std::int32_t* off;
+    const std::int32_t* nbr;
+    const std::uint32_t* ew;
+    const std::int32_t* clusters;
+    const unsigned long long* cw;
+    const std::uint8_t* roles;
+    std::uint64_t cap;
+    std::uint32_t tie_salt;
+    bool filter_roles;
+    bool use_second;
+    std::int32_t* proposals;
+    std::uint64_t* gains;
+    std::uint64_t* regret;
+    std::uint32_t* ties;
+};
+
+// Candidate accumulator: best (value, tie) and the runner-up value of the
+// multiset (aggregation.cuh:97-106); independent of evaluation order.
+struct Top2 {
+    std::uint64_t current, best, tie, second;
+};
+
+__device__ inline void top2_add(const AffinityArgs& a, std::int32_t v, std::int32_t src, std::int32_t c,
+                                std::uint64_t val, Top2& t) {
+    if (c == src) {
+        t.current = val;
+        return;
+    }
+    if (a.filter_roles) {
+        const std::uint64_t base = a.cw[c];
+        if (a.roles[c] || !(base != 0 && base <= a.cap && a.cw[src] <= a.cap - base)) return;
+    }
+    const std::uint32_t h = mix32((std::uint32_t)v ^ mix32((std::uint32_t)c) ^ a.tie_salt);
+    const std::uint64_t key = ((std::uint64_t)h << 32) | (std::uint64_t)(0xffffffffU - (std::uint32_t)c);
+    if (a.use_second) {
+        if (sclp_better(val, key, t.best, t.tie)) {
+            t.second = t.second > t.best ? t.second : t.best;
+            t.best = val;
+            t.tie = key;
+        } else
+            t.second = t.second > val ? t.second : val;
+    } else if (sclp_better(val, key, t.best, t.tie)) {
+        t.best = val;
+        t.tie = key;
+    }
+}
+
+__device__ inline Top2 top2_merge(const Top2& x, const Top2& y) {
+    Top2 r;
+    r.current = x.current > y.current ? x.current : y.current;
+    if (sclp_better(y.best, y.tie, x.best, x.tie)) {
+        r.best = y.best;
+        r.tie = y.tie;
+        std::uint64_t s = x.second > y.second ? x.second : y.second;
+        r.second = s > x.best ? s : x.best;
+    } else {
+        r.best = x.best;
+        r.tie = x.tie;
+        std::uint64_t s = x.second > y.second ? x.second : y.second;
+        r.second = s > y.best ? s : y.best;
+    }
+    return r;
+}
+
+template <int W>
+__device__ inline Top2 top2_group_reduce(Top2 t) {
+    for (int d = W / 2; d; d >>= 1) {
+        Top2 o;
+        o.current = __shfl_xor(t.current, d, W);
+        o.best = __shfl_xor(t.best, d, W);
+        o.tie = __shfl_xor(t.tie, d, W);
+        o.second = __shfl_xor(t.second, d, W);
+        t = top2_merge(t, o);
+    }
+    return t;
+}
+
+// sclp_decode_gain_kernel (aggregation.cuh:513-557) for one mover.
+__device__ inline void decode_proposal(const AffinityArgs& a, std::int32_t v, std::int32_t src, const Top2& t) {
+    if (!(t.best > t.current)) return;
+    const std::int64_t target = (std::int64_t)(0xffffffffULL - (t.tie & 0xffffffffULL));
+    if (target < 0 || target > 0x7fffffffLL || target == src) return;
+    a.proposals[v] = (std::int32_t)target;
+    a.gains[v] = t.best - t.current;
+    if (a.use_second) a.regret[v] = t.best - t.second;
+    a.ties[v] = (std::uint32_t)(t.tie >> 32);
+}
+
+// Group kernel: W lanes per mover, rows of at most W*I entries.
+template <int W, int I>
+__global__ void k_affinity_group(AffinityArgs a, const std::int32_t* list, std::int64_t count) {
+    const int lg = threadIdx.x % W;
+    const int group_base = (threadIdx.x % kWave) / W * W;
+    const std::int64_t groups = (std::int64_t)gridDim.x * (blockDim.x / W);
+    for (std::int64_t idx = blockIdx.x * (std::int64_t)(blockDim.x / W) + threadIdx.x / W; idx < count;
+         idx += groups) {
+        const std::int32_t v = list[idx];
+        const std::int32_t src = a.clusters[v];
+        if (!a.roles[src]) continue;
+        const std::int64_t b = a.off[v], d = a.off[v + 1] - b;
+        std::int32_t k[I];
+        std::uint32_t w[I];
+#pragma unroll
+        for (int it = 0; it < I; ++it) {
+            const std::int64_t x = it * W + lg;
+            if (x < d) {
+                const std::int32_t u = a.nbr[b + x];
+                k[it] = u == v ? kSentinel : a.clusters[u];
+                w[it] = (std::uint32_t)ew_at(a.ew, b + x);
+            } else {
+                k[it] = kSentinel;
+                w[it] = 0;
+            }
+        }
+        group_bitonic_sort<W, I>(k, w, lg);
+        Top2 t{0, 0, 0, 0};
+        group_reduce_runs<W, I>(k, w, lg, group_base, [&](std::int32_t c, std::uint64_t s, int) {
+            top2_add(a, v, src, c, s, t);
+        });
+        t = top2_group_reduce<W>(t);
+        if (lg == 0) decode_proposal(a, v, src, t);
+    }
+}
+
+// Block kernel: one 256-thread block per mover with <= 4096 entries.
+__global__ void __launch_bounds__(kBlockSortThreads) k_affinity_block(AffinityArgs a, const std::int32_t* list,
+                                                                      std::int64_t count) {
+    __shared__ BlockSortSmem s;
+    __shared__ Top2 part[kBlockSortThreads / kWave];
+    for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
+        const std::int32_t v = list[idx];
+        const std::int32_t src = a.clusters[v];
+        if (!a.roles[src]) continue;
+        const std::int64_t b = a.off[v];
+        const int d = (int)(a.off[v + 1] - b);
+        for (int x = threadIdx.x; x < d; x += kBlockSortThreads) {
+            const std::int32_t u = a.nbr[b + x];
+            s.key[x] = u == v ? kSentinel : a.clusters[u];
+            s.val[x] = (std::uint32_t)ew_at(a.ew, b + x);
+        }
+        __syncthreads();
+        Top2 t{0, 0, 0, 0};
+        block_sort_reduce(s, d, [&](std::int32_t c, std::uint64_t sum, int) { top2_add(a, v, src, c, sum, t); });
+        t = top2_group_reduce<kWave>(t);
+        if (lane_id() == 0) part[threadIdx.x / kWave] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            Top2 r = part[0];
+            for (int i = 1; i < kBlockSortThreads / kWave; ++i) r = top2_merge(r, part[i]);
+            decode_proposal(a, v, src, r);
+        }
+        __syncthreads();
+    }
+}
+
+// Hub rows (> 4096 entries): records (row, cluster, w), radix sort, reduce.
+__global__ void k_hub_records(AffinityArgs a, const std::int32_t* list, std::int64_t count,
+                              const std::int64_t* rec_off, int key_bits, std::uint64_t* keys, std::uint32_t* vals) {
+    for (std::int64_t r = blockIdx.x; r < count; r += gridDim.x) {
+        const std::int32_t v = list[r];
+        const std::int64_t b = a.off[v], d = a.off[v + 1] - b;
+        const std::int64_t o = rec_off[r];
+        const bool mover = a.roles[a.clusters[v]] != 0;
+        for (std::int64_t x = threadIdx.x; x < d; x += blockDim.x) {
+            const std::int32_t u = a.nbr[b + x];
+            const bool skip = !mover || u == v;
+            keys[o + x] = skip ? ~0ULL : (((std::uint64_t)r << key_bits) | (std::uint64_t)a.clusters[u]);
+            vals[o + x] = skip ? 0u : (std::uint32_t)ew_at(a.ew, b + x);
+        }
+    }
+}
+
+// Unique (row, key) runs of sorted records: heads, scan, atomic sums.
+__global__ void k_run_heads(const std::uint64_t* keys, std::int64_t n, std::uint8_t* heads) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (std::int64_t)gridDim.x * blockDim.x)
+        heads[i] = keys[i] != ~0ULL && (i == 0 || keys[i] != keys[i - 1]);
+}
+__global__ void k_run_accumulate(const std::uint64_t* keys, const std::uint32_t* vals, std::int64_t n,
+                                 const std::int64_t* pos, const std::uint8_t* heads, std::uint64_t* ukeys,
+                                 unsigned long long* usums) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        if (keys[i] == ~0ULL) continue;
+        const std::int64_t u = pos[i] - 1;  // inclusive count of heads -> run id
+        if (heads[i]) ukeys[u] = keys[i];
+        atomicAdd(&usums[u], (unsigned long long)vals[i]);
+    }
+}
+
+struct HubTop2 {
+    unsigned long long current, best, tie, second;
+};
+
+__global__ void k_hub_best(AffinityArgs a, const std::int32_t* list, const std::uint64_t* ukeys,
+                           const unsigned long long* usums, std::int64_t nu, int key_bits, HubTop2* acc) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < nu;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t r = (std::int64_t)(ukeys[i] >> key_bits);
+        const std::int32_t c = (std::int32_t)(ukeys[i] & ((1ULL << key_bits) - 1));
+        const std::int32_t v = list[r];
+        const std::int32_t src = a.clusters[v];
+        const unsigned long long val = usums[i];
+        if (c == src) {
+            acc[r].current = val;
+            continue;
+        }
+        if (a.filter_roles) {
+            const std::uint64_t base = a.cw[c];
+            if (a.roles[c] || !(base != 0 && base <= a.cap && a.cw[src] <= a.cap - base)) continue;
+        }
+        const unsigned long long prev = atomicMax(&acc[r].best, val);
+        if (a.use_second) atomicMax(&acc[r].second, prev < val ? prev : val);
+    }
+}
+__global__ void k_hub_tie(AffinityArgs a, const std::int32_t* list, const std::uint64_t* ukeys,
+                          const unsigned long long* usums, std::int64_t nu, int key_bits, HubTop2* acc) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < nu;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t r = (std::int64_t)(ukeys[i] >> key_bits);
+        const std::int32_t c = (std::int32_t)(ukeys[i] & ((1ULL << key_bits) - 1));
+        const std::int32_t v = list[r];
+        const std::int32_t src = a.clusters[v];
+        if (c == src || usums[i] != acc[r].best) continue;
+        if (a.filter_roles) {
+            const std::uint64_t base = a.cw[c];
+            if (a.roles[c] || !(base != 0 && base <= a.cap && a.cw[src] <= a.cap - base)) continue;
+        }
+        const std::uint32_t h = mix32((std::uint32_t)v ^ mix32((std::uint32_t)c) ^ a.tie_salt);
+        const unsigned long long key = ((unsigned long long)h << 32) | (unsigned long long)(0xffffffffU - (std::uint32_t)c);
+        atomicMax(&acc[r].tie, key);
+    }
+}
+__global__ void k_hub_decode(AffinityArgs a, const std::int32_t* list, std::int64_t count, const HubTop2* acc) {
+    for (std::int64_t r = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; r < count;
+         r += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t v = list[r];
+        const std::int32_t src = a.clusters[v];
+        if (!a.roles[src]) continue;
+        // A zero best never wins a comparison; keep that rule for the tie too.
+        Top2 t{acc[r].current, acc[r].best, acc[r].best ? acc[r].tie : 0, acc[r].second};
+        decode_proposal(a, v, src, t);
+    }
+}
+
+// ------------------------------------------------------------ admission
+__global__ void k_commit_all(std::int64_t n, const std::int32_t* proposals, const std::uint32_t* vw,
+                             std::int32_t* clusters, unsigned long long* cw) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t t = proposals[v];
+        if (t == kInvalid) continue;
+        clusters[v] = t;
+        cw[v] = 0;
+        atomicAdd(&cw[t], (unsigned long long)vw[v]);
+    }
+}
+
+__global__ void k_valid_flags(std::int64_t n, const std::int32_t* targets, std::uint8_t* flags) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x)
+        flags[v] = targets[v] != kInvalid;
+}
+
+// Composite key, least significant first: (~tie) | (~score masked) << 32 | target << (32+sb).
+__global__ void k_admission_keys(std::int64_t count, const std::int32_t* order, const std::int32_t* targets,
+                                 const std::uint64_t* score, const std::uint32_t* ties, int score_bits, bool with_target,
+                                 std::uint64_t* keys, std::uint32_t* vals) {
+    const std::uint64_t mask = score_bits >= 32 ? 0xffffffffULL : ((1ULL << score_bits) - 1);
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t v = order[i];
+        std::uint64_t key = (std::uint64_t)(~ties[v]);
+        key |= ((~score[v]) & mask) << 32;
+        if (with_target) key |= (std::uint64_t)targets[v] << (32 + score_bits);
+        keys[i] = key;
+        vals[i] = (std::uint32_t)v;
+    }
+}
+// Second LSD stage keys when the composite does not fit 64 bits.
+__global__ void k_key_field(std::int64_t count, const std::uint32_t* vals, const std::int32_t* targets,
+                            const std::uint64_t* score, int mode, int shift, std::uint64_t* keys) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t v = (std::int32_t)vals[i];
+        keys[i] = mode == 0 ? (std::uint64_t)targets[v] : ((~score[v]) >> shift);
+    }
+}
+
+// Segment heads by target in sorted order, per-item weight.
+__global__ void k_seg_items(std::int64_t count, const std::uint32_t* vals, const std::int32_t* targets,
+                            const std::uint32_t* vw, bool ones, SegU64* items) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t v = (std::int32_t)vals[i];
+        const bool head = i == 0 || targets[(std::int32_t)vals[i - 1]] != targets[v];
+        items[i] = SegU64{ones ? 1ULL : (std::uint64_t)vw[v], head ? 1u : 0u};
+    }
+}
+
+__global__ void k_gather_base(std::int64_t count, const std::uint32_t* vals, const std::int32_t* targets,
+                              const unsigned long long* cw, std::uint64_t* base) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x)
+        base[i] = cw[targets[(std::int32_t)vals[i]]];
+}
+
+// sclp_commit_kernel (aggregation.cuh:624-652)
+__global__ void k_admission_commit(std::int64_t count, const std::uint32_t* vals, const std::int32_t* targets,
+                                   const std::uint32_t* vw, const SegU64* prefix, const std::uint64_t* base,
+                                   std::uint64_t cap, std::int32_t* clusters, unsigned long long* cw) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t v = (std::int32_t)vals[i];
+        const std::int32_t t = targets[v];
+        const std::uint64_t w = vw[v];
+        if (base[i] <= cap && prefix[i].v <= cap - base[i]) {
+            const std::int32_t s = clusters[v];
+            atomicAdd(&cw[s], 0ULL - (unsigned long long)w);
+            atomicAdd(&cw[t], (unsigned long long)w);
+            clusters[v] = t;
+        }
+    }
+}
+
+__global__ void k_count_nonzero(std::int64_t n, const unsigned long long* cw, unsigned long long* out) {
+    unsigned long long c = 0;
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x)
+        c += cw[v] != 0;
+    c = wave_sum(c);
+    if (lane_id() == 0 && c) atomicAdd(out, c);
+}
+
+// ------------------------------------------------------------ isolated
+__global__ void k_isolated_flags(std::int64_t n, const std::int32_t* off, std::uint8_t* flags) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x)
+        flags[v] = off[v] == off[v + 1];
+}
+__global__ void k_gather_vw(std::int64_t count, const std::int32_t* ids, const std::uint32_t* vw, std::uint64_t* out) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x)
+        out[i] = vw[ids[i]];
+}
+// isolated_representative_kernel / isolated_commit_kernel (aggregation.cuh:713-745)
+__global__ void k_isolated_rep(std::int64_t count, const std::int32_t* iso, const std::uint64_t* prefix,
+                               const std::uint32_t* vw, std::uint64_t cap, std::int32_t* rep) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t v = iso[i];
+        const std::uint64_t fb = (prefix[i] - vw[v]) / cap, lb = (prefix[i] - 1) / cap;
+        if (fb == lb) atomicMin(&rep[fb], v);
+    }
+}
+__global__ void k_isolated_commit(std::int64_t count, const std::int32_t* iso, const std::uint64_t* prefix,
+                                  const std::uint32_t* vw, std::uint64_t cap, const std::int32_t* rep,
+                                  std::int32_t* clusters, unsigned long long* cw) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t v = iso[i];
+        const std::uint64_t fb = (prefix[i] - vw[v]) / cap, lb = (prefix[i] - 1) / cap;
+        if (fb != lb) continue;
+        const std::int32_t t = rep[fb];
+        if (t == v) continue;
+        clusters[v] = t;
+        cw[v] = 0;
+        atomicAdd(&cw[t], (unsigned long long)vw[v]);
+    }
+}
+
+// ------------------------------------------------------------- two-hop
+__global__ void k_favorites(std::int64_t n, const std::uint32_t* vw, const unsigned long long* cw,
+                            const std::int32_t* clusters, std::int32_t* proposals) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x)
+        if (cw[clusters[v]] != (unsigned long long)vw[v]) proposals[v] = kInvalid;
+}
+// sclp_pair_flags_kernel (aggregation.cuh:666-676); ranks are 1-based.
+__global__ void k_pair_flags(std::int64_t count, const std::uint32_t* vals, const std::uint32_t* vw,
+                             const SegU64* ranks, std::uint64_t cap, std::uint8_t* flags) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const bool second = (ranks[i].v & 1ULL) == 0ULL;
+        flags[i] = second && i > 0 && (std::uint64_t)vw[vals[i - 1]] <= cap - (std::uint64_t)vw[vals[i]];
+    }
+}
+__global__ void k_pair_commit(std::int64_t count, const std::uint32_t* vals, const std::uint8_t* flags,
+                              const std::int64_t* pos, std::int64_t budget, const std::uint32_t* vw,
+                              std::int32_t* clusters, unsigned long long* cw) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        if (!flags[i] || pos[i] >= budget) continue;
+        const std::int32_t v = (std::int32_t)vals[i];
+        const std::int32_t s = clusters[v], t = clusters[vals[i - 1]];
+        if (s == t) continue;
+        const unsigned long long w = vw[v];
+        atomicAdd(&cw[s], 0ULL - w);
+        atomicAdd(&cw[t], w);
+        clusters[v] = t;
+    }
+}
+
+// ----------------------------------------------------------- compaction
+__global__ void k_nonempty(std::int64_t n, const unsigned long long* cw, std::uint8_t* flags) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x)
+        flags[v] = cw[v] > 0;
+}
+__global__ void k_compact_map(std::int64_t n, const std::int32_t* clusters, const std::int64_t* ids,
+                              const std::uint32_t* vw, std::int32_t* map, unsigned long long* sums) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t c = (std::int32_t)ids[clusters[v]];
+        map[v] = c;
+        atomicAdd(&sums[c], (unsigned long long)vw[v]);
+    }
+}
+__global__ void k_narrow_weights(std::int64_t n, const unsigned long long* sums, std::uint32_t* out,
+                                 unsigned long long* maxw) {
+    unsigned long long m = 0;
+    for (std::int64_t c = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (std::int64_t)gridDim.x * blockDim.x) {
+        out[c] = (std::uint32_t)sums[c];
+        m = m > sums[c] ? m : sums[c];
+    }
+    m = wave_max(m);
+    if (lane_id() == 0 && m) atomicMax(maxw, m);
+}
+
+// ----------------------------------------------------------- contraction
+// contract_record_bounds (src/coarsen.cu:101-119) plus raw coarse-row sizes.
+__global__ void k_contract_bounds(std::int64_t n, const std::int32_t* map, const std::int32_t* off,
+                                  const std::uint64_t* fine_bounds, unsigned long long* coarse_bounds,
+                                  unsigned long long* raw_counts) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t c = map[v];
+        atomicAdd(&raw_counts[c], (unsigned long long)(off[v + 1] - off[v]));
+        unsigned long long* target = &coarse_bounds[c];
+        const unsigned long long amount = fine_bounds[v];
+        unsigned long long prev = *target;
+        while (true) {
+            const unsigned long long sum = ~0ULL - prev < amount ? ~0ULL : prev + amount;
+            const unsigned long long seen = atomicCAS(target, prev, sum);
+            if (seen == prev) break;
+            prev = seen;
+        }
+    }
+}
+
+// Scatter fine edges into their coarse rows (bucket order is arbitrary; the
+// row sort makes the output canonical).  Internal edges become kSentinel.
+__global__ void k_contract_scatter(std::int64_t n, const std::int32_t* off, const std::int32_t* nbr,
+                                   const std::uint32_t* ew, const std::int32_t* map, unsigned long long* cursor,
+                                   std::int32_t* rkey, std::uint32_t* rval) {
+    const int lane = lane_id();
+    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane; base < n;
+         base += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t v = base + lane;
+        std::int64_t b = 0, e = 0, o = 0;
+        std::int32_t s = 0;
+        if (v < n) {
+            b = off[v];
+            e = off[v + 1];
+            s = map[v];
+            if (e > b) o = (std::int64_t)atomicAdd(&cursor[s], (unsigned long long)(e - b));
+        }
+        const bool heavy = v < n && e - b > 32;
+        if (v < n && !heavy)
+            for (std::int64_t x = b; x < e; ++x) {
+                const std::int32_t t = map[nbr[x]];
+                rkey[o + (x - b)] = t == s ? kSentinel : t;
+                rval[o + (x - b)] = (std::uint32_t)ew_at(ew, x);
+            }
+        unsigned long long hv = __ballot(heavy);
+        while (hv) {
+            const int src = __ffsll(hv) - 1;
+            hv &= hv - 1;
+            const std::int64_t hb = __shfl(b, src), he = __shfl(e, src), ho = __shfl(o, src);
+            const std::int32_t hs = __shfl(s, src);
+            for (std::int64_t x = hb + lane; x < he; x += kWave) {
+                const std::int32_t t = map[nbr[x]];
+                rkey[ho + (x - hb)] = t == hs ? kSentinel : t;
+                rval[ho + (x - hb)] = (std::uint32_t)ew_at(ew, x);
+            }
+        }
+    }
+}
+
+struct RowArgs {
+    const std::int64_t* raw_off;
+    std::int32_t* rkey;
+    std::uint32_t* rval;
+    unsigned long long* ucount;
+    unsigned long long* overflow;
+};
+
+__device__ inline void emit_row_entry(const RowArgs& r, std::int64_t start, std::int32_t key, std::uint64_t sum,
+                                      int idx) {
+    if (sum > 0xffffffffULL) atomicOr(r.overflow, 1ULL);
+    r.rkey[start + idx] = key;
+    r.rval[start + idx] = (std::uint32_t)sum;
+}
+
+// Sort-reduce coarse rows in place: unique entries written at the row start.
+template <int W, int I>
+__global__ void k_rows_group(RowArgs r, const std::int32_t* list, std::int64_t count) {
+    const int lg = threadIdx.x % W;
+    const int group_base = (threadIdx.x % kWave) / W * W;
+    const std::int64_t groups = (std::int64_t)gridDim.x * (blockDim.x / W);
+    for (std::int64_t idx = blockIdx.x * (std::int64_t)(blockDim.x / W) + threadIdx.x / W; idx < count;
+         idx += groups) {
+        const std::int32_t row = list[idx];
+        const std::int64_t b = r.raw_off[row], d = r.raw_off[row + 1] - b;
+        std::int32_t k[I];
+        std::uint32_t w[I];
+#pragma unroll
+        for (int it = 0; it < I; ++it) {
+            const std::int64_t x = it * W + lg;
+            k[it] = x < d ? r.rkey[b + x] : kSentinel;
+            w[it] = x < d ? r.rval[b + x] : 0u;
+        }
+        group_bitonic_sort<W, I>(k, w, lg);
+        const int u = group_reduce_runs<W, I>(
+            k, w, lg, group_base, [&](std::int32_t key, std::uint64_t s, int i) { emit_row_entry(r, b, key, s, i); });
+        if (lg == 0) r.ucount[row] = (unsigned long long)u;
+    }
+}
+
+__global__ void __launch_bounds__(kBlockSortThreads) k_rows_block(RowArgs r, const std::int32_t* list,
+                                                                  std::int64_t count) {
+    __shared__ BlockSortSmem s;
+    for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
+        const std::int32_t row = list[idx];
+        const std::int64_t b = r.raw_off[row];
+        const int d = (int)(r.raw_off[row + 1] - b);
+        for (int x = threadIdx.x; x < d; x += kBlockSortThreads) {
+            s.key[x] = r.rkey[b + x];
+            s.val[x] = r.rval[b + x];
+        }
+        __syncthreads();
+        const int u = block_sort_reduce(s, d, [&](std::int32_t key, std::uint64_t sum, int i) {
+            emit_row_entry(r, b, key, sum, i);
+        });
+        if (threadIdx.x == 0) r.ucount[row] = (unsigned long long)u;
+        __syncthreads();
+    }
+}
+
+// Long rows: records keyed by (long-row index, target).
+__global__ void k_long_records(RowArgs r, const std::int32_t* list, std::int64_t count, const std::int64_t* rec_off,
+                               int key_bits, std::uint64_t* keys, std::uint32_t* vals) {
+    for (std::int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const std::int32_t row = list[i];
+        const std::int64_t b = r.raw_off[row], d = r.raw_off[row + 1] - b;
+        const std::int64_t o = rec_off[i];
+        for (std::int64_t x = threadIdx.x; x < d; x += blockDim.x) {
+            const std::int32_t key = r.rkey[b + x];
+            keys[o + x] = key == kSentinel ? ~0ULL : (((std::uint64_t)i << key_bits) | (std::uint64_t)key);
+            vals[o + x] = r.rval[b + x];
+        }
+    }
+}
+// Write reduced long-row entries back at their row start.  first/last
+// bracket each long row's contiguous run of unique records.
+__global__ void k_long_bounds(const std::uint64_t* ukeys, std::int64_t nu, int key_bits, unsigned long long* first,
+                              unsigned long long* last) {
+    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < nu;
+         j += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t li = (std::int64_t)(ukeys[j] >> key_bits);
+        atomicMin(&first[li], (unsigned long long)j);
+        atomicMax(&last[li], (unsigned long long)(j + 1));
+    }
+}
+__global__ void k_long_writeback(RowArgs r, const std::int32_t* list, const std::uint64_t* ukeys,
+                                 const unsigned long long* usums, std::int64_t nu, int key_bits,
+                                 const unsigned long long* first, const unsigned long long* last) {
+    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < nu;
+         j += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t li = (std::int64_t)(ukeys[j] >> key_bits);
+        const std::int32_t key = (std::int32_t)(ukeys[j] & ((1ULL << key_bits) - 1));
+        const std::int32_t row = list[li];
+        if ((unsigned long long)j == first[li]) r.ucount[row] = last[li] - first[li];
+        emit_row_entry(r, r.raw_off[row], key, usums[j], (int)(j - (std::int64_t)first[li]));
+    }
+}
+
+// Lengths of listed rows (offsets of type O), for record layouts.
+template <typename O>
+__global__ void k_list_lengths(const std::int32_t* list, std::int64_t count, const O* off, std::int64_t* out) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x)
+        out[i] = (std::int64_t)(off[list[i] + 1] - off[list[i]]);
+}
+
+__global__ void k_copy_rows(std::int64_t rows, const std::int64_t* raw_off, const std::int32_t* rkey,
+                            const std::uint32_t* rval, const std::int64_t* coff64, std::int32_t* off,
+                            std::int32_t* nbr, std::uint32_t* ew, std::int64_t m_total) {
+    const int lane = lane_id();
+    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane; base < rows;
+         base += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t row = base + lane;
+        std::int64_t src = 0, dst = 0, cnt = 0;
+        if (row < rows) {
+            src = raw_off[row];
+            dst = coff64[row];
+            cnt = (row + 1 < rows ? coff64[row + 1] : m_total) - dst;
+            off[row] = (std::int32_t)dst;
+            if (row == rows - 1) off[rows] = (std::int32_t)m_total;
+        }
+        const bool heavy = row < rows && cnt > 16;
+        if (row < rows && !heavy)
+            for (std::int64_t x = 0; x < cnt; ++x) {
+                nbr[dst + x] = rkey[src + x];
+                ew[dst + x] = rval[src + x];
+            }
+        unsigned long long hv = __ballot(heavy);
+        while (hv) {
+            const int sl = __ffsll(hv) - 1;
+            hv &= hv - 1;
+            const std::int64_t hs = __shfl(src, sl), hd = __shfl(dst, sl), hc = __shfl(cnt, sl);
+            for (std::int64_t x = lane; x < hc; x += kWave) {
+                nbr[hd + x] = rkey[hs + x];
+                ew[hd + x] = rval[hs + x];
+            }
+        }
+    }
+}
+
+__global__ void k_row_classes(std::int64_t rows, const std::int64_t* raw_off, std::int32_t* lists, std::int64_t stride,
+                              ClassCounts* counts) {
+    const int lane = lane_id();
+    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane; base < rows;
+         base += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t v = base + lane;
+        const int cls = v < rows ? degree_class(raw_off[v + 1] - raw_off[v]) : -1;
+        for (int c = 0; c < 7; ++c) {
+            const unsigned long long mask = __ballot(cls == c);
+            if (!mask) continue;
+            unsigned long long start = 0;
+            const int leader = __ffsll(mask) - 1;
+            if (lane == leader) start = atomicAdd(&counts->c[c], (unsigned long long)__popcll(mask));
+            start = __shfl(start, leader);
+            if (cls == c) lists[c * stride + start + __popcll(mask & ((1ULL << lane) - 1))] = (std::int32_t)v;
+        }
+    }
+}
+
+inline int grid_for(std::int64_t n, int per_block = 256, int cap = 8192) {
+    return static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(n, per_block), cap)));
+}
+
+// Launch group/block/hub kernels over degree-class lists.
+template <typename GroupFn, typename BlockFn>
+void launch_classes(Context& c, const std::int32_t* lists, std::int64_t stride, const unsigned long long* counts,
+                    GroupFn group, BlockFn block) {
+    (void)c;
+    group(0, lists + 0 * stride, (std::int64_t)counts[0]);
+    group(1, lists + 1 * stride, (std::int64_t)counts[1]);
+    group(2, lists + 2 * stride, (std::int64_t)counts[2]);
+    group(3, lists + 3 * stride, (std::int64_t)counts[3]);
+    group(4, lists + 4 * stride, (std::int64_t)counts[4]);
+    block(lists + 5 * stride, (std::int64_t)counts[5]);
+}
+
+}  // namespace
+
+// ====================================================================== host
+
+// Global reduce of (row,key) records: sort, unique, atomic sums.
+static std::int64_t reduce_records(Context& c, DBuf<std::uint64_t>& keys, DBuf<std::uint32_t>& vals, std::int64_t count,
+                                   int total_bits, DBuf<std::uint64_t>& ukeys, DBuf<unsigned long long>& usums) {
+    radix_sort_pairs(c, keys.get(), vals.get(), count, 0, total_bits);
+    DBuf<std::uint8_t> heads = c.buf<std::uint8_t>(static_cast<std::size_t>(count));
+    DBuf<std::int64_t> pos = c.buf<std::int64_t>(static_cast<std::size_t>(count));
+    DBuf<std::int64_t> total = c.buf<std::int64_t>(1);
+    hipLaunchKernelGGL(k_run_heads, dim3(grid_for(count)), dim3(256), 0, c.stream(), keys.get(), count, heads.get());
+    GP_LAUNCH_CHECK();
+    inclusive_sum<std::int64_t>(c, heads.get(), pos.get(), count, total.get());
+    const std::int64_t nu = c.read_scalar(total.get());
+    ukeys = c.buf<std::uint64_t>(static_cast<std::size_t>(std::max<std::int64_t>(nu, 1)));
+    usums = c.buf<unsigned long long>(static_cast<std::size_t>(std::max<std::int64_t>(nu, 1)));
+    fill<unsigned long long>(c, usums.get(), nu, 0ULL);
+    hipLaunchKernelGGL(k_run_accumulate, dim3(grid_for(count)), dim3(256), 0, c.stream(), keys.get(), vals.get(), count,
+                       pos.get(), heads.get(), ukeys.get(), usums.get());
+    GP_LAUNCH_CHECK();
+    return nu;
+}
+
+void compute_level0_metadata(Context& c, DGraph& g, CoarsenState& st) {
+    st.row_bound = c.buf<std::uint64_t>(static_cast<std::size_t>(g.n));
+    DBuf<MetaOut> out = c.buf<MetaOut>(1);
+    fill<unsigned long long>(c, reinterpret_cast<unsigned long long*>(out.get()), sizeof(MetaOut) / 8, 0ULL);
+    if (g.n > 0) {
+        hipLaunchKernelGGL(k_level0_meta, dim3(grid_for(g.n)), dim3(256), 0, c.stream(), g.n, g.off.get(), g.nbr.get(),
+                           g.ew_ptr(), g.vw.get(), st.row_bound.get(), out.get());
+        GP_LAUNCH_CHECK();
+    }
+    MetaOut h = c.read_scalar(out.get());
+    st.max_row_weight = h.max_row;
+    g.incident_bound = h.max_row;
+    st.sorted_unique_rows = h.unsorted == 0;
+    st.positive_ew = (h.flags & 1ULL) == 0;
+    st.unit_ew = (h.flags & 2ULL) == 0;
+    g.max_degree = h.max_deg;
+    g.has_heavy = h.max_deg >= 4096;
+    st.total_weight = h.total_vw;
+    st.max_vertex_weight = h.max_vw;
+}
+
+AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int parts, std::uint32_t seed, int level,
+                          int level_factor, bool pack_isolated, bool seed_matching, bool fast_path,
+                          DBuf<std::int32_t>& map, DBuf<std::uint32_t>& coarse_vw) {
+    const std::int64_t n = g.n;
+    const hipStream_t s = c.stream();
+    const std::uint64_t total = st.total_weight, maxv = st.max_vertex_weight;
+    const std::uint64_t denom = (std::uint64_t)level_factor * (std::uint64_t)parts;
+    const std::uint64_t cap = std::max(maxv, total / denom + (total % denom != 0));
+    AggregateResult res;
+    res.capacity = cap;
+
+    DBuf<std::int32_t> clusters = c.buf<std::int32_t>(n);
+    DBuf<unsigned long long> cw = c.buf<unsigned long long>(n);
+    DBuf<std::uint8_t> roles = c.buf<std::uint8_t>(n);
+    DBuf<std::int32_t> proposals = c.buf<std::int32_t>(n);
+    DBuf<std::uint64_t> gains = c.buf<std::uint64_t>(n);
+    DBuf<std::uint64_t> regret = c.buf<std::uint64_t>(n);
+    DBuf<std::uint32_t> ties = c.buf<std::uint32_t>(n);
+    DBuf<unsigned long long> scalar = c.buf<unsigned long long>(4);
+    const int vb = grid_for(n);
+
+    std::int64_t cluster_count = n;
+    if (seed_matching) {
+        DBuf<std::int32_t> choices = c.buf<std::int32_t>(n);
+        const std::uint32_t salt = seed ^ ((std::uint32_t)level * 0x9e3779b9U);
+        hipLaunchKernelGGL(k_match_choose, dim3(vb), dim3(256), 0, s, n, g.off.get(), g.nbr.get(), g.ew_ptr(),
+                           g.vw.get(), cap, salt, choices.get());
+        GP_LAUNCH_CHECK();
+        fill<unsigned long long>(c, cw.get(), n, 0ULL);
+        fill<unsigned long long>(c, scalar.get(), 1, 0ULL);
+        hipLaunchKernelGGL(k_match_roots, dim3(vb), dim3(256), 0, s, n, choices.get(), g.vw.get(), clusters.get(),
+                           cw.get(), scalar.get());
+        GP_LAUNCH_CHECK();
+        cluster_count = (std::int64_t)c.read_scalar(scalar.get());
+    } else {
+        hipLaunchKernelGGL(k_init_singletons, dim3(vb), dim3(256), 0, s, n, g.vw.get(), clusters.get(), cw.get());
+        GP_LAUNCH_CHECK();
+    }
+
+    // Degree classes (lists built once per level).
+    DBuf<std::int32_t> lists = c.buf<std::int32_t>(static_cast<std::size_t>(7 * n));
+    DBuf<ClassCounts> ccount = c.buf<ClassCounts>(1);
+    fill<unsigned long long>(c, reinterpret_cast<unsigned long long*>(ccount.get()), 8, 0ULL);
+    hipLaunchKernelGGL(k_degree_classes, dim3(vb), dim3(256), 0, s, n, g.off.get(), lists.get(), n, ccount.get());
+    GP_LAUNCH_CHECK();
+    ClassCounts cc = c.read_scalar(ccount.get());
+    std::uint64_t stats_maxdeg = cc.c[7];
+    bool stats_heavy = cc.c[7] >= 4096;
+    if (g.max_degree <= 8) {  // a bound <= 8 is trusted (src/coarsen.cu:84-90)
+        stats_maxdeg = g.max_degree;
+        stats_heavy = false;
+    }
+    res.stats_max_degree = stats_maxdeg;
+    res.stats_has_heavy = stats_heavy;
+
+    // Hub record layout (rows > 4096) prepared once per level.
+    const std::int64_t hub_count = (std::int64_t)cc.c[6];
+    const std::int32_t* hub_list = lists.get() + 6 * n;
+    DBuf<std::int64_t> hub_off;
+    std::int64_t hub_records = 0;
+    if (hub_count) {
+        hub_off = c.buf<std::int64_t>(static_cast<std::size_t>(hub_count));
+        DBuf<std::int64_t> tot = c.buf<std::int64_t>(1);
+        hipLaunchKernelGGL((k_list_lengths<std::int32_t>), dim3(grid_for(hub_count)), dim3(256), 0, s, hub_list,
+                           hub_count, g.off.get(), hub_off.get());
+        GP_LAUNCH_CHECK();
+        exclusive_sum<std::int64_t>(c, hub_off.get(), hub_off.get(), hub_count, tot.get());
+        hub_records = c.read_scalar(tot.get());
+    }
+    const int cluster_bits = std::max(1, bit_width_u64(static_cast<std::uint64_t>(n)));
+    const int hub_bits = std::max(1, bit_width_u64(static_cast<std::uint64_t>(hub_count)));
+
+    auto form_proposals = [&](int round, bool filter_roles, std::uint32_t threshold, bool need_regret) {
+        const std::uint32_t role_salt =
+            seed ^ ((std::uint32_t)level * 0x9e3779b9U) ^ ((std::uint32_t)round * 0x85ebca6bU) ^ 0x27d4eb2dU;
+        const std::uint32_t tie_salt =
+            seed ^ ((std::uint32_t)level * 0xc2b2ae35U) ^ ((std::uint32_t)round * 0x165667b1U) ^ 0xd3a2646cU;
+        hipLaunchKernelGGL(k_roles, dim3(vb), dim3(256), 0, s, n, clusters.get(), g.vw.get(), cw.get(), role_salt,
+                           threshold, !filter_roles, roles.get(), proposals.get());
+        GP_LAUNCH_CHECK();
+        AffinityArgs a{g.off.get(), g.nbr.get(), g.ew_ptr(), clusters.get(), cw.get(), roles.get(), cap, tie_salt,
+                       filter_roles, filter_roles && need_regret, proposals.get(), gains.get(), regret.get(),
+                       ties.get()};
+        auto group = [&](int cls, const std::int32_t* list, std::int64_t count) {
+            if (!count) return;
+            switch (cls) {
+                case 0:
+                    hipLaunchKernelGGL((k_affinity_group<16, 1>), dim3(grid_for(count, 16)), dim3(256), 0, s, a, list,
+                                       count);
+                    break;
+                case 1:
+                    hipLaunchKernelGGL((k_affinity_group<64, 1>), dim3(grid_for(count, 4)), dim3(256), 0, s, a, list,
+                                       count);
+                    break;
+                case 2:
+                    hipLaunchKernelGGL((k_affinity_group<64, 2>), dim3(grid_for(count, 4)), dim3(256), 0, s, a, list,
+                                       count);
+                    break;
+                case 3:
+                    hipLaunchKernelGGL((k_affinity_group<64, 4>), dim3(grid_for(count, 4)), dim3(256), 0, s, a, list,
+                                       count);
+                    break;
+                default:
+                    hipLaunchKernelGGL((k_affinity_group<64, 8>), dim3(grid_for(count, 4)), dim3(256), 0, s, a, list,
+                                       count);
+                    break;
+            }
+            GP_LAUNCH_CHECK();
+        };
+        auto block = [&](const std::int32_t* list, std::int64_t count) {
+            if (!count) return;
+            hipLaunchKernelGGL(k_affinity_block, dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, s, a, list,
+                               count);
+            GP_LAUNCH_CHECK();
+        };
+        launch_classes(c, lists.get(), n, cc.c, group, block);
+        if (hub_count) {
+            DBuf<std::uint64_t> keys = c.buf<std::uint64_t>(static_cast<std::size_t>(hub_records));
+            DBuf<std::uint32_t> vals = c.buf<std::uint32_t>(static_cast<std::size_t>(hub_records));
+            hipLaunchKernelGGL(k_hub_records, dim3(grid_for(hub_count, 1, 4096)), dim3(256), 0, s, a, hub_list,
+                               hub_count, hub_off.get(), cluster_bits, keys.get(), vals.get());
+            GP_LAUNCH_CHECK();
+            DBuf<std::uint64_t> ukeys;
+            DBuf<unsigned long long> usums;
+            const std::int64_t nu =
+                reduce_records(c, keys, vals, hub_records, cluster_bits + hub_bits, ukeys, usums);
+            DBuf<HubTop2> acc = c.buf<HubTop2>(static_cast<std::size_t>(hub_count));
+            fill<unsigned long long>(c, reinterpret_cast<unsigned long long*>(acc.get()), 4 * hub_count, 0ULL);
+            if (nu) {
+                hipLaunchKernelGGL(k_hub_best, dim3(grid_for(nu)), dim3(256), 0, s, a, hub_list, ukeys.get(),
+                                   usums.get(), nu, cluster_bits, acc.get());
+                GP_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_hub_tie, dim3(grid_for(nu)), dim3(256), 0, s, a, hub_list, ukeys.get(),
+                                   usums.get(), nu, cluster_bits, acc.get());
+                GP_LAUNCH_CHECK();
+            }
+            hipLaunchKernelGGL(k_hub_decode, dim3(grid_for(hub_count)), dim3(256), 0, s, a, hub_list, hub_count,
+                               acc.get());
+            GP_LAUNCH_CHECK();
+        }
+    };
+
+    auto count_clusters = [&]() {
+        fill<unsigned long long>(c, scalar.get(), 1, 0ULL);
+        hipLaunchKernelGGL(k_count_nonzero, dim3(vb), dim3(256), 0, s, n, cw.get(), scalar.get());
+        GP_LAUNCH_CHECK();
+        return (std::int64_t)c.read_scalar(scalar.get());
+    };
+
+    const int score_bits = std::max(1, std::min(64, bit_width_u64(st.max_row_weight)));
+    const int target_bits = std::max(1, bit_width_u64(static_cast<std::uint64_t>(n - 1)));
+    // Stable selection of valid proposals then LSD sort by
+    // (target asc, score desc, tie desc); vertex order breaks ties.
+    auto admission_order = [&](const std::uint64_t* score, DBuf<std::uint64_t>& keys,
+                               DBuf<std::uint32_t>& vals) -> std::int64_t {
+        DBuf<std::uint8_t> flags = c.buf<std::uint8_t>(n);
+        DBuf<std::int32_t> order = c.buf<std::int32_t>(n);
+        DBuf<std::int64_t> cnt = c.buf<std::int64_t>(1);
+        hipLaunchKernelGGL(k_valid_flags, dim3(vb), dim3(256), 0, s, n, proposals.get(), flags.get());
+        GP_LAUNCH_CHECK();
+        stable_select(c, flags.get(), n, order.get(), cnt.get());
+        const std::int64_t count = c.read_scalar(cnt.get());
+        if (!count) return 0;
+        keys = c.buf<std::uint64_t>(static_cast<std::size_t>(count));
+        vals = c.buf<std::uint32_t>(static_cast<std::size_t>(count));
+        const bool fits = 32 + score_bits + target_bits <= 64;
+        const int low_bits = std::min(score_bits, 32);
+        hipLaunchKernelGGL(k_admission_keys, dim3(grid_for(count)), dim3(256), 0, s, count, order.get(),
+                           proposals.get(), score, ties.get(), fits ? score_bits : low_bits, fits, keys.get(),
+                           vals.get());
+        GP_LAUNCH_CHECK();
+        if (fits) {
+            radix_sort_pairs(c, keys.get(), vals.get(), count, 0, 32 + score_bits + target_bits);
+        } else {
+            radix_sort_pairs(c, keys.get(), vals.get(), count, 0, 32 + low_bits);
+            if (score_bits > low_bits) {
+                hipLaunchKernelGGL(k_key_field, dim3(grid_for(count)), dim3(256), 0, s, count, vals.get(),
+                                   proposals.get(), score, 1, low_bits, keys.get());
+                GP_LAUNCH_CHECK();
+                radix_sort_pairs(c, keys.get(), vals.get(), count, 0, score_bits - low_bits);
+            }
+            hipLaunchKernelGGL(k_key_field, dim3(grid_for(count)), dim3(256), 0, s, count, vals.get(),
+                               proposals.get(), score, 0, 0, keys.get());
+            GP_LAUNCH_CHECK();
+            radix_sort_pairs(c, keys.get(), vals.get(), count, 0, target_bits);
+        }
+        return count;
+    };
+
+    const std::int64_t desired = (n + 1) / 2;
+    std::uint64_t bound = seed_matching ? (maxv > cap / 2 ? cap : 2 * maxv) : maxv;
+    for (int round = 0; round < 4; ++round) {
+        if ((double)cluster_count <= 0.60 * (double)n) break;
+        const std::int64_t removable = std::max<std::int64_t>(0, cluster_count - desired);
+        const double frac = std::min(0.50, (double)removable / (double)cluster_count);
+        const std::uint32_t threshold = (std::uint32_t)(frac * 4294967296.0);
+        const std::uint64_t room = cap - bound;
+        const std::uint64_t degree = stats_maxdeg;
+        const bool fit = fast_path && (degree == 0 || bound <= room / maxv / degree);
+        const std::uint64_t next_bound = fit ? bound + bound * degree * maxv : cap;
+        form_proposals(round, true, threshold, !fit);
+        if (fit) {
+            hipLaunchKernelGGL(k_commit_all, dim3(vb), dim3(256), 0, s, n, proposals.get(), g.vw.get(),
+                               clusters.get(), cw.get());
+            GP_LAUNCH_CHECK();
+        } else {
+            DBuf<std::uint64_t> keys;
+            DBuf<std::uint32_t> vals;
+            const std::int64_t count = admission_order(regret.get(), keys, vals);
+            if (count) {
+                DBuf<SegU64> items = c.buf<SegU64>(static_cast<std::size_t>(count));
+                DBuf<std::uint64_t> base = c.buf<std::uint64_t>(static_cast<std::size_t>(count));
+                hipLaunchKernelGGL(k_seg_items, dim3(grid_for(count)), dim3(256), 0, s, count, vals.get(),
+                                   proposals.get(), g.vw.get(), false, items.get());
+                GP_LAUNCH_CHECK();
+                scan<SegPlusOp, SegU64, SegU64, SegU64>(c, items.get(), items.get(), count, false);
+                hipLaunchKernelGGL(k_gather_base, dim3(grid_for(count)), dim3(256), 0, s, count, vals.get(),
+                                   proposals.get(), cw.get(), base.get());
+                GP_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_admission_commit, dim3(grid_for(count)), dim3(256), 0, s, count, vals.get(),
+                                   proposals.get(), g.vw.get(), items.get(), base.get(), cap, clusters.get(),
+                                   cw.get());
+                GP_LAUNCH_CHECK();
+            }
+        }
+        cluster_count = count_clusters();
+        bound = next_bound;
+        if ((double)cluster_count <= 0.60 * (double)n) break;
+    }
+
+    if (pack_isolated) {
+        DBuf<std::uint8_t> flags = c.buf<std::uint8_t>(n);
+        DBuf<std::int32_t> iso = c.buf<std::int32_t>(n);
+        DBuf<std::int64_t> cnt = c.buf<std::int64_t>(1);
+        hipLaunchKernelGGL(k_isolated_flags, dim3(vb), dim3(256), 0, s, n, g.off.get(), flags.get());
+        GP_LAUNCH_CHECK();
+        stable_select(c, flags.get(), n, iso.get(), cnt.get());
+        const std::int64_t count = c.read_scalar(cnt.get());
+        if (count > 1) {
+            DBuf<std::uint64_t> prefix = c.buf<std::uint64_t>(static_cast<std::size_t>(count));
+            DBuf<std::uint64_t> tot = c.buf<std::uint64_t>(1);
+            hipLaunchKernelGGL(k_gather_vw, dim3(grid_for(count)), dim3(256), 0, s, count, iso.get(), g.vw.get(),
+                               prefix.get());
+            GP_LAUNCH_CHECK();
+            inclusive_sum<std::uint64_t>(c, prefix.get(), prefix.get(), count, tot.get());
+            const std::uint64_t t = c.read_scalar(tot.get());
+            const std::uint64_t bins = t / cap + (t % cap != 0);
+            if (!bins) throw std::runtime_error("isolated vertices carry no weight");
+            DBuf<std::int32_t> rep = c.buf<std::int32_t>(static_cast<std::size_t>(bins));
+            fill<std::int32_t>(c, rep.get(), (std::int64_t)bins, 0x7fffffff);
+            hipLaunchKernelGGL(k_isolated_rep, dim3(grid_for(count)), dim3(256), 0, s, count, iso.get(), prefix.get(),
+                               g.vw.get(), cap, rep.get());
+            GP_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_isolated_commit, dim3(grid_for(count)), dim3(256), 0, s, count, iso.get(),
+                               prefix.get(), g.vw.get(), cap, rep.get(), clusters.get(), cw.get());
+            GP_LAUNCH_CHECK();
+            cluster_count = count_clusters();
+        }
+    }
+
+    if ((double)cluster_count > 0.60 * (double)n) {
+        form_proposals(4, false, 0, false);
+        hipLaunchKernelGGL(k_favorites, dim3(vb), dim3(256), 0, s, n, g.vw.get(), cw.get(), clusters.get(),
+                           proposals.get());
+        GP_LAUNCH_CHECK();
+        DBuf<std::uint64_t> keys;
+        DBuf<std::uint32_t> vals;
+        const std::int64_t count = admission_order(gains.get(), keys, vals);
+        if (count) {
+            DBuf<SegU64> ranks = c.buf<SegU64>(static_cast<std::size_t>(count));
+            hipLaunchKernelGGL(k_seg_items, dim3(grid_for(count)), dim3(256), 0, s, count, vals.get(), proposals.get(),
+                               g.vw.get(), true, ranks.get());
+            GP_LAUNCH_CHECK();
+            scan<SegPlusOp, SegU64, SegU64, SegU64>(c, ranks.get(), ranks.get(), count, false);
+            DBuf<std::uint8_t> pflags = c.buf<std::uint8_t>(static_cast<std::size_t>(count));
+            DBuf<std::int64_t> pos = c.buf<std::int64_t>(static_cast<std::size_t>(count));
+            hipLaunchKernelGGL(k_pair_flags, dim3(grid_for(count)), dim3(256), 0, s, count, vals.get(), g.vw.get(),
+                               ranks.get(), cap, pflags.get());
+            GP_LAUNCH_CHECK();
+            exclusive_sum<std::int64_t>(c, pflags.get(), pos.get(), count);
+            const std::int64_t budget = std::max<std::int64_t>(0, cluster_count - desired);
+            if (budget > 0) {
+                hipLaunchKernelGGL(k_pair_commit, dim3(grid_for(count)), dim3(256), 0, s, count, vals.get(),
+                                   pflags.get(), pos.get(), budget, g.vw.get(), clusters.get(), cw.get());
+                GP_LAUNCH_CHECK();
+            }
+        }
+        cluster_count = count_clusters();
+    }
+
+    // Compaction.
+    {
+        DBuf<std::uint8_t> flags = c.buf<std::uint8_t>(n);
+        DBuf<std::int64_t> ids = c.buf<std::int64_t>(n);
+        DBuf<std::int64_t> cnt = c.buf<std::int64_t>(1);
+        hipLaunchKernelGGL(k_nonempty, dim3(vb), dim3(256), 0, s, n, cw.get(), flags.get());
+        GP_LAUNCH_CHECK();
+        exclusive_sum<std::int64_t>(c, flags.get(), ids.get(), n, cnt.get());
+        const std::int64_t nc = c.read_scalar(cnt.get());
+        map = c.buf<std::int32_t>(n);
+        DBuf<unsigned long long> sums = c.buf<unsigned long long>(static_cast<std::size_t>(nc));
+        fill<unsigned long long>(c, sums.get(), nc, 0ULL);
+        hipLaunchKernelGGL(k_compact_map, dim3(vb), dim3(256), 0, s, n, clusters.get(), ids.get(), g.vw.get(),
+                           map.get(), sums.get());
+        GP_LAUNCH_CHECK();
+        coarse_vw = c.buf<std::uint32_t>(static_cast<std::size_t>(nc));
+        fill<unsigned long long>(c, scalar.get(), 1, 0ULL);
+        hipLaunchKernelGGL(k_narrow_weights, dim3(grid_for(nc)), dim3(256), 0, s, nc, sums.get(), coarse_vw.get(),
+                           scalar.get());
+        GP_LAUNCH_CHECK();
+        res.max_weight = c.read_scalar(scalar.get());
+        res.coarse_n = nc;
+        if (res.max_weight > 0xffffffffULL) throw std::runtime_error("coarse vertex weight exceeds uint32");
+        if (res.max_weight > cap) throw std::runtime_error("coarsening exceeded cluster capacity");
+    }
+    return res;
+}
+
+std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& st, const std::int32_t* map,
+                                 std::int64_t nc, DBuf<std::uint32_t>&& coarse_vw) {
+    const std::int64_t n = fine.n;
+    const hipStream_t s = c.stream();
+    auto out = std::make_unique<DGraph>();
+    out->n = nc;
+    // Bounds carried through the map (src/coarsen.cu:1470-1513).
+    DBuf<unsigned long long> next_rb = c.buf<unsigned long long>(static_cast<std::size_t>(nc));
+    DBuf<unsigned long long> rawc = c.buf<unsigned long long>(static_cast<std::size_t>(nc));
+    fill<unsigned long long>(c, next_rb.get(), nc, 0ULL);
+    fill<unsigned long long>(c, rawc.get(), nc, 0ULL);
+    hipLaunchKernelGGL(k_contract_bounds, dim3(grid_for(n)), dim3(256), 0, s, n, map, fine.off.get(), st.row_bound.get(),
+                       next_rb.get(), rawc.get());
+    GP_LAUNCH_CHECK();
+    DBuf<unsigned long long> red = c.buf<unsigned long long>(2);
+    fill<unsigned long long>(c, red.get(), 2, 0ULL);
+    hipLaunchKernelGGL((k_reduce_max<unsigned long long>), dim3(grid_for(nc)), dim3(256), 0, s, next_rb.get(), nc,
+                       red.get());
+    hipLaunchKernelGGL((k_reduce_max<unsigned long long>), dim3(grid_for(nc)), dim3(256), 0, s, rawc.get(), nc,
+                       red.get() + 1);
+    GP_LAUNCH_CHECK();
+    unsigned long long rb[2];
+    c.read(rb, red.get(), sizeof rb);
+    const bool count_raw = st.positive_ew && n >= 65536 && fine.max_degree <= 512;
+    const std::uint64_t crb = rb[0];
+    const std::uint64_t craw = count_raw ? rb[1] : ~0ULL;
+    const std::uint64_t next_degree_bound = count_raw && fine.max_degree <= 128 ? craw : crb;
+
+    // Raw coarse rows.
+    DBuf<std::int64_t> raw_off = c.buf<std::int64_t>(static_cast<std::size_t>(nc + 1));
+    DBuf<std::int64_t> rawtot = c.buf<std::int64_t>(1);
+    exclusive_sum<std::int64_t>(c, rawc.get(), raw_off.get(), nc, rawtot.get());
+    c.copy_d2d(raw_off.get() + nc, rawtot.get(), sizeof(std::int64_t));
+    DBuf<unsigned long long> cursor = c.buf<unsigned long long>(static_cast<std::size_t>(nc));
+    c.copy_d2d(cursor.get(), raw_off.get(), static_cast<std::size_t>(nc) * sizeof(std::int64_t));
+    const std::int64_t m = fine.m;
+    DBuf<std::int32_t> rkey = c.buf<std::int32_t>(static_cast<std::size_t>(std::max<std::int64_t>(m, 1)));
+    DBuf<std::uint32_t> rval = c.buf<std::uint32_t>(static_cast<std::size_t>(std::max<std::int64_t>(m, 1)));
+    if (m > 0) {
+        hipLaunchKernelGGL(k_contract_scatter, dim3(grid_for(n)), dim3(256), 0, s, n, fine.off.get(), fine.nbr.get(),
+                           fine.ew_ptr(), map, cursor.get(), rkey.get(), rval.get());
+        GP_LAUNCH_CHECK();
+    }
+    cursor.reset();
+    // Row classes by raw length.
+    DBuf<std::int32_t> lists = c.buf<std::int32_t>(static_cast<std::size_t>(7 * nc));
+    DBuf<ClassCounts> ccount = c.buf<ClassCounts>(1);
+    fill<unsigned long long>(c, reinterpret_cast<unsigned long long*>(ccount.get()), 8, 0ULL);
+    hipLaunchKernelGGL(k_row_classes, dim3(grid_for(nc)), dim3(256), 0, s, nc, raw_off.get(), lists.get(), nc,
+                       ccount.get());
+    GP_LAUNCH_CHECK();
+    ClassCounts cc = c.read_scalar(ccount.get());
+    DBuf<unsigned long long> ucount = c.buf<unsigned long long>(static_cast<std::size_t>(nc));
+    fill<unsigned long long>(c, ucount.get(), nc, 0ULL);
+    DBuf<unsigned long long> overflow = c.buf<unsigned long long>(1);
+    fill<unsigned long long>(c, overflow.get(), 1, 0ULL);
+    RowArgs r{raw_off.get(), rkey.get(), rval.get(), ucount.get(), overflow.get()};
+    auto group = [&](int cls, const std::int32_t* list, std::int64_t count) {
+        if (!count) return;
+        switch (cls) {
+            case 0:
+                hipLaunchKernelGGL((k_rows_group<16, 1>), dim3(grid_for(count, 16)), dim3(256), 0, s, r, list, count);
+                break;
+            case 1:
+                hipLaunchKernelGGL((k_rows_group<64, 1>), dim3(grid_for(count, 4)), dim3(256), 0, s, r, list, count);
+                break;
+            case 2:
+                hipLaunchKernelGGL((k_rows_group<64, 2>), dim3(grid_for(count, 4)), dim3(256), 0, s, r, list, count);
+                break;
+            case 3:
+                hipLaunchKernelGGL((k_rows_group<64, 4>), dim3(grid_for(count, 4)), dim3(256), 0, s, r, list, count);
+                break;
+            default:
+                hipLaunchKernelGGL((k_rows_group<64, 8>), dim3(grid_for(count, 4)), dim3(256), 0, s, r, list, count);
+                break;
+        }
+        GP_LAUNCH_CHECK();
+    };
+    auto block = [&](const std::int32_t* list, std::int64_t count) {
+        if (!count) return;
+        hipLaunchKernelGGL(k_rows_block, dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, s, r, list, count);
+        GP_LAUNCH_CHECK();
+    };
+    launch_classes(c, lists.get(), nc, cc.c, group, block);
+    const std::int64_t long_count = (std::int64_t)cc.c[6];
+    if (long_count) {
+        const std::int32_t* long_list = lists.get() + 6 * nc;
+        DBuf<std::int64_t> rec_off = c.buf<std::int64_t>(static_cast<std::size_t>(long_count));
+        DBuf<std::int64_t> rtot = c.buf<std::int64_t>(1);
+        hipLaunchKernelGGL((k_list_lengths<std::int64_t>), dim3(grid_for(long_count)), dim3(256), 0, s, long_list,
+                           long_count, raw_off.get(), rec_off.get());
+        GP_LAUNCH_CHECK();
+        exclusive_sum<std::int64_t>(c, rec_off.get(), rec_off.get(), long_count, rtot.get());
+        const std::int64_t records = c.read_scalar(rtot.get());
+        const int key_bits = std::max(1, bit_width_u64(static_cast<std::uint64_t>(nc)));
+        const int row_bits = std::max(1, bit_width_u64(static_cast<std::uint64_t>(long_count)));
+        DBuf<std::uint64_t> keys = c.buf<std::uint64_t>(static_cast<std::size_t>(records));
+        DBuf<std::uint32_t> vals = c.buf<std::uint32_t>(static_cast<std::size_t>(records));
+        hipLaunchKernelGGL(k_long_records, dim3(grid_for(long_count, 1, 4096)), dim3(256), 0, s, r, long_list,
+                           long_count, rec_off.get(), key_bits, keys.get(), vals.get());
+        GP_LAUNCH_CHECK();
+        DBuf<std::uint64_t> ukeys;
+        DBuf<unsigned long long> usums;
+        const std::int64_t nu = reduce_records(c, keys, vals, records, key_bits + row_bits, ukeys, usums);
+        if (nu) {
+            DBuf<unsigned long long> first = c.buf<unsigned long long>(static_cast<std::size_t>(long_count));
+            DBuf<unsigned long long> last = c.buf<unsigned long long>(static_cast<std::size_t>(long_count));
+            fill<unsigned long long>(c, first.get(), long_count, ~0ULL);
+            fill<unsigned long long>(c, last.get(), long_count, 0ULL);
+            hipLaunchKernelGGL(k_long_bounds, dim3(grid_for(nu)), dim3(256), 0, s, ukeys.get(), nu, key_bits,
+                               first.get(), last.get());
+            GP_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_long_writeback, dim3(grid_for(nu)), dim3(256), 0, s, r, long_list, ukeys.get(),
+                               usums.get(), nu, key_bits, first.get(), last.get());
+            GP_LAUNCH_CHECK();
+        }
+    }
+    if (c.read_scalar(overflow.get())) throw std::runtime_error("coarse edge weight exceeds uint32");
+    // Final offsets and copy.
+    DBuf<std::int64_t> coff64 = c.buf<std::int64_t>(static_cast<std::size_t>(nc + 1));
+    DBuf<std::int64_t> mtot = c.buf<std::int64_t>(1);
+    exclusive_sum<std::int64_t>(c, ucount.get(), coff64.get(), nc, mtot.get());
+    const std::int64_t mc = c.read_scalar(mtot.get());
+    if (mc > 0x7fffffffLL) throw std::runtime_error("coarse graph exceeds int32 offsets");
+    out->m = mc;
+    out->off = c.buf<std::int32_t>(static_cast<std::size_t>(nc + 1));
+    out->nbr = c.buf<std::int32_t>(static_cast<std::size_t>(std::max<std::int64_t>(mc, 1)));
+    out->ew = c.buf<std::uint32_t>(static_cast<std::size_t>(std::max<std::int64_t>(mc, 1)));
+    hipLaunchKernelGGL(k_copy_rows, dim3(grid_for(nc)), dim3(256), 0, s, nc, raw_off.get(), rkey.get(), rval.get(),
+                       coff64.get(), out->off.get(), out->nbr.get(), out->ew.get(), mc);
+    GP_LAUNCH_CHECK();
+    out->vw = std::move(coarse_vw);
+    out->incident_bound = crb;
+    if (st.positive_ew) {
+        out->max_degree = next_degree_bound;
+        out->has_heavy = next_degree_bound >= 4096;
+    } else {
+        out->max_degree = ~0ULL;
+        out->has_heavy = true;
+    }
+    st.max_row_weight = crb;
+    st.row_bound = c.buf<std::uint64_t>(static_cast<std::size_t>(nc));
+    c.copy_d2d(st.row_bound.get(), next_rb.get(), static_cast<std::size_t>(nc) * sizeof(std::uint64_t));
+    return out;
+}
+
+Hierarchy coarsen(Context& c, DGraph& input, const CoarsenOptions& o) {
+    if (input.n <= 0 || input.n > 0x7fffffffLL) throw std::invalid_argument("unsupported graph size");
+    if (o.parts < 2 || o.parts > kMaxParts || o.cluster_size_factor < 2 || o.max_levels < 0)
+        throw std::invalid_argument("invalid coarsening options");
+    Hierarchy h;
+    h.input = &input;
+    CoarsenState st;
+    compute_level0_metadata(c, input, st);
+    const int k = o.parts;
+    const std::int64_t cutoff = std::max<std::int64_t>(32, (std::int64_t)o.cluster_size_factor * k);
+    const DGraph* cur = &input;
+    std::uint64_t cur_max_degree = input.max_degree;
+    bool cur_heavy = input.has_heavy;
+    int level = 0;
+    for (; level < o.max_levels && cur->n > cutoff; ++level) {
+        const std::uint32_t level_seed = o.seed + (std::uint32_t)level;
+        int level_factor = o.cluster_size_factor;
+        if (o.progressive) {
+            const std::int64_t ff = std::max<std::int64_t>(o.fine_factor, (512 + k - 1) / k);
+            const std::int64_t den = (std::int64_t)o.capacity_growth * k;
+            const std::int64_t remaining = (cur->n + den - 1) / den;
+            level_factor = (int)std::max<std::int64_t>(level_factor, std::min(ff, remaining));
+        }
+        const bool try_matching = o.seed_matching && cur->n >= 65536 && !cur_heavy &&
+                                  (cur->m / cur->n < 8 || cur_max_degree <= 256) &&
+                                  cur->m / cur->n < o.hem_degree_limit && st.sorted_unique_rows && st.positive_ew;
+        DBuf<std::int32_t> map;
+        DBuf<std::uint32_t> cvw;
+        // Metadata of the level as seen by aggregate (bound, not rescanned).
+        DGraph* mut = const_cast<DGraph*>(cur);
+        mut->max_degree = cur_max_degree;
+        mut->has_heavy = cur_heavy;
+        AggregateResult r = aggregate(c, *cur, st, k, level_seed, level, level_factor, o.pack_isolated, try_matching,
+                                      o.fast_path, map, cvw);
+        cur_max_degree = r.stats_max_degree;
+        cur_heavy = r.stats_has_heavy;
+        mut->max_degree = cur_max_degree;
+        mut->has_heavy = cur_heavy;
+        const double ratio = (double)r.coarse_n / (double)cur->n;
+        if (r.coarse_n >= cur->n || ratio > o.stop_ratio) {
+            h.stop_reason = "insufficient_contraction";
+            break;
+        }
+        auto coarse = contract(c, *cur, st, map.get(), r.coarse_n, std::move(cvw));
+        st.max_vertex_weight = r.max_weight;
+        h.maps.push_back(std::move(map));
+        h.coarse.push_back(std::move(coarse));
+        cur = h.coarse.back().get();
+        cur_max_degree = cur->max_degree;
+        cur_heavy = cur->has_heavy;
+        st.sorted_unique_rows = true;
+    }
+    if (level == o.max_levels && cur->n > cutoff) h.stop_reason = "max_levels";
+    return h;
+}
+
+}  // namespace gp

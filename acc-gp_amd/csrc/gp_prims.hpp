@@ -1,0 +1,321 @@
+// Device primitives used by the partitioner: fills, reduce-then-scan scans
+// (plain and segmented), stable index selection and an LSD radix sort of
+// (u64 key, u32 value) pairs with wave64 multi-split ranking.  All launches go
+// to the context stream; nothing here allocates outside the arena.
+#pragma once
+#include "gp_runtime.hpp"
+
+namespace gp {
+
+// ------------------------------------------------------------------ fill
+template <typename T>
+__global__ void k_fill(T* p, std::int64_t n, T v) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (std::int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+template <typename T>
+inline void fill(Context& c, T* p, std::int64_t n, T v) {
+    if (n <= 0) return;
+    const int blocks = static_cast<int>(std::min<std::int64_t>(ceil_div(n, 256), 4096));
+    hipLaunchKernelGGL(k_fill<T>, dim3(blocks), dim3(256), 0, c.stream(), p, n, v);
+    GP_LAUNCH_CHECK();
+}
+
+template <typename T>
+__global__ void k_iota(T* p, std::int64_t n) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (std::int64_t)gridDim.x * blockDim.x)
+        p[i] = static_cast<T>(i);
+}
+template <typename T>
+inline void iota(Context& c, T* p, std::int64_t n) {
+    if (n <= 0) return;
+    const int blocks = static_cast<int>(std::min<std::int64_t>(ceil_div(n, 256), 4096));
+    hipLaunchKernelGGL(k_iota<T>, dim3(blocks), dim3(256), 0, c.stream(), p, n);
+    GP_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ scans
+struct PlusOp {
+    template <typename T>
+    __device__ static T id() { return T(0); }
+    template <typename T>
+    __device__ static T op(T a, T b) { return a + b; }
+};
+struct MaxOp {
+    template <typename T>
+    __device__ static T id() { return T(0); }
+    template <typename T>
+    __device__ static T op(T a, T b) { return a > b ? a : b; }
+};
+
+// Segmented value: head marks the first element of a segment.
+struct SegU64 {
+    std::uint64_t v;
+    std::uint32_t head;
+};
+struct SegPlusOp {
+    template <typename T>
+    __device__ static T id() { return T{0, 0}; }
+    template <typename T>
+    __device__ static T op(T a, T b) { return b.head ? b : T{a.v + b.v, a.head | b.head}; }
+};
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// Block-wide inclusive scan of one value per thread (Hillis-Steele in LDS).
+template <typename Op, typename T>
+__device__ inline T block_inclusive_scan(T x, T* lds) {
+    const int t = threadIdx.x;
+    lds[t] = x;
+    __syncthreads();
+    for (int d = 1; d < kScanThreads; d <<= 1) {
+        T y = t >= d ? Op::template op<T>(lds[t - d], lds[t]) : lds[t];
+        __syncthreads();
+        lds[t] = y;
+        __syncthreads();
+    }
+    T r = lds[t];
+    __syncthreads();
+    return r;
+}
+
+template <typename Op, typename T, typename In>
+__global__ void k_scan_reduce(const In* in, std::int64_t n, T* partial) {
+    __shared__ T lds[kScanThreads];
+    const std::int64_t base = blockIdx.x * (std::int64_t)kScanTile;
+    T acc = Op::template id<T>();
+    for (int i = 0; i < kScanItems; ++i) {
+        const std::int64_t idx = base + threadIdx.x * kScanItems + i;
+        if (idx < n) acc = Op::template op<T>(acc, static_cast<T>(in[idx]));
+    }
+    T r = block_inclusive_scan<Op>(acc, lds);
+    if (threadIdx.x == kScanThreads - 1) partial[blockIdx.x] = r;
+}
+
+// Exclusive scan of the block partials by one block (loops over chunks).
+template <typename Op, typename T>
+__global__ void k_scan_partials(T* partial, std::int64_t nb, T* total) {
+    __shared__ T lds[kScanThreads];
+    T carry = Op::template id<T>();
+    for (std::int64_t base = 0; base < nb; base += kScanThreads) {
+        const std::int64_t idx = base + threadIdx.x;
+        T x = idx < nb ? partial[idx] : Op::template id<T>();
+        T inc = block_inclusive_scan<Op>(x, lds);
+        lds[threadIdx.x] = inc;
+        __syncthreads();
+        // exclusive = carry (+) inclusive of the previous thread
+        T ex = threadIdx.x ? Op::template op<T>(carry, lds[threadIdx.x - 1]) : carry;
+        T block_total = lds[kScanThreads - 1];
+        __syncthreads();
+        if (idx < nb) partial[idx] = ex;
+        carry = Op::template op<T>(carry, block_total);
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+// Final pass; exclusive=true writes out[i] = prefix before i.
+template <typename Op, typename T, typename In, typename Out>
+__global__ void k_scan_apply(const In* in, Out* out, std::int64_t n, const T* partial, bool exclusive) {
+    __shared__ T lds[kScanThreads];
+    const std::int64_t base = blockIdx.x * (std::int64_t)kScanTile;
+    T vals[kScanItems];
+    T acc = Op::template id<T>();
+    for (int i = 0; i < kScanItems; ++i) {
+        const std::int64_t idx = base + threadIdx.x * kScanItems + i;
+        vals[i] = idx < n ? static_cast<T>(in[idx]) : Op::template id<T>();
+        acc = Op::template op<T>(acc, vals[i]);
+    }
+    T inc = block_inclusive_scan<Op>(acc, lds);
+    lds[threadIdx.x] = inc;
+    __syncthreads();
+    T run = Op::template op<T>(partial[blockIdx.x], threadIdx.x ? lds[threadIdx.x - 1] : Op::template id<T>());
+    for (int i = 0; i < kScanItems; ++i) {
+        const std::int64_t idx = base + threadIdx.x * kScanItems + i;
+        T next = Op::template op<T>(run, vals[i]);
+        if (idx < n) out[idx] = static_cast<Out>(exclusive ? run : next);
+        run = next;
+    }
+}
+
+// Scan n elements of In into Out (in == out allowed).  Accumulates in T.
+// total (device, optional) receives the reduction of all elements.
+template <typename Op, typename T, typename In, typename Out>
+inline void scan(Context& c, const In* in, Out* out, std::int64_t n, bool exclusive, T* total = nullptr) {
+    if (n <= 0) {
+        if (total) fill<T>(c, total, 1, T{});
+        return;
+    }
+    const std::int64_t nb = (n + kScanTile - 1) / kScanTile;
+    DBuf<T> partial = c.buf<T>(static_cast<std::size_t>(nb));
+    hipLaunchKernelGGL((k_scan_reduce<Op, T, In>), dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0,
+                       c.stream(), in, n, partial.get());
+    GP_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_scan_partials<Op, T>), dim3(1), dim3(kScanThreads), 0, c.stream(), partial.get(), nb,
+                       total);
+    GP_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_scan_apply<Op, T, In, Out>), dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0,
+                       c.stream(), in, out, n, partial.get(), exclusive);
+    GP_LAUNCH_CHECK();
+}
+
+template <typename T, typename In, typename Out>
+inline void exclusive_sum(Context& c, const In* in, Out* out, std::int64_t n, T* total = nullptr) {
+    scan<PlusOp, T, In, Out>(c, in, out, n, true, total);
+}
+template <typename T, typename In, typename Out>
+inline void inclusive_sum(Context& c, const In* in, Out* out, std::int64_t n, T* total = nullptr) {
+    scan<PlusOp, T, In, Out>(c, in, out, n, false, total);
+}
+
+// ------------------------------------------------------ stable selection
+// out[pos[i]] = i for flagged i, pos = exclusive scan of flags; count -> *count.
+static __global__ void k_select_scatter(const std::uint8_t* flags, const std::int64_t* pos, std::int64_t n,
+                                        std::int32_t* out) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (std::int64_t)gridDim.x * blockDim.x)
+        if (flags[i]) out[pos[i]] = static_cast<std::int32_t>(i);
+}
+inline void stable_select(Context& c, const std::uint8_t* flags, std::int64_t n, std::int32_t* out,
+                          std::int64_t* count_dev) {
+    DBuf<std::int64_t> pos = c.buf<std::int64_t>(static_cast<std::size_t>(n > 0 ? n : 1));
+    exclusive_sum<std::int64_t>(c, flags, pos.get(), n, count_dev);
+    if (n > 0) {
+        const int blocks = static_cast<int>(std::min<std::int64_t>(ceil_div(n, 256), 8192));
+        hipLaunchKernelGGL(k_select_scatter, dim3(blocks), dim3(256), 0, c.stream(), flags, pos.get(), n, out);
+        GP_LAUNCH_CHECK();
+    }
+}
+
+// ---------------------------------------------------------------- radix sort
+constexpr int kRadixThreads = 256;
+constexpr int kRadixItems = 16;
+constexpr int kRadixTile = kRadixThreads * kRadixItems;
+constexpr int kRadixBins = 256;
+
+static __global__ void k_radix_hist(const std::uint64_t* keys, std::int64_t n, int shift, std::uint32_t* hist,
+                                    int nblocks) {
+    __shared__ std::uint32_t h[kRadixBins];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const std::int64_t base = blockIdx.x * (std::int64_t)kRadixTile;
+    for (int r = 0; r < kRadixItems; ++r) {
+        const std::int64_t i = base + r * kRadixThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xff], 1u);
+    }
+    __syncthreads();
+    hist[(std::int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter.  Elements of a tile are ranked in index order: rounds of
+// 256 consecutive keys, wave-level peers via 8 ballots, per-wave digit counts.
+static __global__ void k_radix_scatter(const std::uint64_t* keys_in, const std::uint32_t* vals_in,
+                                       std::uint64_t* keys_out, std::uint32_t* vals_out, std::int64_t n, int shift,
+                                       const std::uint32_t* offsets, int nblocks) {
+    constexpr int kWaves = kRadixThreads / kWave;
+    __shared__ std::uint32_t running[kRadixBins];
+    __shared__ std::uint32_t wcount[kWaves][kRadixBins];
+    const int t = threadIdx.x, w = t / kWave, lane = t % kWave;
+    running[t] = offsets[(std::int64_t)t * nblocks + blockIdx.x];
+    for (int i = 0; i < kWaves; ++i) wcount[i][t] = 0;
+    __syncthreads();
+    const std::int64_t base = blockIdx.x * (std::int64_t)kRadixTile;
+    const unsigned long long lt = (1ULL << lane) - 1ULL;
+    for (int r = 0; r < kRadixItems; ++r) {
+        const std::int64_t i = base + r * kRadixThreads + t;
+        const bool valid = i < n;
+        std::uint64_t key = valid ? keys_in[i] : 0;
+        std::uint32_t val = valid && vals_in ? vals_in[i] : 0;
+        const std::uint32_t d = valid ? static_cast<std::uint32_t>((key >> shift) & 0xff) : 0;
+        unsigned long long peers = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const std::uint32_t rank = static_cast<std::uint32_t>(__popcll(peers & lt));
+        if (valid && rank == 0) wcount[w][d] = static_cast<std::uint32_t>(__popcll(peers));
+        __syncthreads();
+        if (valid) {
+            std::uint32_t off = running[d];
+            for (int j = 0; j < w; ++j) off += wcount[j][d];
+            off += rank;
+            keys_out[off] = key;
+            if (vals_out) vals_out[off] = val;
+        }
+        __syncthreads();
+        std::uint32_t add = 0;
+        for (int j = 0; j < kWaves; ++j) {
+            add += wcount[j][t];
+            wcount[j][t] = 0;
+        }
+        running[t] += add;
+        __syncthreads();
+    }
+}
+
+// Sorts (keys, vals) by bits [begin_bit, end_bit).  Uses alt buffers; the
+// result is copied back into keys/vals.  vals may be null.
+inline void radix_sort_pairs(Context& c, std::uint64_t* keys, std::uint32_t* vals, std::int64_t n, int begin_bit,
+                             int end_bit) {
+    if (n <= 1 || end_bit <= begin_bit) return;
+    if (n > 0xffffffffLL) throw std::invalid_argument("radix sort size exceeds 32-bit offsets");
+    const int nblocks = ceil_div(n, kRadixTile);
+    DBuf<std::uint64_t> k2 = c.buf<std::uint64_t>(static_cast<std::size_t>(n));
+    DBuf<std::uint32_t> v2;
+    if (vals) v2 = c.buf<std::uint32_t>(static_cast<std::size_t>(n));
+    DBuf<std::uint32_t> hist = c.buf<std::uint32_t>(static_cast<std::size_t>(kRadixBins) * nblocks);
+    std::uint64_t* kin = keys;
+    std::uint64_t* kout = k2.get();
+    std::uint32_t* vin = vals;
+    std::uint32_t* vout = v2.get();
+    for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kRadixThreads), 0, c.stream(), kin, n, shift,
+                           hist.get(), nblocks);
+        GP_LAUNCH_CHECK();
+        exclusive_sum<std::uint32_t>(c, hist.get(), hist.get(), static_cast<std::int64_t>(kRadixBins) * nblocks);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kRadixThreads), 0, c.stream(), kin, vin, kout,
+                           vout, n, shift, hist.get(), nblocks);
+        GP_LAUNCH_CHECK();
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    if (kin != keys) {
+        c.copy_d2d(keys, kin, static_cast<std::size_t>(n) * sizeof(std::uint64_t));
+        if (vals) c.copy_d2d(vals, vin, static_cast<std::size_t>(n) * sizeof(std::uint32_t));
+    }
+}
+
+inline int bit_width_u64(std::uint64_t v) {
+    int b = 0;
+    while (v) {
+        ++b;
+        v >>= 1;
+    }
+    return b;
+}
+
+// ---------------------------------------------------------- reductions
+template <typename T>
+__global__ void k_reduce_max(const T* in, std::int64_t n, unsigned long long* out) {
+    unsigned long long m = 0;
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (std::int64_t)gridDim.x * blockDim.x)
+        m = m > (unsigned long long)in[i] ? m : (unsigned long long)in[i];
+    m = wave_max(m);
+    if (lane_id() == 0 && m) atomicMax(out, m);
+}
+template <typename T>
+__global__ void k_reduce_sum(const T* in, std::int64_t n, unsigned long long* out) {
+    unsigned long long s = 0;
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (std::int64_t)gridDim.x * blockDim.x)
+        s += (unsigned long long)in[i];
+    s = wave_sum(s);
+    if (lane_id() == 0 && s) atomicAdd(out, s);
+}
+
+}  // namespace gp

@@ -1,0 +1,1970 @@
+// Refinement on gfx950: projection, label propagation with per-vertex
+// part-connectivity tables in LDS, afterburner priority filter, exact
+// incremental cut/load updates, deterministic capacity repair and the
+// capacity-complementary exchange.  Restates src/refine.cu and
+// src/refinement/*.cuh of the reference; order-dependent steps follow the
+// canonical vertex-id order.
+#include "gp_refine.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+
+namespace gp {
+namespace {
+
+constexpr int kLightDeg = 64;     // 16-lane groups
+constexpr int kHeavyDeg = 1024;   // one block per row above this
+constexpr int kBalanceBuckets = 50;
+constexpr int kBalanceSections = 16;
+constexpr int kBalanceWidth = kBalanceBuckets * kBalanceSections;
+constexpr int kJointSeeds = 128;
+constexpr int kJointSide = 4;
+constexpr int kJointArchive = 4;
+
+// Counter slots (device array of u64).
+enum Ctr {
+    kFrontier = 0,
+    kCand = 1,
+    kDefer = 2,
+    kMovers = 3,
+    kDelta = 4,
+    kDonors = 5,
+    kCtrCount = 8
+};
+
+struct RepairSample {
+    unsigned long long loss;    // modular sum of -gain
+    unsigned long long weight;
+};
+
+struct Pow15 {
+    double p[48];  // 1.5^j for j=-23..24 at index j+23
+};
+
+struct RefArgs {
+    std::int64_t n;
+    const std::int32_t* off;
+    const std::int32_t* nbr;
+    const std::uint32_t* ew;
+    const std::uint32_t* vw;
+    int k;
+    std::uint64_t cap;
+    double filter;
+    std::int32_t* labels;
+    unsigned long long* loads;
+    const double* prices;  // null unless vertex weights exceed n
+    std::int32_t* targets;
+    std::int64_t* gains;
+    std::int32_t* locks;
+    std::int32_t* flags;
+    std::int32_t* member;
+    std::int32_t* frontier;
+    std::int32_t* cand;
+    std::int32_t* movers;
+    std::int32_t* defer;
+    unsigned long long* ctr;
+    RepairSample* samples;
+    // repair
+    const unsigned long long* loads0;
+    std::int32_t* bucket;
+    unsigned long long* counts;
+    Pow15 pow15;
+};
+
+__device__ inline void touch(const RefArgs& a, std::int32_t u) {
+    if (a.member[u] == 0 && atomicExch(&a.member[u], 1) == 0)
+        a.frontier[atomicAdd(&a.ctr[kFrontier], 1ULL)] = u;
+}
+
+__device__ inline void append(std::int32_t* list, unsigned long long* count, std::int32_t v) {
+    list[atomicAdd(count, 1ULL)] = v;
+}
+
+// Part-connectivity table of one vertex in group-private LDS; returns the
+// incident weight excluding self loops (src/refinement/affinity.cuh:9-66).
+template <int W>
+__device__ inline std::uint64_t group_histogram(const RefArgs& a, const std::int32_t* labels, std::int64_t v,
+                                                unsigned long long* hist, int lg) {
+    for (int p = lg; p < a.k; p += W) hist[p] = 0;
+    wave_sync();
+    std::uint64_t total = 0;
+    const std::int64_t b = a.off[v], e = a.off[v + 1];
+    for (std::int64_t x = b + lg; x < e; x += W) {
+        const std::int32_t u = a.nbr[x];
+        if (u == v) continue;
+        const std::uint64_t w = ew_at(a.ew, x);
+        atomicAdd(&hist[labels[u]], (unsigned long long)w);
+        total += w;
+    }
+    wave_sync();
+    return group_sum<W>(total);
+}
+
+__device__ inline std::uint64_t block_histogram(const RefArgs& a, const std::int32_t* labels, std::int64_t v,
+                                                unsigned long long* hist, unsigned long long* red) {
+    for (int p = threadIdx.x; p < a.k; p += blockDim.x) hist[p] = 0;
+    __syncthreads();
+    std::uint64_t total = 0;
+    const std::int64_t b = a.off[v], e = a.off[v + 1];
+    for (std::int64_t x = b + threadIdx.x; x < e; x += blockDim.x) {
+        const std::int32_t u = a.nbr[x];
+        if (u == v) continue;
+        const std::uint64_t w = ew_at(a.ew, x);
+        atomicAdd(&hist[labels[u]], (unsigned long long)w);
+        total += w;
+    }
+    total = wave_sum(total);
+    if (lane_id() == 0) red[threadIdx.x / kWave] = total;
+    __syncthreads();
+    std::uint64_t sum = 0;
+    for (int i = 0; i < (int)(blockDim.x / kWave); ++i) sum += red[i];
+    __syncthreads();
+    return sum;
+}
+
+// estimated_repair_cost (src/refinement/incremental.cuh:156-159).
+__device__ inline double repair_cost(const RefArgs& a, int part, std::uint64_t weight) {
+    const std::uint64_t slack = a.cap - a.loads[part];
+    return weight > slack ? a.prices[part] * (double)(weight - slack) : 0.0;
+}
+
+struct Cand {
+    int p;
+    std::uint64_t conn;
+    double score;
+};
+
+__device__ inline bool cand_better(const Cand& x, const Cand& y, bool priced) {
+    if (x.p < 0) return false;
+    if (y.p < 0) return true;
+    if (priced) return x.score > y.score || (x.score == y.score && x.p < y.p);
+    return x.conn > y.conn || (x.conn == y.conn && x.p < y.p);
+}
+
+template <int W>
+__device__ inline Cand cand_group_reduce(Cand c, bool priced) {
+    for (int d = W / 2; d; d >>= 1) {
+        Cand o;
+        o.p = __shfl_xor(c.p, d, W);
+        o.conn = __shfl_xor(c.conn, d, W);
+        o.score = __shfl_xor(c.score, d, W);
+        if (cand_better(o, c, priced)) c = o;
+    }
+    return c;
+}
+
+// LP proposal (incremental.cuh:166-198, 229-296) given the table.
+__device__ inline Cand lp_scan_parts(const RefArgs& a, const unsigned long long* hist, int s, std::uint64_t w, int first,
+                                     int step) {
+    Cand best{-1, 0, 0.0};
+    const bool priced = a.prices != nullptr;
+    for (int p = first; p < a.k; p += step) {
+        const std::uint64_t c = hist[p];
+        if (p == s || !c) continue;
+        Cand x{p, c, priced ? (double)c - repair_cost(a, p, w) : 0.0};
+        if (cand_better(x, best, priced)) best = x;
+    }
+    return best;
+}
+
+__device__ inline void lp_accept(const RefArgs& a, std::int64_t v, const Cand& best, std::uint64_t current) {
+    if (best.p >= 0 && (best.conn >= current || current - best.conn < (std::uint64_t)(a.filter * (double)current))) {
+        a.targets[v] = best.p;
+        a.gains[v] = best.conn >= current ? (std::int64_t)(best.conn - current) : -(std::int64_t)(current - best.conn);
+        append(a.cand, &a.ctr[kCand], (std::int32_t)v);
+    }
+}
+
+template <int W>
+__global__ void k_lp_propose(RefArgs a, const std::int32_t* list, const unsigned long long* count_ptr,
+                             unsigned long long* defer_count) {
+    extern __shared__ unsigned long long smem[];
+    const int lg = threadIdx.x % W, gi = threadIdx.x / W;
+    unsigned long long* hist = smem + gi * a.k;
+    const std::int64_t count = (std::int64_t)*count_ptr;
+    const std::int64_t gpb = blockDim.x / W;
+    for (std::int64_t idx = blockIdx.x * gpb + gi; idx < count; idx += (std::int64_t)gridDim.x * gpb) {
+        const std::int64_t v = list[idx];
+        const std::int64_t deg = a.off[v + 1] - a.off[v];
+        if (deg > kHeavyDeg) continue;
+        if (W == 16 && deg > kLightDeg) {
+            if (lg == 0) append(a.defer, defer_count, (std::int32_t)v);
+            continue;
+        }
+        int locked = 0;
+        if (lg == 0) {
+            a.targets[v] = -1;
+            a.gains[v] = INT64_MIN;
+            a.flags[v] = 0;
+            locked = a.locks[v];
+            a.locks[v] = 0;
+        }
+        locked = __shfl(locked, 0, W);
+        if (locked) continue;
+        const std::uint64_t total = group_histogram<W>(a, a.labels, v, hist, lg);
+        const int s = a.labels[v];
+        const std::uint64_t current = hist[s];
+        if (current == total) continue;
+        Cand best = lp_scan_parts(a, hist, s, a.vw[v], lg, W);
+        best = cand_group_reduce<W>(best, a.prices != nullptr);
+        if (lg == 0) lp_accept(a, v, best, current);
+    }
+}
+
+__device__ inline Cand cand_block_reduce(Cand c, bool priced, Cand* red) {
+    c = cand_group_reduce<kWave>(c, priced);
+    if (lane_id() == 0) red[threadIdx.x / kWave] = c;
+    __syncthreads();
+    Cand r = red[0];
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i)
+        if (cand_better(red[i], r, priced)) r = red[i];
+    __syncthreads();
+    return r;
+}
+
+// Heavy rows: one block per vertex of the static heavy list.
+__global__ void k_lp_propose_block(RefArgs a, const std::int32_t* heavy, std::int64_t count) {
+    extern __shared__ unsigned long long smem[];
+    __shared__ unsigned long long red[16];
+    __shared__ Cand cred[16];
+    __shared__ int locked_s;
+    for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
+        const std::int64_t v = heavy[idx];
+        if (threadIdx.x == 0) {
+            a.targets[v] = -1;
+            a.gains[v] = INT64_MIN;
+            a.flags[v] = 0;
+            locked_s = a.locks[v];
+            a.locks[v] = 0;
+        }
+        __syncthreads();
+        const int locked = locked_s;
+        __syncthreads();
+        if (locked) continue;
+        const std::uint64_t total = block_histogram(a, a.labels, v, smem, red);
+        const int s = a.labels[v];
+        const std::uint64_t current = smem[s];
+        if (current == total) {
+            __syncthreads();
+            continue;
+        }
+        Cand best = lp_scan_parts(a, smem, s, a.vw[v], threadIdx.x, blockDim.x);
+        best = cand_block_reduce(best, a.prices != nullptr, cred);
+        if (threadIdx.x == 0) lp_accept(a, v, best, current);
+        __syncthreads();
+    }
+}
+
+// Afterburner (incremental.cuh:361-438).
+__device__ inline std::int64_t after_edge(const RefArgs& a, std::int64_t v, std::int64_t x, int s, int t,
+                                          std::int64_t gv) {
+    const std::int32_t u = a.nbr[x];
+    if (u == v) return 0;
+    const int tu = a.targets[u];
+    if (tu < 0) return 0;
+    const std::int64_t gu = a.gains[u];
+    if (!(gu > gv || (gu == gv && u < v))) return 0;
+    const int lu = a.labels[u];
+    return (std::int64_t)ew_at(a.ew, x) * ((tu == t) - (tu == s) - (lu == t) + (lu == s));
+}
+
+__device__ inline void after_decide(const RefArgs& a, std::int64_t v, int s, int t, std::int64_t gv, std::int64_t corr) {
+    bool keep;
+    if (a.prices) {
+        const std::uint64_t w = a.vw[v];
+        const double adj = repair_cost(a, s, w) - repair_cost(a, t, w);
+        keep = (double)(gv + corr) + adj >= 0;
+    } else
+        keep = gv + corr >= 0;
+    a.flags[v] = keep;
+    if (keep) append(a.movers, &a.ctr[kMovers], (std::int32_t)v);
+}
+
+template <int W>
+__global__ void k_afterburner(RefArgs a, const std::int32_t* list, const unsigned long long* count_ptr,
+                              unsigned long long* defer_count) {
+    const int lg = threadIdx.x % W, gi = threadIdx.x / W;
+    const std::int64_t count = (std::int64_t)*count_ptr;
+    const std::int64_t gpb = blockDim.x / W;
+    for (std::int64_t idx = blockIdx.x * gpb + gi; idx < count; idx += (std::int64_t)gridDim.x * gpb) {
+        const std::int64_t v = list[idx];
+        const std::int64_t b = a.off[v], e = a.off[v + 1];
+        if (e - b > kHeavyDeg) continue;
+        if (W == 16 && e - b > kLightDeg) {
+            if (lg == 0) append(a.defer, defer_count, (std::int32_t)v);
+            continue;
+        }
+        const int s = a.labels[v], t = a.targets[v];
+        const std::int64_t gv = a.gains[v];
+        std::int64_t corr = 0;
+        for (std::int64_t x = b + lg; x < e; x += W) corr += after_edge(a, v, x, s, t, gv);
+        corr = group_sum<W>(corr);
+        if (lg == 0) after_decide(a, v, s, t, gv, corr);
+    }
+}
+
+__global__ void k_afterburner_block(RefArgs a, const std::int32_t* heavy, std::int64_t count) {
+    __shared__ long long red[16];
+    for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
+        const std::int64_t v = heavy[idx];
+        if (a.targets[v] < 0) continue;
+        const int s = a.labels[v], t = a.targets[v];
+        const std::int64_t gv = a.gains[v];
+        std::int64_t corr = 0;
+        for (std::int64_t x = a.off[v] + threadIdx.x; x < a.off[v + 1]; x += blockDim.x)
+            corr += after_edge(a, v, x, s, t, gv);
+        corr = wave_sum(corr);
+        if (lane_id() == 0) red[threadIdx.x / kWave] = corr;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            std::int64_t sum = 0;
+            for (int i = 0; i < (int)(blockDim.x / kWave); ++i) sum += red[i];
+            after_decide(a, v, s, t, gv, sum);
+        }
+        __syncthreads();
+    }
+}
+
+// Exact cut delta of one edge (incremental.cuh:440-470): an undirected edge
+// is owned by the endpoint that moves alone, or by the larger id.
+__device__ inline std::int64_t update_edge(const RefArgs& a, std::int64_t v, std::int64_t x, int s, int t) {
+    const std::int32_t u = a.nbr[x];
+    if (u == v) return 0;
+    const int old_u = a.labels[u];
+    const bool moves = a.flags[u] != 0;
+    const int new_u = moves ? a.targets[u] : old_u;
+    touch(a, u);
+    if (!moves || u > v) return (std::int64_t)ew_at(a.ew, x) * ((s != old_u) - (t != new_u));
+    return 0;
+}
+
+__device__ inline void update_finish(const RefArgs& a, std::int64_t v, int s, std::int64_t gain, bool samples) {
+    if (gain) atomicAdd(&a.ctr[kDelta], (unsigned long long)gain);
+    if (samples && a.samples) {
+        atomicAdd(&a.samples[s].loss, (unsigned long long)(-gain));
+        atomicAdd(&a.samples[s].weight, (unsigned long long)a.vw[v]);
+    }
+}
+
+template <int W>
+__global__ void k_update(RefArgs a, const std::int32_t* list, const unsigned long long* count_ptr,
+                         unsigned long long* defer_count, bool samples) {
+    const int lg = threadIdx.x % W, gi = threadIdx.x / W;
+    const std::int64_t count = (std::int64_t)*count_ptr;
+    const std::int64_t gpb = blockDim.x / W;
+    for (std::int64_t idx = blockIdx.x * gpb + gi; idx < count; idx += (std::int64_t)gridDim.x * gpb) {
+        const std::int64_t v = list[idx];
+        const std::int64_t b = a.off[v], e = a.off[v + 1];
+        if (e - b > kHeavyDeg) continue;
+        if (W == 16 && e - b > kLightDeg) {
+            if (lg == 0) append(a.defer, defer_count, (std::int32_t)v);
+            continue;
+        }
+        const int s = a.labels[v], t = a.targets[v];
+        std::int64_t gain = 0;
+        for (std::int64_t x = b + lg; x < e; x += W) gain += update_edge(a, v, x, s, t);
+        gain = group_sum<W>(gain);
+        if (lg == 0) update_finish(a, v, s, gain, samples);
+    }
+}
+
+__global__ void k_update_block(RefArgs a, const std::int32_t* heavy, std::int64_t count, bool samples) {
+    __shared__ long long red[16];
+    for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
+        const std::int64_t v = heavy[idx];
+        if (!a.flags[v]) continue;
+        const int s = a.labels[v], t = a.targets[v];
+        std::int64_t gain = 0;
+        for (std::int64_t x = a.off[v] + threadIdx.x; x < a.off[v + 1]; x += blockDim.x)
+            gain += update_edge(a, v, x, s, t);
+        gain = wave_sum(gain);
+        if (lane_id() == 0) red[threadIdx.x / kWave] = gain;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            std::int64_t sum = 0;
+            for (int i = 0; i < (int)(blockDim.x / kWave); ++i) sum += red[i];
+            update_finish(a, v, s, sum, samples);
+        }
+        __syncthreads();
+    }
+}
+
+// incremental_commit (incremental.cuh:532-547).
+__global__ void k_commit(RefArgs a, const std::int32_t* list, const unsigned long long* count_ptr, bool lp) {
+    const std::int64_t count = (std::int64_t)*count_ptr;
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t v = list[i];
+        const int s = a.labels[v], t = a.targets[v];
+        const unsigned long long w = a.vw[v];
+        atomicAdd(&a.loads[s], 0ULL - w);
+        atomicAdd(&a.loads[t], w);
+        a.labels[v] = t;
+        if (lp) a.locks[v] = 1;
+        touch(a, v);
+    }
+}
+
+// Boundary frontier and per-level resets (incremental.cuh:214-225).
+template <int W>
+__global__ void k_frontier_init(RefArgs a) {
+    extern __shared__ unsigned long long smem[];
+    const int lg = threadIdx.x % W, gi = threadIdx.x / W;
+    unsigned long long* hist = smem + gi * a.k;
+    const std::int64_t gpb = blockDim.x / W;
+    for (std::int64_t v = blockIdx.x * gpb + gi; v < a.n; v += (std::int64_t)gridDim.x * gpb) {
+        if (lg == 0) {
+            a.targets[v] = -1;
+            a.gains[v] = INT64_MIN;
+            a.flags[v] = 0;
+            a.locks[v] = 0;
+            a.member[v] = 0;
+        }
+        const std::int64_t deg = a.off[v + 1] - a.off[v];
+        bool boundary = false;
+        // Boundary test without the table: any neighbour with another label.
+        const int s = a.labels[v];
+        for (std::int64_t x = a.off[v] + lg; x < a.off[v + 1]; x += W) {
+            const std::int32_t u = a.nbr[x];
+            if (u != v && a.labels[u] != s) boundary = true;
+        }
+        (void)deg;
+        (void)hist;
+        const unsigned long long any = (__ballot(boundary) >> ((threadIdx.x % kWave) / W * W)) &
+                                       (W == 64 ? ~0ULL : ((1ULL << W) - 1));
+        if (lg == 0 && any) {
+            a.member[v] = 1;
+            append(a.frontier, &a.ctr[kFrontier], (std::int32_t)v);
+        }
+    }
+}
+
+// ------------------------------------------------------------- repair
+__device__ inline int gain_bucket(const RefArgs& a, double gain) {
+    if (gain > 0) return 0;
+    if (gain == 0) return 1;
+    const double x = -gain;
+    for (int j = -23; j <= 23; ++j)
+        if (x <= a.pow15.p[j + 23]) return 25 + j < 2 ? 2 : 25 + j;
+    return 49;
+}
+
+__global__ void k_repair_donors(RefArgs a) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < a.n;
+         v += (std::int64_t)gridDim.x * blockDim.x) {
+        a.flags[v] = 0;
+        a.bucket[v] = -1;
+        if (a.loads0[a.labels[v]] > a.cap) append(a.cand, &a.ctr[kDonors], (std::int32_t)v);
+    }
+}
+
+struct RepairPick {
+    int p;
+    std::uint64_t conn, slack;
+};
+__device__ inline bool pick_better(const RepairPick& x, const RepairPick& y) {
+    if (x.p < 0) return false;
+    if (y.p < 0) return true;
+    return x.conn > y.conn || (x.conn == y.conn && (x.slack > y.slack || (x.slack == y.slack && x.p < y.p)));
+}
+
+// enumerate_repair_destination (rebalance_select.cuh:8-75) on frozen loads.
+__device__ inline RepairPick repair_scan(const RefArgs& a, const unsigned long long* hist, int s, std::uint64_t w,
+                                         int first, int step) {
+    RepairPick best{-1, 0, 0};
+    for (int p = first; p < a.k; p += step) {
+        if (p == s || a.loads0[p] >= a.cap || w > a.cap - a.loads0[p]) continue;
+        const std::uint64_t c = hist[p];
+        if (!c) continue;
+        RepairPick x{p, c, a.cap - a.loads0[p]};
+        if (pick_better(x, best)) best = x;
+    }
+    return best;
+}
+template <int W>
+__device__ inline RepairPick pick_group_reduce(RepairPick c) {
+    for (int d = W / 2; d; d >>= 1) {
+        RepairPick o;
+        o.p = __shfl_xor(c.p, d, W);
+        o.conn = __shfl_xor(c.conn, d, W);
+        o.slack = __shfl_xor(c.slack, d, W);
+        if (pick_better(o, c)) c = o;
+    }
+    return c;
+}
+
+__device__ inline void repair_finish(const RefArgs& a, std::int64_t v, int s, std::uint64_t w, std::uint64_t current,
+                                     RepairPick best) {
+    if (best.p < 0) {
+        // Slack-weighted hash routing in part order.
+        std::uint64_t tot = 0;
+        for (int p = 0; p < a.k; ++p)
+            if (p != s && a.loads0[p] < a.cap && w <= a.cap - a.loads0[p]) {
+                const std::uint64_t sl = a.cap - a.loads0[p];
+                tot = ~0ULL - tot < sl ? ~0ULL : tot + sl;
+            }
+        best.conn = 0;
+        if (tot) {
+            std::uint64_t ticket = (std::uint64_t)mix32((std::uint32_t)v) % tot;
+            for (int p = 0; p < a.k; ++p)
+                if (p != s && a.loads0[p] < a.cap && w <= a.cap - a.loads0[p]) {
+                    const std::uint64_t sl = a.cap - a.loads0[p];
+                    if (ticket < sl) {
+                        best.p = p;
+                        break;
+                    }
+                    ticket -= sl;
+                }
+        }
+    }
+    a.targets[v] = best.p;
+    if (best.p < 0) return;
+    const double gain = best.conn >= current ? (double)(best.conn - current) : -(double)(current - best.conn);
+    const std::uint64_t excess = a.loads0[s] - a.cap;
+    const std::uint64_t relief = w < excess ? w : excess;
+    const double gpw = gain / (double)relief;
+    const int b = s * kBalanceWidth + gain_bucket(a, gpw) * kBalanceSections +
+                  (int)(mix32((std::uint32_t)v) % kBalanceSections);
+    a.bucket[v] = b;
+    atomicAdd(&a.counts[b], (unsigned long long)w);
+}
+
+template <int W>
+__global__ void k_repair_propose(RefArgs a, const std::int32_t* list, const unsigned long long* count_ptr,
+                                 unsigned long long* defer_count) {
+    extern __shared__ unsigned long long smem[];
+    const int lg = threadIdx.x % W, gi = threadIdx.x / W;
+    unsigned long long* hist = smem + gi * a.k;
+    const std::int64_t count = (std::int64_t)*count_ptr;
+    const std::int64_t gpb = blockDim.x / W;
+    for (std::int64_t idx = blockIdx.x * gpb + gi; idx < count; idx += (std::int64_t)gridDim.x * gpb) {
+        const std::int64_t v = list[idx];
+        const std::int64_t deg = a.off[v + 1] - a.off[v];
+        if (deg > kHeavyDeg) continue;
+        if (W == 16 && deg > kLightDeg) {
+            if (lg == 0) append(a.defer, defer_count, (std::int32_t)v);
+            continue;
+        }
+        const std::uint64_t total = group_histogram<W>(a, a.labels, v, hist, lg);
+        const int s = a.labels[v];
+        const std::uint64_t w = a.vw[v];
+        const std::uint64_t current = hist[s];
+        RepairPick best{-1, 0, 0};
+        if (current != total) best = repair_scan(a, hist, s, w, lg, W);
+        best = pick_group_reduce<W>(best);
+        if (lg == 0) repair_finish(a, v, s, w, current, best);
+    }
+}
+
+__global__ void k_repair_propose_block(RefArgs a, const std::int32_t* heavy, std::int64_t count) {
+    extern __shared__ unsigned long long smem[];
+    __shared__ unsigned long long red[16];
+    __shared__ RepairPick pred[16];
+    for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
+        const std::int64_t v = heavy[idx];
+        if (!(a.loads0[a.labels[v]] > a.cap)) continue;
+        const std::uint64_t total = block_histogram(a, a.labels, v, smem, red);
+        const int s = a.labels[v];
+        const std::uint64_t w = a.vw[v];
+        const std::uint64_t current = smem[s];
+        RepairPick best{-1, 0, 0};
+        if (current != total) best = repair_scan(a, smem, s, w, threadIdx.x, blockDim.x);
+        best = pick_group_reduce<kWave>(best);
+        if (lane_id() == 0) pred[threadIdx.x / kWave] = best;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            RepairPick r = pred[0];
+            for (int i = 1; i < (int)(blockDim.x / kWave); ++i)
+                if (pick_better(pred[i], r)) r = pred[i];
+            repair_finish(a, v, s, w, current, r);
+        }
+        __syncthreads();
+    }
+}
+
+// Per overloaded part: exclusive prefix over the 800 buckets and the bucket
+// that contains the excess threshold (balance_scan_buckets, incremental.cuh:628-652).
+__global__ void k_bucket_scan(RefArgs a, unsigned long long* prefix, std::int32_t* bstar, unsigned long long* excess) {
+    const int p = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    if (!(a.loads0[p] > a.cap)) {
+        bstar[p] = -1;
+        excess[p] = 0;
+        return;
+    }
+    const unsigned long long ex = a.loads0[p] - a.cap;
+    unsigned long long sum = 0;
+    int star = kBalanceWidth;
+    for (int b = 0; b < kBalanceWidth; ++b) {
+        prefix[p * kBalanceWidth + b] = sum;
+        const unsigned long long cnt = a.counts[p * kBalanceWidth + b];
+        if (star == kBalanceWidth && cnt && sum < ex && ex <= sum + cnt) star = b;
+        sum += cnt;
+    }
+    bstar[p] = star;
+    excess[p] = ex;
+}
+
+__global__ void k_repair_flag(RefArgs a, const std::int32_t* bstar, std::uint8_t* f) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < a.n;
+         v += (std::int64_t)gridDim.x * blockDim.x) {
+        const int b = a.bucket[v];
+        f[v] = b >= 0 && (b % kBalanceWidth) <= bstar[b / kBalanceWidth];
+    }
+}
+
+template <int W>
+__global__ void k_conn_rows(RefArgs a, const std::int32_t* list, const std::int64_t* count_ptr,
+                            unsigned long long* rows) {
+    extern __shared__ unsigned long long smem[];
+    const int lg = threadIdx.x % W, gi = threadIdx.x / W;
+    unsigned long long* hist = smem + gi * a.k;
+    const std::int64_t count = *count_ptr;
+    const std::int64_t gpb = blockDim.x / W;
+    for (std::int64_t idx = blockIdx.x * gpb + gi; idx < count; idx += (std::int64_t)gridDim.x * gpb) {
+        const std::int64_t v = list[idx];
+        group_histogram<W>(a, a.labels, v, hist, lg);
+        for (int p = lg; p < a.k; p += W) rows[idx * a.k + p] = hist[p];
+        wave_sync();
+    }
+}
+
+// Receiver reservation (rebalance_graph.cuh:86-156) in canonical order: one
+// workgroup walks the candidate donors in vertex order; within each attempt a
+// part's requesters are ranked by vertex id on top of the start-of-attempt
+// request total.  Also decides bucket eligibility (prefix + rank < excess).
+constexpr int kReserveThreads = 1024;
+constexpr int kReserveWaves = kReserveThreads / kWave;
+
+struct ReserveArgs {
+    const std::int32_t* list;
+    const std::int64_t* count_ptr;
+    const unsigned long long* conn;  // count x k
+    const unsigned long long* prefix;
+    const std::int32_t* bstar;
+    const unsigned long long* excess;
+    std::uint8_t* elig;
+    std::uint8_t* done;
+    std::int32_t* choice;
+    unsigned long long* tried;  // count x 2
+};
+
+// Per wave: peers with the same key and the exclusive weighted prefix.
+__device__ inline void wave_keyed_prefix(bool has, int key, std::uint64_t w, std::uint64_t& excl,
+                                         std::uint64_t& total, bool& leader) {
+    const int lane = lane_id();
+    unsigned long long peers = __ballot(has);
+    for (int b = 0; b < 7; ++b) {
+        const bool bit = (key >> b) & 1;
+        const unsigned long long bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+    }
+    excl = 0;
+    total = 0;
+    for (int j = 0; j < kWave; ++j) {
+        const std::uint64_t wj = __shfl(w, j);
+        if ((peers >> j) & 1ULL) {
+            total += wj;
+            if (j < lane) excl += wj;
+        }
+    }
+    leader = has && (__ffsll(peers) - 1) == lane;
+}
+
+__global__ void __launch_bounds__(kReserveThreads) k_reserve(RefArgs a, ReserveArgs r) {
+    __shared__ unsigned long long wtot[kReserveWaves][kMaxParts];
+    __shared__ unsigned long long running[kMaxParts];
+    __shared__ unsigned long long requests[kMaxParts];
+    __shared__ unsigned long long snapshot[kMaxParts];
+    __shared__ unsigned long long slack[kMaxParts];
+    __shared__ int any;
+    const int t = threadIdx.x, wave = t / kWave;
+    const std::int64_t count = *r.count_ptr;
+    const int k = a.k;
+    for (int p = t; p < k; p += kReserveThreads) {
+        running[p] = 0;
+        requests[p] = 0;
+        slack[p] = a.loads0[p] < a.cap ? a.cap - a.loads0[p] : 0;
+        for (int i = 0; i < kReserveWaves; ++i) wtot[i][p] = 0;
+    }
+    __syncthreads();
+    // Phase 1: eligibility with ranks inside the boundary bucket.
+    for (std::int64_t base = 0; base < count; base += kReserveThreads) {
+        const std::int64_t i = base + t;
+        const bool valid = i < count;
+        std::int32_t v = 0;
+        int b = 0, s = 0, lb = 0;
+        std::uint64_t w = 0;
+        if (valid) {
+            v = r.list[i];
+            b = a.bucket[v];
+            s = b / kBalanceWidth;
+            lb = b % kBalanceWidth;
+            w = a.vw[v];
+        }
+        const bool isb = valid && lb == r.bstar[s];
+        std::uint64_t excl, total;
+        bool leader;
+        wave_keyed_prefix(isb, s, w, excl, total, leader);
+        if (leader) wtot[wave][s] = total;
+        __syncthreads();
+        if (valid) {
+            bool e = lb < r.bstar[s];
+            if (isb) {
+                std::uint64_t rank = running[s] + excl;
+                for (int j = 0; j < wave; ++j) rank += wtot[j][s];
+                e = r.prefix[b] + rank < r.excess[s];
+            }
+            r.elig[i] = e;
+            r.done[i] = 0;
+            r.tried[2 * i] = 0;
+            r.tried[2 * i + 1] = 0;
+        }
+        __syncthreads();
+        for (int p = t; p < k; p += kReserveThreads) {
+            unsigned long long add = 0;
+            for (int j = 0; j < kReserveWaves; ++j) {
+                add += wtot[j][p];
+                wtot[j][p] = 0;
+            }
+            running[p] += add;
+        }
+        __syncthreads();
+    }
+    // Phase 2: attempts.
+    for (int attempt = 0; attempt < k; ++attempt) {
+        for (int p = t; p < k; p += kReserveThreads) {
+            snapshot[p] = requests[p];
+            running[p] = 0;
+        }
+        if (t == 0) any = 0;
+        __syncthreads();
+        for (std::int64_t i = t; i < count; i += kReserveThreads) {
+            int c = -1;
+            if (r.elig[i] && !r.done[i]) {
+                const std::int32_t v = r.list[i];
+                if (attempt == 0)
+                    c = a.targets[v];
+                else {
+                    const std::uint64_t w = a.vw[v];
+                    const int lv = a.labels[v];
+                    unsigned long long best = 0, room = 0;
+                    for (int p = 0; p < k; ++p) {
+                        if (p == lv || ((r.tried[2 * i + (p >> 6)] >> (p & 63)) & 1ULL)) continue;
+                        const unsigned long long avail = snapshot[p] < slack[p] ? slack[p] - snapshot[p] : 0;
+                        if (avail < w) continue;
+                        const unsigned long long cn = r.conn[i * k + p];
+                        if (c < 0 || cn > best || (cn == best && avail > room)) {
+                            c = p;
+                            best = cn;
+                            room = avail;
+                        }
+                    }
+                }
+                if (c >= 0) any = 1;
+            }
+            r.choice[i] = c;
+        }
+        __syncthreads();
+        if (!any) break;
+        for (std::int64_t base = 0; base < count; base += kReserveThreads) {
+            const std::int64_t i = base + t;
+            const bool valid = i < count;
+            int c = valid ? r.choice[i] : -1;
+            const bool has = c >= 0;
+            std::int32_t v = valid ? r.list[i] : 0;
+            const std::uint64_t w = has ? a.vw[v] : 0;
+            std::uint64_t excl, total;
+            bool leader;
+            wave_keyed_prefix(has, has ? c : 0, w, excl, total, leader);
+            if (leader) wtot[wave][c] = total;
+            __syncthreads();
+            if (has) {
+                std::uint64_t begin = requests[c] + running[c] + excl;
+                for (int j = 0; j < wave; ++j) begin += wtot[j][c];
+                r.tried[2 * i + (c >> 6)] |= 1ULL << (c & 63);
+                if (begin <= slack[c] && w <= slack[c] - begin) {
+                    a.targets[v] = c;
+                    a.flags[v] = 1;
+                    r.done[i] = 1;
+                    append(a.movers, &a.ctr[kMovers], v);
+                }
+            }
+            __syncthreads();
+            for (int p = t; p < k; p += kReserveThreads) {
+                unsigned long long add = 0;
+                for (int j = 0; j < kReserveWaves; ++j) {
+                    add += wtot[j][p];
+                    wtot[j][p] = 0;
+                }
+                running[p] += add;
+            }
+            __syncthreads();
+        }
+        for (int p = t; p < k; p += kReserveThreads) requests[p] += running[p];
+        __syncthreads();
+    }
+}
+
+// Donors that were not admitted carry no target (rebalance_graph.cuh:155).
+__global__ void k_repair_clear(RefArgs a, const std::int32_t* donors, const unsigned long long* count_ptr) {
+    const std::int64_t count = (std::int64_t)*count_ptr;
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t v = donors[i];
+        if (!a.flags[v]) a.targets[v] = -1;
+    }
+}
+
+// ------------------------------------------------------------ exchange
+struct JMove {
+    std::int32_t v[2 * kJointSide];
+    std::int32_t size, a, b, id;
+    std::int64_t gain, net;
+};
+
+__host__ __device__ inline JMove jmove_empty() {
+    JMove m;
+    for (int j = 0; j < 2 * kJointSide; ++j) m.v[j] = -1;
+    m.size = 0;
+    m.a = -1;
+    m.b = -1;
+    m.id = 0;
+    m.gain = 0;
+    m.net = 0;
+    return m;
+}
+
+__device__ inline std::int64_t jgap(std::int64_t net, std::int64_t lo, std::int64_t hi) {
+    return net < lo ? lo - net : net > hi ? net - hi : 0;
+}
+
+template <int W>
+__global__ void k_dense_affinity(RefArgs a, unsigned long long* A) {
+    extern __shared__ unsigned long long smem[];
+    const int lg = threadIdx.x % W, gi = threadIdx.x / W;
+    unsigned long long* hist = smem + gi * a.k;
+    const std::int64_t gpb = blockDim.x / W;
+    for (std::int64_t v = blockIdx.x * gpb + gi; v < a.n; v += (std::int64_t)gridDim.x * gpb) {
+        group_histogram<W>(a, a.labels, v, hist, lg);
+        for (int p = lg; p < a.k; p += W) A[v * a.k + p] = hist[p];
+        wave_sync();
+    }
+}
+
+__global__ void k_quotient(RefArgs a, const unsigned long long* A, unsigned long long* Q) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < a.n;
+         v += (std::int64_t)gridDim.x * blockDim.x) {
+        const int la = a.labels[v];
+        for (int b = la + 1; b < a.k; ++b) {
+            const unsigned long long x = A[v * a.k + b];
+            if (x) atomicAdd(&Q[la * a.k + b], x);
+        }
+    }
+}
+
+// joint_match (exchange.cuh:55-83): one block, one thread per part.
+__global__ void k_joint_match(int k, const unsigned long long* Q, std::int32_t* partner, std::int32_t* visits) {
+    __shared__ int choice[kMaxParts];
+    __shared__ int progress;
+    const int p = threadIdx.x;
+    if (p < k) partner[p] = -1;
+    __syncthreads();
+    for (int round = 0; round < k / 2; ++round) {
+        if (p == 0) progress = 0;
+        int best = -1;
+        double score = -1;
+        if (p < k && partner[p] < 0)
+            for (int q = 0; q < k; ++q) {
+                if (q == p || partner[q] >= 0) continue;
+                const int pair = (p < q ? p : q) * k + (p < q ? q : p);
+                if (!Q[pair]) continue;
+                const double penalty = 1.0 + visits[pair];
+                const double value = (double)Q[pair] / (penalty * penalty);
+                if (value > score || (value == score && q < best)) {
+                    score = value;
+                    best = q;
+                }
+            }
+        if (p < k) choice[p] = best;
+        __syncthreads();
+        if (p < k && best >= 0 && choice[best] == p) {
+            partner[p] = best;
+            if (p < best) {
+                ++visits[p * k + best];
+                atomicAdd(&progress, 1);
+            }
+        }
+        __syncthreads();
+        const int gained = progress;
+        __syncthreads();
+        if (!gained) break;
+    }
+}
+
+__global__ void k_joint_seeds(RefArgs a, const unsigned long long* A, const std::int32_t* partner,
+                              unsigned long long* seeds, int sw, int pass) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < a.n;
+         v += (std::int64_t)gridDim.x * blockDim.x) {
+        const int p = a.labels[v], q = partner[p];
+        if (q < 0 || !A[v * a.k + q]) continue;
+        const std::uint32_t h = mix32((std::uint32_t)v ^ mix32((std::uint32_t)pass + 17));
+        atomicMin(&seeds[p * sw + h % (std::uint32_t)sw], ((unsigned long long)h << 32) | (std::uint32_t)v);
+    }
+}
+
+// joint_grow (exchange.cuh:96-204): one wave per (seed, policy) slot.
+__global__ void k_joint_grow(RefArgs a, const unsigned long long* A, int slots, int sw,
+                             const unsigned long long* seeds, const std::int32_t* partner, JMove* moves,
+                             JMove* archive) {
+    __shared__ std::int32_t mem[4][8];
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    const int slot = blockIdx.x * (blockDim.x / kWave) + wave;
+    if (slot >= slots) return;
+    const int policy = slot % 2;
+    const int k = a.k;
+    JMove best = jmove_empty();
+    best.id = slot;
+    if (lane == 0) {
+        moves[slot] = best;
+        for (int j = 0; j < kJointArchive; ++j) archive[slot * kJointArchive + j] = jmove_empty();
+    }
+    const unsigned long long enc = seeds[slot / 2];
+    if (enc == ~0ULL) return;
+    const std::int32_t root = (std::int32_t)(std::uint32_t)enc;
+    const int source = a.labels[root], target = partner[source];
+    if (target < 0) return;
+    std::int32_t* members = mem[wave];
+    if (lane < 8) members[lane] = lane == 0 ? root : -1;
+    wave_sync();
+    int size = 1, c0 = 1, c1 = 0;
+    std::int64_t net = a.vw[root];
+    const std::int64_t lo = (std::int64_t)a.loads[source] - (std::int64_t)a.cap;
+    const std::int64_t hi = (std::int64_t)a.cap - (std::int64_t)a.loads[target];
+    std::int64_t gain = (std::int64_t)A[(std::int64_t)root * k + target] - (std::int64_t)A[(std::int64_t)root * k + source];
+    const int pa = source < target ? source : target, pb = source < target ? target : source;
+    for (int step = 0; step < 8; ++step) {
+        if (lane == 0 && gain > 0 && size <= kJointArchive) {
+            JMove st = jmove_empty();
+            st.size = size;
+            st.a = pa;
+            st.b = pb;
+            st.gain = gain;
+            st.net = source == pa ? net : -net;
+            st.id = slot * kJointArchive + step;
+            for (int j = 0; j < size; ++j) {
+                int pos = j;
+                while (pos && st.v[pos - 1] > members[j]) {
+                    st.v[pos] = st.v[pos - 1];
+                    --pos;
+                }
+                st.v[pos] = members[j];
+            }
+            archive[slot * kJointArchive + step] = st;
+        }
+        if (gain > best.gain && !jgap(net, lo, hi)) {
+            best.size = size;
+            best.a = pa;
+            best.b = pb;
+            best.gain = gain;
+            best.net = source == pa ? net : -net;
+            for (int j = 0; j < 8; ++j) best.v[j] = j < size ? members[j] : -1;
+        }
+        if (size == 8) break;
+        std::int64_t cg = LLONG_MIN, cgap = LLONG_MAX;
+        std::int32_t chosen = INT_MAX;
+        const bool gap_first = policy == 0 && jgap(net, lo, hi) != 0;
+        for (int origin = -1; origin < size; ++origin) {
+            const bool rev = origin < 0;
+            if (rev && c1) continue;
+            const std::int32_t mm = rev ? -1 : members[origin];
+            const std::int64_t begin = rev ? 0 : a.off[mm];
+            std::int64_t end = rev ? sw : a.off[mm + 1];
+            if (!rev && end > begin + 128) end = begin + 128;
+            const int lm = rev ? -1 : a.labels[mm];
+            for (std::int64_t e = begin + lane; e < end; e += kWave) {
+                std::int32_t u;
+                if (rev) {
+                    const unsigned long long val = seeds[(std::int64_t)target * sw + e];
+                    if (val == ~0ULL) continue;
+                    u = (std::int32_t)(std::uint32_t)val;
+                } else
+                    u = a.nbr[e];
+                const int p = a.labels[u];
+                if (p != source && p != target) continue;
+                const int side = p == source ? 0 : 1;
+                if ((side ? c1 : c0) >= kJointSide) continue;
+                if (!rev && p != lm) continue;
+                bool dup = false;
+                for (int j = 0; j < size; ++j) dup |= u == members[j];
+                if (dup || a.off[u + 1] - a.off[u] > 1024) continue;
+                const std::int64_t cnet = net + (side ? -(std::int64_t)a.vw[u] : (std::int64_t)a.vw[u]);
+                const std::int64_t gp = jgap(cnet, lo, hi);
+                if (net > hi && !side) continue;
+                if (net < lo && side) continue;
+                std::int64_t marginal = (std::int64_t)A[(std::int64_t)u * k + (side ? source : target)] -
+                                        (std::int64_t)A[(std::int64_t)u * k + p];
+                if (!rev) marginal += 2 * (std::int64_t)ew_at(a.ew, e);
+                const bool better = gap_first
+                    ? (gp < cgap || (gp == cgap && (marginal > cg || (marginal == cg && u < chosen))))
+                    : (marginal > cg || (marginal == cg && (gp < cgap || (gp == cgap && u < chosen))));
+                if (better) {
+                    chosen = u;
+                    cg = marginal;
+                    cgap = gp;
+                }
+            }
+        }
+        for (int d = kWave / 2; d; d >>= 1) {
+            const std::int32_t ou = __shfl_xor(chosen, d);
+            const std::int64_t og = __shfl_xor(cg, d);
+            const std::int64_t ogp = __shfl_xor(cgap, d);
+            const bool better = gap_first
+                ? (ogp < cgap || (ogp == cgap && (og > cg || (og == cg && ou < chosen))))
+                : (og > cg || (og == cg && (ogp < cgap || (ogp == cgap && ou < chosen))));
+            if (better) {
+                chosen = ou;
+                cg = og;
+                cgap = ogp;
+            }
+        }
+        if (chosen == INT_MAX) break;
+        const int lc = a.labels[chosen];
+        const int side = lc == source ? 0 : 1;
+        std::int64_t corr = 0;
+        for (std::int64_t e = a.off[chosen] + lane; e < a.off[chosen + 1]; e += kWave)
+            for (int j = 0; j < size; ++j)
+                if (a.nbr[e] == members[j])
+                    corr += (a.labels[members[j]] == lc ? 2 : -2) * (std::int64_t)ew_at(a.ew, e);
+        corr = wave_sum(corr);
+        const std::int64_t cgain = (std::int64_t)A[(std::int64_t)chosen * k + (side ? source : target)] -
+                                   (std::int64_t)A[(std::int64_t)chosen * k + lc] + corr;
+        wave_sync();
+        if (lane == 0) members[size] = chosen;
+        wave_sync();
+        ++size;
+        if (side) ++c1; else ++c0;
+        gain += cgain;
+        net += side ? -(std::int64_t)a.vw[chosen] : (std::int64_t)a.vw[chosen];
+    }
+    if (lane == 0) moves[slot] = best;
+}
+
+// Orders for the archive and the transaction pool (complementary_moves.cuh:5-14, exchange.cuh:33-40).
+struct NetOrder {
+    __device__ bool operator()(const JMove& x, const JMove& y) const {  // x before y
+        if (x.a != y.a) return x.a < y.a;
+        if (x.net != y.net) return x.net < y.net;
+        if (x.gain != y.gain) return x.gain > y.gain;
+        if (x.size != y.size) return x.size < y.size;
+        for (int j = 0; j < x.size; ++j)
+            if (x.v[j] != y.v[j]) return x.v[j] < y.v[j];
+        return x.id < y.id;
+    }
+};
+struct JointOrder {
+    __device__ bool operator()(const JMove& x, const JMove& y) const {
+        if ((x.size != 0) != (y.size != 0)) return x.size != 0;
+        if (x.a != y.a) return x.a < y.a;
+        if (x.gain != y.gain) return x.gain > y.gain;
+        return x.id < y.id;
+    }
+};
+
+template <typename Cmp>
+__device__ inline bool idx_after(int x, int y, const JMove* m, Cmp cmp) {  // x sorts after y
+    if (x < 0) return y >= 0;
+    if (y < 0) return false;
+    return cmp(m[y], m[x]);
+}
+
+template <typename Cmp>
+__global__ void k_bitonic_global(int* idx, int N, int j, int kk, const JMove* m, Cmp cmp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int l = i ^ j;
+    if (l <= i) return;
+    const int x = idx[i], y = idx[l];
+    const bool asc = (i & kk) == 0;
+    if (asc ? idx_after(x, y, m, cmp) : idx_after(y, x, m, cmp)) {
+        idx[i] = y;
+        idx[l] = x;
+    }
+}
+template <typename Cmp>
+__global__ void k_bitonic_local(int* idx, int N, const JMove* m, Cmp cmp) {
+    __shared__ int s[1024];
+    for (int i = threadIdx.x; i < N; i += blockDim.x) s[i] = idx[i];
+    __syncthreads();
+    for (int kk = 2; kk <= N; kk <<= 1)
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < N; i += blockDim.x) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const int x = s[i], y = s[l];
+                    const bool asc = (i & kk) == 0;
+                    if (asc ? idx_after(x, y, m, cmp) : idx_after(y, x, m, cmp)) {
+                        s[i] = y;
+                        s[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < N; i += blockDim.x) idx[i] = s[i];
+}
+
+__global__ void k_nonempty_moves(const JMove* m, int count, std::uint8_t* f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) f[i] = m[i].size != 0;
+}
+__global__ void k_gather_moves(const JMove* src, const int* idx, int count, JMove* dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = src[idx[i]];
+}
+// exchange.cuh:19-32 equality of consecutive sorted archive entries.
+__global__ void k_unique_flags(const JMove* m, int count, std::uint8_t* f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    bool keep = true;
+    if (i > 0) {
+        const JMove& x = m[i - 1];
+        const JMove& y = m[i];
+        bool same = x.size && x.size == y.size && x.a == y.a && x.b == y.b && x.gain == y.gain && x.net == y.net;
+        for (int j = 0; same && j < x.size; ++j) {
+            bool found = false;
+            for (int t = 0; t < y.size; ++t) found |= x.v[j] == y.v[t];
+            same = found;
+        }
+        keep = !same;
+    }
+    f[i] = keep;
+}
+
+__device__ inline int jnet_bound(const JMove* r, int count, int pa, std::int64_t net, bool upper) {
+    int lo = 0, hi = count;
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        const JMove& x = r[mid];
+        if (x.a < pa || (x.a == pa && (x.net < net || (upper && x.net == net)))) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// joint_combine (complementary_moves.cuh:31-75): one wave per fragment.
+__global__ void k_joint_combine(RefArgs a, int count, const JMove* archive, int id_base, JMove* out) {
+    const int lane = lane_id();
+    const int i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (i >= count) return;
+    const JMove x = archive[i];
+    const std::int64_t lo = (std::int64_t)a.loads[x.a] - (std::int64_t)a.cap;
+    const std::int64_t hi = (std::int64_t)a.cap - (std::int64_t)a.loads[x.b];
+    if (lane == 0) {
+        JMove o = x;
+        o.id = id_base + i;
+        if (jgap(x.net, lo, hi)) o.size = 0;
+        out[i] = o;
+        out[count + i] = jmove_empty();
+    }
+    const int begin = jnet_bound(archive, count, x.a, lo - x.net, false);
+    const int end = jnet_bound(archive, count, x.a, hi - x.net, true);
+    std::int64_t best_gain = 0;
+    int best = -1, previous = -1;
+    for (int r = 0; r < 4 && begin < end; ++r) {
+        const int j = begin + (int)(((std::int64_t)(end - begin - 1) * r) / 3);
+        if (j == previous) continue;
+        previous = j;
+        if (j == i) continue;
+        const JMove& y = archive[j];
+        if (y.a != x.a || y.b != x.b || x.size + y.size > 8) continue;
+        bool overlap = false;
+        for (int u = 0; u < x.size; ++u)
+            for (int t = 0; t < y.size; ++t) overlap |= x.v[u] == y.v[t];
+        if (overlap) continue;
+        std::int64_t corr = 0;
+        for (int u = 0; u < x.size; ++u) {
+            const std::int32_t v = x.v[u];
+            const int lv = a.labels[v];
+            for (std::int64_t e = a.off[v] + lane; e < a.off[v + 1]; e += kWave) {
+                const std::int32_t nb = a.nbr[e];
+                for (int t = 0; t < y.size; ++t)
+                    if (nb == y.v[t]) corr += (lv == a.labels[nb] ? 2 : -2) * (std::int64_t)ew_at(a.ew, e);
+            }
+        }
+        corr = wave_sum(corr);
+        const std::int64_t gain = x.gain + y.gain + corr;
+        if (gain > best_gain || (gain == best_gain && gain > 0 && j < best)) {
+            best_gain = gain;
+            best = j;
+        }
+    }
+    if (lane == 0 && best >= 0) {
+        JMove res = x;
+        const JMove y = archive[best];
+        for (int j = 0; j < y.size; ++j) res.v[res.size++] = y.v[j];
+        res.gain = best_gain;
+        res.net += y.net;
+        res.id = id_base + count + i;
+        out[count + i] = res;
+    }
+}
+
+__global__ void k_joint_claim(const JMove* m, int count, std::int32_t* owners) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        for (int j = 0; j < m[i].size; ++j) atomicMin(&owners[m[i].v[j]], i);
+}
+__global__ void k_joint_resolve(JMove* m, int count, const std::int32_t* owners) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    for (int j = 0; j < m[i].size; ++j)
+        if (owners[m[i].v[j]] != i) {
+            m[i].size = 0;
+            break;
+        }
+}
+__global__ void k_joint_mark(RefArgs a, const JMove* m, int count, std::int32_t* owners) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    for (int j = 0; j < m[i].size; ++j) {
+        const std::int32_t v = m[i].v[j];
+        owners[v] = i;
+        a.targets[v] = a.labels[v] == m[i].a ? m[i].b : m[i].a;
+    }
+}
+// joint_events (exchange.cuh:227-262): one wave per transaction.
+__global__ void k_joint_events(RefArgs a, const JMove* m, int count, const std::int32_t* owners,
+                               unsigned long long* events) {
+    const int lane = lane_id();
+    const int i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (i >= count || !m[i].size) return;
+    unsigned long long own = 0;
+    const JMove mv = m[i];
+    for (int j = 0; j < mv.size; ++j) {
+        const std::int32_t v = mv.v[j];
+        for (std::int64_t e = a.off[v] + lane; e < a.off[v + 1]; e += kWave) {
+            const std::int32_t u = a.nbr[e];
+            const std::int32_t other = owners[u];
+            if (u == v || (other != INT_MAX && v > u)) continue;
+            if (other != INT_MAX && a.labels[u] != mv.a && a.labels[u] != mv.b) continue;
+            const std::int64_t w = (std::int64_t)ew_at(a.ew, e);
+            const int before = a.labels[v] != a.labels[u];
+            if (other == i || other == INT_MAX) {
+                const int after = a.targets[v] != (other == i ? a.targets[u] : a.labels[u]);
+                own += (unsigned long long)((before - after) * w);
+            } else {
+                const int middle = i < other ? a.targets[v] != a.labels[u] : a.labels[v] != a.targets[u];
+                const int after = a.targets[v] != a.targets[u];
+                const unsigned long long first = (unsigned long long)((before - middle) * w);
+                const unsigned long long last = (unsigned long long)((middle - after) * w);
+                own += i < other ? first : last;
+                atomicAdd(&events[other], i < other ? last : first);
+            }
+        }
+    }
+    own = wave_sum(own);
+    if (lane == 0 && own) atomicAdd(&events[i], own);
+}
+// joint_prefix / joint_accept (exchange.cuh:263-289).
+__global__ void k_joint_prefix(RefArgs a, const JMove* m, int count, const unsigned long long* events,
+                               std::int32_t* stops) {
+    const int pa = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pa >= a.k) return;
+    std::int64_t gain = 0, net = 0, best = 0;
+    int stop = -1;
+    int low = 0, high = count;
+    while (low < high) {
+        const int mid = (low + high) / 2;
+        if (m[mid].a < pa) low = mid + 1;
+        else high = mid;
+    }
+    for (int i = low; i < count && m[i].a == pa; ++i) {
+        if (!m[i].size) continue;
+        gain += (std::int64_t)events[i];
+        net += m[i].net;
+        const std::int64_t lo = (std::int64_t)a.loads[pa] - (std::int64_t)a.cap;
+        const std::int64_t hi = (std::int64_t)a.cap - (std::int64_t)a.loads[m[i].b];
+        if (!jgap(net, lo, hi) && gain > best) {
+            best = gain;
+            stop = i;
+        }
+    }
+    stops[pa] = stop;
+}
+__global__ void k_joint_accept(RefArgs a, const JMove* m, int count, const std::int32_t* stops) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count && m[i].size && i <= stops[m[i].a])
+        for (int j = 0; j < m[i].size; ++j) a.flags[m[i].v[j]] = 1;
+}
+__global__ void k_flag_list(RefArgs a, unsigned long long* count) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < a.n;
+         v += (std::int64_t)gridDim.x * blockDim.x)
+        if (a.flags[v]) append(a.movers, count, (std::int32_t)v);
+}
+
+// ---------------------------------------------------------- basics
+__global__ void k_project(std::int64_t n, const std::int32_t* map, const std::int32_t* coarse, std::int32_t* fine) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x)
+        fine[v] = coarse[map[v]];
+}
+__global__ void k_cut(std::int64_t n, const std::int32_t* off, const std::int32_t* nbr, const std::uint32_t* ew,
+                      const std::int32_t* labels, unsigned long long* out) {
+    const int lane = lane_id();
+    unsigned long long local = 0;
+    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane; base < n;
+         base += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t v = base + lane;
+        std::int64_t b = 0, e = 0;
+        if (v < n) {
+            b = off[v];
+            e = off[v + 1];
+        }
+        const bool heavy = v < n && e - b > 64;
+        if (v < n && !heavy) {
+            const int s = labels[v];
+            for (std::int64_t x = b; x < e; ++x)
+                if (labels[nbr[x]] != s) local += ew_at(ew, x);
+        }
+        unsigned long long hv = __ballot(heavy);
+        while (hv) {
+            const int src = __ffsll(hv) - 1;
+            hv &= hv - 1;
+            const std::int64_t hvx = __shfl(v, src);
+            const int s = labels[hvx];
+            for (std::int64_t x = off[hvx] + lane; x < off[hvx + 1]; x += kWave)
+                if (labels[nbr[x]] != s) local += ew_at(ew, x);
+        }
+    }
+    local = wave_sum(local);
+    if (lane == 0 && local) atomicAdd(out, local);
+}
+__global__ void k_loads(std::int64_t n, const std::uint32_t* vw, const std::int32_t* labels, unsigned long long* loads) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&loads[labels[v]], (unsigned long long)vw[v]);
+}
+__global__ void k_heavy_flags(std::int64_t n, const std::int32_t* off, std::uint8_t* f) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x)
+        f[v] = off[v + 1] - off[v] > kHeavyDeg;
+}
+__global__ void k_max_u32(std::int64_t n, const std::uint32_t* x, unsigned long long* out) {
+    unsigned long long m = 0;
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (std::int64_t)gridDim.x * blockDim.x)
+        m = m > x[i] ? m : x[i];
+    m = wave_max(m);
+    if (lane_id() == 0 && m) atomicMax(out, m);
+}
+
+inline int grid_for(std::int64_t n, int per_block = 256, int cap = 8192) {
+    return static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(n, per_block), cap)));
+}
+
+}  // namespace
+
+// ====================================================================== host
+
+void project(Context& c, std::int64_t n, const std::int32_t* map, const std::int32_t* coarse, std::int32_t* fine) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_project, dim3(grid_for(n)), dim3(256), 0, c.stream(), n, map, coarse, fine);
+    GP_LAUNCH_CHECK();
+}
+
+std::uint64_t device_cut(Context& c, const DGraph& g, const std::int32_t* labels) {
+    DBuf<unsigned long long> out = c.buf<unsigned long long>(1);
+    fill<unsigned long long>(c, out.get(), 1, 0ULL);
+    if (g.n > 0) {
+        hipLaunchKernelGGL(k_cut, dim3(grid_for(g.n)), dim3(256), 0, c.stream(), g.n, g.off.get(), g.nbr.get(),
+                           g.ew_ptr(), labels, out.get());
+        GP_LAUNCH_CHECK();
+    }
+    const unsigned long long d = c.read_scalar(out.get());
+    if (d & 1ULL) throw std::runtime_error("refinement cut is not symmetric");
+    return d / 2;
+}
+
+std::uint64_t device_edge_weight_sum(Context& c, const DGraph& g) {
+    if (!g.ew) return static_cast<std::uint64_t>(g.m);
+    DBuf<unsigned long long> out = c.buf<unsigned long long>(1);
+    fill<unsigned long long>(c, out.get(), 1, 0ULL);
+    if (g.m > 0) {
+        hipLaunchKernelGGL((k_reduce_sum<std::uint32_t>), dim3(grid_for(g.m)), dim3(256), 0, c.stream(), g.ew.get(), g.m,
+                           out.get());
+        GP_LAUNCH_CHECK();
+    }
+    return c.read_scalar(out.get());
+}
+
+void device_loads(Context& c, const DGraph& g, const std::int32_t* labels, int k, std::uint64_t* loads_host) {
+    DBuf<unsigned long long> out = c.buf<unsigned long long>(static_cast<std::size_t>(k));
+    fill<unsigned long long>(c, out.get(), k, 0ULL);
+    hipLaunchKernelGGL(k_loads, dim3(grid_for(g.n)), dim3(256), 0, c.stream(), g.n, g.vw.get(), labels, out.get());
+    GP_LAUNCH_CHECK();
+    c.read(loads_host, out.get(), static_cast<std::size_t>(k) * sizeof(unsigned long long));
+}
+
+namespace {
+
+class LevelRefiner {
+public:
+    LevelRefiner(Context& c, const DGraph& g, std::int32_t* labels, std::vector<std::uint64_t>& loads,
+                 const RefineOptions& o, std::uint64_t cap, LevelStats& st)
+        : c_(c), g_(g), loads_host_(loads), o_(o), st_(st) {
+        const std::int64_t n = g.n;
+        const int k = o.parts;
+        std::memset(&a_, 0, sizeof a_);
+        a_.n = n;
+        a_.off = g.off.get();
+        a_.nbr = g.nbr.get();
+        a_.ew = g.ew_ptr();
+        a_.vw = g.vw.get();
+        a_.k = k;
+        a_.cap = cap;
+        a_.labels = labels;
+        targets_ = c.buf<std::int32_t>(n);
+        gains_ = c.buf<std::int64_t>(n);
+        locks_ = c.buf<std::int32_t>(n);
+        flags_ = c.buf<std::int32_t>(n);
+        member_ = c.buf<std::int32_t>(n);
+        frontier_ = c.buf<std::int32_t>(n);
+        cand_ = c.buf<std::int32_t>(n);
+        movers_ = c.buf<std::int32_t>(n);
+        defer_ = c.buf<std::int32_t>(n);
+        ctr_ = c.buf<unsigned long long>(kCtrCount);
+        loads_ = c.buf<unsigned long long>(static_cast<std::size_t>(k));
+        loads0_ = c.buf<unsigned long long>(static_cast<std::size_t>(k));
+        bucket_ = c.buf<std::int32_t>(n);
+        counts_ = c.buf<unsigned long long>(static_cast<std::size_t>(k) * kBalanceWidth);
+        a_.targets = targets_.get();
+        a_.gains = gains_.get();
+        a_.locks = locks_.get();
+        a_.flags = flags_.get();
+        a_.member = member_.get();
+        a_.frontier = frontier_.get();
+        a_.cand = cand_.get();
+        a_.movers = movers_.get();
+        a_.defer = defer_.get();
+        a_.ctr = ctr_.get();
+        a_.loads = loads_.get();
+        a_.loads0 = loads0_.get();
+        a_.bucket = bucket_.get();
+        a_.counts = counts_.get();
+        a_.pow15.p[23] = 1.0;
+        for (int j = 1; j <= 24; ++j) a_.pow15.p[23 + j] = a_.pow15.p[22 + j] * 1.5;
+        for (int j = 1; j <= 23; ++j) a_.pow15.p[23 - j] = a_.pow15.p[24 - j] / 1.5;
+        c.upload(loads_.get(), loads.data(), static_cast<std::size_t>(k) * sizeof(std::uint64_t));
+        fill<unsigned long long>(c, ctr_.get(), kCtrCount, 0ULL);
+        std::uint64_t total = 0;
+        for (auto x : loads) total += x;
+        if (total > static_cast<std::uint64_t>(n)) {
+            prices_ = c.buf<double>(static_cast<std::size_t>(k));
+            samples_ = c.buf<RepairSample>(static_cast<std::size_t>(k));
+            fill<double>(c, prices_.get(), k, 0.0);
+            a_.prices = prices_.get();
+            a_.samples = samples_.get();
+        }
+        // Static heavy-row list.
+        DBuf<std::uint8_t> hf = c.buf<std::uint8_t>(n);
+        heavy_ = c.buf<std::int32_t>(n);
+        DBuf<std::int64_t> hc = c.buf<std::int64_t>(1);
+        hipLaunchKernelGGL(k_heavy_flags, dim3(grid_for(n)), dim3(256), 0, c.stream(), n, g.off.get(), hf.get());
+        GP_LAUNCH_CHECK();
+        stable_select(c, hf.get(), n, heavy_.get(), hc.get());
+        heavy_count_ = c.read_scalar(hc.get());
+        // Frontier of boundary vertices, resets.
+        hipLaunchKernelGGL((k_frontier_init<16>), dim3(grid_for(n, 16)), dim3(256), 0, c.stream(), a_);
+        GP_LAUNCH_CHECK();
+        // filter constant (incremental.cuh:897-899)
+        std::uint64_t max_edge = 1;
+        if (g.ew) {
+            DBuf<unsigned long long> mx = c.buf<unsigned long long>(1);
+            fill<unsigned long long>(c, mx.get(), 1, 0ULL);
+            if (g.m > 0) {
+                hipLaunchKernelGGL(k_max_u32, dim3(grid_for(g.m)), dim3(256), 0, c.stream(), g.m, g.ew.get(), mx.get());
+                GP_LAUNCH_CHECK();
+            }
+            max_edge = c.read_scalar(mx.get());
+        }
+        a_.filter = max_edge <= 1 ? 0.25 : 0.75;
+    }
+
+    std::uint64_t max_load() const {
+        std::uint64_t m = 0;
+        for (auto x : loads_host_) m = std::max(m, x);
+        return m;
+    }
+
+    void sync_loads() {
+        c_.read(loads_host_.data(), loads_.get(), loads_host_.size() * sizeof(std::uint64_t));
+    }
+
+    std::size_t hist_smem(int groups) const { return static_cast<std::size_t>(groups) * a_.k * sizeof(unsigned long long); }
+
+    // Counter-driven three-tier launch (16-lane groups, deferred waves, heavy blocks).
+    template <typename Light, typename Medium, typename Heavy>
+    void tiers(Light light, Medium medium, Heavy heavy) {
+        fill<unsigned long long>(c_, ctr_.get() + kDefer, 1, 0ULL);
+        light();
+        medium();
+        if (heavy_count_) heavy();
+    }
+
+    // One LP round; returns the number of movers and the cut delta.
+    std::int64_t lp_round(std::int64_t& delta) {
+        const hipStream_t s = c_.stream();
+        const std::int64_t n = a_.n;
+        if (samples_) fill<unsigned long long>(c_, reinterpret_cast<unsigned long long*>(samples_.get()), 2 * a_.k, 0ULL);
+        fill<unsigned long long>(c_, ctr_.get() + kCand, 1, 0ULL);
+        fill<unsigned long long>(c_, ctr_.get() + kMovers, 1, 0ULL);
+        fill<unsigned long long>(c_, ctr_.get() + kDelta, 1, 0ULL);
+        const int gb = grid_for(n, 16, 2048);
+        tiers(
+            [&] {
+                hipLaunchKernelGGL((k_lp_propose<16>), dim3(gb), dim3(256), hist_smem(16), s, a_, frontier_.get(),
+                                   ctr_.get() + kFrontier, ctr_.get() + kDefer);
+                GP_LAUNCH_CHECK();
+            },
+            [&] {
+                hipLaunchKernelGGL((k_lp_propose<64>), dim3(grid_for(n, 4, 1024)), dim3(256), hist_smem(4), s, a_,
+                                   defer_.get(), ctr_.get() + kDefer, ctr_.get() + kDefer);
+                GP_LAUNCH_CHECK();
+            },
+            [&] {
+                hipLaunchKernelGGL(k_lp_propose_block, dim3(grid_for(heavy_count_, 1, 2048)), dim3(256), hist_smem(1),
+                                   s, a_, heavy_.get(), heavy_count_);
+                GP_LAUNCH_CHECK();
+            });
+        tiers(
+            [&] {
+                hipLaunchKernelGGL((k_afterburner<16>), dim3(gb), dim3(256), 0, s, a_, cand_.get(), ctr_.get() + kCand,
+                                   ctr_.get() + kDefer);
+                GP_LAUNCH_CHECK();
+            },
+            [&] {
+                hipLaunchKernelGGL((k_afterburner<64>), dim3(grid_for(n, 4, 1024)), dim3(256), 0, s, a_, defer_.get(),
+                                   ctr_.get() + kDefer, ctr_.get() + kDefer);
+                GP_LAUNCH_CHECK();
+            },
+            [&] {
+                hipLaunchKernelGGL(k_afterburner_block, dim3(grid_for(heavy_count_, 1, 2048)), dim3(256), 0, s, a_,
+                                   heavy_.get(), heavy_count_);
+                GP_LAUNCH_CHECK();
+            });
+        return apply(true, false, delta);
+    }
+
+    // Update + commit of the current mover list.
+    std::int64_t apply(bool lp, bool samples, std::int64_t& delta) {
+        const hipStream_t s = c_.stream();
+        const std::int64_t n = a_.n;
+        const int gb = grid_for(n, 16, 2048);
+        fill<unsigned long long>(c_, ctr_.get() + kDelta, 1, 0ULL);
+        tiers(
+            [&] {
+                hipLaunchKernelGGL((k_update<16>), dim3(gb), dim3(256), 0, s, a_, movers_.get(), ctr_.get() + kMovers,
+                                   ctr_.get() + kDefer, samples);
+                GP_LAUNCH_CHECK();
+            },
+            [&] {
+                hipLaunchKernelGGL((k_update<64>), dim3(grid_for(n, 4, 1024)), dim3(256), 0, s, a_, defer_.get(),
+                                   ctr_.get() + kDefer, ctr_.get() + kDefer, samples);
+                GP_LAUNCH_CHECK();
+            },
+            [&] {
+                hipLaunchKernelGGL(k_update_block, dim3(grid_for(heavy_count_, 1, 2048)), dim3(256), 0, s, a_,
+                                   heavy_.get(), heavy_count_, samples);
+                GP_LAUNCH_CHECK();
+            });
+        hipLaunchKernelGGL(k_commit, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, a_, movers_.get(),
+                           ctr_.get() + kMovers, lp);
+        GP_LAUNCH_CHECK();
+        unsigned long long h[kCtrCount];
+        c_.read(h, ctr_.get(), sizeof h);
+        sync_loads();
+        delta = (std::int64_t)h[kDelta];
+        return (std::int64_t)h[kMovers];
+    }
+
+    bool update_cut(std::uint64_t& cut, std::int64_t delta) {
+        if (delta >= 0) {
+            if ((std::uint64_t)delta > cut) throw std::runtime_error("incremental cut underflow");
+            cut -= (std::uint64_t)delta;
+        } else
+            cut += (std::uint64_t)(-delta);
+        return true;
+    }
+
+    // Capacity repair loop (rebalance_graph.cuh:160-280); returns feasibility.
+    bool rebalance(std::uint64_t& cut) {
+        const hipStream_t s = c_.stream();
+        const std::int64_t n = a_.n;
+        const int k = a_.k;
+        int iterations = 0;
+        bool feasible = false;
+        std::int64_t count = 0;
+        DBuf<unsigned long long> prefix = c_.buf<unsigned long long>(static_cast<std::size_t>(k) * kBalanceWidth);
+        DBuf<std::int32_t> bstar = c_.buf<std::int32_t>(static_cast<std::size_t>(k));
+        DBuf<unsigned long long> excess = c_.buf<unsigned long long>(static_cast<std::size_t>(k));
+        DBuf<std::uint8_t> f = c_.buf<std::uint8_t>(n);
+        DBuf<std::int32_t> list = c_.buf<std::int32_t>(n);
+        DBuf<std::int64_t> lcount = c_.buf<std::int64_t>(1);
+        do {
+            c_.copy_d2d(loads0_.get(), loads_.get(), static_cast<std::size_t>(k) * sizeof(unsigned long long));
+            fill<unsigned long long>(c_, counts_.get(), (std::int64_t)k * kBalanceWidth, 0ULL);
+            fill<unsigned long long>(c_, ctr_.get() + kDonors, 1, 0ULL);
+            fill<unsigned long long>(c_, ctr_.get() + kMovers, 1, 0ULL);
+            hipLaunchKernelGGL(k_repair_donors, dim3(grid_for(n)), dim3(256), 0, s, a_);
+            GP_LAUNCH_CHECK();
+            const int gb = grid_for(n, 16, 2048);
+            tiers(
+                [&] {
+                    hipLaunchKernelGGL((k_repair_propose<16>), dim3(gb), dim3(256), hist_smem(16), s, a_, cand_.get(),
+                                       ctr_.get() + kDonors, ctr_.get() + kDefer);
+                    GP_LAUNCH_CHECK();
+                },
+                [&] {
+                    hipLaunchKernelGGL((k_repair_propose<64>), dim3(grid_for(n, 4, 1024)), dim3(256), hist_smem(4), s,
+                                       a_, defer_.get(), ctr_.get() + kDefer, ctr_.get() + kDefer);
+                    GP_LAUNCH_CHECK();
+                },
+                [&] {
+                    hipLaunchKernelGGL(k_repair_propose_block, dim3(grid_for(heavy_count_, 1, 2048)), dim3(256),
+                                       hist_smem(1), s, a_, heavy_.get(), heavy_count_);
+                    GP_LAUNCH_CHECK();
+                });
+            hipLaunchKernelGGL(k_bucket_scan, dim3(k), dim3(64), 0, s, a_, prefix.get(), bstar.get(), excess.get());
+            GP_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_repair_flag, dim3(grid_for(n)), dim3(256), 0, s, a_, bstar.get(), f.get());
+            GP_LAUNCH_CHECK();
+            stable_select(c_, f.get(), n, list.get(), lcount.get());
+            const std::int64_t nl = c_.read_scalar(lcount.get());
+            if (nl > 0) {
+                DBuf<unsigned long long> conn = c_.buf<unsigned long long>(static_cast<std::size_t>(nl) * k);
+                DBuf<std::uint8_t> elig = c_.buf<std::uint8_t>(static_cast<std::size_t>(nl));
+                DBuf<std::uint8_t> done = c_.buf<std::uint8_t>(static_cast<std::size_t>(nl));
+                DBuf<std::int32_t> choice = c_.buf<std::int32_t>(static_cast<std::size_t>(nl));
+                DBuf<unsigned long long> tried = c_.buf<unsigned long long>(static_cast<std::size_t>(nl) * 2);
+                hipLaunchKernelGGL((k_conn_rows<64>), dim3(grid_for(nl, 4, 2048)), dim3(256), hist_smem(4), s, a_,
+                                   list.get(), lcount.get(), conn.get());
+                GP_LAUNCH_CHECK();
+                ReserveArgs r{list.get(), lcount.get(), conn.get(), prefix.get(), bstar.get(), excess.get(),
+                              elig.get(), done.get(), choice.get(), tried.get()};
+                hipLaunchKernelGGL(k_reserve, dim3(1), dim3(kReserveThreads), 0, s, a_, r);
+                GP_LAUNCH_CHECK();
+            }
+            hipLaunchKernelGGL(k_repair_clear, dim3(grid_for(n)), dim3(256), 0, s, a_, cand_.get(),
+                               ctr_.get() + kDonors);
+            GP_LAUNCH_CHECK();
+            std::int64_t delta = 0;
+            count = apply(false, true, delta);
+            ++iterations;
+            if (count) {
+                update_cut(cut, delta);
+                st_.repair_moves += static_cast<std::uint64_t>(count);
+            }
+            feasible = max_load() <= a_.cap;
+        } while (!feasible && count && iterations < 2 * k + 4);
+        st_.repair_iterations += iterations;
+        return feasible;
+    }
+
+    // Capacity-complementary exchange (incremental.cuh:974-1041).
+    void exchange(int passes, std::uint64_t& cut, std::uint64_t& best_cut) {
+        const hipStream_t s = c_.stream();
+        const std::int64_t n = a_.n;
+        const int k = a_.k;
+        int sw = std::min(kJointSeeds, std::max(32, 4096 / k));
+        const int slots = k * sw * 2;
+        DBuf<unsigned long long> A = c_.buf<unsigned long long>(static_cast<std::size_t>(n) * k);
+        DBuf<unsigned long long> Q = c_.buf<unsigned long long>(static_cast<std::size_t>(k) * k);
+        DBuf<std::int32_t> visits = c_.buf<std::int32_t>(static_cast<std::size_t>(k) * k);
+        DBuf<std::int32_t> partner = c_.buf<std::int32_t>(static_cast<std::size_t>(k));
+        DBuf<unsigned long long> seeds = c_.buf<unsigned long long>(static_cast<std::size_t>(k) * sw);
+        DBuf<JMove> grown = c_.buf<JMove>(static_cast<std::size_t>(slots));
+        DBuf<JMove> archive = c_.buf<JMove>(static_cast<std::size_t>(slots) * kJointArchive);
+        DBuf<std::int32_t> owners = c_.buf<std::int32_t>(n);
+        DBuf<std::int32_t> stops = c_.buf<std::int32_t>(static_cast<std::size_t>(k));
+        auto dense = [&] {
+            hipLaunchKernelGGL((k_dense_affinity<64>), dim3(grid_for(n, 4, 4096)), dim3(256), hist_smem(4), s, a_,
+                               A.get());
+            GP_LAUNCH_CHECK();
+        };
+        dense();
+        fill<unsigned long long>(c_, Q.get(), (std::int64_t)k * k, 0ULL);
+        fill<std::int32_t>(c_, visits.get(), (std::int64_t)k * k, 0);
+        hipLaunchKernelGGL(k_quotient, dim3(grid_for(n)), dim3(256), 0, s, a_, A.get(), Q.get());
+        GP_LAUNCH_CHECK();
+        int empty = 0;
+        for (int pass = 0; pass < passes; ++pass) {
+            hipLaunchKernelGGL(k_joint_match, dim3(1), dim3(kMaxParts), 0, s, k, Q.get(), partner.get(), visits.get());
+            GP_LAUNCH_CHECK();
+            fill<unsigned long long>(c_, seeds.get(), (std::int64_t)k * sw, ~0ULL);
+            hipLaunchKernelGGL(k_joint_seeds, dim3(grid_for(n)), dim3(256), 0, s, a_, A.get(), partner.get(),
+                               seeds.get(), sw, pass);
+            GP_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_joint_grow, dim3(ceil_div(slots, 4)), dim3(256), 0, s, a_, A.get(), slots, sw,
+                               seeds.get(), partner.get(), grown.get(), archive.get());
+            GP_LAUNCH_CHECK();
+            // Archive: non-empty, sorted by net order, unique.
+            const int an = slots * kJointArchive;
+            DBuf<JMove> frag;
+            const int fragments = compact_sort(archive.get(), an, NetOrder{}, frag, true);
+            const int pool_cap = slots + 2 * fragments;
+            DBuf<JMove> pool = c_.buf<JMove>(static_cast<std::size_t>(std::max(pool_cap, 1)));
+            c_.copy_d2d(pool.get(), grown.get(), static_cast<std::size_t>(slots) * sizeof(JMove));
+            if (fragments) {
+                hipLaunchKernelGGL(k_joint_combine, dim3(ceil_div(fragments, 4)), dim3(256), 0, s, a_, fragments,
+                                   frag.get(), slots, pool.get() + slots);
+                GP_LAUNCH_CHECK();
+            }
+            DBuf<JMove> moves;
+            const int active = compact_sort(pool.get(), pool_cap, JointOrder{}, moves, false);
+            if (!active) {
+                if (++empty >= 2) break;
+                continue;
+            }
+            fill<std::int32_t>(c_, owners.get(), n, INT_MAX);
+            hipLaunchKernelGGL(k_joint_claim, dim3(ceil_div(active, 256)), dim3(256), 0, s, moves.get(), active,
+                               owners.get());
+            hipLaunchKernelGGL(k_joint_resolve, dim3(ceil_div(active, 256)), dim3(256), 0, s, moves.get(), active,
+                               owners.get());
+            fill<std::int32_t>(c_, owners.get(), n, INT_MAX);
+            hipLaunchKernelGGL(k_joint_mark, dim3(ceil_div(active, 256)), dim3(256), 0, s, a_, moves.get(), active,
+                               owners.get());
+            GP_LAUNCH_CHECK();
+            DBuf<unsigned long long> events = c_.buf<unsigned long long>(static_cast<std::size_t>(active));
+            fill<unsigned long long>(c_, events.get(), active, 0ULL);
+            hipLaunchKernelGGL(k_joint_events, dim3(ceil_div(active, 4)), dim3(256), 0, s, a_, moves.get(), active,
+                               owners.get(), events.get());
+            hipLaunchKernelGGL(k_joint_prefix, dim3(ceil_div(k, 128)), dim3(128), 0, s, a_, moves.get(), active,
+                               events.get(), stops.get());
+            fill<std::int32_t>(c_, flags_.get(), n, 0);
+            hipLaunchKernelGGL(k_joint_accept, dim3(ceil_div(active, 256)), dim3(256), 0, s, a_, moves.get(), active,
+                               stops.get());
+            fill<unsigned long long>(c_, ctr_.get() + kMovers, 1, 0ULL);
+            hipLaunchKernelGGL(k_flag_list, dim3(grid_for(n)), dim3(256), 0, s, a_, ctr_.get() + kMovers);
+            GP_LAUNCH_CHECK();
+            const unsigned long long cnt = c_.read_scalar(ctr_.get() + kMovers);
+            if (!cnt) {
+                if (++empty >= 2) break;
+                continue;
+            }
+            empty = 0;
+            std::int64_t delta = 0;
+            apply(false, false, delta);
+            if (delta <= 0) throw std::runtime_error("non-improving boundary group batch");
+            update_cut(cut, delta);
+            st_.group_moves += cnt;
+            if (max_load() > a_.cap || cut >= best_cut) throw std::runtime_error("capacity exchange violated cut/capacity");
+            best_cut = cut;
+            dense();
+        }
+    }
+
+    // Stable compaction of non-empty moves, sort by cmp, optional unique.
+    template <typename Cmp>
+    int compact_sort(const JMove* src, int count, Cmp cmp, DBuf<JMove>& out, bool unique) {
+        const hipStream_t s = c_.stream();
+        if (count <= 0) return 0;
+        DBuf<std::uint8_t> f = c_.buf<std::uint8_t>(static_cast<std::size_t>(count));
+        DBuf<std::int32_t> idx = c_.buf<std::int32_t>(static_cast<std::size_t>(count));
+        DBuf<std::int64_t> cnt = c_.buf<std::int64_t>(1);
+        hipLaunchKernelGGL(k_nonempty_moves, dim3(ceil_div(count, 256)), dim3(256), 0, s, src, count, f.get());
+        GP_LAUNCH_CHECK();
+        stable_select(c_, f.get(), count, idx.get(), cnt.get());
+        const int m = (int)c_.read_scalar(cnt.get());
+        if (!m) return 0;
+        int N = 1;
+        while (N < m) N <<= 1;
+        DBuf<std::int32_t> sidx = c_.buf<std::int32_t>(static_cast<std::size_t>(N));
+        fill<std::int32_t>(c_, sidx.get(), N, -1);
+        c_.copy_d2d(sidx.get(), idx.get(), static_cast<std::size_t>(m) * sizeof(std::int32_t));
+        if (N <= 1024) {
+            hipLaunchKernelGGL((k_bitonic_local<Cmp>), dim3(1), dim3(1024), 0, s, sidx.get(), N, src, cmp);
+            GP_LAUNCH_CHECK();
+        } else {
+            for (int kk = 2; kk <= N; kk <<= 1)
+                for (int j = kk >> 1; j > 0; j >>= 1) {
+                    hipLaunchKernelGGL((k_bitonic_global<Cmp>), dim3(ceil_div(N, 256)), dim3(256), 0, s, sidx.get(), N,
+                                       j, kk, src, cmp);
+                    GP_LAUNCH_CHECK();
+                }
+        }
+        DBuf<JMove> sorted = c_.buf<JMove>(static_cast<std::size_t>(m));
+        hipLaunchKernelGGL(k_gather_moves, dim3(ceil_div(m, 256)), dim3(256), 0, s, src, sidx.get(), m, sorted.get());
+        GP_LAUNCH_CHECK();
+        if (!unique) {
+            out = std::move(sorted);
+            return m;
+        }
+        DBuf<std::uint8_t> uf = c_.buf<std::uint8_t>(static_cast<std::size_t>(m));
+        DBuf<std::int32_t> uidx = c_.buf<std::int32_t>(static_cast<std::size_t>(m));
+        hipLaunchKernelGGL(k_unique_flags, dim3(ceil_div(m, 256)), dim3(256), 0, s, sorted.get(), m, uf.get());
+        GP_LAUNCH_CHECK();
+        stable_select(c_, uf.get(), m, uidx.get(), cnt.get());
+        const int u = (int)c_.read_scalar(cnt.get());
+        out = c_.buf<JMove>(static_cast<std::size_t>(u));
+        hipLaunchKernelGGL(k_gather_moves, dim3(ceil_div(u, 256)), dim3(256), 0, s, sorted.get(), uidx.get(), u,
+                           out.get());
+        GP_LAUNCH_CHECK();
+        return u;
+    }
+
+    void publish_prices() {
+        if (!prices_) return;
+        std::vector<RepairSample> sm(static_cast<std::size_t>(a_.k));
+        c_.read(sm.data(), samples_.get(), sm.size() * sizeof(RepairSample));
+        std::vector<double> pr(static_cast<std::size_t>(a_.k));
+        c_.read(pr.data(), prices_.get(), pr.size() * sizeof(double));
+        for (int p = 0; p < a_.k; ++p) {
+            const RepairSample& x = sm[static_cast<std::size_t>(p)];
+            if (!x.weight) continue;
+            const long long loss = (long long)x.loss;
+            pr[static_cast<std::size_t>(p)] = loss > 0 ? (double)loss / (double)x.weight : 0.0;
+        }
+        c_.upload(prices_.get(), pr.data(), pr.size() * sizeof(double));
+    }
+
+    void reset_samples() {
+        if (samples_) fill<unsigned long long>(c_, reinterpret_cast<unsigned long long*>(samples_.get()), 2 * a_.k, 0ULL);
+    }
+
+    Context& c_;
+    const DGraph& g_;
+    std::vector<std::uint64_t>& loads_host_;
+    const RefineOptions& o_;
+    LevelStats& st_;
+    RefArgs a_;
+    DBuf<std::int32_t> targets_, locks_, flags_, member_, frontier_, cand_, movers_, defer_, bucket_, heavy_;
+    DBuf<std::int64_t> gains_;
+    DBuf<unsigned long long> ctr_, loads_, loads0_, counts_;
+    DBuf<double> prices_;
+    DBuf<RepairSample> samples_;
+    std::int64_t heavy_count_ = 0;
+};
+
+}  // namespace
+
+LevelStats refine_level(Context& c, const DGraph& g, std::int32_t* labels, std::vector<std::uint64_t>& loads,
+                        const RefineOptions& o, std::uint64_t cap, int rounds, bool has_known_cut,
+                        std::uint64_t known_cut) {
+    LevelStats st;
+    st.n = g.n;
+    st.m = g.m;
+    st.refined = 1;
+    const int k = o.parts;
+    LevelRefiner R(c, g, labels, loads, o, cap, st);
+    std::uint64_t cut = has_known_cut ? known_cut : device_cut(c, g, labels);
+    st.initial_cut = cut;
+    if (R.max_load() > cap) {
+        R.reset_samples();
+        if (!R.rebalance(cut)) {
+            st.initial_repair_failed = 1;
+            st.final_cut = cut;
+            return st;
+        }
+    }
+    const std::uint64_t feasible_start = cut;
+    std::uint64_t best_cut = cut;
+    DBuf<std::int32_t> best_labels = c.buf<std::int32_t>(static_cast<std::size_t>(g.n));
+    std::vector<std::uint64_t> best_loads = loads;
+    c.copy_d2d(best_labels.get(), labels, static_cast<std::size_t>(g.n) * sizeof(std::int32_t));
+    const bool graph_mode = g.n >= o.graph_loop_threshold;
+    int stale = 0, empty = 0;
+    for (int phase = 0; phase < rounds && stale < o.stale_rounds; ++phase) {
+        st.rounds = phase + 1;
+        R.reset_samples();
+        std::int64_t delta = 0;
+        const std::int64_t count = R.lp_round(delta);
+        if (!count) {
+            if (++empty >= 2) break;
+            ++stale;
+            continue;
+        }
+        empty = 0;
+        R.update_cut(cut, delta);
+        st.lp_moves += static_cast<std::uint64_t>(count);
+        if (R.max_load() > cap && !R.rebalance(cut)) break;
+        R.publish_prices();
+        ++stale;
+        if (cut < best_cut) {
+            if (graph_mode) {
+                if (best_cut - cut > best_cut / 1000) stale = 0;
+            } else if ((long double)cut < 0.999L * (long double)best_cut)
+                stale = 0;
+            best_cut = cut;
+            c.copy_d2d(best_labels.get(), labels, static_cast<std::size_t>(g.n) * sizeof(std::int32_t));
+            best_loads = loads;
+        }
+    }
+    c.copy_d2d(labels, best_labels.get(), static_cast<std::size_t>(g.n) * sizeof(std::int32_t));
+    loads = best_loads;
+    c.upload(R.loads_.get(), loads.data(), static_cast<std::size_t>(k) * sizeof(std::uint64_t));
+    cut = best_cut;
+    if (rounds > 0 && o.exchange_rounds > 0 && g.n <= o.exchange_max_n) {
+        fill<std::int32_t>(c, R.flags_.get(), g.n, 0);
+        fill<std::int32_t>(c, R.targets_.get(), g.n, -1);
+        R.exchange(o.exchange_rounds, cut, best_cut);
+    }
+    st.final_cut = cut;
+    std::uint64_t mx = 0;
+    for (auto x : loads) mx = std::max(mx, x);
+    if (cut != best_cut || cut > feasible_start || mx > cap)
+        throw std::runtime_error("incremental refinement postcondition failed");
+    return st;
+}
+
+UncoarsenResult uncoarsen(Context& c, const Hierarchy& h, const std::int32_t* coarsest_labels, const RefineOptions& o) {
+    const int k = o.parts;
+    UncoarsenResult res;
+    const DGraph& cg = h.level(h.levels() - 1);
+    DBuf<std::int32_t> labels = c.buf<std::int32_t>(static_cast<std::size_t>(cg.n));
+    c.copy_d2d(labels.get(), coarsest_labels, static_cast<std::size_t>(cg.n) * sizeof(std::int32_t));
+    std::vector<std::uint64_t> loads(static_cast<std::size_t>(k), 0);
+    device_loads(c, cg, labels.get(), k, loads.data());
+    std::uint64_t total = 0;
+    for (auto x : loads) total += x;
+    const long double raw = (long double)total * o.imbalance / k;
+    const std::uint64_t cap = (std::uint64_t)std::floor(raw);
+    if (cap < total / (unsigned)k + (total % (unsigned)k != 0))
+        throw std::invalid_argument("integer partition capacity cannot hold total weight");
+    LevelStats st = refine_level(c, cg, labels.get(), loads, o, cap, o.max_rounds, false, 0);
+    res.levels.push_back(st);
+    std::uint64_t last_cut = st.final_cut;
+    std::int64_t last_refined = cg.n;
+    for (int lvl = h.levels() - 2; lvl >= 0; --lvl) {
+        const DGraph& fg = h.level(lvl);
+        DBuf<std::int32_t> fl = c.buf<std::int32_t>(static_cast<std::size_t>(fg.n));
+        project(c, fg.n, h.maps[static_cast<std::size_t>(lvl)].get(), labels.get(), fl.get());
+        labels = std::move(fl);
+        std::uint64_t maxload = 0;
+        for (auto x : loads) maxload = std::max(maxload, x);
+        const bool refine = lvl == 0 || maxload > cap || (double)fg.n >= o.growth_ratio * (double)last_refined;
+        LevelStats ls;
+        ls.n = fg.n;
+        ls.m = fg.m;
+        if (refine) {
+            ls = refine_level(c, fg, labels.get(), loads, o, cap, o.max_rounds, true, last_cut);
+            if (!ls.initial_repair_failed) last_refined = fg.n;
+        } else {
+            ls.initial_cut = ls.final_cut = last_cut;
+        }
+        last_cut = ls.final_cut;
+        res.levels.push_back(ls);
+    }
+    std::uint64_t maxload = 0;
+    for (auto x : loads) maxload = std::max(maxload, x);
+    if (maxload > cap) throw std::runtime_error("final original partition violates capacity");
+    res.labels = std::move(labels);
+    res.loads = loads;
+    res.cut = last_cut;
+    return res;
+}
+
+}  // namespace gp

@@ -1,0 +1,269 @@
+// Device runtime for one GPU: a HIP stream, a stream-ordered caching arena and
+// pinned staging for the few scalars the host loop control reads back.
+// (Role of include/detail/scratch_pool.hpp:22-185 in the reference; here the
+// arena lives in the context so repeated partitions reuse HBM without hipMalloc.)
+#pragma once
+#include "gp_common.hpp"
+
+#include <cstring>
+#include <map>
+#include <string>
+#include <mutex>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+namespace gp {
+
+class Arena {
+public:
+    explicit Arena(hipStream_t stream) : stream_(stream) {}
+    Arena(const Arena&) = delete;
+    Arena& operator=(const Arena&) = delete;
+    ~Arena() { release(); }
+
+    static std::size_t size_class(std::size_t bytes) {
+        if (bytes < 256) bytes = 256;
+        std::size_t p = 256;
+        while (p < bytes && p < (std::size_t(1) << 20)) p <<= 1;
+        if (p >= bytes) return p;
+        // Above 1 MiB: round to 1/8 of the leading power of two.
+        std::size_t lead = std::size_t(1) << 20;
+        while (lead * 2 <= bytes) lead <<= 1;
+        const std::size_t step = lead / 8;
+        return (bytes + step - 1) / step * step;
+    }
+
+    void* alloc(std::size_t bytes) {
+        if (!bytes) return nullptr;
+        const std::size_t cls = size_class(bytes);
+        std::lock_guard<std::mutex> lock(mutex_);
+        auto it = free_.find(cls);
+        if (it != free_.end() && !it->second.empty()) {
+            void* p = it->second.back();
+            it->second.pop_back();
+            live_[p] = cls;
+            return p;
+        }
+        void* p = nullptr;
+        hipError_t err = hipMalloc(&p, cls);
+        if (err != hipSuccess) {
+            // Drop the cache once and retry before failing.
+            (void)hipGetLastError();
+            trim_locked();
+            GP_HIP(hipMalloc(&p, cls));
+        }
+        total_ += cls;
+        live_[p] = cls;
+        return p;
+    }
+
+    void free(void* p) {
+        if (!p) return;
+        std::lock_guard<std::mutex> lock(mutex_);
+        auto it = live_.find(p);
+        if (it == live_.end()) throw std::logic_error("arena free of unknown pointer");
+        free_[it->second].push_back(p);
+        live_.erase(it);
+    }
+
+    // Return all cached (not live) blocks to HIP after the stream drains.
+    void trim() {
+        std::lock_guard<std::mutex> lock(mutex_);
+        trim_locked();
+    }
+    std::size_t reserved_bytes() const { return total_; }
+
+private:
+    void trim_locked() {
+        (void)hipStreamSynchronize(stream_);
+        for (auto& kv : free_)
+            for (void* p : kv.second) {
+                (void)hipFree(p);
+                total_ -= kv.first;
+            }
+        free_.clear();
+    }
+    void release() {
+        (void)hipStreamSynchronize(stream_);
+        for (auto& kv : free_)
+            for (void* p : kv.second) (void)hipFree(p);
+        for (auto& kv : live_) (void)hipFree(kv.first);
+        free_.clear();
+        live_.clear();
+        total_ = 0;
+    }
+    hipStream_t stream_;
+    std::mutex mutex_;
+    std::map<std::size_t, std::vector<void*>> free_;
+    std::unordered_map<void*, std::size_t> live_;
+    std::size_t total_ = 0;
+};
+
+// Move-only device array owned by an arena.
+template <typename T>
+class DBuf {
+public:
+    DBuf() = default;
+    DBuf(Arena* a, std::size_t n) : a_(a), n_(n) { p_ = static_cast<T*>(a->alloc(n * sizeof(T))); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept { swap(o); }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) {
+            reset();
+            swap(o);
+        }
+        return *this;
+    }
+    ~DBuf() { reset(); }
+    void reset() {
+        if (a_ && p_) a_->free(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    void swap(DBuf& o) noexcept {
+        std::swap(a_, o.a_);
+        std::swap(p_, o.p_);
+        std::swap(n_, o.n_);
+    }
+    // Grow-only resize; contents are discarded on growth.
+    void ensure(Arena* a, std::size_t n) {
+        if (n <= n_ && p_) return;
+        reset();
+        a_ = a;
+        n_ = n;
+        p_ = static_cast<T*>(a->alloc((n ? n : 1) * sizeof(T)));
+    }
+    T* get() const { return p_; }
+    std::size_t size() const { return n_; }
+    explicit operator bool() const { return p_ != nullptr; }
+
+private:
+    Arena* a_ = nullptr;
+    T* p_ = nullptr;
+    std::size_t n_ = 0;
+};
+
+// Accumulated device time and algorithmic bytes per kernel class, measured
+// with HIP events recorded on the context stream around each launch.
+struct ProfileEntry {
+    double ms = 0.0;
+    double bytes = 0.0;
+    long long launches = 0;
+};
+
+class Context {
+public:
+    explicit Context(int device) : device_(device) {
+        GP_HIP(hipSetDevice(device));
+        GP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        arena_ = new Arena(stream_);
+        GP_HIP(hipHostMalloc(&pinned_, kPinnedBytes, hipHostMallocDefault));
+    }
+    ~Context() {
+        delete arena_;
+        if (pinned_) (void)hipHostFree(pinned_);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+
+    hipStream_t stream() const { return stream_; }
+    Arena* arena() { return arena_; }
+    int device() const { return device_; }
+    void activate() const { GP_HIP(hipSetDevice(device_)); }
+
+    void sync() { GP_HIP(hipStreamSynchronize(stream_)); }
+
+    // D2H of a small region through pinned memory; synchronizes the stream.
+    void read(void* host, const void* dev, std::size_t bytes) {
+        if (!bytes) return;
+        if (bytes > kPinnedBytes) {
+            GP_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream_));
+            sync();
+            return;
+        }
+        GP_HIP(hipMemcpyAsync(pinned_, dev, bytes, hipMemcpyDeviceToHost, stream_));
+        sync();
+        std::memcpy(host, pinned_, bytes);
+    }
+    template <typename T>
+    T read_scalar(const T* dev) {
+        T v;
+        read(&v, dev, sizeof(T));
+        return v;
+    }
+    void upload(void* dev, const void* host, std::size_t bytes) {
+        if (!bytes) return;
+        GP_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, stream_));
+        sync();
+    }
+    void download(void* host, const void* dev, std::size_t bytes) {
+        if (!bytes) return;
+        GP_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream_));
+        sync();
+    }
+    void copy_d2d(void* dst, const void* src, std::size_t bytes) {
+        if (!bytes) return;
+        GP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_));
+    }
+
+    template <typename T>
+    DBuf<T> buf(std::size_t n) { return DBuf<T>(arena_, n); }
+
+    bool profiling() const { return profiling_; }
+    void set_profiling(bool on) { profiling_ = on; }
+    // Wraps one launch; bytes are the launch's algorithmic traffic.
+    template <typename Fn>
+    void timed(const char* name, double bytes, Fn&& fn) {
+        if (!profiling_) {
+            fn();
+            return;
+        }
+        hipEvent_t a, b;
+        GP_HIP(hipEventCreate(&a));
+        GP_HIP(hipEventCreate(&b));
+        GP_HIP(hipEventRecord(a, stream_));
+        fn();
+        GP_HIP(hipEventRecord(b, stream_));
+        pending_.push_back(Pending{name, bytes, a, b});
+    }
+    // Synchronizes and folds pending events into the totals.
+    void resolve_profile() {
+        if (pending_.empty()) return;
+        sync();
+        for (auto& p : pending_) {
+            float ms = 0.f;
+            GP_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+            ProfileEntry& e = profile_[p.name];
+            e.ms += ms;
+            e.bytes += p.bytes;
+            e.launches += 1;
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        pending_.clear();
+    }
+    std::map<std::string, ProfileEntry>& profile() {
+        resolve_profile();
+        return profile_;
+    }
+
+private:
+    struct Pending {
+        std::string name;
+        double bytes;
+        hipEvent_t a, b;
+    };
+    static constexpr std::size_t kPinnedBytes = 1 << 16;
+    bool profiling_ = false;
+    std::vector<Pending> pending_;
+    std::map<std::string, ProfileEntry> profile_;
+    int device_;
+    hipStream_t stream_ = nullptr;
+    Arena* arena_ = nullptr;
+    void* pinned_ = nullptr;
+};
+
+}  // namespace gp

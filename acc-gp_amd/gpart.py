@@ -1,0 +1,339 @@
+"""ctypes binding of libgpart_mi355x.so (include/gpart_mi355x.h).
+
+Mirrors the reference's C++ entry points (coarsen, initial_partition_metis,
+uncoarsen, refine_partition, partition_capacity) for Python callers.  Raises
+RuntimeError if the library has not been built; there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpart_mi355x.so")
+_LIB = None
+
+GP_OK = 0
+
+
+class CoarsenOptions(ctypes.Structure):
+    _fields_ = [
+        ("parts", ctypes.c_int), ("seed", ctypes.c_uint32), ("stop_ratio", ctypes.c_double),
+        ("max_levels", ctypes.c_int), ("cluster_size_factor", ctypes.c_int), ("pack_isolated", ctypes.c_int),
+        ("progressive", ctypes.c_int), ("fine_factor", ctypes.c_int), ("capacity_growth", ctypes.c_int),
+        ("seed_matching", ctypes.c_int), ("fast_path", ctypes.c_int), ("hem_degree_limit", ctypes.c_int),
+    ]
+
+
+class RefineOptions(ctypes.Structure):
+    _fields_ = [
+        ("parts", ctypes.c_int), ("imbalance", ctypes.c_double), ("max_rounds", ctypes.c_int),
+        ("stale_rounds", ctypes.c_int), ("growth_ratio", ctypes.c_double), ("exchange_rounds", ctypes.c_int),
+    ]
+
+
+class LevelStats(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int64), ("m", ctypes.c_int64), ("refined", ctypes.c_int),
+        ("initial_repair_failed", ctypes.c_int), ("rounds", ctypes.c_int),
+        ("initial_cut", ctypes.c_uint64), ("final_cut", ctypes.c_uint64),
+        ("lp_moves", ctypes.c_uint64), ("repair_moves", ctypes.c_uint64), ("group_moves", ctypes.c_uint64),
+        ("repair_iterations", ctypes.c_int),
+    ]
+
+
+class PartitionStats(ctypes.Structure):
+    _fields_ = [
+        ("coarsen_seconds", ctypes.c_double), ("initial_partition_seconds", ctypes.c_double),
+        ("uncoarsen_seconds", ctypes.c_double), ("algorithm_total_seconds", ctypes.c_double),
+        ("levels", ctypes.c_int), ("coarsest_vertices", ctypes.c_int64), ("final_cut", ctypes.c_uint64),
+        ("max_part_weight", ctypes.c_uint64), ("capacity", ctypes.c_uint64),
+        ("total_edge_weight", ctypes.c_uint64),
+    ]
+
+
+# Every symbol declared in include/gpart_mi355x.h.
+EXPORTS = [
+    "gp_last_error", "gp_context_create", "gp_context_destroy", "gp_context_reserved_bytes",
+    "gp_graph_create", "gp_graph_info", "gp_graph_download", "gp_graph_destroy", "gp_capacity",
+    "gp_default_coarsen_options", "gp_coarsen", "gp_hierarchy_levels", "gp_hierarchy_level",
+    "gp_hierarchy_map_download", "gp_hierarchy_level_info", "gp_hierarchy_destroy",
+    "gp_initial_partition_host", "gp_initial_partition", "gp_default_refine_options", "gp_uncoarsen",
+    "gp_result_labels", "gp_result_cut", "gp_result_loads", "gp_result_levels", "gp_result_level_stats",
+    "gp_result_destroy", "gp_refine_partition", "gp_partition", "gp_generate_grid", "gp_generate_rmat",
+    "gp_generate_rgg2d", "gp_generate_knn3d", "gp_free_host", "gp_load_csr", "gp_save_csr",
+    "gp_profile_enable", "gp_profile_query", "gp_profile_reset",
+]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libgpart_mi355x.so is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, u64, i32, u32, dbl = (ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int,
+                                       ctypes.c_uint32, ctypes.c_double)
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        L.gp_last_error.restype = ctypes.c_char_p
+        L.gp_context_create.argtypes = [i32, pp]
+        L.gp_context_destroy.argtypes = [vp]
+        L.gp_context_reserved_bytes.argtypes = [vp]
+        L.gp_context_reserved_bytes.restype = u64
+        L.gp_graph_create.argtypes = [vp, i64, i64, vp, vp, vp, vp, pp]
+        L.gp_graph_info.argtypes = [vp, vp, vp]
+        L.gp_graph_download.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.gp_graph_destroy.argtypes = [vp]
+        L.gp_capacity.argtypes = [u64, i32, dbl, vp]
+        L.gp_default_coarsen_options.argtypes = [i32, u32, vp]
+        L.gp_coarsen.argtypes = [vp, vp, vp, pp]
+        L.gp_hierarchy_levels.argtypes = [vp]
+        L.gp_hierarchy_level.argtypes = [vp, i32]
+        L.gp_hierarchy_level.restype = vp
+        L.gp_hierarchy_map_download.argtypes = [vp, vp, i32, vp]
+        L.gp_hierarchy_level_info.argtypes = [vp, i32, vp, vp, vp]
+        L.gp_hierarchy_destroy.argtypes = [vp]
+        L.gp_initial_partition_host.argtypes = [i64, vp, vp, vp, vp, i32, dbl, u32, vp]
+        L.gp_initial_partition.argtypes = [vp, vp, i32, dbl, u32, vp]
+        L.gp_default_refine_options.argtypes = [i32, dbl, vp]
+        L.gp_uncoarsen.argtypes = [vp, vp, vp, vp, pp]
+        L.gp_result_labels.argtypes = [vp, vp, vp]
+        L.gp_result_cut.argtypes = [vp]
+        L.gp_result_cut.restype = u64
+        L.gp_result_loads.argtypes = [vp, vp]
+        L.gp_result_levels.argtypes = [vp]
+        L.gp_result_level_stats.argtypes = [vp, i32, vp]
+        L.gp_result_destroy.argtypes = [vp]
+        L.gp_refine_partition.argtypes = [vp, vp, vp, vp, vp]
+        L.gp_partition.argtypes = [vp, vp, i32, dbl, u32, vp, vp]
+        L.gp_generate_grid.argtypes = [i64, i64, vp, vp, vp, vp]
+        L.gp_generate_rmat.argtypes = [i32, i32, dbl, dbl, dbl, u64, vp, vp, vp, vp]
+        L.gp_generate_rgg2d.argtypes = [i64, dbl, u64, vp, vp, vp, vp]
+        L.gp_generate_knn3d.argtypes = [i64, i32, u64, vp, vp, vp, vp, vp, vp]
+        L.gp_free_host.argtypes = [vp]
+        L.gp_load_csr.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, vp, vp, vp]
+        L.gp_save_csr.argtypes = [ctypes.c_char_p, ctypes.c_char_p, i64, i64, vp, vp]
+        L.gp_profile_enable.argtypes = [vp, i32]
+        L.gp_profile_query.argtypes = [vp, ctypes.c_char_p, vp, vp, vp]
+        L.gp_profile_reset.argtypes = [vp]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def check(rc):
+    if rc != GP_OK:
+        raise RuntimeError("gpart error %d: %s" % (rc, lib().gp_last_error().decode()))
+
+
+def capacity(total, parts, imbalance=1.03):
+    out = ctypes.c_uint64()
+    check(lib().gp_capacity(total, parts, imbalance, ctypes.byref(out)))
+    return out.value
+
+
+# ------------------------------------------------------------- host graphs
+def _take_csr(n, m, off_p, nbr_p):
+    off = np.ctypeslib.as_array(ctypes.cast(off_p, ctypes.POINTER(ctypes.c_int32)), shape=(n.value + 1,)).copy()
+    nbr = np.ctypeslib.as_array(ctypes.cast(nbr_p, ctypes.POINTER(ctypes.c_int32)),
+                                shape=(max(m.value, 1),))[:m.value].copy()
+    lib().gp_free_host(off_p)
+    lib().gp_free_host(nbr_p)
+    return off, nbr
+
+
+def generate_grid(rows, cols):
+    n, m, a, b = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_void_p(), ctypes.c_void_p()
+    check(lib().gp_generate_grid(rows, cols, ctypes.byref(n), ctypes.byref(m), ctypes.byref(a), ctypes.byref(b)))
+    return _take_csr(n, m, a, b)
+
+
+def generate_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1):
+    n, m, x, y = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_void_p(), ctypes.c_void_p()
+    check(lib().gp_generate_rmat(scale, edge_factor, a, b, c, seed, ctypes.byref(n), ctypes.byref(m),
+                                 ctypes.byref(x), ctypes.byref(y)))
+    return _take_csr(n, m, x, y)
+
+
+def generate_rgg2d(points, radius, seed=5):
+    n, m, x, y = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_void_p(), ctypes.c_void_p()
+    check(lib().gp_generate_rgg2d(points, radius, seed, ctypes.byref(n), ctypes.byref(m), ctypes.byref(x),
+                                  ctypes.byref(y)))
+    return _take_csr(n, m, x, y)
+
+
+def generate_knn3d(points, knn=10, seed=2):
+    n, m, x, y, ew, vw = (ctypes.c_int64(), ctypes.c_int64(), ctypes.c_void_p(), ctypes.c_void_p(),
+                          ctypes.c_void_p(), ctypes.c_void_p())
+    check(lib().gp_generate_knn3d(points, knn, seed, ctypes.byref(n), ctypes.byref(m), ctypes.byref(x),
+                                  ctypes.byref(y), ctypes.byref(ew), ctypes.byref(vw)))
+    mm = m.value
+    e = np.ctypeslib.as_array(ctypes.cast(ew, ctypes.POINTER(ctypes.c_uint32)), shape=(max(mm, 1),))[:mm].copy()
+    v = np.ctypeslib.as_array(ctypes.cast(vw, ctypes.POINTER(ctypes.c_uint32)), shape=(n.value,)).copy()
+    lib().gp_free_host(ew)
+    lib().gp_free_host(vw)
+    off, nbr = _take_csr(n, m, x, y)
+    return off, nbr, e, v
+
+
+def load_csr(indptr, indices):
+    n, m, x, y = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_void_p(), ctypes.c_void_p()
+    check(lib().gp_load_csr(indptr.encode(), indices.encode(), ctypes.byref(n), ctypes.byref(m),
+                            ctypes.byref(x), ctypes.byref(y)))
+    return _take_csr(n, m, x, y)
+
+
+def save_csr(indptr, indices, off, nbr):
+    off = np.ascontiguousarray(off, np.int32)
+    nbr = np.ascontiguousarray(nbr, np.int32)
+    check(lib().gp_save_csr(indptr.encode(), indices.encode(), len(off) - 1, len(nbr), _p(off), _p(nbr)))
+
+
+def initial_partition_host(off, nbr, ew, vw, parts, imbalance=1.03, seed=0):
+    off = np.ascontiguousarray(off, np.int32)
+    nbr = np.ascontiguousarray(nbr, np.int32)
+    ew = None if ew is None else np.ascontiguousarray(ew, np.uint32)
+    vw = None if vw is None else np.ascontiguousarray(vw, np.uint32)
+    out = np.zeros(len(off) - 1, np.int32)
+    check(lib().gp_initial_partition_host(len(off) - 1, _p(off), _p(nbr), _p(ew), _p(vw), parts, imbalance, seed,
+                                          _p(out)))
+    return out
+
+
+def coarsen_options(parts, seed=0, **overrides):
+    o = CoarsenOptions()
+    lib().gp_default_coarsen_options(parts, seed, ctypes.byref(o))
+    for key, value in overrides.items():
+        setattr(o, key, value)
+    return o
+
+
+def refine_options(parts, imbalance=1.03, **overrides):
+    o = RefineOptions()
+    lib().gp_default_refine_options(parts, imbalance, ctypes.byref(o))
+    for key, value in overrides.items():
+        setattr(o, key, value)
+    return o
+
+
+# ------------------------------------------------------------ device side
+class Context:
+    def __init__(self, device=0):
+        self.h = ctypes.c_void_p()
+        check(lib().gp_context_create(device, ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().gp_context_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def profile(self, enable=True):
+        check(lib().gp_profile_enable(self.h, int(enable)))
+
+    def profile_query(self, name):
+        ms, by, ln = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        check(lib().gp_profile_query(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(by), ctypes.byref(ln)))
+        return ms.value, by.value, ln.value
+
+    def profile_reset(self):
+        check(lib().gp_profile_reset(self.h))
+
+
+class Graph:
+    def __init__(self, ctx, off, nbr, ew=None, vw=None):
+        self.ctx = ctx
+        off = np.ascontiguousarray(off, np.int32)
+        nbr = np.ascontiguousarray(nbr, np.int32)
+        ew = None if ew is None else np.ascontiguousarray(ew, np.uint32)
+        vw = None if vw is None else np.ascontiguousarray(vw, np.uint32)
+        self.n = len(off) - 1
+        self.m = len(nbr)
+        self.h = ctypes.c_void_p()
+        check(lib().gp_graph_create(ctx.h, self.n, self.m, _p(off), _p(nbr), _p(ew), _p(vw), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().gp_graph_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def partition(self, parts, imbalance=1.03, seed=0, download=True):
+        labels = np.zeros(self.n, np.int32) if download else None
+        st = PartitionStats()
+        check(lib().gp_partition(self.ctx.h, self.h, parts, imbalance, seed, _p(labels), ctypes.byref(st)))
+        return labels, dict((f, getattr(st, f)) for f, _ in PartitionStats._fields_)
+
+    def refine(self, labels, ropts):
+        out = np.array(labels, np.int32, copy=True)
+        st = LevelStats()
+        check(lib().gp_refine_partition(self.ctx.h, self.h, _p(out), ctypes.byref(ropts), ctypes.byref(st)))
+        return out, dict((f, getattr(st, f)) for f, _ in LevelStats._fields_)
+
+
+def download_graph(ctx, g_handle):
+    n, m = ctypes.c_int64(), ctypes.c_int64()
+    check(lib().gp_graph_info(g_handle, ctypes.byref(n), ctypes.byref(m)))
+    off = np.zeros(n.value + 1, np.int32)
+    nbr = np.zeros(max(m.value, 1), np.int32)
+    ew = np.zeros(max(m.value, 1), np.uint32)
+    vw = np.zeros(n.value, np.uint32)
+    check(lib().gp_graph_download(ctx.h, g_handle, _p(off), _p(nbr), _p(ew), _p(vw)))
+    return off, nbr[:m.value], ew[:m.value], vw
+
+
+class Hierarchy:
+    def __init__(self, ctx, graph, opts):
+        self.ctx = ctx
+        self.graph = graph
+        self.h = ctypes.c_void_p()
+        check(lib().gp_coarsen(ctx.h, graph.h, ctypes.byref(opts), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().gp_hierarchy_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    @property
+    def levels(self):
+        return lib().gp_hierarchy_levels(self.h)
+
+    def graph_at(self, level):
+        return download_graph(self.ctx, lib().gp_hierarchy_level(self.h, level))
+
+    def map(self, level):
+        n = ctypes.c_int64()
+        check(lib().gp_graph_info(lib().gp_hierarchy_level(self.h, level), ctypes.byref(n), None))
+        out = np.zeros(n.value, np.int32)
+        check(lib().gp_hierarchy_map_download(self.ctx.h, self.h, level, _p(out)))
+        return out
+
+    def info(self, level):
+        md, hh, ib = ctypes.c_uint64(), ctypes.c_int(), ctypes.c_uint64()
+        check(lib().gp_hierarchy_level_info(self.h, level, ctypes.byref(md), ctypes.byref(hh), ctypes.byref(ib)))
+        return dict(max_degree=md.value, has_heavy=hh.value, incident_bound=ib.value)
+
+    def uncoarsen(self, coarsest_labels, ropts):
+        labels = np.ascontiguousarray(coarsest_labels, np.int32)
+        r = ctypes.c_void_p()
+        check(lib().gp_uncoarsen(self.ctx.h, self.h, _p(labels), ctypes.byref(ropts), ctypes.byref(r)))
+        try:
+            out = np.zeros(self.graph.n, np.int32)
+            check(lib().gp_result_labels(self.ctx.h, r, _p(out)))
+            loads = np.zeros(ropts.parts, np.uint64)
+            check(lib().gp_result_loads(r, _p(loads)))
+            stats = []
+            for i in range(lib().gp_result_levels(r)):
+                st = LevelStats()
+                check(lib().gp_result_level_stats(r, i, ctypes.byref(st)))
+                stats.append(dict((f, getattr(st, f)) for f, _ in LevelStats._fields_))
+            return out, lib().gp_result_cut(r), loads, stats
+        finally:
+            lib().gp_result_destroy(r)

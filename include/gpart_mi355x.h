@@ -1,0 +1,177 @@
+/*
+ * gpart_mi355x.h -- C ABI of the MI355X multilevel k-way graph partitioner.
+ *
+ * The reference (Shixg-cnic/Acc-GP) exposes a C++ API over thrust device
+ * vectors and has no C binding; each entry point below names the reference
+ * interface it replaces.  Plain pointers and sizes only.  Every function
+ * returns 0 (GP_OK) on success or a gp_status code, with a message available
+ * from gp_last_error() on the calling thread (the reference throws
+ * std::runtime_error / invalid_argument / overflow_error instead,
+ * include/check.hpp:8-18).
+ *
+ * Graph type: int32 vertices and offsets, uint32 vertex and edge weights,
+ * 64-bit cut / load sums, 2 <= parts <= 128 (include/graph_types.hpp:4-10).
+ */
+#ifndef GPART_MI355X_H
+#define GPART_MI355X_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gp_status {
+    GP_OK = 0,
+    GP_ERR_INVALID = 1,   /* std::invalid_argument in the reference */
+    GP_ERR_RUNTIME = 2,   /* std::runtime_error / overflow_error */
+    GP_ERR_HIP = 3,       /* device failure (CUDA_CHECK in the reference) */
+    GP_ERR_MEMORY = 4
+} gp_status;
+
+typedef struct gp_context gp_context;     /* one GPU: stream + HBM arena */
+typedef struct gp_graph gp_graph;         /* device-resident weighted CSR */
+typedef struct gp_hierarchy gp_hierarchy; /* coarsening levels + maps */
+typedef struct gp_result gp_result;       /* uncoarsening result (labels on device) */
+
+typedef struct gp_coarsen_options {       /* include/coarsen.hpp:15-38 */
+    int parts;
+    uint32_t seed;
+    double stop_ratio;
+    int max_levels;
+    int cluster_size_factor;
+    int pack_isolated;
+    int progressive;
+    int fine_factor;
+    int capacity_growth;
+    int seed_matching;
+    int fast_path;
+    int hem_degree_limit;
+} gp_coarsen_options;
+
+typedef struct gp_refine_options {        /* include/refine.hpp:13-26 */
+    int parts;
+    double imbalance;
+    int max_rounds;
+    int stale_rounds;
+    double growth_ratio;
+    int exchange_rounds;
+} gp_refine_options;
+
+typedef struct gp_level_stats {           /* RefineLevelResult, include/refine.hpp:60-68 */
+    int64_t n, m;
+    int refined;
+    int initial_repair_failed;
+    int rounds;
+    uint64_t initial_cut, final_cut;
+    uint64_t lp_moves, repair_moves, group_moves;
+    int repair_iterations;
+} gp_level_stats;
+
+typedef struct gp_partition_stats {       /* partition_timing line, app/partition_main.cu:163-177 */
+    double coarsen_seconds;
+    double initial_partition_seconds;
+    double uncoarsen_seconds;
+    double algorithm_total_seconds;
+    int levels;
+    int64_t coarsest_vertices;
+    uint64_t final_cut;
+    uint64_t max_part_weight;
+    uint64_t capacity;
+    uint64_t total_edge_weight;
+} gp_partition_stats;
+
+const char* gp_last_error(void);
+
+/* ---- context -------------------------------------------------------- */
+/* Replaces the thread-local ScratchPoolSession (include/detail/scratch_pool.hpp:115-130). */
+int gp_context_create(int device, gp_context** out);
+void gp_context_destroy(gp_context* ctx);
+/* Cached HBM bytes held by the context arena. */
+uint64_t gp_context_reserved_bytes(const gp_context* ctx);
+
+/* ---- graphs (include/graph.hpp:59-76) -------------------------------- */
+/* make_device_weighted: uploads a host CSR.  ew==NULL or all-ones edge weights
+ * are stored implicitly; vw==NULL means unit vertex weights. */
+int gp_graph_create(gp_context* ctx, int64_t n, int64_t m, const int32_t* offsets, const int32_t* neighbors,
+                    const uint32_t* edge_weights, const uint32_t* vertex_weights, gp_graph** out);
+int gp_graph_info(const gp_graph* g, int64_t* n, int64_t* m);
+/* copy_device_weighted: any output pointer may be NULL; ew expands unit weights. */
+int gp_graph_download(gp_context* ctx, const gp_graph* g, int32_t* offsets, int32_t* neighbors,
+                      uint32_t* edge_weights, uint32_t* vertex_weights);
+void gp_graph_destroy(gp_graph* g);
+
+/* partition_capacity (include/partition_capacity.hpp:14-25). */
+int gp_capacity(uint64_t total, int parts, double imbalance, uint64_t* out);
+
+/* ---- coarsening (include/coarsen.hpp:94-112) -------------------------- */
+void gp_default_coarsen_options(int parts, uint32_t seed, gp_coarsen_options* out);
+/* coarsen(DeviceWeightedGraph&&, CoarsenOptions): the input graph becomes
+ * level 0 and must outlive the hierarchy (it is borrowed, not consumed). */
+int gp_coarsen(gp_context* ctx, gp_graph* input, const gp_coarsen_options* opts, gp_hierarchy** out);
+int gp_hierarchy_levels(const gp_hierarchy* h);
+/* Borrowed view of level i (0 = input). */
+const gp_graph* gp_hierarchy_level(const gp_hierarchy* h, int level);
+/* fine_to_coarse map of level i (length = vertices of level i), i < levels-1. */
+int gp_hierarchy_map_download(gp_context* ctx, const gp_hierarchy* h, int level, int32_t* map);
+/* Level metadata carried by contraction: max degree bound, heavy rows, incident weight bound. */
+int gp_hierarchy_level_info(const gp_hierarchy* h, int level, uint64_t* max_degree, int* has_heavy,
+                            uint64_t* incident_bound);
+void gp_hierarchy_destroy(gp_hierarchy* h);
+
+/* ---- initial partition (include/initial_partition.hpp:6-8) ------------ */
+/* Host partitioner of a host CSR (not METIS: recursive bisection). */
+int gp_initial_partition_host(int64_t n, const int32_t* offsets, const int32_t* neighbors,
+                              const uint32_t* edge_weights, const uint32_t* vertex_weights, int parts,
+                              double imbalance, uint32_t seed, int32_t* labels);
+/* initial_partition_metis(graph, parts, eps, seed): downloads the device graph
+ * and partitions it on the host; labels are host output. */
+int gp_initial_partition(gp_context* ctx, const gp_graph* g, int parts, double imbalance, uint32_t seed,
+                         int32_t* labels);
+
+/* ---- uncoarsening / refinement (include/uncoarsen.hpp:17-20, refine.hpp:84-98) */
+void gp_default_refine_options(int parts, double imbalance, gp_refine_options* out);
+/* uncoarsen(DeviceHierarchy&, coarsest labels, RefineOptions). */
+int gp_uncoarsen(gp_context* ctx, const gp_hierarchy* h, const int32_t* coarsest_labels,
+                 const gp_refine_options* opts, gp_result** out);
+int gp_result_labels(gp_context* ctx, const gp_result* r, int32_t* labels);
+uint64_t gp_result_cut(const gp_result* r);
+int gp_result_loads(const gp_result* r, uint64_t* loads);
+int gp_result_levels(const gp_result* r);
+int gp_result_level_stats(const gp_result* r, int index, gp_level_stats* out);
+void gp_result_destroy(gp_result* r);
+/* refine_partition(graph, labels, options): one level, labels in/out (host). */
+int gp_refine_partition(gp_context* ctx, const gp_graph* g, int32_t* labels, const gp_refine_options* opts,
+                        gp_level_stats* stats);
+
+/* ---- whole pipeline: the CLI's algorithm_total_seconds scope ---------- */
+/* coarsen -> host initial partition -> uncoarsen, input resident on device.
+ * labels may be NULL (result stays on device and is released). */
+int gp_partition(gp_context* ctx, gp_graph* g, int parts, double imbalance, uint32_t seed, int32_t* labels,
+                 gp_partition_stats* stats);
+
+/* ---- synthetic inputs and checks (host only) -------------------------- */
+/* Generated CSR arrays are allocated by the library; release with gp_free_host. */
+int gp_generate_grid(int64_t rows, int64_t cols, int64_t* n, int64_t* m, int32_t** offsets, int32_t** neighbors);
+int gp_generate_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed, int64_t* n, int64_t* m,
+                     int32_t** offsets, int32_t** neighbors);
+int gp_generate_rgg2d(int64_t n_points, double radius, uint64_t seed, int64_t* n, int64_t* m, int32_t** offsets,
+                      int32_t** neighbors);
+int gp_generate_knn3d(int64_t n_points, int knn, uint64_t seed, int64_t* n, int64_t* m, int32_t** offsets,
+                      int32_t** neighbors, uint32_t** edge_weights, uint32_t** vertex_weights);
+void gp_free_host(void* p);
+/* Reference int64 binary CSR reader/writer (src/graph.cpp:47-116). */
+int gp_load_csr(const char* indptr_path, const char* indices_path, int64_t* n, int64_t* m, int32_t** offsets,
+                int32_t** neighbors);
+int gp_save_csr(const char* indptr_path, const char* indices_path, int64_t n, int64_t m, const int32_t* offsets,
+                const int32_t* neighbors);
+
+/* Kernel-time accounting (HIP events on the context stream) for roofline reports. */
+int gp_profile_enable(gp_context* ctx, int enable);
+/* name: kernel class; returns accumulated ms, algorithmic bytes and launches. */
+int gp_profile_query(gp_context* ctx, const char* name, double* ms, double* bytes, int64_t* launches);
+int gp_profile_reset(gp_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

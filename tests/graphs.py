@@ -1,0 +1,47 @@
+"""Small deterministic test graphs (numpy), independent of the library."""
+import numpy as np
+
+
+def csr_from_edges(n, edges, weights=None):
+    """Symmetric CSR with sorted rows from undirected (u, v[, w]) edges."""
+    rows = [[] for _ in range(n)]
+    for i, (u, v) in enumerate(edges):
+        w = 1 if weights is None else weights[i]
+        rows[u].append((v, w))
+        rows[v].append((u, w))
+    off = [0]
+    nbr, ew = [], []
+    for r in rows:
+        r.sort()
+        for v, w in r:
+            nbr.append(v)
+            ew.append(w)
+        off.append(len(nbr))
+    return np.array(off, np.int32), np.array(nbr, np.int32), np.array(ew, np.uint32)
+
+
+def grid(rows, cols):
+    edges = []
+    for r in range(rows):
+        for c in range(cols):
+            v = r * cols + c
+            if c + 1 < cols:
+                edges.append((v, v + 1))
+            if r + 1 < rows:
+                edges.append((v, v + cols))
+    off, nbr, _ = csr_from_edges(rows * cols, edges)
+    return off, nbr
+
+
+def random_graph(n, m, seed, max_weight=1):
+    rng = np.random.default_rng(seed)
+    seen = set()
+    edges, weights = [], []
+    while len(edges) < m:
+        u, v = (int(x) for x in rng.integers(0, n, 2))
+        if u == v or (min(u, v), max(u, v)) in seen:
+            continue
+        seen.add((min(u, v), max(u, v)))
+        edges.append((u, v))
+        weights.append(int(rng.integers(1, max_weight + 1)))
+    return csr_from_edges(n, edges, weights)

@@ -1,0 +1,112 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Deterministic mode is the only mode, so every comparison is bit-exact:
+coarsening maps and coarse CSR arrays per level, refined labels, cut and
+part loads.  Full-size configurations are covered by size-independent
+properties in test_gpu_properties.py.
+"""
+import numpy as np
+import pytest
+
+from tests import graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _cases(gp):
+    cases = []
+    off, nbr = graphs.grid(32, 32)
+    cases.append(("grid32_k4", off, nbr, None, None, 4))
+    off, nbr = gp.generate_grid(256, 256)
+    cases.append(("grid256_k8", off, nbr, None, None, 8))
+    off, nbr = gp.generate_rmat(12, 16, seed=3)
+    cases.append(("rmat12_k16", off, nbr, None, None, 16))
+    off, nbr = gp.generate_rgg2d(1 << 13, (8.0 / (3.14159265 * (1 << 13))) ** 0.5, seed=5)
+    cases.append(("rgg13_k8", off, nbr, None, None, 8))
+    off, nbr, ew, vw = gp.generate_knn3d(3000, 6, seed=2)
+    cases.append(("knn3000_k16_weighted", off, nbr, ew, vw, 16))
+    off, nbr = gp.generate_rmat(15, 16, seed=1)
+    cases.append(("rmat15_k64", off, nbr, None, None, 64))
+    return cases
+
+
+@pytest.fixture(scope="module")
+def cases(gp):
+    return _cases(gp)
+
+
+def _compare_hierarchy(gp, og, ctx, name, off, nbr, ew, vw, k):
+    _say("case %s: coarsen n=%d m=%d k=%d" % (name, len(off) - 1, len(nbr), k))
+    g = gp.Graph(ctx, off, nbr, ew, vw)
+    h = gp.Hierarchy(ctx, g, gp.coarsen_options(k))
+    oh = og.Hierarchy(off, nbr, ew, vw, og.coarsen_opts(k))
+    assert h.levels == oh.levels, (name, h.levels, oh.levels)
+    for lvl in range(h.levels):
+        a = h.graph_at(lvl)
+        b = oh.graph(lvl)
+        for x, y, what in zip(a, b, ("offsets", "neighbors", "edge_weights", "vertex_weights")):
+            assert np.array_equal(x, y), (name, lvl, what)
+        if lvl + 1 < h.levels:
+            assert np.array_equal(h.map(lvl), oh.map(lvl)), (name, lvl, "map")
+        gi, oi = h.info(lvl), oh.info(lvl)
+        assert gi["max_degree"] == oi["max_degree"] and gi["has_heavy"] == oi["has_heavy"], (name, lvl, gi, oi)
+    return g, h, oh
+
+
+def test_coarsen_and_uncoarsen_bit_exact(gp, og, ctx, cases):
+    for name, off, nbr, ew, vw, k in cases:
+        g, h, oh = _compare_hierarchy(gp, og, ctx, name, off, nbr, ew, vw, k)
+        coff, cnbr, cew, cvw = oh.graph(oh.levels - 1)
+        init = gp.initial_partition_host(coff, cnbr, cew, cvw, k)
+        _say("case %s: uncoarsen from %d coarse vertices" % (name, len(coff) - 1))
+        lab, cut, loads, st = h.uncoarsen(init, gp.refine_options(k))
+        olab, ocut, oloads, ost = oh.uncoarsen(init, og.refine_opts(k))
+        assert cut == ocut, (name, cut, ocut)
+        assert np.array_equal(loads, oloads), name
+        assert np.array_equal(lab, olab), (name, int((lab != olab).sum()))
+        for a, b in zip(st, ost):
+            for key in ("n", "refined", "initial_repair_failed", "rounds", "initial_cut", "final_cut",
+                        "lp_moves", "repair_moves", "group_moves", "repair_iterations"):
+                assert a[key] == b[key], (name, key, a, b)
+        # independent recount and strict capacity
+        assert og.cut(off, nbr, ew, lab) == cut
+        w = np.ones(len(off) - 1, np.uint64) if vw is None else vw.astype(np.uint64)
+        assert np.array_equal(np.bincount(lab, weights=w, minlength=k).astype(np.uint64), loads)
+        assert int(loads.max()) <= og.capacity(int(w.sum()), k)
+        h.close()
+        g.close()
+        _say("case %s: ok cut=%d" % (name, cut))
+
+
+def test_full_partition_matches_stages(gp, og, ctx):
+    off, nbr = gp.generate_rmat(13, 16, seed=7)
+    k = 32
+    _say("case rmat13_k32: gp_partition")
+    g = gp.Graph(ctx, off, nbr)
+    labels, st = g.partition(k)
+    oh = og.Hierarchy(off, nbr, None, None, og.coarsen_opts(k))
+    coff, cnbr, cew, cvw = oh.graph(oh.levels - 1)
+    init = gp.initial_partition_host(coff, cnbr, cew, cvw, k)
+    olab, ocut, _, _ = oh.uncoarsen(init, og.refine_opts(k))
+    assert np.array_equal(labels, olab)
+    assert st["final_cut"] == ocut
+    assert st["max_part_weight"] <= st["capacity"]
+    g.close()
+
+
+def test_refine_partition_single_level(gp, og, ctx):
+    off, nbr, ew = graphs.random_graph(3000, 12000, seed=11, max_weight=9)
+    k = 8
+    rng = np.random.default_rng(4)
+    init = rng.integers(0, k, len(off) - 1).astype(np.int32)
+    _say("case random3000_refine")
+    g = gp.Graph(ctx, off, nbr, ew, None)
+    lab, st = g.refine(init, gp.refine_options(k))
+    olab, ost = og.refine_graph(off, nbr, ew, None, init, og.refine_opts(k))
+    assert np.array_equal(lab, olab)
+    assert st["final_cut"] == ost["final_cut"]
+    g.close()

@@ -5,6 +5,7 @@
 // of the reference with wave64 kernels and a canonical (vertex-id) order.
 #include "gp_coarsen.hpp"
 #include "gp_segsort.hpp"
+#include "gp_rows.hpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -38,72 +39,81 @@ __device__ inline std::uint64_t sat_add(std::uint64_t a, std::uint64_t b) {
     return ~0ULL - a < b ? ~0ULL : a + b;
 }
 
-__global__ void k_level0_meta(std::int64_t n, const std::int32_t* off, const std::int32_t* nbr,
-                              const std::uint32_t* ew, const std::uint32_t* vw, std::uint64_t* row_bound,
-                              MetaOut* out) {
-    const int lane = lane_id();
+// Row sums (exact: u32 weights over int32 rows stay below 2^63) and the
+// sorted-row check; the body's accumulator counts unsorted rows.
+struct MetaBody {
+    const std::int32_t* off;
+    const std::int32_t* nbr;
+    const std::uint32_t* ew;
+    std::uint64_t* row_bound;
+    __device__ std::int64_t degree(std::int32_t v) const { return off[v + 1] - off[v]; }
+    template <class Team>
+    __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long*) const {
+        const std::int64_t b = off[v], e = off[v + 1];
+        unsigned long long d = 0;
+        unsigned long long unsorted = 0;
+        for (std::int64_t x = b + t.rank(); x < e; x += Team::size) {
+            d += ew_at(ew, x);
+            if (x > b && nbr[x] <= nbr[x - 1]) unsorted = 1;
+        }
+        d = t.sum(d);
+        unsorted = t.sum(unsorted);
+        if (t.leader()) row_bound[v] = d;
+        return RowOut{false, unsorted ? 1 : 0};
+    }
+};
+
+// Per-vertex maxima and sums; one atomic per block and statistic.
+__global__ void __launch_bounds__(256) k_vertex_meta(std::int64_t n, const std::int32_t* off, const std::uint32_t* vw,
+                                                     const std::uint64_t* row_bound, MetaOut* out) {
+    __shared__ unsigned long long part[4][4];
     unsigned long long maxrow = 0, maxdeg = 0, totalvw = 0, maxvw = 0;
-    unsigned flags = 0, unsorted = 0;
-    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane; base < n;
-         base += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int64_t v = base + lane;
-        std::int64_t b = 0, e = 0;
-        if (v < n) {
-            b = off[v];
-            e = off[v + 1];
-            totalvw += vw[v];
-            maxvw = maxvw > vw[v] ? maxvw : vw[v];
-            maxdeg = maxdeg > (unsigned long long)(e - b) ? maxdeg : (unsigned long long)(e - b);
-        }
-        const bool heavy = v < n && e - b > 64;
-        std::uint64_t d = 0;
-        if (v < n && !heavy) {
-            for (std::int64_t x = b; x < e; ++x) {
-                const std::uint64_t w = ew_at(ew, x);
-                d = sat_add(d, w);
-                if (x > b && nbr[x] <= nbr[x - 1]) unsorted = 1;
-                if (!w) flags |= 1u;
-                if (w != 1) flags |= 2u;
-            }
-            row_bound[v] = d;
-            maxrow = maxrow > d ? maxrow : d;
-        }
-        unsigned long long hv = __ballot(heavy);
-        while (hv) {
-            const int src = __ffsll(hv) - 1;
-            hv &= hv - 1;
-            const std::int64_t hvx = __shfl(v, src);
-            const std::int64_t hb = off[hvx], he = off[hvx + 1];
-            std::uint64_t hd = 0;
-            for (std::int64_t x = hb + lane; x < he; x += kWave) {
-                const std::uint64_t w = ew_at(ew, x);
-                hd = sat_add(hd, w);
-                if (x > hb && nbr[x] <= nbr[x - 1]) unsorted = 1;
-                if (!w) flags |= 1u;
-                if (w != 1) flags |= 2u;
-            }
-            for (int dd = kWave / 2; dd; dd >>= 1) hd = sat_add(hd, __shfl_xor(hd, dd));
-            if (lane == src) {
-                row_bound[hvx] = hd;
-                maxrow = maxrow > hd ? maxrow : hd;
-            }
-        }
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long d = (unsigned long long)(off[v + 1] - off[v]);
+        const unsigned long long w = vw[v];
+        maxdeg = maxdeg > d ? maxdeg : d;
+        maxvw = maxvw > w ? maxvw : w;
+        totalvw += w;
+        maxrow = maxrow > row_bound[v] ? maxrow : row_bound[v];
     }
     maxrow = wave_max(maxrow);
     maxdeg = wave_max(maxdeg);
     maxvw = wave_max(maxvw);
     totalvw = wave_sum(totalvw);
-    const unsigned long long anyuns = __ballot(unsorted != 0);
-    const unsigned long long anyzero = __ballot((flags & 1u) != 0);
-    const unsigned long long anynon = __ballot((flags & 2u) != 0);
-    if (lane == 0) {
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        part[w][0] = maxrow;
+        part[w][1] = maxdeg;
+        part[w][2] = maxvw;
+        part[w][3] = totalvw;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < 4; ++i) {
+            maxrow = maxrow > part[i][0] ? maxrow : part[i][0];
+            maxdeg = maxdeg > part[i][1] ? maxdeg : part[i][1];
+            maxvw = maxvw > part[i][2] ? maxvw : part[i][2];
+            totalvw += part[i][3];
+        }
         if (maxrow) atomicMax(&out->max_row, maxrow);
         if (maxdeg) atomicMax(&out->max_deg, maxdeg);
         if (maxvw) atomicMax(&out->max_vw, maxvw);
         if (totalvw) atomicAdd(&out->total_vw, totalvw);
-        if (anyuns) atomicOr(&out->unsorted, 1ULL);
-        if (anyzero || anynon) atomicOr(&out->flags, (anyzero ? 1ULL : 0ULL) | (anynon ? 2ULL : 0ULL));
     }
+}
+
+// Edge-weight flags: bit0 some weight is zero, bit1 some weight is not one.
+__global__ void __launch_bounds__(256) k_edge_flags(std::int64_t m, const std::uint32_t* ew, MetaOut* out) {
+    bool zero = false, non_unit = false;
+    for (std::int64_t x = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; x < m;
+         x += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::uint32_t w = ew[x];
+        zero |= w == 0;
+        non_unit |= w != 1;
+    }
+    const unsigned long long z = __ballot(zero), nu = __ballot(non_unit);
+    if (lane_id() == 0 && (z || nu)) atomicOr(&out->flags, (z ? 1ULL : 0ULL) | (nu ? 2ULL : 0ULL));
 }
 
 // Degree classes for this level (src/coarsening/aggregation.cuh:5-71).
@@ -116,28 +126,63 @@ __device__ inline int degree_class(std::int64_t d) {
     return d <= kLowDegree ? 0 : d <= 64 ? 1 : d <= 128 ? 2 : d <= 256 ? 3 : d <= kWaveDegree ? 4 : d <= kBlockDegree ? 5 : 6;
 }
 
-__global__ void k_degree_classes(std::int64_t n, const std::int32_t* off, std::int32_t* lists, std::int64_t stride,
-                                 ClassCounts* counts) {
+// Row-length classification into seven lists.  A block classifies a tile
+// of 256 x kClassItems rows: ranks come from wave ballots and an LDS
+// counter per class, and each class costs one global atomic per tile.
+constexpr int kClassItems = 8;
+
+template <typename Off>
+__global__ void __launch_bounds__(256) k_classify(std::int64_t rows, const Off* off, std::int32_t* lists,
+                                                  std::int64_t stride, ClassCounts* counts) {
+    __shared__ unsigned int lc[8];
+    __shared__ unsigned long long gb[8];
     const int lane = lane_id();
+    const unsigned long long lt = (1ULL << lane) - 1ULL;
     unsigned long long maxdeg = 0;
-    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane; base < n;
-         base += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int64_t v = base + lane;
-        const std::int64_t d = v < n ? off[v + 1] - off[v] : 0;
-        maxdeg = maxdeg > (unsigned long long)d ? maxdeg : (unsigned long long)d;
-        const int cls = v < n ? degree_class(d) : -1;
-        for (int c = 0; c < 7; ++c) {
-            const unsigned long long mask = __ballot(cls == c);
-            if (!mask) continue;
-            unsigned long long start = 0;
-            const int leader = __ffsll(mask) - 1;
-            if (lane == leader) start = atomicAdd(&counts->c[c], (unsigned long long)__popcll(mask));
-            start = __shfl(start, leader);
-            if (cls == c) lists[c * stride + start + __popcll(mask & ((1ULL << lane) - 1))] = (std::int32_t)v;
+    for (std::int64_t base = blockIdx.x * (std::int64_t)(256 * kClassItems); base < rows;
+         base += (std::int64_t)gridDim.x * (256 * kClassItems)) {
+        if (threadIdx.x < 8) lc[threadIdx.x] = 0;
+        __syncthreads();
+        int cls[kClassItems];
+        unsigned int rank[kClassItems];
+#pragma unroll
+        for (int it = 0; it < kClassItems; ++it) {
+            const std::int64_t v = base + it * 256 + threadIdx.x;
+            const std::int64_t d = v < rows ? (std::int64_t)(off[v + 1] - off[v]) : 0;
+            maxdeg = maxdeg > (unsigned long long)d ? maxdeg : (unsigned long long)d;
+            cls[it] = v < rows ? degree_class(d) : -1;
+            rank[it] = 0;
+            for (int c = 0; c < 7; ++c) {
+                const unsigned long long mask = __ballot(cls[it] == c);
+                if (!mask) continue;
+                const int leader = __ffsll(mask) - 1;
+                unsigned int wb = 0;
+                if (lane == leader) wb = atomicAdd(&lc[c], (unsigned int)__popcll(mask));
+                wb = __shfl(wb, leader);
+                if (cls[it] == c) rank[it] = wb + (unsigned int)__popcll(mask & lt);
+            }
         }
+        __syncthreads();
+        if (threadIdx.x < 7)
+            gb[threadIdx.x] = lc[threadIdx.x] ? atomicAdd(&counts->c[threadIdx.x], (unsigned long long)lc[threadIdx.x]) : 0;
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < kClassItems; ++it)
+            if (cls[it] >= 0)
+                lists[cls[it] * stride + gb[cls[it]] + rank[it]] = (std::int32_t)(base + it * 256 + threadIdx.x);
+        __syncthreads();
     }
     maxdeg = wave_max(maxdeg);
     if (lane == 0 && maxdeg) atomicMax(&counts->c[7], maxdeg);
+}
+
+template <typename Off>
+void classify(Context& c, std::int64_t rows, const Off* off, std::int32_t* lists, std::int64_t stride,
+              ClassCounts* counts) {
+    if (rows <= 0) return;
+    const int blocks = static_cast<int>(std::min<std::int64_t>(ceil_div(rows, 256 * kClassItems), 2048));
+    hipLaunchKernelGGL((k_classify<Off>), dim3(blocks), dim3(256), 0, c.stream(), rows, off, lists, stride, counts);
+    GP_LAUNCH_CHECK();
 }
 
 // -------------------------------------------------------- seed matching
@@ -431,59 +476,125 @@ __global__ void k_run_heads(const std::uint64_t* keys, std::int64_t n, std::uint
          i += (std::int64_t)gridDim.x * blockDim.x)
         heads[i] = keys[i] != ~0ULL && (i == 0 || keys[i] != keys[i - 1]);
 }
-__global__ void k_run_accumulate(const std::uint64_t* keys, const std::uint32_t* vals, std::int64_t n,
-                                 const std::int64_t* pos, const std::uint8_t* heads, std::uint64_t* ukeys,
-                                 unsigned long long* usums) {
+__global__ void k_run_items(const std::uint64_t* keys, const std::uint32_t* vals, std::int64_t n, SegU64* items) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (std::int64_t)gridDim.x * blockDim.x)
+        items[i] = SegU64{vals[i], (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u};
+}
+// The last record of a run carries the run's sum after the segmented scan.
+__global__ void k_run_tails(const std::uint64_t* keys, std::int64_t n, const std::int64_t* pos, const SegU64* sums,
+                            std::uint64_t* ukeys, unsigned long long* usums) {
     for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
          i += (std::int64_t)gridDim.x * blockDim.x) {
-        if (keys[i] == ~0ULL) continue;
+        if (keys[i] == ~0ULL || (i + 1 < n && keys[i + 1] == keys[i])) continue;
         const std::int64_t u = pos[i] - 1;  // inclusive count of heads -> run id
-        if (heads[i]) ukeys[u] = keys[i];
-        atomicAdd(&usums[u], (unsigned long long)vals[i]);
+        ukeys[u] = keys[i];
+        usums[u] = sums[i].v;
     }
+}
+
+__device__ inline std::int64_t lower_bound_u64(const std::uint64_t* a, std::int64_t n, std::uint64_t key) {
+    std::int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const std::int64_t mid = lo + (hi - lo) / 2;
+        if (a[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Record range [first, last) of list row li in keys sorted by (li, key).
+__device__ inline void block_record_range(const std::uint64_t* ukeys, std::int64_t nu, std::int64_t li, int key_bits,
+                                          std::int64_t* range) {
+    __syncthreads();
+    if (threadIdx.x == 0) range[0] = lower_bound_u64(ukeys, nu, (std::uint64_t)li << key_bits);
+    if (threadIdx.x == kWave) range[1] = lower_bound_u64(ukeys, nu, (std::uint64_t)(li + 1) << key_bits);
+    __syncthreads();
 }
 
 struct HubTop2 {
     unsigned long long current, best, tie, second;
 };
 
-__global__ void k_hub_best(AffinityArgs a, const std::int32_t* list, const std::uint64_t* ukeys,
-                           const unsigned long long* usums, std::int64_t nu, int key_bits, HubTop2* acc) {
-    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < nu;
-         i += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int64_t r = (std::int64_t)(ukeys[i] >> key_bits);
-        const std::int32_t c = (std::int32_t)(ukeys[i] & ((1ULL << key_bits) - 1));
+// Per hub row (one block): the weight toward its own cluster, the best and
+// second-best admissible weights (order statistics, so the result does not
+// depend on visiting order) and the hashed tie key among the best.
+__global__ void __launch_bounds__(256) k_hub_reduce(AffinityArgs a, const std::int32_t* list, std::int64_t count,
+                                                   const std::uint64_t* ukeys, const unsigned long long* usums,
+                                                   std::int64_t nu, int key_bits, HubTop2* acc) {
+    __shared__ std::int64_t range[2];
+    __shared__ unsigned long long part[4][3];
+    const std::uint64_t mask = (1ULL << key_bits) - 1ULL;
+    const int w = threadIdx.x / kWave;
+    for (std::int64_t r = blockIdx.x; r < count; r += gridDim.x) {
+        block_record_range(ukeys, nu, r, key_bits, range);
+        const std::int64_t first = range[0], last = range[1];
         const std::int32_t v = list[r];
         const std::int32_t src = a.clusters[v];
-        const unsigned long long val = usums[i];
-        if (c == src) {
-            acc[r].current = val;
-            continue;
-        }
-        if (a.filter_roles) {
+        auto admissible = [&](std::int32_t c) {
+            if (!a.filter_roles) return true;
             const std::uint64_t base = a.cw[c];
-            if (a.roles[c] || !(base != 0 && base <= a.cap && a.cw[src] <= a.cap - base)) continue;
+            return !a.roles[c] && base != 0 && base <= a.cap && a.cw[src] <= a.cap - base;
+        };
+        unsigned long long cur = 0, b1 = 0, b2 = 0;
+        for (std::int64_t j = first + threadIdx.x; j < last; j += blockDim.x) {
+            const std::int32_t c = (std::int32_t)(ukeys[j] & mask);
+            const unsigned long long val = usums[j];
+            if (c == src) {
+                cur = val;
+                continue;
+            }
+            if (!admissible(c)) continue;
+            if (val > b1) {
+                b2 = b1;
+                b1 = val;
+            } else if (val > b2)
+                b2 = val;
         }
-        const unsigned long long prev = atomicMax(&acc[r].best, val);
-        if (a.use_second) atomicMax(&acc[r].second, prev < val ? prev : val);
-    }
-}
-__global__ void k_hub_tie(AffinityArgs a, const std::int32_t* list, const std::uint64_t* ukeys,
-                          const unsigned long long* usums, std::int64_t nu, int key_bits, HubTop2* acc) {
-    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < nu;
-         i += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int64_t r = (std::int64_t)(ukeys[i] >> key_bits);
-        const std::int32_t c = (std::int32_t)(ukeys[i] & ((1ULL << key_bits) - 1));
-        const std::int32_t v = list[r];
-        const std::int32_t src = a.clusters[v];
-        if (c == src || usums[i] != acc[r].best) continue;
-        if (a.filter_roles) {
-            const std::uint64_t base = a.cw[c];
-            if (a.roles[c] || !(base != 0 && base <= a.cap && a.cw[src] <= a.cap - base)) continue;
+        for (int d = kWave / 2; d; d >>= 1) {
+            const unsigned long long o1 = __shfl_xor(b1, d), o2 = __shfl_xor(b2, d), oc = __shfl_xor(cur, d);
+            const unsigned long long lo = b1 < o1 ? b1 : o1, hi2 = b2 > o2 ? b2 : o2;
+            b1 = b1 > o1 ? b1 : o1;
+            b2 = lo > hi2 ? lo : hi2;
+            cur = cur > oc ? cur : oc;
         }
-        const std::uint32_t h = mix32((std::uint32_t)v ^ mix32((std::uint32_t)c) ^ a.tie_salt);
-        const unsigned long long key = ((unsigned long long)h << 32) | (unsigned long long)(0xffffffffU - (std::uint32_t)c);
-        atomicMax(&acc[r].tie, key);
+        if (lane_id() == 0) {
+            part[w][0] = b1;
+            part[w][1] = b2;
+            part[w][2] = cur;
+        }
+        __syncthreads();
+        b1 = part[0][0];
+        b2 = part[0][1];
+        cur = part[0][2];
+        for (int i = 1; i < 4; ++i) {
+            const unsigned long long o1 = part[i][0], o2 = part[i][1];
+            const unsigned long long lo = b1 < o1 ? b1 : o1, hi2 = b2 > o2 ? b2 : o2;
+            b1 = b1 > o1 ? b1 : o1;
+            b2 = lo > hi2 ? lo : hi2;
+            cur = cur > part[i][2] ? cur : part[i][2];
+        }
+        unsigned long long tie = 0;
+        if (b1)
+            for (std::int64_t j = first + threadIdx.x; j < last; j += blockDim.x) {
+                const std::int32_t c = (std::int32_t)(ukeys[j] & mask);
+                if (c == src || usums[j] != b1 || !admissible(c)) continue;
+                const std::uint32_t h = mix32((std::uint32_t)v ^ mix32((std::uint32_t)c) ^ a.tie_salt);
+                const unsigned long long key =
+                    ((unsigned long long)h << 32) | (unsigned long long)(0xffffffffU - (std::uint32_t)c);
+                tie = tie > key ? tie : key;
+            }
+        tie = wave_max(tie);
+        __syncthreads();
+        if (lane_id() == 0) part[w][0] = tie;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 1; i < 4; ++i) tie = tie > part[i][0] ? tie : part[i][0];
+            acc[r] = HubTop2{cur, b1, tie, a.use_second ? b2 : 0ULL};
+        }
+        __syncthreads();
     }
 }
 __global__ void k_hub_decode(AffinityArgs a, const std::int32_t* list, std::int64_t count, const HubTop2* acc) {
@@ -706,42 +817,31 @@ __global__ void k_contract_bounds(std::int64_t n, const std::int32_t* map, const
 
 // Scatter fine edges into their coarse rows (bucket order is arbitrary; the
 // row sort makes the output canonical).  Internal edges become kSentinel.
-__global__ void k_contract_scatter(std::int64_t n, const std::int32_t* off, const std::int32_t* nbr,
-                                   const std::uint32_t* ew, const std::int32_t* map, unsigned long long* cursor,
-                                   std::int32_t* rkey, std::uint32_t* rval) {
-    const int lane = lane_id();
-    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane; base < n;
-         base += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int64_t v = base + lane;
-        std::int64_t b = 0, e = 0, o = 0;
-        std::int32_t s = 0;
-        if (v < n) {
-            b = off[v];
-            e = off[v + 1];
-            s = map[v];
-            if (e > b) o = (std::int64_t)atomicAdd(&cursor[s], (unsigned long long)(e - b));
+struct ScatterBody {
+    const std::int32_t* off;
+    const std::int32_t* nbr;
+    const std::uint32_t* ew;
+    const std::int32_t* map;
+    unsigned long long* cursor;
+    std::int32_t* rkey;
+    std::uint32_t* rval;
+    __device__ std::int64_t degree(std::int32_t v) const { return off[v + 1] - off[v]; }
+    template <class Team>
+    __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long*) const {
+        const std::int64_t b = off[v], e = off[v + 1];
+        if (e == b) return RowOut{false, 0};
+        const std::int32_t s = map[v];
+        unsigned long long o = 0;
+        if (t.leader()) o = atomicAdd(&cursor[s], (unsigned long long)(e - b));
+        o = t.bcast(o);
+        for (std::int64_t x = b + t.rank(); x < e; x += Team::size) {
+            const std::int32_t tc = map[nbr[x]];
+            rkey[o + (x - b)] = tc == s ? kSentinel : tc;
+            rval[o + (x - b)] = (std::uint32_t)ew_at(ew, x);
         }
-        const bool heavy = v < n && e - b > 32;
-        if (v < n && !heavy)
-            for (std::int64_t x = b; x < e; ++x) {
-                const std::int32_t t = map[nbr[x]];
-                rkey[o + (x - b)] = t == s ? kSentinel : t;
-                rval[o + (x - b)] = (std::uint32_t)ew_at(ew, x);
-            }
-        unsigned long long hv = __ballot(heavy);
-        while (hv) {
-            const int src = __ffsll(hv) - 1;
-            hv &= hv - 1;
-            const std::int64_t hb = __shfl(b, src), he = __shfl(e, src), ho = __shfl(o, src);
-            const std::int32_t hs = __shfl(s, src);
-            for (std::int64_t x = hb + lane; x < he; x += kWave) {
-                const std::int32_t t = map[nbr[x]];
-                rkey[ho + (x - hb)] = t == hs ? kSentinel : t;
-                rval[ho + (x - hb)] = (std::uint32_t)ew_at(ew, x);
-            }
-        }
+        return RowOut{false, 0};
     }
-}
+};
 
 struct RowArgs {
     const std::int64_t* raw_off;
@@ -817,27 +917,18 @@ __global__ void k_long_records(RowArgs r, const std::int32_t* list, std::int64_t
         }
     }
 }
-// Write reduced long-row entries back at their row start.  first/last
-// bracket each long row's contiguous run of unique records.
-__global__ void k_long_bounds(const std::uint64_t* ukeys, std::int64_t nu, int key_bits, unsigned long long* first,
-                              unsigned long long* last) {
-    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < nu;
-         j += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int64_t li = (std::int64_t)(ukeys[j] >> key_bits);
-        atomicMin(&first[li], (unsigned long long)j);
-        atomicMax(&last[li], (unsigned long long)(j + 1));
-    }
-}
-__global__ void k_long_writeback(RowArgs r, const std::int32_t* list, const std::uint64_t* ukeys,
-                                 const unsigned long long* usums, std::int64_t nu, int key_bits,
-                                 const unsigned long long* first, const unsigned long long* last) {
-    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < nu;
-         j += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int64_t li = (std::int64_t)(ukeys[j] >> key_bits);
-        const std::int32_t key = (std::int32_t)(ukeys[j] & ((1ULL << key_bits) - 1));
+// Write reduced long-row entries back at their row start; one block per row.
+__global__ void k_long_writeback(RowArgs r, const std::int32_t* list, std::int64_t count, const std::uint64_t* ukeys,
+                                 const unsigned long long* usums, std::int64_t nu, int key_bits) {
+    __shared__ std::int64_t range[2];
+    for (std::int64_t li = blockIdx.x; li < count; li += gridDim.x) {
+        block_record_range(ukeys, nu, li, key_bits, range);
+        const std::int64_t first = range[0], last = range[1];
         const std::int32_t row = list[li];
-        if ((unsigned long long)j == first[li]) r.ucount[row] = last[li] - first[li];
-        emit_row_entry(r, r.raw_off[row], key, usums[j], (int)(j - (std::int64_t)first[li]));
+        const std::int64_t start = r.raw_off[row];
+        if (threadIdx.x == 0) r.ucount[row] = (unsigned long long)(last - first);
+        for (std::int64_t j = first + threadIdx.x; j < last; j += blockDim.x)
+            emit_row_entry(r, start, (std::int32_t)(ukeys[j] & ((1ULL << key_bits) - 1)), usums[j], (int)(j - first));
     }
 }
 
@@ -849,58 +940,31 @@ __global__ void k_list_lengths(const std::int32_t* list, std::int64_t count, con
         out[i] = (std::int64_t)(off[list[i] + 1] - off[list[i]]);
 }
 
-__global__ void k_copy_rows(std::int64_t rows, const std::int64_t* raw_off, const std::int32_t* rkey,
-                            const std::uint32_t* rval, const std::int64_t* coff64, std::int32_t* off,
-                            std::int32_t* nbr, std::uint32_t* ew, std::int64_t m_total) {
-    const int lane = lane_id();
-    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane; base < rows;
-         base += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int64_t row = base + lane;
-        std::int64_t src = 0, dst = 0, cnt = 0;
-        if (row < rows) {
-            src = raw_off[row];
-            dst = coff64[row];
-            cnt = (row + 1 < rows ? coff64[row + 1] : m_total) - dst;
+// Compacts reduced rows into the final CSR; coff has rows + 1 entries.
+struct CopyBody {
+    const std::int64_t* raw_off;
+    const std::int32_t* rkey;
+    const std::uint32_t* rval;
+    const std::int64_t* coff;
+    std::int32_t* off;
+    std::int32_t* nbr;
+    std::uint32_t* ew;
+    std::int64_t rows;
+    __device__ std::int64_t degree(std::int32_t r) const { return coff[r + 1] - coff[r]; }
+    template <class Team>
+    __device__ RowOut operator()(const Team& t, std::int32_t row, unsigned long long*) const {
+        const std::int64_t src = raw_off[row], dst = coff[row], cnt = coff[row + 1] - dst;
+        if (t.leader()) {
             off[row] = (std::int32_t)dst;
-            if (row == rows - 1) off[rows] = (std::int32_t)m_total;
+            if (row == rows - 1) off[rows] = (std::int32_t)coff[rows];
         }
-        const bool heavy = row < rows && cnt > 16;
-        if (row < rows && !heavy)
-            for (std::int64_t x = 0; x < cnt; ++x) {
-                nbr[dst + x] = rkey[src + x];
-                ew[dst + x] = rval[src + x];
-            }
-        unsigned long long hv = __ballot(heavy);
-        while (hv) {
-            const int sl = __ffsll(hv) - 1;
-            hv &= hv - 1;
-            const std::int64_t hs = __shfl(src, sl), hd = __shfl(dst, sl), hc = __shfl(cnt, sl);
-            for (std::int64_t x = lane; x < hc; x += kWave) {
-                nbr[hd + x] = rkey[hs + x];
-                ew[hd + x] = rval[hs + x];
-            }
+        for (std::int64_t x = t.rank(); x < cnt; x += Team::size) {
+            nbr[dst + x] = rkey[src + x];
+            ew[dst + x] = rval[src + x];
         }
+        return RowOut{false, 0};
     }
-}
-
-__global__ void k_row_classes(std::int64_t rows, const std::int64_t* raw_off, std::int32_t* lists, std::int64_t stride,
-                              ClassCounts* counts) {
-    const int lane = lane_id();
-    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane; base < rows;
-         base += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int64_t v = base + lane;
-        const int cls = v < rows ? degree_class(raw_off[v + 1] - raw_off[v]) : -1;
-        for (int c = 0; c < 7; ++c) {
-            const unsigned long long mask = __ballot(cls == c);
-            if (!mask) continue;
-            unsigned long long start = 0;
-            const int leader = __ffsll(mask) - 1;
-            if (lane == leader) start = atomicAdd(&counts->c[c], (unsigned long long)__popcll(mask));
-            start = __shfl(start, leader);
-            if (cls == c) lists[c * stride + start + __popcll(mask & ((1ULL << lane) - 1))] = (std::int32_t)v;
-        }
-    }
-}
+};
 
 inline int grid_for(std::int64_t n, int per_block = 256, int cap = 8192) {
     return static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(n, per_block), cap)));
@@ -936,9 +1000,13 @@ static std::int64_t reduce_records(Context& c, DBuf<std::uint64_t>& keys, DBuf<s
     const std::int64_t nu = c.read_scalar(total.get());
     ukeys = c.buf<std::uint64_t>(static_cast<std::size_t>(std::max<std::int64_t>(nu, 1)));
     usums = c.buf<unsigned long long>(static_cast<std::size_t>(std::max<std::int64_t>(nu, 1)));
-    fill<unsigned long long>(c, usums.get(), nu, 0ULL);
-    hipLaunchKernelGGL(k_run_accumulate, dim3(grid_for(count)), dim3(256), 0, c.stream(), keys.get(), vals.get(), count,
-                       pos.get(), heads.get(), ukeys.get(), usums.get());
+    DBuf<SegU64> items = c.buf<SegU64>(static_cast<std::size_t>(count));
+    hipLaunchKernelGGL(k_run_items, dim3(grid_for(count)), dim3(256), 0, c.stream(), keys.get(), vals.get(), count,
+                       items.get());
+    GP_LAUNCH_CHECK();
+    scan<SegPlusOp, SegU64>(c, items.get(), items.get(), count, false);
+    hipLaunchKernelGGL(k_run_tails, dim3(grid_for(count)), dim3(256), 0, c.stream(), keys.get(), count, pos.get(),
+                       items.get(), ukeys.get(), usums.get());
     GP_LAUNCH_CHECK();
     return nu;
 }
@@ -948,9 +1016,20 @@ void compute_level0_metadata(Context& c, DGraph& g, CoarsenState& st) {
     DBuf<MetaOut> out = c.buf<MetaOut>(1);
     fill<unsigned long long>(c, reinterpret_cast<unsigned long long*>(out.get()), sizeof(MetaOut) / 8, 0ULL);
     if (g.n > 0) {
-        hipLaunchKernelGGL(k_level0_meta, dim3(grid_for(g.n)), dim3(256), 0, c.stream(), g.n, g.off.get(), g.nbr.get(),
-                           g.ew_ptr(), g.vw.get(), st.row_bound.get(), out.get());
+        DBuf<std::int32_t> ta = c.buf<std::int32_t>(static_cast<std::size_t>(g.n));
+        DBuf<std::int32_t> tb = c.buf<std::int32_t>(static_cast<std::size_t>(g.n));
+        DBuf<unsigned long long> tn = c.buf<unsigned long long>(2);
+        const MetaBody body{g.off.get(), g.nbr.get(), g.ew_ptr(), st.row_bound.get()};
+        run_tiers(c, body, nullptr, g.n, nullptr, ta.get(), tb.get(), tn.get(), nullptr, nullptr, reinterpret_cast<unsigned long long*>(out.get()) + 1,
+                  0, g.n);
+        hipLaunchKernelGGL(k_vertex_meta, dim3(grid_for(g.n, 256, 1024)), dim3(256), 0, c.stream(), g.n, g.off.get(),
+                           g.vw.get(), st.row_bound.get(), out.get());
         GP_LAUNCH_CHECK();
+        if (g.ew && g.m > 0) {
+            hipLaunchKernelGGL(k_edge_flags, dim3(grid_for(g.m, 256, 1024)), dim3(256), 0, c.stream(), g.m, g.ew.get(),
+                               out.get());
+            GP_LAUNCH_CHECK();
+        }
     }
     MetaOut h = c.read_scalar(out.get());
     st.max_row_weight = h.max_row;
@@ -1007,7 +1086,7 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
     DBuf<std::int32_t> lists = c.buf<std::int32_t>(static_cast<std::size_t>(7 * n));
     DBuf<ClassCounts> ccount = c.buf<ClassCounts>(1);
     fill<unsigned long long>(c, reinterpret_cast<unsigned long long*>(ccount.get()), 8, 0ULL);
-    hipLaunchKernelGGL(k_degree_classes, dim3(vb), dim3(256), 0, s, n, g.off.get(), lists.get(), n, ccount.get());
+    classify(c, n, g.off.get(), lists.get(), n, ccount.get());
     GP_LAUNCH_CHECK();
     ClassCounts cc = c.read_scalar(ccount.get());
     std::uint64_t stats_maxdeg = cc.c[7];
@@ -1091,15 +1170,9 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
             const std::int64_t nu =
                 reduce_records(c, keys, vals, hub_records, cluster_bits + hub_bits, ukeys, usums);
             DBuf<HubTop2> acc = c.buf<HubTop2>(static_cast<std::size_t>(hub_count));
-            fill<unsigned long long>(c, reinterpret_cast<unsigned long long*>(acc.get()), 4 * hub_count, 0ULL);
-            if (nu) {
-                hipLaunchKernelGGL(k_hub_best, dim3(grid_for(nu)), dim3(256), 0, s, a, hub_list, ukeys.get(),
-                                   usums.get(), nu, cluster_bits, acc.get());
-                GP_LAUNCH_CHECK();
-                hipLaunchKernelGGL(k_hub_tie, dim3(grid_for(nu)), dim3(256), 0, s, a, hub_list, ukeys.get(),
-                                   usums.get(), nu, cluster_bits, acc.get());
-                GP_LAUNCH_CHECK();
-            }
+            hipLaunchKernelGGL(k_hub_reduce, dim3(grid_for(hub_count, 1, 4096)), dim3(256), 0, s, a, hub_list,
+                               hub_count, ukeys.get(), usums.get(), nu, cluster_bits, acc.get());
+            GP_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_hub_decode, dim3(grid_for(hub_count)), dim3(256), 0, s, a, hub_list, hub_count,
                                acc.get());
             GP_LAUNCH_CHECK();
@@ -1298,9 +1371,9 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
     GP_LAUNCH_CHECK();
     DBuf<unsigned long long> red = c.buf<unsigned long long>(2);
     fill<unsigned long long>(c, red.get(), 2, 0ULL);
-    hipLaunchKernelGGL((k_reduce_max<unsigned long long>), dim3(grid_for(nc)), dim3(256), 0, s, next_rb.get(), nc,
+    hipLaunchKernelGGL((k_reduce_max<unsigned long long>), dim3(grid_for(nc, 256, 1024)), dim3(256), 0, s, next_rb.get(), nc,
                        red.get());
-    hipLaunchKernelGGL((k_reduce_max<unsigned long long>), dim3(grid_for(nc)), dim3(256), 0, s, rawc.get(), nc,
+    hipLaunchKernelGGL((k_reduce_max<unsigned long long>), dim3(grid_for(nc, 256, 1024)), dim3(256), 0, s, rawc.get(), nc,
                        red.get() + 1);
     GP_LAUNCH_CHECK();
     unsigned long long rb[2];
@@ -1318,21 +1391,26 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
     DBuf<unsigned long long> cursor = c.buf<unsigned long long>(static_cast<std::size_t>(nc));
     c.copy_d2d(cursor.get(), raw_off.get(), static_cast<std::size_t>(nc) * sizeof(std::int64_t));
     const std::int64_t m = fine.m;
+    DBuf<std::int32_t> tier_a = c.buf<std::int32_t>(static_cast<std::size_t>(std::max(n, nc)));
+    DBuf<std::int32_t> tier_b = c.buf<std::int32_t>(static_cast<std::size_t>(std::max(n, nc)));
+    DBuf<unsigned long long> tier_n = c.buf<unsigned long long>(2);
     DBuf<std::int32_t> rkey = c.buf<std::int32_t>(static_cast<std::size_t>(std::max<std::int64_t>(m, 1)));
     DBuf<std::uint32_t> rval = c.buf<std::uint32_t>(static_cast<std::size_t>(std::max<std::int64_t>(m, 1)));
     if (m > 0) {
-        hipLaunchKernelGGL(k_contract_scatter, dim3(grid_for(n)), dim3(256), 0, s, n, fine.off.get(), fine.nbr.get(),
-                           fine.ew_ptr(), map, cursor.get(), rkey.get(), rval.get());
-        GP_LAUNCH_CHECK();
+        // Algorithmic traffic: offsets, map[v], neighbours, map[neighbour],
+        // optional edge weights, and the (key, weight) record written per entry.
+        const double bytes = 8.0 * (double)n + 4.0 + (double)m * (8.0 + (fine.ew ? 4.0 : 0.0) + 8.0);
+        const ScatterBody body{fine.off.get(), fine.nbr.get(), fine.ew_ptr(), map, cursor.get(), rkey.get(), rval.get()};
+        c.timed("contract_scatter", bytes,
+                [&] { run_tiers(c, body, nullptr, n, nullptr, tier_a.get(), tier_b.get(), tier_n.get(), nullptr, nullptr,
+                                nullptr, 0, n); });
     }
     cursor.reset();
     // Row classes by raw length.
     DBuf<std::int32_t> lists = c.buf<std::int32_t>(static_cast<std::size_t>(7 * nc));
     DBuf<ClassCounts> ccount = c.buf<ClassCounts>(1);
     fill<unsigned long long>(c, reinterpret_cast<unsigned long long*>(ccount.get()), 8, 0ULL);
-    hipLaunchKernelGGL(k_row_classes, dim3(grid_for(nc)), dim3(256), 0, s, nc, raw_off.get(), lists.get(), nc,
-                       ccount.get());
-    GP_LAUNCH_CHECK();
+    classify(c, nc, raw_off.get(), lists.get(), nc, ccount.get());
     ClassCounts cc = c.read_scalar(ccount.get());
     DBuf<unsigned long long> ucount = c.buf<unsigned long long>(static_cast<std::size_t>(nc));
     fill<unsigned long long>(c, ucount.get(), nc, 0ULL);
@@ -1387,15 +1465,8 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
         DBuf<unsigned long long> usums;
         const std::int64_t nu = reduce_records(c, keys, vals, records, key_bits + row_bits, ukeys, usums);
         if (nu) {
-            DBuf<unsigned long long> first = c.buf<unsigned long long>(static_cast<std::size_t>(long_count));
-            DBuf<unsigned long long> last = c.buf<unsigned long long>(static_cast<std::size_t>(long_count));
-            fill<unsigned long long>(c, first.get(), long_count, ~0ULL);
-            fill<unsigned long long>(c, last.get(), long_count, 0ULL);
-            hipLaunchKernelGGL(k_long_bounds, dim3(grid_for(nu)), dim3(256), 0, s, ukeys.get(), nu, key_bits,
-                               first.get(), last.get());
-            GP_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_long_writeback, dim3(grid_for(nu)), dim3(256), 0, s, r, long_list, ukeys.get(),
-                               usums.get(), nu, key_bits, first.get(), last.get());
+            hipLaunchKernelGGL(k_long_writeback, dim3(grid_for(long_count, 1, 4096)), dim3(256), 0, s, r, long_list,
+                               long_count, ukeys.get(), usums.get(), nu, key_bits);
             GP_LAUNCH_CHECK();
         }
     }
@@ -1404,15 +1475,16 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
     DBuf<std::int64_t> coff64 = c.buf<std::int64_t>(static_cast<std::size_t>(nc + 1));
     DBuf<std::int64_t> mtot = c.buf<std::int64_t>(1);
     exclusive_sum<std::int64_t>(c, ucount.get(), coff64.get(), nc, mtot.get());
+    c.copy_d2d(coff64.get() + nc, mtot.get(), sizeof(std::int64_t));
     const std::int64_t mc = c.read_scalar(mtot.get());
     if (mc > 0x7fffffffLL) throw std::runtime_error("coarse graph exceeds int32 offsets");
     out->m = mc;
     out->off = c.buf<std::int32_t>(static_cast<std::size_t>(nc + 1));
     out->nbr = c.buf<std::int32_t>(static_cast<std::size_t>(std::max<std::int64_t>(mc, 1)));
     out->ew = c.buf<std::uint32_t>(static_cast<std::size_t>(std::max<std::int64_t>(mc, 1)));
-    hipLaunchKernelGGL(k_copy_rows, dim3(grid_for(nc)), dim3(256), 0, s, nc, raw_off.get(), rkey.get(), rval.get(),
-                       coff64.get(), out->off.get(), out->nbr.get(), out->ew.get(), mc);
-    GP_LAUNCH_CHECK();
+    const CopyBody copy{raw_off.get(), rkey.get(), rval.get(), coff64.get(), out->off.get(), out->nbr.get(),
+                        out->ew.get(), nc};
+    run_tiers(c, copy, nullptr, nc, nullptr, tier_a.get(), tier_b.get(), tier_n.get(), nullptr, nullptr, nullptr, 0, nc);
     out->vw = std::move(coarse_vw);
     out->incident_bound = crb;
     if (st.positive_ew) {

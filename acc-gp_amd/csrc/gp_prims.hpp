@@ -299,23 +299,36 @@ inline int bit_width_u64(std::uint64_t v) {
 }
 
 // ---------------------------------------------------------- reductions
+// One global atomic per block (blocks of at most 1024 threads).
 template <typename T>
 __global__ void k_reduce_max(const T* in, std::int64_t n, unsigned long long* out) {
+    __shared__ unsigned long long part[16];
     unsigned long long m = 0;
     for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
          i += (std::int64_t)gridDim.x * blockDim.x)
         m = m > (unsigned long long)in[i] ? m : (unsigned long long)in[i];
     m = wave_max(m);
-    if (lane_id() == 0 && m) atomicMax(out, m);
+    if (lane_id() == 0) part[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (unsigned w = 1; w < blockDim.x / kWave; ++w) m = m > part[w] ? m : part[w];
+        if (m) atomicMax(out, m);
+    }
 }
 template <typename T>
 __global__ void k_reduce_sum(const T* in, std::int64_t n, unsigned long long* out) {
+    __shared__ unsigned long long part[16];
     unsigned long long s = 0;
     for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
          i += (std::int64_t)gridDim.x * blockDim.x)
         s += (unsigned long long)in[i];
     s = wave_sum(s);
-    if (lane_id() == 0 && s) atomicAdd(out, s);
+    if (lane_id() == 0) part[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (unsigned w = 1; w < blockDim.x / kWave; ++w) s += part[w];
+        if (s) atomicAdd(out, s);
+    }
 }
 
 }  // namespace gp

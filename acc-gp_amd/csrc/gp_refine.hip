@@ -5,6 +5,7 @@
 // src/refinement/*.cuh of the reference; order-dependent steps follow the
 // canonical vertex-id order.
 #include "gp_refine.hpp"
+#include "gp_rows.hpp"
 
 #include <algorithm>
 #include <cfloat>
@@ -15,8 +16,6 @@
 namespace gp {
 namespace {
 
-constexpr int kLightDeg = 64;     // 16-lane groups
-constexpr int kHeavyDeg = 1024;   // one block per row above this
 constexpr int kBalanceBuckets = 50;
 constexpr int kBalanceSections = 16;
 constexpr int kBalanceWidth = kBalanceBuckets * kBalanceSections;
@@ -28,10 +27,11 @@ constexpr int kJointArchive = 4;
 enum Ctr {
     kFrontier = 0,
     kCand = 1,
-    kDefer = 2,
-    kMovers = 3,
-    kDelta = 4,
-    kDonors = 5,
+    kTier = 2,  // two slots: medium, heavy
+    kMovers = 4,
+    kDelta = 5,
+    kDonors = 6,
+    kActive = 7,
     kCtrCount = 8
 };
 
@@ -61,10 +61,6 @@ struct RefArgs {
     std::int32_t* locks;
     std::int32_t* flags;
     std::int32_t* member;
-    std::int32_t* frontier;
-    std::int32_t* cand;
-    std::int32_t* movers;
-    std::int32_t* defer;
     unsigned long long* ctr;
     RepairSample* samples;
     // repair
@@ -74,55 +70,25 @@ struct RefArgs {
     Pow15 pow15;
 };
 
-__device__ inline void touch(const RefArgs& a, std::int32_t u) {
-    if (a.member[u] == 0 && atomicExch(&a.member[u], 1) == 0)
-        a.frontier[atomicAdd(&a.ctr[kFrontier], 1ULL)] = u;
-}
-
-__device__ inline void append(std::int32_t* list, unsigned long long* count, std::int32_t v) {
-    list[atomicAdd(count, 1ULL)] = v;
-}
-
-// Part-connectivity table of one vertex in group-private LDS; returns the
+// Part-connectivity table of one vertex in team-private LDS; returns the
 // incident weight excluding self loops (src/refinement/affinity.cuh:9-66).
-template <int W>
-__device__ inline std::uint64_t group_histogram(const RefArgs& a, const std::int32_t* labels, std::int64_t v,
-                                                unsigned long long* hist, int lg) {
-    for (int p = lg; p < a.k; p += W) hist[p] = 0;
-    wave_sync();
+template <class Team>
+__device__ inline std::uint64_t team_histogram(const Team& t, const RefArgs& a, std::int64_t v,
+                                               unsigned long long* hist) {
+    t.sync();  // earlier readers of the table are done
+    for (int p = t.rank(); p < a.k; p += Team::size) hist[p] = 0;
+    t.sync();
     std::uint64_t total = 0;
     const std::int64_t b = a.off[v], e = a.off[v + 1];
-    for (std::int64_t x = b + lg; x < e; x += W) {
+    for (std::int64_t x = b + t.rank(); x < e; x += Team::size) {
         const std::int32_t u = a.nbr[x];
         if (u == v) continue;
         const std::uint64_t w = ew_at(a.ew, x);
-        atomicAdd(&hist[labels[u]], (unsigned long long)w);
+        atomicAdd(&hist[a.labels[u]], (unsigned long long)w);
         total += w;
     }
-    wave_sync();
-    return group_sum<W>(total);
-}
-
-__device__ inline std::uint64_t block_histogram(const RefArgs& a, const std::int32_t* labels, std::int64_t v,
-                                                unsigned long long* hist, unsigned long long* red) {
-    for (int p = threadIdx.x; p < a.k; p += blockDim.x) hist[p] = 0;
-    __syncthreads();
-    std::uint64_t total = 0;
-    const std::int64_t b = a.off[v], e = a.off[v + 1];
-    for (std::int64_t x = b + threadIdx.x; x < e; x += blockDim.x) {
-        const std::int32_t u = a.nbr[x];
-        if (u == v) continue;
-        const std::uint64_t w = ew_at(a.ew, x);
-        atomicAdd(&hist[labels[u]], (unsigned long long)w);
-        total += w;
-    }
-    total = wave_sum(total);
-    if (lane_id() == 0) red[threadIdx.x / kWave] = total;
-    __syncthreads();
-    std::uint64_t sum = 0;
-    for (int i = 0; i < (int)(blockDim.x / kWave); ++i) sum += red[i];
-    __syncthreads();
-    return sum;
+    t.sync();
+    return t.sum(total);
 }
 
 // estimated_repair_cost (src/refinement/incremental.cuh:156-159).
@@ -155,291 +121,182 @@ __device__ inline Cand cand_group_reduce(Cand c, bool priced) {
     }
     return c;
 }
-
-// LP proposal (incremental.cuh:166-198, 229-296) given the table.
-__device__ inline Cand lp_scan_parts(const RefArgs& a, const unsigned long long* hist, int s, std::uint64_t w, int first,
-                                     int step) {
-    Cand best{-1, 0, 0.0};
-    const bool priced = a.prices != nullptr;
-    for (int p = first; p < a.k; p += step) {
-        const std::uint64_t c = hist[p];
-        if (p == s || !c) continue;
-        Cand x{p, c, priced ? (double)c - repair_cost(a, p, w) : 0.0};
-        if (cand_better(x, best, priced)) best = x;
-    }
-    return best;
-}
-
-__device__ inline void lp_accept(const RefArgs& a, std::int64_t v, const Cand& best, std::uint64_t current) {
-    if (best.p >= 0 && (best.conn >= current || current - best.conn < (std::uint64_t)(a.filter * (double)current))) {
-        a.targets[v] = best.p;
-        a.gains[v] = best.conn >= current ? (std::int64_t)(best.conn - current) : -(std::int64_t)(current - best.conn);
-        append(a.cand, &a.ctr[kCand], (std::int32_t)v);
-    }
-}
-
 template <int W>
-__global__ void k_lp_propose(RefArgs a, const std::int32_t* list, const unsigned long long* count_ptr,
-                             unsigned long long* defer_count) {
-    extern __shared__ unsigned long long smem[];
-    const int lg = threadIdx.x % W, gi = threadIdx.x / W;
-    unsigned long long* hist = smem + gi * a.k;
-    const std::int64_t count = (std::int64_t)*count_ptr;
-    const std::int64_t gpb = blockDim.x / W;
-    for (std::int64_t idx = blockIdx.x * gpb + gi; idx < count; idx += (std::int64_t)gridDim.x * gpb) {
-        const std::int64_t v = list[idx];
-        const std::int64_t deg = a.off[v + 1] - a.off[v];
-        if (deg > kHeavyDeg) continue;
-        if (W == 16 && deg > kLightDeg) {
-            if (lg == 0) append(a.defer, defer_count, (std::int32_t)v);
-            continue;
-        }
-        int locked = 0;
-        if (lg == 0) {
-            a.targets[v] = -1;
-            a.gains[v] = INT64_MIN;
-            a.flags[v] = 0;
-            locked = a.locks[v];
-            a.locks[v] = 0;
-        }
-        locked = __shfl(locked, 0, W);
-        if (locked) continue;
-        const std::uint64_t total = group_histogram<W>(a, a.labels, v, hist, lg);
-        const int s = a.labels[v];
-        const std::uint64_t current = hist[s];
-        if (current == total) continue;
-        Cand best = lp_scan_parts(a, hist, s, a.vw[v], lg, W);
-        best = cand_group_reduce<W>(best, a.prices != nullptr);
-        if (lg == 0) lp_accept(a, v, best, current);
-    }
+__device__ inline Cand team_reduce(const GroupTeam<W>&, Cand c, bool priced) {
+    return cand_group_reduce<W>(c, priced);
 }
-
-__device__ inline Cand cand_block_reduce(Cand c, bool priced, Cand* red) {
+__device__ inline Cand team_reduce(const BlockTeam& t, Cand c, bool priced) {
     c = cand_group_reduce<kWave>(c, priced);
-    if (lane_id() == 0) red[threadIdx.x / kWave] = c;
     __syncthreads();
-    Cand r = red[0];
-    for (int i = 1; i < (int)(blockDim.x / kWave); ++i)
-        if (cand_better(red[i], r, priced)) r = red[i];
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        t.red[3 * w] = (unsigned long long)(long long)c.p;
+        t.red[3 * w + 1] = c.conn;
+        t.red[3 * w + 2] = __double_as_longlong(c.score);
+    }
+    __syncthreads();
+    Cand r{(int)(long long)t.red[0], t.red[1], __longlong_as_double((long long)t.red[2])};
+    for (int i = 1; i < kTierThreads / kWave; ++i) {
+        Cand o{(int)(long long)t.red[3 * i], t.red[3 * i + 1], __longlong_as_double((long long)t.red[3 * i + 2])};
+        if (cand_better(o, r, priced)) r = o;
+    }
     __syncthreads();
     return r;
 }
 
-// Heavy rows: one block per vertex of the static heavy list.
-__global__ void k_lp_propose_block(RefArgs a, const std::int32_t* heavy, std::int64_t count) {
-    extern __shared__ unsigned long long smem[];
-    __shared__ unsigned long long red[16];
-    __shared__ Cand cred[16];
-    __shared__ int locked_s;
-    for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
-        const std::int64_t v = heavy[idx];
-        if (threadIdx.x == 0) {
+struct RowBody {
+    RefArgs a;
+    __device__ std::int64_t degree(std::int32_t v) const { return a.off[v + 1] - a.off[v]; }
+};
+
+// LP proposal (incremental.cuh:166-198, 229-296).
+struct LpProposeBody : RowBody {
+    template <class Team>
+    __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long* hist) const {
+        const int locked = a.locks[v];
+        t.sync();
+        if (t.leader()) {
             a.targets[v] = -1;
             a.gains[v] = INT64_MIN;
             a.flags[v] = 0;
-            locked_s = a.locks[v];
             a.locks[v] = 0;
         }
-        __syncthreads();
-        const int locked = locked_s;
-        __syncthreads();
-        if (locked) continue;
-        const std::uint64_t total = block_histogram(a, a.labels, v, smem, red);
+        if (locked) return RowOut{false, 0};
+        const std::uint64_t total = team_histogram(t, a, v, hist);
         const int s = a.labels[v];
-        const std::uint64_t current = smem[s];
-        if (current == total) {
-            __syncthreads();
-            continue;
+        const std::uint64_t current = hist[s];
+        if (current == total) return RowOut{false, 0};
+        const std::uint64_t w = a.vw[v];
+        const bool priced = a.prices != nullptr;
+        Cand best{-1, 0, 0.0};
+        for (int p = t.rank(); p < a.k; p += Team::size) {
+            const std::uint64_t c = hist[p];
+            if (p == s || !c) continue;
+            Cand x{p, c, priced ? (double)c - repair_cost(a, p, w) : 0.0};
+            if (cand_better(x, best, priced)) best = x;
         }
-        Cand best = lp_scan_parts(a, smem, s, a.vw[v], threadIdx.x, blockDim.x);
-        best = cand_block_reduce(best, a.prices != nullptr, cred);
-        if (threadIdx.x == 0) lp_accept(a, v, best, current);
-        __syncthreads();
+        best = team_reduce(t, best, priced);
+        const bool accept = best.p >= 0 && (best.conn >= current ||
+                                            current - best.conn < (std::uint64_t)(a.filter * (double)current));
+        if (accept && t.leader()) {
+            a.targets[v] = best.p;
+            a.gains[v] = best.conn >= current ? (std::int64_t)(best.conn - current) : -(std::int64_t)(current - best.conn);
+        }
+        return RowOut{accept, 0};
     }
-}
+};
 
 // Afterburner (incremental.cuh:361-438).
-__device__ inline std::int64_t after_edge(const RefArgs& a, std::int64_t v, std::int64_t x, int s, int t,
-                                          std::int64_t gv) {
-    const std::int32_t u = a.nbr[x];
-    if (u == v) return 0;
-    const int tu = a.targets[u];
-    if (tu < 0) return 0;
-    const std::int64_t gu = a.gains[u];
-    if (!(gu > gv || (gu == gv && u < v))) return 0;
-    const int lu = a.labels[u];
-    return (std::int64_t)ew_at(a.ew, x) * ((tu == t) - (tu == s) - (lu == t) + (lu == s));
-}
-
-__device__ inline void after_decide(const RefArgs& a, std::int64_t v, int s, int t, std::int64_t gv, std::int64_t corr) {
-    bool keep;
-    if (a.prices) {
-        const std::uint64_t w = a.vw[v];
-        const double adj = repair_cost(a, s, w) - repair_cost(a, t, w);
-        keep = (double)(gv + corr) + adj >= 0;
-    } else
-        keep = gv + corr >= 0;
-    a.flags[v] = keep;
-    if (keep) append(a.movers, &a.ctr[kMovers], (std::int32_t)v);
-}
-
-template <int W>
-__global__ void k_afterburner(RefArgs a, const std::int32_t* list, const unsigned long long* count_ptr,
-                              unsigned long long* defer_count) {
-    const int lg = threadIdx.x % W, gi = threadIdx.x / W;
-    const std::int64_t count = (std::int64_t)*count_ptr;
-    const std::int64_t gpb = blockDim.x / W;
-    for (std::int64_t idx = blockIdx.x * gpb + gi; idx < count; idx += (std::int64_t)gridDim.x * gpb) {
-        const std::int64_t v = list[idx];
-        const std::int64_t b = a.off[v], e = a.off[v + 1];
-        if (e - b > kHeavyDeg) continue;
-        if (W == 16 && e - b > kLightDeg) {
-            if (lg == 0) append(a.defer, defer_count, (std::int32_t)v);
-            continue;
-        }
-        const int s = a.labels[v], t = a.targets[v];
+struct AfterBody : RowBody {
+    template <class Team>
+    __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long*) const {
+        const int s = a.labels[v], tv = a.targets[v];
         const std::int64_t gv = a.gains[v];
         std::int64_t corr = 0;
-        for (std::int64_t x = b + lg; x < e; x += W) corr += after_edge(a, v, x, s, t, gv);
-        corr = group_sum<W>(corr);
-        if (lg == 0) after_decide(a, v, s, t, gv, corr);
-    }
-}
-
-__global__ void k_afterburner_block(RefArgs a, const std::int32_t* heavy, std::int64_t count) {
-    __shared__ long long red[16];
-    for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
-        const std::int64_t v = heavy[idx];
-        if (a.targets[v] < 0) continue;
-        const int s = a.labels[v], t = a.targets[v];
-        const std::int64_t gv = a.gains[v];
-        std::int64_t corr = 0;
-        for (std::int64_t x = a.off[v] + threadIdx.x; x < a.off[v + 1]; x += blockDim.x)
-            corr += after_edge(a, v, x, s, t, gv);
-        corr = wave_sum(corr);
-        if (lane_id() == 0) red[threadIdx.x / kWave] = corr;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            std::int64_t sum = 0;
-            for (int i = 0; i < (int)(blockDim.x / kWave); ++i) sum += red[i];
-            after_decide(a, v, s, t, gv, sum);
+        for (std::int64_t x = a.off[v] + t.rank(); x < a.off[v + 1]; x += Team::size) {
+            const std::int32_t u = a.nbr[x];
+            if (u == v) continue;
+            const int tu = a.targets[u];
+            if (tu < 0) continue;
+            const std::int64_t gu = a.gains[u];
+            if (!(gu > gv || (gu == gv && u < v))) continue;
+            const int lu = a.labels[u];
+            corr += (std::int64_t)ew_at(a.ew, x) * ((tu == tv) - (tu == s) - (lu == tv) + (lu == s));
         }
-        __syncthreads();
+        corr = t.sum(corr);
+        bool keep;
+        if (a.prices) {
+            const std::uint64_t w = a.vw[v];
+            const double adj = repair_cost(a, s, w) - repair_cost(a, tv, w);
+            keep = (double)(gv + corr) + adj >= 0;
+        } else
+            keep = gv + corr >= 0;
+        if (t.leader()) a.flags[v] = keep;
+        return RowOut{keep, 0};
     }
-}
+};
 
-// Exact cut delta of one edge (incremental.cuh:440-470): an undirected edge
-// is owned by the endpoint that moves alone, or by the larger id.
-__device__ inline std::int64_t update_edge(const RefArgs& a, std::int64_t v, std::int64_t x, int s, int t) {
-    const std::int32_t u = a.nbr[x];
-    if (u == v) return 0;
-    const int old_u = a.labels[u];
-    const bool moves = a.flags[u] != 0;
-    const int new_u = moves ? a.targets[u] : old_u;
-    touch(a, u);
-    if (!moves || u > v) return (std::int64_t)ew_at(a.ew, x) * ((s != old_u) - (t != new_u));
-    return 0;
-}
-
-__device__ inline void update_finish(const RefArgs& a, std::int64_t v, int s, std::int64_t gain, bool samples) {
-    if (gain) atomicAdd(&a.ctr[kDelta], (unsigned long long)gain);
-    if (samples && a.samples) {
-        atomicAdd(&a.samples[s].loss, (unsigned long long)(-gain));
-        atomicAdd(&a.samples[s].weight, (unsigned long long)a.vw[v]);
-    }
-}
-
-template <int W>
-__global__ void k_update(RefArgs a, const std::int32_t* list, const unsigned long long* count_ptr,
-                         unsigned long long* defer_count, bool samples) {
-    const int lg = threadIdx.x % W, gi = threadIdx.x / W;
-    const std::int64_t count = (std::int64_t)*count_ptr;
-    const std::int64_t gpb = blockDim.x / W;
-    for (std::int64_t idx = blockIdx.x * gpb + gi; idx < count; idx += (std::int64_t)gridDim.x * gpb) {
-        const std::int64_t v = list[idx];
-        const std::int64_t b = a.off[v], e = a.off[v + 1];
-        if (e - b > kHeavyDeg) continue;
-        if (W == 16 && e - b > kLightDeg) {
-            if (lg == 0) append(a.defer, defer_count, (std::int32_t)v);
-            continue;
-        }
-        const int s = a.labels[v], t = a.targets[v];
+// Exact cut delta (incremental.cuh:440-547): an undirected edge is owned by
+// the endpoint that moves alone, or by the larger id when both move.
+struct UpdateBody : RowBody {
+    bool samples;
+    template <class Team>
+    __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long*) const {
+        const int s = a.labels[v], tv = a.targets[v];
         std::int64_t gain = 0;
-        for (std::int64_t x = b + lg; x < e; x += W) gain += update_edge(a, v, x, s, t);
-        gain = group_sum<W>(gain);
-        if (lg == 0) update_finish(a, v, s, gain, samples);
-    }
-}
-
-__global__ void k_update_block(RefArgs a, const std::int32_t* heavy, std::int64_t count, bool samples) {
-    __shared__ long long red[16];
-    for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
-        const std::int64_t v = heavy[idx];
-        if (!a.flags[v]) continue;
-        const int s = a.labels[v], t = a.targets[v];
-        std::int64_t gain = 0;
-        for (std::int64_t x = a.off[v] + threadIdx.x; x < a.off[v + 1]; x += blockDim.x)
-            gain += update_edge(a, v, x, s, t);
-        gain = wave_sum(gain);
-        if (lane_id() == 0) red[threadIdx.x / kWave] = gain;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            std::int64_t sum = 0;
-            for (int i = 0; i < (int)(blockDim.x / kWave); ++i) sum += red[i];
-            update_finish(a, v, s, sum, samples);
+        for (std::int64_t x = a.off[v] + t.rank(); x < a.off[v + 1]; x += Team::size) {
+            const std::int32_t u = a.nbr[x];
+            if (u == v) continue;
+            const int old_u = a.labels[u];
+            const bool moves = a.flags[u] != 0;
+            const int new_u = moves ? a.targets[u] : old_u;
+            a.member[u] = 1;
+            if (!moves || u > v) gain += (std::int64_t)ew_at(a.ew, x) * ((s != old_u) - (tv != new_u));
         }
-        __syncthreads();
+        gain = t.sum(gain);
+        if (samples && a.samples && t.leader()) {
+            atomicAdd(&a.samples[s].loss, (unsigned long long)(-gain));
+            atomicAdd(&a.samples[s].weight, (unsigned long long)a.vw[v]);
+        }
+        return RowOut{false, gain};
     }
-}
+};
 
-// incremental_commit (incremental.cuh:532-547).
+// Boundary test for the frontier (incremental.cuh:214-225).
+struct BoundaryBody : RowBody {
+    template <class Team>
+    __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long*) const {
+        const int s = a.labels[v];
+        int any = 0;
+        for (std::int64_t x = a.off[v] + t.rank(); x < a.off[v + 1]; x += Team::size) {
+            const std::int32_t u = a.nbr[x];
+            if (u != v && a.labels[u] != s) any = 1;
+        }
+        any = t.sum(any);
+        if (t.leader()) {
+            a.member[v] = any != 0;
+            a.targets[v] = -1;
+            a.gains[v] = INT64_MIN;
+            a.flags[v] = 0;
+            a.locks[v] = 0;
+        }
+        return RowOut{false, 0};
+    }
+};
+
+// incremental_commit (incremental.cuh:532-547) with per-block load deltas.
 __global__ void k_commit(RefArgs a, const std::int32_t* list, const unsigned long long* count_ptr, bool lp) {
+    __shared__ unsigned long long delta[kMaxParts];
+    for (int p = threadIdx.x; p < a.k; p += blockDim.x) delta[p] = 0;
+    __syncthreads();
     const std::int64_t count = (std::int64_t)*count_ptr;
     for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
          i += (std::int64_t)gridDim.x * blockDim.x) {
         const std::int32_t v = list[i];
         const int s = a.labels[v], t = a.targets[v];
         const unsigned long long w = a.vw[v];
-        atomicAdd(&a.loads[s], 0ULL - w);
-        atomicAdd(&a.loads[t], w);
+        atomicAdd(&delta[s], 0ULL - w);
+        atomicAdd(&delta[t], w);
         a.labels[v] = t;
         if (lp) a.locks[v] = 1;
-        touch(a, v);
+        a.member[v] = 1;
     }
+    __syncthreads();
+    for (int p = threadIdx.x; p < a.k; p += blockDim.x)
+        if (delta[p]) atomicAdd(&a.loads[p], delta[p]);
 }
 
-// Boundary frontier and per-level resets (incremental.cuh:214-225).
-template <int W>
-__global__ void k_frontier_init(RefArgs a) {
-    extern __shared__ unsigned long long smem[];
-    const int lg = threadIdx.x % W, gi = threadIdx.x / W;
-    unsigned long long* hist = smem + gi * a.k;
-    const std::int64_t gpb = blockDim.x / W;
-    for (std::int64_t v = blockIdx.x * gpb + gi; v < a.n; v += (std::int64_t)gridDim.x * gpb) {
-        if (lg == 0) {
-            a.targets[v] = -1;
-            a.gains[v] = INT64_MIN;
-            a.flags[v] = 0;
-            a.locks[v] = 0;
-            a.member[v] = 0;
-        }
-        const std::int64_t deg = a.off[v + 1] - a.off[v];
-        bool boundary = false;
-        // Boundary test without the table: any neighbour with another label.
-        const int s = a.labels[v];
-        for (std::int64_t x = a.off[v] + lg; x < a.off[v + 1]; x += W) {
-            const std::int32_t u = a.nbr[x];
-            if (u != v && a.labels[u] != s) boundary = true;
-        }
-        (void)deg;
-        (void)hist;
-        const unsigned long long any = (__ballot(boundary) >> ((threadIdx.x % kWave) / W * W)) &
-                                       (W == 64 ? ~0ULL : ((1ULL << W) - 1));
-        if (lg == 0 && any) {
-            a.member[v] = 1;
-            append(a.frontier, &a.ctr[kFrontier], (std::int32_t)v);
-        }
+// Frontier list from member flags (order irrelevant: per-vertex decisions).
+__global__ void k_compact_flags(std::int64_t n, const std::int32_t* flags, std::int32_t* out,
+                                unsigned long long* count) {
+    __shared__ std::int32_t sb[kTierThreads / kWave][kStage];
+    Stage st{sb[threadIdx.x / kWave], out, count, 0};
+    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane_id(); base < n;
+         base += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t v = base + lane_id();
+        st.push(v < n && flags[v] != 0, (std::int32_t)v);
     }
+    st.flush();
 }
 
 // ------------------------------------------------------------- repair
@@ -452,13 +309,22 @@ __device__ inline int gain_bucket(const RefArgs& a, double gain) {
     return 49;
 }
 
-__global__ void k_repair_donors(RefArgs a) {
-    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < a.n;
-         v += (std::int64_t)gridDim.x * blockDim.x) {
-        a.flags[v] = 0;
-        a.bucket[v] = -1;
-        if (a.loads0[a.labels[v]] > a.cap) append(a.cand, &a.ctr[kDonors], (std::int32_t)v);
+// Donor flags, resets (flags, bucket) on frozen loads.
+__global__ void k_repair_donors(RefArgs a, std::int32_t* donors, unsigned long long* count) {
+    __shared__ std::int32_t sb[kTierThreads / kWave][kStage];
+    Stage st{sb[threadIdx.x / kWave], donors, count, 0};
+    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane_id(); base < a.n;
+         base += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t v = base + lane_id();
+        bool d = false;
+        if (v < a.n) {
+            a.flags[v] = 0;
+            a.bucket[v] = -1;
+            d = a.loads0[a.labels[v]] > a.cap;
+        }
+        st.push(d, (std::int32_t)v);
     }
+    st.flush();
 }
 
 struct RepairPick {
@@ -469,20 +335,6 @@ __device__ inline bool pick_better(const RepairPick& x, const RepairPick& y) {
     if (x.p < 0) return false;
     if (y.p < 0) return true;
     return x.conn > y.conn || (x.conn == y.conn && (x.slack > y.slack || (x.slack == y.slack && x.p < y.p)));
-}
-
-// enumerate_repair_destination (rebalance_select.cuh:8-75) on frozen loads.
-__device__ inline RepairPick repair_scan(const RefArgs& a, const unsigned long long* hist, int s, std::uint64_t w,
-                                         int first, int step) {
-    RepairPick best{-1, 0, 0};
-    for (int p = first; p < a.k; p += step) {
-        if (p == s || a.loads0[p] >= a.cap || w > a.cap - a.loads0[p]) continue;
-        const std::uint64_t c = hist[p];
-        if (!c) continue;
-        RepairPick x{p, c, a.cap - a.loads0[p]};
-        if (pick_better(x, best)) best = x;
-    }
-    return best;
 }
 template <int W>
 __device__ inline RepairPick pick_group_reduce(RepairPick c) {
@@ -495,7 +347,31 @@ __device__ inline RepairPick pick_group_reduce(RepairPick c) {
     }
     return c;
 }
+template <int W>
+__device__ inline RepairPick team_reduce(const GroupTeam<W>&, RepairPick c) {
+    return pick_group_reduce<W>(c);
+}
+__device__ inline RepairPick team_reduce(const BlockTeam& t, RepairPick c) {
+    c = pick_group_reduce<kWave>(c);
+    __syncthreads();
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        t.red[3 * w] = (unsigned long long)(long long)c.p;
+        t.red[3 * w + 1] = c.conn;
+        t.red[3 * w + 2] = c.slack;
+    }
+    __syncthreads();
+    RepairPick r{(int)(long long)t.red[0], t.red[1], t.red[2]};
+    for (int i = 1; i < kTierThreads / kWave; ++i) {
+        RepairPick o{(int)(long long)t.red[3 * i], t.red[3 * i + 1], t.red[3 * i + 2]};
+        if (pick_better(o, r)) r = o;
+    }
+    __syncthreads();
+    return r;
+}
 
+// enumerate_repair_destination (rebalance_select.cuh:8-75) on frozen loads,
+// then the gain bucket (incremental.cuh:559-626).
 __device__ inline void repair_finish(const RefArgs& a, std::int64_t v, int s, std::uint64_t w, std::uint64_t current,
                                      RepairPick best) {
     if (best.p < 0) {
@@ -532,58 +408,27 @@ __device__ inline void repair_finish(const RefArgs& a, std::int64_t v, int s, st
     atomicAdd(&a.counts[b], (unsigned long long)w);
 }
 
-template <int W>
-__global__ void k_repair_propose(RefArgs a, const std::int32_t* list, const unsigned long long* count_ptr,
-                                 unsigned long long* defer_count) {
-    extern __shared__ unsigned long long smem[];
-    const int lg = threadIdx.x % W, gi = threadIdx.x / W;
-    unsigned long long* hist = smem + gi * a.k;
-    const std::int64_t count = (std::int64_t)*count_ptr;
-    const std::int64_t gpb = blockDim.x / W;
-    for (std::int64_t idx = blockIdx.x * gpb + gi; idx < count; idx += (std::int64_t)gridDim.x * gpb) {
-        const std::int64_t v = list[idx];
-        const std::int64_t deg = a.off[v + 1] - a.off[v];
-        if (deg > kHeavyDeg) continue;
-        if (W == 16 && deg > kLightDeg) {
-            if (lg == 0) append(a.defer, defer_count, (std::int32_t)v);
-            continue;
-        }
-        const std::uint64_t total = group_histogram<W>(a, a.labels, v, hist, lg);
+struct RepairProposeBody : RowBody {
+    template <class Team>
+    __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long* hist) const {
+        const std::uint64_t total = team_histogram(t, a, v, hist);
         const int s = a.labels[v];
         const std::uint64_t w = a.vw[v];
         const std::uint64_t current = hist[s];
         RepairPick best{-1, 0, 0};
-        if (current != total) best = repair_scan(a, hist, s, w, lg, W);
-        best = pick_group_reduce<W>(best);
-        if (lg == 0) repair_finish(a, v, s, w, current, best);
+        if (current != total)
+            for (int p = t.rank(); p < a.k; p += Team::size) {
+                if (p == s || a.loads0[p] >= a.cap || w > a.cap - a.loads0[p]) continue;
+                const std::uint64_t c = hist[p];
+                if (!c) continue;
+                RepairPick x{p, c, a.cap - a.loads0[p]};
+                if (pick_better(x, best)) best = x;
+            }
+        best = team_reduce(t, best);
+        if (t.leader()) repair_finish(a, v, s, w, current, best);
+        return RowOut{false, 0};
     }
-}
-
-__global__ void k_repair_propose_block(RefArgs a, const std::int32_t* heavy, std::int64_t count) {
-    extern __shared__ unsigned long long smem[];
-    __shared__ unsigned long long red[16];
-    __shared__ RepairPick pred[16];
-    for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
-        const std::int64_t v = heavy[idx];
-        if (!(a.loads0[a.labels[v]] > a.cap)) continue;
-        const std::uint64_t total = block_histogram(a, a.labels, v, smem, red);
-        const int s = a.labels[v];
-        const std::uint64_t w = a.vw[v];
-        const std::uint64_t current = smem[s];
-        RepairPick best{-1, 0, 0};
-        if (current != total) best = repair_scan(a, smem, s, w, threadIdx.x, blockDim.x);
-        best = pick_group_reduce<kWave>(best);
-        if (lane_id() == 0) pred[threadIdx.x / kWave] = best;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            RepairPick r = pred[0];
-            for (int i = 1; i < (int)(blockDim.x / kWave); ++i)
-                if (pick_better(pred[i], r)) r = pred[i];
-            repair_finish(a, v, s, w, current, r);
-        }
-        __syncthreads();
-    }
-}
+};
 
 // Per overloaded part: exclusive prefix over the 800 buckets and the bucket
 // that contains the excess threshold (balance_scan_buckets, incremental.cuh:628-652).
@@ -617,195 +462,154 @@ __global__ void k_repair_flag(RefArgs a, const std::int32_t* bstar, std::uint8_t
 }
 
 template <int W>
-__global__ void k_conn_rows(RefArgs a, const std::int32_t* list, const std::int64_t* count_ptr,
-                            unsigned long long* rows) {
+__global__ void k_conn_rows(RefArgs a, const std::int32_t* list, std::int64_t count, unsigned long long* rows) {
     extern __shared__ unsigned long long smem[];
     const int lg = threadIdx.x % W, gi = threadIdx.x / W;
     unsigned long long* hist = smem + gi * a.k;
-    const std::int64_t count = *count_ptr;
     const std::int64_t gpb = blockDim.x / W;
     for (std::int64_t idx = blockIdx.x * gpb + gi; idx < count; idx += (std::int64_t)gridDim.x * gpb) {
         const std::int64_t v = list[idx];
-        group_histogram<W>(a, a.labels, v, hist, lg);
+        team_histogram(GroupTeam<W>{lg}, a, v, hist);
         for (int p = lg; p < a.k; p += W) rows[idx * a.k + p] = hist[p];
         wave_sync();
     }
 }
 
-// Receiver reservation (rebalance_graph.cuh:86-156) in canonical order: one
-// workgroup walks the candidate donors in vertex order; within each attempt a
-// part's requesters are ranked by vertex id on top of the start-of-attempt
-// request total.  Also decides bucket eligibility (prefix + rank < excess).
-constexpr int kReserveThreads = 1024;
-constexpr int kReserveWaves = kReserveThreads / kWave;
+// Receiver reservation (rebalance_graph.cuh:86-156) in canonical order:
+// requesters of one part are ranked by vertex id on top of the part's
+// request total at the start of the attempt.  Ranks come from a stable
+// one-pass radix sort by key followed by a segmented scan of weights.
+constexpr std::uint64_t kNoKey = 0xff;
 
-struct ReserveArgs {
-    const std::int32_t* list;
-    const std::int64_t* count_ptr;
-    const unsigned long long* conn;  // count x k
-    const unsigned long long* prefix;
-    const std::int32_t* bstar;
-    const unsigned long long* excess;
-    std::uint8_t* elig;
-    std::uint8_t* done;
-    std::int32_t* choice;
-    unsigned long long* tried;  // count x 2
-};
-
-// Per wave: peers with the same key and the exclusive weighted prefix.
-__device__ inline void wave_keyed_prefix(bool has, int key, std::uint64_t w, std::uint64_t& excl,
-                                         std::uint64_t& total, bool& leader) {
-    const int lane = lane_id();
-    unsigned long long peers = __ballot(has);
-    for (int b = 0; b < 7; ++b) {
-        const bool bit = (key >> b) & 1;
-        const unsigned long long bal = __ballot(bit);
-        peers &= bit ? bal : ~bal;
+// Eligibility keys: part of a donor in its part's boundary bucket.
+__global__ void k_elig_keys(RefArgs a, const std::int32_t* L, std::int64_t nl, const std::int32_t* bstar,
+                            std::uint64_t* keys, std::uint32_t* vals) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < nl;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const int b = a.bucket[L[i]];
+        const int s = b / kBalanceWidth;
+        keys[i] = (b % kBalanceWidth) == bstar[s] ? (std::uint64_t)s : kNoKey;
+        vals[i] = (std::uint32_t)i;
     }
-    excl = 0;
-    total = 0;
-    for (int j = 0; j < kWave; ++j) {
-        const std::uint64_t wj = __shfl(w, j);
-        if ((peers >> j) & 1ULL) {
-            total += wj;
-            if (j < lane) excl += wj;
+}
+// Segment items over keys sorted ascending; w(i) supplied by the caller list.
+__global__ void k_key_items(const std::uint64_t* keys, const std::uint32_t* vals, std::int64_t count,
+                            const std::int32_t* verts, const std::uint32_t* vw, SegU64* items) {
+    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
+         j += (std::int64_t)gridDim.x * blockDim.x) {
+        const bool head = j == 0 || keys[j] != keys[j - 1];
+        const std::uint64_t w = keys[j] == kNoKey ? 0 : (std::uint64_t)vw[verts[vals[j]]];
+        items[j] = SegU64{w, head ? 1u : 0u};
+    }
+}
+__global__ void k_eligible(RefArgs a, const std::int32_t* L, const std::uint64_t* keys, const std::uint32_t* vals,
+                           const SegU64* P, std::int64_t nl, const unsigned long long* prefix,
+                           const std::int32_t* bstar, const unsigned long long* excess, std::uint8_t* elig) {
+    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < nl;
+         j += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::uint32_t i = vals[j];
+        const std::int32_t v = L[i];
+        const int b = a.bucket[v];
+        const int s = b / kBalanceWidth;
+        bool e = (b % kBalanceWidth) < bstar[s];
+        if (keys[j] != kNoKey) {
+            const std::uint64_t rank = P[j].v - a.vw[v];
+            e = prefix[b] + rank < excess[s];
         }
+        elig[i] = e;
     }
-    leader = has && (__ffsll(peers) - 1) == lane;
+}
+__global__ void k_gather_i32(const std::int32_t* src, const std::int32_t* idx, std::int64_t count, std::int32_t* dst) {
+    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
+         j += (std::int64_t)gridDim.x * blockDim.x)
+        dst[j] = src[idx[j]];
 }
 
-__global__ void __launch_bounds__(kReserveThreads) k_reserve(RefArgs a, ReserveArgs r) {
-    __shared__ unsigned long long wtot[kReserveWaves][kMaxParts];
-    __shared__ unsigned long long running[kMaxParts];
-    __shared__ unsigned long long requests[kMaxParts];
-    __shared__ unsigned long long snapshot[kMaxParts];
-    __shared__ unsigned long long slack[kMaxParts];
-    __shared__ int any;
-    const int t = threadIdx.x, wave = t / kWave;
-    const std::int64_t count = *r.count_ptr;
-    const int k = a.k;
-    for (int p = t; p < k; p += kReserveThreads) {
-        running[p] = 0;
-        requests[p] = 0;
-        slack[p] = a.loads0[p] < a.cap ? a.cap - a.loads0[p] : 0;
-        for (int i = 0; i < kReserveWaves; ++i) wtot[i][p] = 0;
-    }
-    __syncthreads();
-    // Phase 1: eligibility with ranks inside the boundary bucket.
-    for (std::int64_t base = 0; base < count; base += kReserveThreads) {
-        const std::int64_t i = base + t;
-        const bool valid = i < count;
-        std::int32_t v = 0;
-        int b = 0, s = 0, lb = 0;
-        std::uint64_t w = 0;
-        if (valid) {
-            v = r.list[i];
-            b = a.bucket[v];
-            s = b / kBalanceWidth;
-            lb = b % kBalanceWidth;
-            w = a.vw[v];
-        }
-        const bool isb = valid && lb == r.bstar[s];
-        std::uint64_t excl, total;
-        bool leader;
-        wave_keyed_prefix(isb, s, w, excl, total, leader);
-        if (leader) wtot[wave][s] = total;
-        __syncthreads();
-        if (valid) {
-            bool e = lb < r.bstar[s];
-            if (isb) {
-                std::uint64_t rank = running[s] + excl;
-                for (int j = 0; j < wave; ++j) rank += wtot[j][s];
-                e = r.prefix[b] + rank < r.excess[s];
-            }
-            r.elig[i] = e;
-            r.done[i] = 0;
-            r.tried[2 * i] = 0;
-            r.tried[2 * i + 1] = 0;
-        }
-        __syncthreads();
-        for (int p = t; p < k; p += kReserveThreads) {
-            unsigned long long add = 0;
-            for (int j = 0; j < kReserveWaves; ++j) {
-                add += wtot[j][p];
-                wtot[j][p] = 0;
-            }
-            running[p] += add;
-        }
-        __syncthreads();
-    }
-    // Phase 2: attempts.
-    for (int attempt = 0; attempt < k; ++attempt) {
-        for (int p = t; p < k; p += kReserveThreads) {
-            snapshot[p] = requests[p];
-            running[p] = 0;
-        }
-        if (t == 0) any = 0;
-        __syncthreads();
-        for (std::int64_t i = t; i < count; i += kReserveThreads) {
-            int c = -1;
-            if (r.elig[i] && !r.done[i]) {
-                const std::int32_t v = r.list[i];
-                if (attempt == 0)
-                    c = a.targets[v];
-                else {
-                    const std::uint64_t w = a.vw[v];
-                    const int lv = a.labels[v];
-                    unsigned long long best = 0, room = 0;
-                    for (int p = 0; p < k; ++p) {
-                        if (p == lv || ((r.tried[2 * i + (p >> 6)] >> (p & 63)) & 1ULL)) continue;
-                        const unsigned long long avail = snapshot[p] < slack[p] ? slack[p] - snapshot[p] : 0;
-                        if (avail < w) continue;
-                        const unsigned long long cn = r.conn[i * k + p];
-                        if (c < 0 || cn > best || (cn == best && avail > room)) {
-                            c = p;
-                            best = cn;
-                            room = avail;
-                        }
+struct ReserveState {
+    const std::int32_t* E;           // eligible donors in vertex order
+    std::int64_t ne;
+    const unsigned long long* conn;  // ne x k
+    unsigned long long* requests;    // k, start-of-attempt totals
+    const unsigned long long* slack; // k
+    std::uint8_t* done;
+    unsigned long long* tried;       // ne x 2
+};
+
+__global__ void k_choose(RefArgs a, ReserveState r, int attempt, std::uint64_t* keys, std::uint32_t* vals,
+                         unsigned long long* active) {
+    unsigned long long cnt = 0;
+    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane_id(); base < r.ne;
+         base += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t e = base + lane_id();
+        if (e >= r.ne) continue;
+        int c = -1;
+        if (!r.done[e]) {
+            const std::int32_t v = r.E[e];
+            if (attempt == 0)
+                c = a.targets[v];
+            else {
+                const std::uint64_t w = a.vw[v];
+                const int lv = a.labels[v];
+                unsigned long long best = 0, room = 0;
+                for (int p = 0; p < a.k; ++p) {
+                    if (p == lv || ((r.tried[2 * e + (p >> 6)] >> (p & 63)) & 1ULL)) continue;
+                    const unsigned long long avail = r.requests[p] < r.slack[p] ? r.slack[p] - r.requests[p] : 0;
+                    if (avail < w) continue;
+                    const unsigned long long cn = r.conn[e * a.k + p];
+                    if (c < 0 || cn > best || (cn == best && avail > room)) {
+                        c = p;
+                        best = cn;
+                        room = avail;
                     }
                 }
-                if (c >= 0) any = 1;
             }
-            r.choice[i] = c;
         }
-        __syncthreads();
-        if (!any) break;
-        for (std::int64_t base = 0; base < count; base += kReserveThreads) {
-            const std::int64_t i = base + t;
-            const bool valid = i < count;
-            int c = valid ? r.choice[i] : -1;
-            const bool has = c >= 0;
-            std::int32_t v = valid ? r.list[i] : 0;
-            const std::uint64_t w = has ? a.vw[v] : 0;
-            std::uint64_t excl, total;
-            bool leader;
-            wave_keyed_prefix(has, has ? c : 0, w, excl, total, leader);
-            if (leader) wtot[wave][c] = total;
-            __syncthreads();
-            if (has) {
-                std::uint64_t begin = requests[c] + running[c] + excl;
-                for (int j = 0; j < wave; ++j) begin += wtot[j][c];
-                r.tried[2 * i + (c >> 6)] |= 1ULL << (c & 63);
-                if (begin <= slack[c] && w <= slack[c] - begin) {
-                    a.targets[v] = c;
-                    a.flags[v] = 1;
-                    r.done[i] = 1;
-                    append(a.movers, &a.ctr[kMovers], v);
-                }
+        keys[e] = c >= 0 ? (std::uint64_t)c : kNoKey;
+        vals[e] = (std::uint32_t)e;
+        cnt += c >= 0;
+    }
+    cnt = wave_sum(cnt);
+    if (lane_id() == 0 && cnt) atomicAdd(active, cnt);
+}
+
+__global__ void k_admit(RefArgs a, ReserveState r, const std::uint64_t* keys, const std::uint32_t* vals,
+                        const SegU64* P, std::int32_t* movers, unsigned long long* mcount) {
+    __shared__ std::int32_t sb[kTierThreads / kWave][kStage];
+    Stage st{sb[threadIdx.x / kWave], movers, mcount, 0};
+    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane_id(); base < r.ne;
+         base += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int64_t j = base + lane_id();
+        bool admit = false;
+        std::int32_t v = 0;
+        if (j < r.ne && keys[j] != kNoKey) {
+            const int c = (int)keys[j];
+            const std::uint32_t e = vals[j];
+            v = r.E[e];
+            const std::uint64_t w = a.vw[v];
+            const std::uint64_t begin = r.requests[c] + P[j].v - w;
+            r.tried[2 * e + (c >> 6)] |= 1ULL << (c & 63);
+            if (begin <= r.slack[c] && w <= r.slack[c] - begin) {
+                a.targets[v] = c;
+                a.flags[v] = 1;
+                r.done[e] = 1;
+                admit = true;
             }
-            __syncthreads();
-            for (int p = t; p < k; p += kReserveThreads) {
-                unsigned long long add = 0;
-                for (int j = 0; j < kReserveWaves; ++j) {
-                    add += wtot[j][p];
-                    wtot[j][p] = 0;
-                }
-                running[p] += add;
-            }
-            __syncthreads();
         }
-        for (int p = t; p < k; p += kReserveThreads) requests[p] += running[p];
-        __syncthreads();
+        st.push(admit, v);
+    }
+    st.flush();
+}
+__global__ void k_requests_add(ReserveState r, const std::uint64_t* keys, const SegU64* P) {
+    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < r.ne;
+         j += (std::int64_t)gridDim.x * blockDim.x) {
+        if (keys[j] == kNoKey) continue;
+        if (j + 1 == r.ne || keys[j + 1] != keys[j]) r.requests[keys[j]] += P[j].v;
+    }
+}
+__global__ void k_slack(RefArgs a, unsigned long long* slack, unsigned long long* requests) {
+    for (int p = threadIdx.x; p < a.k; p += blockDim.x) {
+        slack[p] = a.loads0[p] < a.cap ? a.cap - a.loads0[p] : 0;
+        requests[p] = 0;
     }
 }
 
@@ -849,7 +653,7 @@ __global__ void k_dense_affinity(RefArgs a, unsigned long long* A) {
     unsigned long long* hist = smem + gi * a.k;
     const std::int64_t gpb = blockDim.x / W;
     for (std::int64_t v = blockIdx.x * gpb + gi; v < a.n; v += (std::int64_t)gridDim.x * gpb) {
-        group_histogram<W>(a, a.labels, v, hist, lg);
+        team_histogram(GroupTeam<W>{lg}, a, v, hist);
         for (int p = lg; p < a.k; p += W) A[v * a.k + p] = hist[p];
         wave_sync();
     }
@@ -1300,11 +1104,6 @@ __global__ void k_joint_accept(RefArgs a, const JMove* m, int count, const std::
     if (i < count && m[i].size && i <= stops[m[i].a])
         for (int j = 0; j < m[i].size; ++j) a.flags[m[i].v[j]] = 1;
 }
-__global__ void k_flag_list(RefArgs a, unsigned long long* count) {
-    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < a.n;
-         v += (std::int64_t)gridDim.x * blockDim.x)
-        if (a.flags[v]) append(a.movers, count, (std::int32_t)v);
-}
 
 // ---------------------------------------------------------- basics
 __global__ void k_project(std::int64_t n, const std::int32_t* map, const std::int32_t* coarse, std::int32_t* fine) {
@@ -1347,11 +1146,6 @@ __global__ void k_loads(std::int64_t n, const std::uint32_t* vw, const std::int3
     for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
          v += (std::int64_t)gridDim.x * blockDim.x)
         atomicAdd(&loads[labels[v]], (unsigned long long)vw[v]);
-}
-__global__ void k_heavy_flags(std::int64_t n, const std::int32_t* off, std::uint8_t* f) {
-    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
-         v += (std::int64_t)gridDim.x * blockDim.x)
-        f[v] = off[v + 1] - off[v] > kHeavyDeg;
 }
 __global__ void k_max_u32(std::int64_t n, const std::uint32_t* x, unsigned long long* out) {
     unsigned long long m = 0;
@@ -1435,9 +1229,10 @@ public:
         frontier_ = c.buf<std::int32_t>(n);
         cand_ = c.buf<std::int32_t>(n);
         movers_ = c.buf<std::int32_t>(n);
-        defer_ = c.buf<std::int32_t>(n);
-        ctr_ = c.buf<unsigned long long>(kCtrCount);
-        loads_ = c.buf<unsigned long long>(static_cast<std::size_t>(k));
+        medium_ = c.buf<std::int32_t>(n);
+        heavy_ = c.buf<std::int32_t>(n);
+        // Counters and part loads share one allocation: one read returns both.
+        ctr_ = c.buf<unsigned long long>(static_cast<std::size_t>(kCtrCount + k));
         loads0_ = c.buf<unsigned long long>(static_cast<std::size_t>(k));
         bucket_ = c.buf<std::int32_t>(n);
         counts_ = c.buf<unsigned long long>(static_cast<std::size_t>(k) * kBalanceWidth);
@@ -1446,19 +1241,15 @@ public:
         a_.locks = locks_.get();
         a_.flags = flags_.get();
         a_.member = member_.get();
-        a_.frontier = frontier_.get();
-        a_.cand = cand_.get();
-        a_.movers = movers_.get();
-        a_.defer = defer_.get();
         a_.ctr = ctr_.get();
-        a_.loads = loads_.get();
+        a_.loads = ctr_.get() + kCtrCount;
         a_.loads0 = loads0_.get();
         a_.bucket = bucket_.get();
         a_.counts = counts_.get();
         a_.pow15.p[23] = 1.0;
         for (int j = 1; j <= 24; ++j) a_.pow15.p[23 + j] = a_.pow15.p[22 + j] * 1.5;
         for (int j = 1; j <= 23; ++j) a_.pow15.p[23 - j] = a_.pow15.p[24 - j] / 1.5;
-        c.upload(loads_.get(), loads.data(), static_cast<std::size_t>(k) * sizeof(std::uint64_t));
+        c.upload(a_.loads, loads.data(), static_cast<std::size_t>(k) * sizeof(std::uint64_t));
         fill<unsigned long long>(c, ctr_.get(), kCtrCount, 0ULL);
         std::uint64_t total = 0;
         for (auto x : loads) total += x;
@@ -1469,17 +1260,8 @@ public:
             a_.prices = prices_.get();
             a_.samples = samples_.get();
         }
-        // Static heavy-row list.
-        DBuf<std::uint8_t> hf = c.buf<std::uint8_t>(n);
-        heavy_ = c.buf<std::int32_t>(n);
-        DBuf<std::int64_t> hc = c.buf<std::int64_t>(1);
-        hipLaunchKernelGGL(k_heavy_flags, dim3(grid_for(n)), dim3(256), 0, c.stream(), n, g.off.get(), hf.get());
-        GP_LAUNCH_CHECK();
-        stable_select(c, hf.get(), n, heavy_.get(), hc.get());
-        heavy_count_ = c.read_scalar(hc.get());
-        // Frontier of boundary vertices, resets.
-        hipLaunchKernelGGL((k_frontier_init<16>), dim3(grid_for(n, 16)), dim3(256), 0, c.stream(), a_);
-        GP_LAUNCH_CHECK();
+        // Boundary vertices seed the frontier; per-level resets.
+        tiers(BoundaryBody{{a_}}, nullptr, n, nullptr, nullptr, nullptr, nullptr, 0);
         // filter constant (incremental.cuh:897-899)
         std::uint64_t max_edge = 1;
         if (g.ew) {
@@ -1500,93 +1282,41 @@ public:
         return m;
     }
 
-    void sync_loads() {
-        c_.read(loads_host_.data(), loads_.get(), loads_host_.size() * sizeof(std::uint64_t));
+    unsigned long long* loads_dev() const { return a_.loads; }
+
+    template <class Body>
+    void tiers(const Body& body, const std::int32_t* list, std::int64_t n, const unsigned long long* count,
+               std::int32_t* out, unsigned long long* out_count, unsigned long long* acc, int scratch) {
+        run_tiers(c_, body, list, n, count, medium_.get(), heavy_.get(), ctr_.get() + kTier, out, out_count, acc,
+                  scratch, a_.n);
     }
 
-    std::size_t hist_smem(int groups) const { return static_cast<std::size_t>(groups) * a_.k * sizeof(unsigned long long); }
-
-    // Counter-driven three-tier launch (16-lane groups, deferred waves, heavy blocks).
-    template <typename Light, typename Medium, typename Heavy>
-    void tiers(Light light, Medium medium, Heavy heavy) {
-        fill<unsigned long long>(c_, ctr_.get() + kDefer, 1, 0ULL);
-        light();
-        medium();
-        if (heavy_count_) heavy();
-    }
-
-    // One LP round; returns the number of movers and the cut delta.
+    // One LP round over the frontier; returns the number of movers and the cut delta.
     std::int64_t lp_round(std::int64_t& delta) {
         const hipStream_t s = c_.stream();
         const std::int64_t n = a_.n;
-        if (samples_) fill<unsigned long long>(c_, reinterpret_cast<unsigned long long*>(samples_.get()), 2 * a_.k, 0ULL);
-        fill<unsigned long long>(c_, ctr_.get() + kCand, 1, 0ULL);
-        fill<unsigned long long>(c_, ctr_.get() + kMovers, 1, 0ULL);
-        fill<unsigned long long>(c_, ctr_.get() + kDelta, 1, 0ULL);
-        const int gb = grid_for(n, 16, 2048);
-        tiers(
-            [&] {
-                hipLaunchKernelGGL((k_lp_propose<16>), dim3(gb), dim3(256), hist_smem(16), s, a_, frontier_.get(),
-                                   ctr_.get() + kFrontier, ctr_.get() + kDefer);
-                GP_LAUNCH_CHECK();
-            },
-            [&] {
-                hipLaunchKernelGGL((k_lp_propose<64>), dim3(grid_for(n, 4, 1024)), dim3(256), hist_smem(4), s, a_,
-                                   defer_.get(), ctr_.get() + kDefer, ctr_.get() + kDefer);
-                GP_LAUNCH_CHECK();
-            },
-            [&] {
-                hipLaunchKernelGGL(k_lp_propose_block, dim3(grid_for(heavy_count_, 1, 2048)), dim3(256), hist_smem(1),
-                                   s, a_, heavy_.get(), heavy_count_);
-                GP_LAUNCH_CHECK();
-            });
-        tiers(
-            [&] {
-                hipLaunchKernelGGL((k_afterburner<16>), dim3(gb), dim3(256), 0, s, a_, cand_.get(), ctr_.get() + kCand,
-                                   ctr_.get() + kDefer);
-                GP_LAUNCH_CHECK();
-            },
-            [&] {
-                hipLaunchKernelGGL((k_afterburner<64>), dim3(grid_for(n, 4, 1024)), dim3(256), 0, s, a_, defer_.get(),
-                                   ctr_.get() + kDefer, ctr_.get() + kDefer);
-                GP_LAUNCH_CHECK();
-            },
-            [&] {
-                hipLaunchKernelGGL(k_afterburner_block, dim3(grid_for(heavy_count_, 1, 2048)), dim3(256), 0, s, a_,
-                                   heavy_.get(), heavy_count_);
-                GP_LAUNCH_CHECK();
-            });
+        fill<unsigned long long>(c_, ctr_.get(), kCtrCount, 0ULL);
+        hipLaunchKernelGGL(k_compact_flags, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, n, member_.get(),
+                           frontier_.get(), ctr_.get() + kFrontier);
+        GP_LAUNCH_CHECK();
+        tiers(LpProposeBody{{a_}}, frontier_.get(), 0, ctr_.get() + kFrontier, cand_.get(), ctr_.get() + kCand,
+              nullptr, a_.k);
+        tiers(AfterBody{{a_}}, cand_.get(), 0, ctr_.get() + kCand, movers_.get(), ctr_.get() + kMovers, nullptr, 0);
         return apply(true, false, delta);
     }
 
     // Update + commit of the current mover list.
     std::int64_t apply(bool lp, bool samples, std::int64_t& delta) {
         const hipStream_t s = c_.stream();
-        const std::int64_t n = a_.n;
-        const int gb = grid_for(n, 16, 2048);
         fill<unsigned long long>(c_, ctr_.get() + kDelta, 1, 0ULL);
-        tiers(
-            [&] {
-                hipLaunchKernelGGL((k_update<16>), dim3(gb), dim3(256), 0, s, a_, movers_.get(), ctr_.get() + kMovers,
-                                   ctr_.get() + kDefer, samples);
-                GP_LAUNCH_CHECK();
-            },
-            [&] {
-                hipLaunchKernelGGL((k_update<64>), dim3(grid_for(n, 4, 1024)), dim3(256), 0, s, a_, defer_.get(),
-                                   ctr_.get() + kDefer, ctr_.get() + kDefer, samples);
-                GP_LAUNCH_CHECK();
-            },
-            [&] {
-                hipLaunchKernelGGL(k_update_block, dim3(grid_for(heavy_count_, 1, 2048)), dim3(256), 0, s, a_,
-                                   heavy_.get(), heavy_count_, samples);
-                GP_LAUNCH_CHECK();
-            });
-        hipLaunchKernelGGL(k_commit, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, a_, movers_.get(),
+        tiers(UpdateBody{{a_}, samples}, movers_.get(), 0, ctr_.get() + kMovers, nullptr, nullptr,
+              ctr_.get() + kDelta, 0);
+        hipLaunchKernelGGL(k_commit, dim3(grid_for(a_.n, 256, 1024)), dim3(256), 0, s, a_, movers_.get(),
                            ctr_.get() + kMovers, lp);
         GP_LAUNCH_CHECK();
-        unsigned long long h[kCtrCount];
-        c_.read(h, ctr_.get(), sizeof h);
-        sync_loads();
+        std::vector<unsigned long long> h(static_cast<std::size_t>(kCtrCount + a_.k));
+        c_.read(h.data(), ctr_.get(), h.size() * sizeof(unsigned long long));
+        for (int p = 0; p < a_.k; ++p) loads_host_[static_cast<std::size_t>(p)] = h[static_cast<std::size_t>(kCtrCount + p)];
         delta = (std::int64_t)h[kDelta];
         return (std::int64_t)h[kMovers];
     }
@@ -1598,6 +1328,68 @@ public:
         } else
             cut += (std::uint64_t)(-delta);
         return true;
+    }
+
+    // Ranks of keyed items: stable sort by key, then an inclusive segmented
+    // scan of weights; P[j] is the weight of items with the same key up to j.
+    void keyed_prefix(std::uint64_t* keys, std::uint32_t* vals, std::int64_t count, const std::int32_t* verts,
+                      SegU64* P) {
+        radix_sort_pairs(c_, keys, vals, count, 0, 8);
+        hipLaunchKernelGGL(k_key_items, dim3(grid_for(count)), dim3(256), 0, c_.stream(), keys, vals, count, verts,
+                           a_.vw, P);
+        GP_LAUNCH_CHECK();
+        scan<SegPlusOp, SegU64>(c_, P, P, count, false);
+    }
+
+    // Receiver reservation over the flagged donors L (vertex order).
+    void reserve(const std::int32_t* L, std::int64_t nl, const unsigned long long* prefix, const std::int32_t* bstar,
+                 const unsigned long long* excess) {
+        const hipStream_t s = c_.stream();
+        const int k = a_.k;
+        DBuf<std::uint64_t> keys = c_.buf<std::uint64_t>(static_cast<std::size_t>(nl));
+        DBuf<std::uint32_t> vals = c_.buf<std::uint32_t>(static_cast<std::size_t>(nl));
+        DBuf<SegU64> P = c_.buf<SegU64>(static_cast<std::size_t>(nl));
+        DBuf<std::uint8_t> elig = c_.buf<std::uint8_t>(static_cast<std::size_t>(nl));
+        hipLaunchKernelGGL(k_elig_keys, dim3(grid_for(nl)), dim3(256), 0, s, a_, L, nl, bstar, keys.get(), vals.get());
+        GP_LAUNCH_CHECK();
+        keyed_prefix(keys.get(), vals.get(), nl, L, P.get());
+        hipLaunchKernelGGL(k_eligible, dim3(grid_for(nl)), dim3(256), 0, s, a_, L, keys.get(), vals.get(), P.get(), nl,
+                           prefix, bstar, excess, elig.get());
+        GP_LAUNCH_CHECK();
+        DBuf<std::int32_t> idx = c_.buf<std::int32_t>(static_cast<std::size_t>(nl));
+        DBuf<std::int64_t> cnt = c_.buf<std::int64_t>(1);
+        stable_select(c_, elig.get(), nl, idx.get(), cnt.get());
+        const std::int64_t ne = c_.read_scalar(cnt.get());
+        if (ne <= 0) return;
+        DBuf<std::int32_t> E = c_.buf<std::int32_t>(static_cast<std::size_t>(ne));
+        hipLaunchKernelGGL(k_gather_i32, dim3(grid_for(ne)), dim3(256), 0, s, L, idx.get(), ne, E.get());
+        GP_LAUNCH_CHECK();
+        DBuf<unsigned long long> conn = c_.buf<unsigned long long>(static_cast<std::size_t>(ne) * k);
+        hipLaunchKernelGGL((k_conn_rows<kWave>), dim3(grid_for(ne, 4, 2048)), dim3(256), 4 * k * sizeof(unsigned long long),
+                           s, a_, E.get(), ne, conn.get());
+        GP_LAUNCH_CHECK();
+        DBuf<unsigned long long> requests = c_.buf<unsigned long long>(static_cast<std::size_t>(k));
+        DBuf<unsigned long long> slack = c_.buf<unsigned long long>(static_cast<std::size_t>(k));
+        DBuf<std::uint8_t> done = c_.buf<std::uint8_t>(static_cast<std::size_t>(ne));
+        DBuf<unsigned long long> tried = c_.buf<unsigned long long>(static_cast<std::size_t>(ne) * 2);
+        fill<std::uint8_t>(c_, done.get(), ne, 0);
+        fill<unsigned long long>(c_, tried.get(), 2 * ne, 0ULL);
+        hipLaunchKernelGGL(k_slack, dim3(1), dim3(kMaxParts), 0, s, a_, slack.get(), requests.get());
+        GP_LAUNCH_CHECK();
+        ReserveState r{E.get(), ne, conn.get(), requests.get(), slack.get(), done.get(), tried.get()};
+        for (int attempt = 0; attempt < k; ++attempt) {
+            fill<unsigned long long>(c_, ctr_.get() + kActive, 1, 0ULL);
+            hipLaunchKernelGGL(k_choose, dim3(grid_for(ne)), dim3(256), 0, s, a_, r, attempt, keys.get(), vals.get(),
+                               ctr_.get() + kActive);
+            GP_LAUNCH_CHECK();
+            if (!c_.read_scalar(ctr_.get() + kActive)) break;
+            keyed_prefix(keys.get(), vals.get(), ne, E.get(), P.get());
+            hipLaunchKernelGGL(k_admit, dim3(grid_for(ne)), dim3(256), 0, s, a_, r, keys.get(), vals.get(), P.get(),
+                               movers_.get(), ctr_.get() + kMovers);
+            GP_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_requests_add, dim3(grid_for(ne)), dim3(256), 0, s, r, keys.get(), P.get());
+            GP_LAUNCH_CHECK();
+        }
     }
 
     // Capacity repair loop (rebalance_graph.cuh:160-280); returns feasibility.
@@ -1615,49 +1407,20 @@ public:
         DBuf<std::int32_t> list = c_.buf<std::int32_t>(n);
         DBuf<std::int64_t> lcount = c_.buf<std::int64_t>(1);
         do {
-            c_.copy_d2d(loads0_.get(), loads_.get(), static_cast<std::size_t>(k) * sizeof(unsigned long long));
+            c_.copy_d2d(loads0_.get(), a_.loads, static_cast<std::size_t>(k) * sizeof(unsigned long long));
             fill<unsigned long long>(c_, counts_.get(), (std::int64_t)k * kBalanceWidth, 0ULL);
-            fill<unsigned long long>(c_, ctr_.get() + kDonors, 1, 0ULL);
-            fill<unsigned long long>(c_, ctr_.get() + kMovers, 1, 0ULL);
-            hipLaunchKernelGGL(k_repair_donors, dim3(grid_for(n)), dim3(256), 0, s, a_);
+            fill<unsigned long long>(c_, ctr_.get(), kCtrCount, 0ULL);
+            hipLaunchKernelGGL(k_repair_donors, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, a_, cand_.get(),
+                               ctr_.get() + kDonors);
             GP_LAUNCH_CHECK();
-            const int gb = grid_for(n, 16, 2048);
-            tiers(
-                [&] {
-                    hipLaunchKernelGGL((k_repair_propose<16>), dim3(gb), dim3(256), hist_smem(16), s, a_, cand_.get(),
-                                       ctr_.get() + kDonors, ctr_.get() + kDefer);
-                    GP_LAUNCH_CHECK();
-                },
-                [&] {
-                    hipLaunchKernelGGL((k_repair_propose<64>), dim3(grid_for(n, 4, 1024)), dim3(256), hist_smem(4), s,
-                                       a_, defer_.get(), ctr_.get() + kDefer, ctr_.get() + kDefer);
-                    GP_LAUNCH_CHECK();
-                },
-                [&] {
-                    hipLaunchKernelGGL(k_repair_propose_block, dim3(grid_for(heavy_count_, 1, 2048)), dim3(256),
-                                       hist_smem(1), s, a_, heavy_.get(), heavy_count_);
-                    GP_LAUNCH_CHECK();
-                });
+            tiers(RepairProposeBody{{a_}}, cand_.get(), 0, ctr_.get() + kDonors, nullptr, nullptr, nullptr, a_.k);
             hipLaunchKernelGGL(k_bucket_scan, dim3(k), dim3(64), 0, s, a_, prefix.get(), bstar.get(), excess.get());
             GP_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_repair_flag, dim3(grid_for(n)), dim3(256), 0, s, a_, bstar.get(), f.get());
             GP_LAUNCH_CHECK();
             stable_select(c_, f.get(), n, list.get(), lcount.get());
             const std::int64_t nl = c_.read_scalar(lcount.get());
-            if (nl > 0) {
-                DBuf<unsigned long long> conn = c_.buf<unsigned long long>(static_cast<std::size_t>(nl) * k);
-                DBuf<std::uint8_t> elig = c_.buf<std::uint8_t>(static_cast<std::size_t>(nl));
-                DBuf<std::uint8_t> done = c_.buf<std::uint8_t>(static_cast<std::size_t>(nl));
-                DBuf<std::int32_t> choice = c_.buf<std::int32_t>(static_cast<std::size_t>(nl));
-                DBuf<unsigned long long> tried = c_.buf<unsigned long long>(static_cast<std::size_t>(nl) * 2);
-                hipLaunchKernelGGL((k_conn_rows<64>), dim3(grid_for(nl, 4, 2048)), dim3(256), hist_smem(4), s, a_,
-                                   list.get(), lcount.get(), conn.get());
-                GP_LAUNCH_CHECK();
-                ReserveArgs r{list.get(), lcount.get(), conn.get(), prefix.get(), bstar.get(), excess.get(),
-                              elig.get(), done.get(), choice.get(), tried.get()};
-                hipLaunchKernelGGL(k_reserve, dim3(1), dim3(kReserveThreads), 0, s, a_, r);
-                GP_LAUNCH_CHECK();
-            }
+            if (nl > 0) reserve(list.get(), nl, prefix.get(), bstar.get(), excess.get());
             hipLaunchKernelGGL(k_repair_clear, dim3(grid_for(n)), dim3(256), 0, s, a_, cand_.get(),
                                ctr_.get() + kDonors);
             GP_LAUNCH_CHECK();
@@ -1691,7 +1454,7 @@ public:
         DBuf<std::int32_t> owners = c_.buf<std::int32_t>(n);
         DBuf<std::int32_t> stops = c_.buf<std::int32_t>(static_cast<std::size_t>(k));
         auto dense = [&] {
-            hipLaunchKernelGGL((k_dense_affinity<64>), dim3(grid_for(n, 4, 4096)), dim3(256), hist_smem(4), s, a_,
+            hipLaunchKernelGGL((k_dense_affinity<64>), dim3(grid_for(n, 4, 4096)), dim3(256), 4 * k * sizeof(unsigned long long), s, a_,
                                A.get());
             GP_LAUNCH_CHECK();
         };
@@ -1748,7 +1511,8 @@ public:
             hipLaunchKernelGGL(k_joint_accept, dim3(ceil_div(active, 256)), dim3(256), 0, s, a_, moves.get(), active,
                                stops.get());
             fill<unsigned long long>(c_, ctr_.get() + kMovers, 1, 0ULL);
-            hipLaunchKernelGGL(k_flag_list, dim3(grid_for(n)), dim3(256), 0, s, a_, ctr_.get() + kMovers);
+            hipLaunchKernelGGL(k_compact_flags, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, n, flags_.get(),
+                               movers_.get(), ctr_.get() + kMovers);
             GP_LAUNCH_CHECK();
             const unsigned long long cnt = c_.read_scalar(ctr_.get() + kMovers);
             if (!cnt) {
@@ -1841,12 +1605,11 @@ public:
     const RefineOptions& o_;
     LevelStats& st_;
     RefArgs a_;
-    DBuf<std::int32_t> targets_, locks_, flags_, member_, frontier_, cand_, movers_, defer_, bucket_, heavy_;
+    DBuf<std::int32_t> targets_, locks_, flags_, member_, frontier_, cand_, movers_, medium_, heavy_, bucket_;
     DBuf<std::int64_t> gains_;
-    DBuf<unsigned long long> ctr_, loads_, loads0_, counts_;
+    DBuf<unsigned long long> ctr_, loads0_, counts_;
     DBuf<double> prices_;
     DBuf<RepairSample> samples_;
-    std::int64_t heavy_count_ = 0;
 };
 
 }  // namespace
@@ -1905,7 +1668,7 @@ LevelStats refine_level(Context& c, const DGraph& g, std::int32_t* labels, std::
     }
     c.copy_d2d(labels, best_labels.get(), static_cast<std::size_t>(g.n) * sizeof(std::int32_t));
     loads = best_loads;
-    c.upload(R.loads_.get(), loads.data(), static_cast<std::size_t>(k) * sizeof(std::uint64_t));
+    c.upload(R.loads_dev(), loads.data(), static_cast<std::size_t>(k) * sizeof(std::uint64_t));
     cut = best_cut;
     if (rounds > 0 && o.exchange_rounds > 0 && g.n <= o.exchange_max_n) {
         fill<std::int32_t>(c, R.flags_.get(), g.n, 0);

@@ -1,0 +1,234 @@
+// Three-tier row processing for skewed degree distributions.
+//
+// A "body" processes one CSR row with a team of lanes: a 16-lane group
+// (rows <= kTierLight entries), a full wave (rows <= kTierMedium) or a
+// 256-thread block (longer rows).  Light kernels defer longer rows to a
+// medium list and medium kernels defer to a heavy list, with wave-aggregated
+// appends (one atomic per wave).  Loops are wave-uniform so every lane of a
+// wave reaches the ballots.
+#pragma once
+#include "gp_prims.hpp"
+
+namespace gp {
+
+constexpr int kTierLight = 64;
+constexpr int kTierMedium = 4096;
+constexpr int kTierThreads = 256;
+constexpr int kBlockRed = 32;  // u64 words of block-team reduction scratch
+
+// Appends v for every lane with pred set; all lanes of the wave must call it.
+__device__ inline void wave_append(bool pred, std::int32_t v, std::int32_t* list, unsigned long long* count) {
+    const unsigned long long mask = __ballot(pred);
+    if (!mask) return;
+    const int lane = lane_id();
+    const int leader = __ffsll(mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader);
+    if (pred) list[base + __popcll(mask & ((1ULL << lane) - 1ULL))] = v;
+}
+
+// Wave-private LDS staging of list appends: one global atomic per kStage
+// entries instead of one per wave step.  List order is unspecified; every
+// list built this way is consumed by per-vertex work.  All lanes of the
+// wave call push() and flush() together.
+constexpr int kStage = 256;
+
+struct Stage {
+    std::int32_t* buf;  // kStage LDS entries owned by one wave
+    std::int32_t* list;
+    unsigned long long* count;
+    int fill;
+
+    __device__ void flush() {
+        if (!fill) return;
+        wave_sync();
+        unsigned long long base = 0;
+        if (lane_id() == 0) base = atomicAdd(count, (unsigned long long)fill);
+        base = __shfl(base, 0);
+        for (int i = lane_id(); i < fill; i += kWave) list[base + i] = buf[i];
+        wave_sync();
+        fill = 0;
+    }
+    __device__ void push(bool pred, std::int32_t v) {
+        if (!list) return;
+        const unsigned long long mask = __ballot(pred);
+        if (!mask) return;
+        const int cnt = __popcll(mask);
+        if (fill + cnt > kStage) flush();
+        if (pred) buf[fill + __popcll(mask & ((1ULL << lane_id()) - 1ULL))] = v;
+        fill += cnt;
+    }
+};
+
+template <int W>
+struct GroupTeam {
+    int lg;
+    static constexpr int size = W;
+    __device__ int rank() const { return lg; }
+    template <typename T>
+    __device__ T sum(T x) const { return group_sum<W>(x); }
+    __device__ void sync() const { wave_sync(); }
+    __device__ bool leader() const { return lg == 0; }
+    template <typename T>
+    __device__ T bcast(T x) const { return __shfl(x, 0, W); }
+};
+
+struct BlockTeam {
+    unsigned long long* red;  // LDS scratch of kBlockRed entries
+    static constexpr int size = kTierThreads;
+    __device__ int rank() const { return threadIdx.x; }
+    template <typename T>
+    __device__ T sum(T x) const {
+        x = wave_sum(x);
+        __syncthreads();
+        if (lane_id() == 0) red[threadIdx.x / kWave] = (unsigned long long)x;
+        __syncthreads();
+        T s = 0;
+        for (int i = 0; i < kTierThreads / kWave; ++i) s += (T)red[i];
+        __syncthreads();
+        return s;
+    }
+    __device__ void sync() const { __syncthreads(); }
+    __device__ bool leader() const { return threadIdx.x == 0; }
+    template <typename T>
+    __device__ T bcast(T x) const {
+        static_assert(sizeof(T) <= sizeof(unsigned long long), "bcast of a word");
+        __syncthreads();
+        if (threadIdx.x == 0) red[0] = (unsigned long long)x;
+        __syncthreads();
+        const T r = (T)red[0];
+        __syncthreads();
+        return r;
+    }
+};
+
+struct RowOut {
+    bool append;
+    std::int64_t acc;
+};
+
+struct TierLists {
+    std::int32_t* medium;
+    unsigned long long* medium_count;
+    std::int32_t* heavy;
+    unsigned long long* heavy_count;
+    std::int32_t* out;          // rows whose body asked for an append
+    unsigned long long* out_count;
+    unsigned long long* acc;    // sum of body accumulators (modular)
+};
+
+// Light tier: list (or the range [0, n) when list is null) of count rows,
+// count read from count_ptr when non-null.
+template <class Body>
+__global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const std::int32_t* list, std::int64_t n,
+                                                             const unsigned long long* count_ptr, TierLists t,
+                                                             int scratch_per_group) {
+    extern __shared__ unsigned long long smem[];
+    __shared__ std::int32_t stage_buf[kTierThreads / kWave][2][kStage];
+    constexpr int W = 16, G = kWave / W;
+    const int lane = lane_id(), gw = lane / W, lg = lane % W;
+    unsigned long long* scratch = smem + (threadIdx.x / W) * scratch_per_group;
+    Stage defer_stage{stage_buf[threadIdx.x / kWave][0], t.medium, t.medium_count, 0};
+    Stage out_stage{stage_buf[threadIdx.x / kWave][1], t.out, t.out_count, 0};
+    const std::int64_t count = count_ptr ? (std::int64_t)*count_ptr : n;
+    const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
+    const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;
+    std::int64_t acc = 0;
+    for (std::int64_t base = wave * G; base < count; base += waves * G) {
+        const std::int64_t idx = base + gw;
+        std::int32_t v = 0;
+        bool defer = false;
+        RowOut r{false, 0};
+        if (idx < count) {
+            v = list ? list[idx] : (std::int32_t)idx;
+            const std::int64_t deg = body.degree(v);
+            if (deg > kTierLight)
+                defer = true;
+            else
+                r = body(GroupTeam<W>{lg}, v, scratch);
+        }
+        if (lg == 0) acc += r.acc;
+        defer_stage.push(lg == 0 && defer, v);
+        out_stage.push(lg == 0 && r.append, v);
+    }
+    defer_stage.flush();
+    out_stage.flush();
+    if (t.acc) {
+        acc = wave_sum(acc);
+        if (lane == 0 && acc) atomicAdd(t.acc, (unsigned long long)acc);
+    }
+}
+
+template <class Body>
+__global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLists t, int scratch_per_group) {
+    extern __shared__ unsigned long long smem[];
+    __shared__ std::int32_t stage_buf[kTierThreads / kWave][2][kStage];
+    const int lane = lane_id();
+    unsigned long long* scratch = smem + (threadIdx.x / kWave) * scratch_per_group;
+    Stage defer_stage{stage_buf[threadIdx.x / kWave][0], t.heavy, t.heavy_count, 0};
+    Stage out_stage{stage_buf[threadIdx.x / kWave][1], t.out, t.out_count, 0};
+    const std::int64_t count = (std::int64_t)*t.medium_count;
+    const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
+    const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;
+    std::int64_t acc = 0;
+    for (std::int64_t idx = wave; idx < count; idx += waves) {
+        const std::int32_t v = t.medium[idx];
+        bool defer = false;
+        RowOut r{false, 0};
+        if (body.degree(v) > kTierMedium)
+            defer = true;
+        else
+            r = body(GroupTeam<kWave>{lane}, v, scratch);
+        if (lane == 0) acc += r.acc;
+        defer_stage.push(lane == 0 && defer, v);
+        out_stage.push(lane == 0 && r.append, v);
+    }
+    defer_stage.flush();
+    out_stage.flush();
+    if (t.acc && lane == 0 && acc) atomicAdd(t.acc, (unsigned long long)acc);
+}
+
+template <class Body>
+__global__ void __launch_bounds__(kTierThreads) k_tier_heavy(Body body, TierLists t) {
+    extern __shared__ unsigned long long smem[];
+    __shared__ unsigned long long red[kBlockRed];
+    const std::int64_t count = (std::int64_t)*t.heavy_count;
+    std::int64_t acc = 0;
+    for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
+        const std::int32_t v = t.heavy[idx];
+        RowOut r = body(BlockTeam{red}, v, smem);
+        if (threadIdx.x == 0) {
+            acc += r.acc;
+            if (t.out && r.append) t.out[atomicAdd(t.out_count, 1ULL)] = v;
+        }
+        __syncthreads();
+    }
+    if (t.acc && threadIdx.x == 0 && acc) atomicAdd(t.acc, (unsigned long long)acc);
+}
+
+// Host launcher: three launches, counters in dev[0] (medium) and dev[1]
+// (heavy) are reset here.  scratch is per-team LDS in u64 units.
+template <class Body>
+void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int64_t n, const unsigned long long* count_ptr,
+               std::int32_t* medium, std::int32_t* heavy, unsigned long long* tier_counts, std::int32_t* out,
+               unsigned long long* out_count, unsigned long long* acc, int scratch, std::int64_t max_rows) {
+    const hipStream_t s = c.stream();
+    fill<unsigned long long>(c, tier_counts, 2, 0ULL);
+    TierLists t{medium, tier_counts, heavy, tier_counts + 1, out, out_count, acc};
+    const int light_blocks =
+        static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(max_rows, kTierThreads / 16), 4096)));
+    hipLaunchKernelGGL((k_tier_light<Body>), dim3(light_blocks), dim3(kTierThreads),
+                       (kTierThreads / 16) * scratch * sizeof(unsigned long long), s, body, list, n, count_ptr, t, scratch);
+    GP_LAUNCH_CHECK();
+    const int medium_blocks =
+        static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(max_rows, kTierThreads / kWave), 2048)));
+    hipLaunchKernelGGL((k_tier_medium<Body>), dim3(medium_blocks), dim3(kTierThreads),
+                       (kTierThreads / kWave) * scratch * sizeof(unsigned long long), s, body, t, scratch);
+    GP_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_tier_heavy<Body>), dim3(256), dim3(kTierThreads), scratch * sizeof(unsigned long long), s, body,
+                       t);
+    GP_LAUNCH_CHECK();
+}
+
+}  // namespace gp

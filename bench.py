@@ -1,0 +1,140 @@
+"""Benchmark: edges partitioned per second on one MI355X.
+
+Workload: R-MAT scale 22 (4,194,304 vertices, edge factor 16,
+a=0.57 b=0.19 c=0.19 d=0.05, symmetrised, deduplicated, no self loops, unit
+weights), k=64, imbalance 1.03, seed 0.  One step = one full partition
+(coarsen -> host initial partition -> uncoarsen) with the input resident in
+HBM, i.e. the reference CLI's algorithm_total_seconds scope.  Prints one JSON
+line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "edges partitioned/s at k=64 ε=3%, batched vs plain; cut ratio; p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--parts", type=int, default=64)
+    ap.add_argument("--cpu-scale", type=int, default=19, help="R-MAT scale of the bounded CPU-oracle sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only; multi-GPU sharding is a design note, not a run.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from tests.conftest import load_package
+    gp = load_package()
+
+    t = time.time()
+    off, nbr = gp.generate_rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=1)
+    n, m = len(off) - 1, len(nbr)
+    edges = m // 2
+    print("bench: rmat scale %d n=%d undirected_edges=%d generated in %.1fs" % (args.scale, n, edges, time.time() - t),
+          file=sys.stderr, flush=True)
+
+    ctx = gp.Context(0)
+    g = gp.Graph(ctx, off, nbr)
+    for i in range(args.warmup):
+        _, st = g.partition(args.parts, 1.03, 0, download=False)
+        print("bench: warmup %d %.3fs cut=%d" % (i, st["algorithm_total_seconds"], st["final_cut"]),
+              file=sys.stderr, flush=True)
+
+    ctx.profile(True)
+    ctx.profile_reset()
+    per_step = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        s0 = time.perf_counter()
+        _, st = g.partition(args.parts, 1.03, 0, download=False)
+        per_step.append(time.perf_counter() - s0)
+        print("bench: step %d %.3fs (algorithm %.3fs) cut=%d" % (i, per_step[-1], st["algorithm_total_seconds"],
+                                                                   st["final_cut"]), file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    ms, byts, launches = ctx.profile_query("contract_scatter")
+    ctx.profile(False)
+
+    # Verification of one result: exact recount and strict capacity.
+    labels, st = g.partition(args.parts, 1.03, 0, download=True)
+    from oracle import oracle as og
+    recount = og.cut(off, nbr, None, labels)
+    loads = np.bincount(labels, minlength=args.parts)
+    cap = gp.capacity(n, args.parts, 1.03)
+    if recount != st["final_cut"] or loads.max() > cap:
+        raise RuntimeError("bench: result failed verification")
+
+    value = edges * args.steps / elapsed
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": {
+            "workload": "rmat-scale%d-ef16-plain-k%d" % (args.scale, args.parts),
+            "vertices": n,
+            "undirected_edges": edges,
+            "parts": args.parts,
+            "imbalance": 1.03,
+            "seed": 0,
+            "cut": int(st["final_cut"]),
+            "cut_ratio": st["final_cut"] / edges,
+            "max_part_weight": int(st["max_part_weight"]),
+            "capacity": int(cap),
+            "levels": st["levels"],
+            "p50_step_ms": 1000.0 * float(np.percentile(per_step, 50)),
+            "p99_step_ms": 1000.0 * float(np.percentile(per_step, 99)),
+        },
+    }
+    if launches:
+        avg_s = ms / 1000.0 / launches
+        avg_bytes = byts / launches
+        achieved = avg_bytes / avg_s / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": "k_contract_scatter", "achieved": achieved,
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                           "traffic": None, "avg_launch_us": avg_s * 1e6, "launches": launches}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(gp, og, args.cpu_scale, args.parts)
+    g.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(gp, og, scale, parts):
+    """Single-threaded CPU oracle on a bounded R-MAT sample (same schedule)."""
+    off, nbr = gp.generate_rmat(scale, 16, 0.57, 0.19, 0.19, seed=1)
+    t = time.perf_counter()
+    oh = og.Hierarchy(off, nbr, None, None, og.coarsen_opts(parts))
+    coff, cnbr, cew, cvw = oh.graph(oh.levels - 1)
+    init = gp.initial_partition_host(coff, cnbr, cew, cvw, parts)
+    oh.uncoarsen(init, og.refine_opts(parts))
+    secs = time.perf_counter() - t
+    edges = len(nbr) // 2
+    return {"value": edges / secs, "unit": "edges/s", "cores": 1, "kind": "port",
+            "sample": "R-MAT scale %d ef16 (%d undirected edges) k=%d, oracle coarsen + host initial partition + "
+                      "oracle uncoarsen, %.1fs" % (scale, edges, parts, secs)}
+
+
+if __name__ == "__main__":
+    main()

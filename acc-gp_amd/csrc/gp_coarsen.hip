@@ -470,26 +470,130 @@ __global__ void k_hub_records(AffinityArgs a, const std::int32_t* list, std::int
     }
 }
 
+// Record pre-reduction for long rows: a block sort-reduces one chunk of
+// up to kBlockSortMax entries of one listed row in LDS and appends its
+// unique (list index, key) records with their sums.  Chunk order in the
+// output is arbitrary; the global sort that follows makes it canonical.
+// chunk_off[i] is the first chunk of list row i (count + 1 entries).
+// Partial sums above 2^32 - 1 raise *overflow (the caller then reduces the
+// raw records instead).
+template <class Src>
+__global__ void __launch_bounds__(kBlockSortThreads) k_chunk_records(Src src, const std::int64_t* chunk_off,
+                                                                     std::int64_t count, int key_bits,
+                                                                     std::uint64_t* keys, std::uint32_t* vals,
+                                                                     unsigned long long* out_count,
+                                                                     unsigned long long* overflow) {
+    __shared__ BlockSortSmem s;
+    __shared__ unsigned long long obase;
+    __shared__ std::int64_t row_of;
+    const std::int64_t chunks = chunk_off[count];
+    for (std::int64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+        if (threadIdx.x == 0) {
+            std::int64_t lo = 0, hi = count;  // last i with chunk_off[i] <= ch
+            while (hi - lo > 1) {
+                const std::int64_t mid = lo + (hi - lo) / 2;
+                if (chunk_off[mid] <= ch)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            row_of = lo;
+        }
+        __syncthreads();
+        const std::int64_t li = row_of;
+        const std::int64_t b = src.begin(li), e = src.end(li);
+        const std::int64_t x0 = b + (ch - chunk_off[li]) * kBlockSortMax;
+        const int d = (int)((e - x0) < kBlockSortMax ? (e - x0) : kBlockSortMax);
+        for (int x = threadIdx.x; x < d; x += kBlockSortThreads) {
+            s.key[x] = src.key(li, x0 + x);
+            s.val[x] = src.val(li, x0 + x);
+        }
+        __syncthreads();
+        block_sort_reduce(
+            s, d,
+            [&](std::int32_t key, std::uint64_t sum, int i) {
+                if (sum > 0xffffffffULL) atomicOr(overflow, 1ULL);
+                keys[obase + i] = ((std::uint64_t)li << key_bits) | (std::uint64_t)key;
+                vals[obase + i] = (std::uint32_t)sum;
+            },
+            [&](int total) {
+                if (threadIdx.x == 0) obase = total ? atomicAdd(out_count, (unsigned long long)total) : 0ULL;
+                __syncthreads();
+            });
+        __syncthreads();
+    }
+}
+
+// Chunks per listed row (zero for rows the source skips).
+template <class Src>
+__global__ void k_chunk_counts(Src src, std::int64_t count, std::int64_t* out) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x)
+        out[i] = src.active(i) ? (src.end(i) - src.begin(i) + kBlockSortMax - 1) / kBlockSortMax : 0;
+}
+
+// Hub affinity records: neighbour clusters of mover hubs.
+struct HubSrc {
+    AffinityArgs a;
+    const std::int32_t* list;
+    __device__ bool active(std::int64_t i) const { return a.roles[a.clusters[list[i]]] != 0; }
+    __device__ std::int64_t begin(std::int64_t i) const { return a.off[list[i]]; }
+    __device__ std::int64_t end(std::int64_t i) const { return a.off[list[i] + 1]; }
+    __device__ std::int32_t key(std::int64_t i, std::int64_t x) const {
+        const std::int32_t u = a.nbr[x];
+        return u == list[i] ? kSentinel : a.clusters[u];
+    }
+    __device__ std::uint32_t val(std::int64_t, std::int64_t x) const { return (std::uint32_t)ew_at(a.ew, x); }
+};
+
 // Unique (row, key) runs of sorted records: heads, scan, atomic sums.
 __global__ void k_run_heads(const std::uint64_t* keys, std::int64_t n, std::uint8_t* heads) {
     for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
          i += (std::int64_t)gridDim.x * blockDim.x)
         heads[i] = keys[i] != ~0ULL && (i == 0 || keys[i] != keys[i - 1]);
 }
-__global__ void k_run_items(const std::uint64_t* keys, const std::uint32_t* vals, std::int64_t n, SegU64* items) {
-    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (std::int64_t)gridDim.x * blockDim.x)
-        items[i] = SegU64{vals[i], (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u};
-}
-// The last record of a run carries the run's sum after the segmented scan.
-__global__ void k_run_tails(const std::uint64_t* keys, std::int64_t n, const std::int64_t* pos, const SegU64* sums,
-                            std::uint64_t* ukeys, unsigned long long* usums) {
-    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (std::int64_t)gridDim.x * blockDim.x) {
-        if (keys[i] == ~0ULL || (i + 1 < n && keys[i + 1] == keys[i])) continue;
-        const std::int64_t u = pos[i] - 1;  // inclusive count of heads -> run id
-        ukeys[u] = keys[i];
-        usums[u] = sums[i].v;
+// Sums of runs of equal keys in sorted records.  A block reduces one tile;
+// runs inside the tile are stored, runs crossing a tile edge add their
+// partial sums atomically into the zero-filled usums.
+__global__ void __launch_bounds__(kScanThreads) k_run_reduce(const std::uint64_t* keys, const std::uint32_t* vals,
+                                                            std::int64_t n, const std::int64_t* pos,
+                                                            std::uint64_t* ukeys, unsigned long long* usums) {
+    __shared__ SegU64 wsum[kScanThreads / kWave];
+    const int lane = lane_id(), w = threadIdx.x / kWave;
+    const std::int64_t tile = blockIdx.x * (std::int64_t)kScanTile;
+    const std::int64_t end = tile + kScanTile < n ? tile + kScanTile : n;
+    const std::int64_t base = tile + w * (kScanItems * kWave);
+    SegU64 inc[kScanItems];
+    SegU64 acc{0, 0};
+#pragma unroll
+    for (int r = 0; r < kScanItems; ++r) {
+        const std::int64_t idx = base + r * kWave + lane;
+        SegU64 x{0, 0};
+        if (idx < end) x = SegU64{vals[idx], (idx == tile || keys[idx] != keys[idx - 1]) ? 1u : 0u};
+        inc[r] = wave_inclusive_scan<SegPlusOp>(x);
+        acc = SegPlusOp::op<SegU64>(acc, shfl_val(inc[r], kWave - 1));
+    }
+    if (lane == 0) wsum[w] = acc;
+    __syncthreads();
+    SegU64 carry{0, 0};
+    for (int i = 0; i < w; ++i) carry = SegPlusOp::op<SegU64>(carry, wsum[i]);
+    const std::uint64_t left = tile > 0 ? keys[tile - 1] : ~0ULL;
+#pragma unroll
+    for (int r = 0; r < kScanItems; ++r) {
+        const std::int64_t idx = base + r * kWave + lane;
+        const SegU64 full = SegPlusOp::op<SegU64>(carry, inc[r]);
+        carry = SegPlusOp::op<SegU64>(carry, shfl_val(inc[r], kWave - 1));
+        if (idx >= end) continue;
+        const std::uint64_t key = keys[idx];
+        if (key == ~0ULL) continue;
+        const bool tail = idx + 1 == n || keys[idx + 1] != key;
+        if (!tail && idx + 1 != end) continue;
+        const std::int64_t u = pos[idx] - 1;  // inclusive count of heads -> run id
+        if (tail) ukeys[u] = key;
+        if (tail && key != left)
+            usums[u] = full.v;
+        else
+            atomicAdd(&usums[u], (unsigned long long)full.v);
     }
 }
 
@@ -834,11 +938,12 @@ struct ScatterBody {
         unsigned long long o = 0;
         if (t.leader()) o = atomicAdd(&cursor[s], (unsigned long long)(e - b));
         o = t.bcast(o);
-        for (std::int64_t x = b + t.rank(); x < e; x += Team::size) {
-            const std::int32_t tc = map[nbr[x]];
-            rkey[o + (x - b)] = tc == s ? kSentinel : tc;
-            rval[o + (x - b)] = (std::uint32_t)ew_at(ew, x);
-        }
+        row_batches(
+            t, b, e, [&](std::int64_t x) { return map[nbr[x]]; },
+            [&](std::int64_t x, std::int32_t tc) {
+                rkey[o + (x - b)] = tc == s ? kSentinel : tc;
+                rval[o + (x - b)] = (std::uint32_t)ew_at(ew, x);
+            });
         return RowOut{false, 0};
     }
 };
@@ -849,6 +954,17 @@ struct RowArgs {
     std::uint32_t* rval;
     unsigned long long* ucount;
     unsigned long long* overflow;
+};
+
+// Raw long coarse rows as record sources.
+struct LongSrc {
+    RowArgs r;
+    const std::int32_t* list;
+    __device__ bool active(std::int64_t) const { return true; }
+    __device__ std::int64_t begin(std::int64_t i) const { return r.raw_off[list[i]]; }
+    __device__ std::int64_t end(std::int64_t i) const { return r.raw_off[list[i] + 1]; }
+    __device__ std::int32_t key(std::int64_t, std::int64_t x) const { return r.rkey[x]; }
+    __device__ std::uint32_t val(std::int64_t, std::int64_t x) const { return r.rval[x]; }
 };
 
 __device__ inline void emit_row_entry(const RowArgs& r, std::int64_t start, std::int32_t key, std::uint64_t sum,
@@ -987,7 +1103,7 @@ void launch_classes(Context& c, const std::int32_t* lists, std::int64_t stride, 
 
 // ====================================================================== host
 
-// Global reduce of (row,key) records: sort, unique, atomic sums.
+// Global reduce of (row,key) records: sort, unique, per-run sums.
 static std::int64_t reduce_records(Context& c, DBuf<std::uint64_t>& keys, DBuf<std::uint32_t>& vals, std::int64_t count,
                                    int total_bits, DBuf<std::uint64_t>& ukeys, DBuf<unsigned long long>& usums) {
     radix_sort_pairs(c, keys.get(), vals.get(), count, 0, total_bits);
@@ -1000,15 +1116,47 @@ static std::int64_t reduce_records(Context& c, DBuf<std::uint64_t>& keys, DBuf<s
     const std::int64_t nu = c.read_scalar(total.get());
     ukeys = c.buf<std::uint64_t>(static_cast<std::size_t>(std::max<std::int64_t>(nu, 1)));
     usums = c.buf<unsigned long long>(static_cast<std::size_t>(std::max<std::int64_t>(nu, 1)));
-    DBuf<SegU64> items = c.buf<SegU64>(static_cast<std::size_t>(count));
-    hipLaunchKernelGGL(k_run_items, dim3(grid_for(count)), dim3(256), 0, c.stream(), keys.get(), vals.get(), count,
-                       items.get());
-    GP_LAUNCH_CHECK();
-    scan<SegPlusOp, SegU64>(c, items.get(), items.get(), count, false);
-    hipLaunchKernelGGL(k_run_tails, dim3(grid_for(count)), dim3(256), 0, c.stream(), keys.get(), count, pos.get(),
-                       items.get(), ukeys.get(), usums.get());
+    fill<unsigned long long>(c, usums.get(), nu, 0ULL);
+    if (count > 0 && nu > 0)
+        hipLaunchKernelGGL(k_run_reduce, dim3(static_cast<unsigned>(ceil_div(count, kScanTile))), dim3(kScanThreads), 0,
+                       c.stream(), keys.get(), vals.get(), count, pos.get(), ukeys.get(), usums.get());
     GP_LAUNCH_CHECK();
     return nu;
+}
+
+// Pre-reduces the records of the listed long rows chunk by chunk, then
+// reduces them globally.  Returns false (nothing reduced) when a partial
+// sum does not fit the 32-bit record weight; the caller then reduces the
+// raw records.
+template <class Src>
+static bool reduce_long_rows(Context& c, const Src& src, std::int64_t count, std::int64_t raw_records, int key_bits,
+                             int total_bits, DBuf<std::uint64_t>& ukeys, DBuf<unsigned long long>& usums,
+                             std::int64_t& nu) {
+    DBuf<std::int64_t> chunk_off = c.buf<std::int64_t>(static_cast<std::size_t>(count + 1));
+    DBuf<std::int64_t> ctot = c.buf<std::int64_t>(1);
+    hipLaunchKernelGGL((k_chunk_counts<Src>), dim3(grid_for(count)), dim3(256), 0, c.stream(), src, count,
+                       chunk_off.get());
+    GP_LAUNCH_CHECK();
+    exclusive_sum<std::int64_t>(c, chunk_off.get(), chunk_off.get(), count, ctot.get());
+    c.copy_d2d(chunk_off.get() + count, ctot.get(), sizeof(std::int64_t));
+    DBuf<unsigned long long> ctr = c.buf<unsigned long long>(2);
+    fill<unsigned long long>(c, ctr.get(), 2, 0ULL);
+    DBuf<std::uint64_t> keys = c.buf<std::uint64_t>(static_cast<std::size_t>(std::max<std::int64_t>(raw_records, 1)));
+    DBuf<std::uint32_t> vals = c.buf<std::uint32_t>(static_cast<std::size_t>(std::max<std::int64_t>(raw_records, 1)));
+    const std::int64_t max_chunks = raw_records / kBlockSortMax + count;
+    hipLaunchKernelGGL((k_chunk_records<Src>), dim3(grid_for(max_chunks, 1, 4096)), dim3(kBlockSortThreads), 0,
+                       c.stream(), src, chunk_off.get(), count, key_bits, keys.get(), vals.get(), ctr.get(),
+                       ctr.get() + 1);
+    GP_LAUNCH_CHECK();
+    unsigned long long h[2];
+    c.read(h, ctr.get(), sizeof h);
+    if (h[1]) return false;
+    nu = h[0] ? reduce_records(c, keys, vals, (std::int64_t)h[0], total_bits, ukeys, usums) : 0;
+    if (!h[0]) {
+        ukeys = c.buf<std::uint64_t>(1);
+        usums = c.buf<unsigned long long>(1);
+    }
+    return true;
 }
 
 void compute_level0_metadata(Context& c, DGraph& g, CoarsenState& st) {
@@ -1160,15 +1308,18 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
         };
         launch_classes(c, lists.get(), n, cc.c, group, block);
         if (hub_count) {
-            DBuf<std::uint64_t> keys = c.buf<std::uint64_t>(static_cast<std::size_t>(hub_records));
-            DBuf<std::uint32_t> vals = c.buf<std::uint32_t>(static_cast<std::size_t>(hub_records));
-            hipLaunchKernelGGL(k_hub_records, dim3(grid_for(hub_count, 1, 4096)), dim3(256), 0, s, a, hub_list,
-                               hub_count, hub_off.get(), cluster_bits, keys.get(), vals.get());
-            GP_LAUNCH_CHECK();
             DBuf<std::uint64_t> ukeys;
             DBuf<unsigned long long> usums;
-            const std::int64_t nu =
-                reduce_records(c, keys, vals, hub_records, cluster_bits + hub_bits, ukeys, usums);
+            std::int64_t nu = 0;
+            if (!reduce_long_rows(c, HubSrc{a, hub_list}, hub_count, hub_records, cluster_bits, cluster_bits + hub_bits,
+                                  ukeys, usums, nu)) {
+                DBuf<std::uint64_t> keys = c.buf<std::uint64_t>(static_cast<std::size_t>(hub_records));
+                DBuf<std::uint32_t> vals = c.buf<std::uint32_t>(static_cast<std::size_t>(hub_records));
+                hipLaunchKernelGGL(k_hub_records, dim3(grid_for(hub_count, 1, 4096)), dim3(256), 0, s, a, hub_list,
+                                   hub_count, hub_off.get(), cluster_bits, keys.get(), vals.get());
+                GP_LAUNCH_CHECK();
+                nu = reduce_records(c, keys, vals, hub_records, cluster_bits + hub_bits, ukeys, usums);
+            }
             DBuf<HubTop2> acc = c.buf<HubTop2>(static_cast<std::size_t>(hub_count));
             hipLaunchKernelGGL(k_hub_reduce, dim3(grid_for(hub_count, 1, 4096)), dim3(256), 0, s, a, hub_list,
                                hub_count, ukeys.get(), usums.get(), nu, cluster_bits, acc.get());
@@ -1456,14 +1607,18 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
         const std::int64_t records = c.read_scalar(rtot.get());
         const int key_bits = std::max(1, bit_width_u64(static_cast<std::uint64_t>(nc)));
         const int row_bits = std::max(1, bit_width_u64(static_cast<std::uint64_t>(long_count)));
-        DBuf<std::uint64_t> keys = c.buf<std::uint64_t>(static_cast<std::size_t>(records));
-        DBuf<std::uint32_t> vals = c.buf<std::uint32_t>(static_cast<std::size_t>(records));
-        hipLaunchKernelGGL(k_long_records, dim3(grid_for(long_count, 1, 4096)), dim3(256), 0, s, r, long_list,
-                           long_count, rec_off.get(), key_bits, keys.get(), vals.get());
-        GP_LAUNCH_CHECK();
         DBuf<std::uint64_t> ukeys;
         DBuf<unsigned long long> usums;
-        const std::int64_t nu = reduce_records(c, keys, vals, records, key_bits + row_bits, ukeys, usums);
+        std::int64_t nu = 0;
+        if (!reduce_long_rows(c, LongSrc{r, long_list}, long_count, records, key_bits, key_bits + row_bits, ukeys,
+                              usums, nu)) {
+            DBuf<std::uint64_t> keys = c.buf<std::uint64_t>(static_cast<std::size_t>(records));
+            DBuf<std::uint32_t> vals = c.buf<std::uint32_t>(static_cast<std::size_t>(records));
+            hipLaunchKernelGGL(k_long_records, dim3(grid_for(long_count, 1, 4096)), dim3(256), 0, s, r, long_list,
+                               long_count, rec_off.get(), key_bits, keys.get(), vals.get());
+            GP_LAUNCH_CHECK();
+            nu = reduce_records(c, keys, vals, records, key_bits + row_bits, ukeys, usums);
+        }
         if (nu) {
             hipLaunchKernelGGL(k_long_writeback, dim3(grid_for(long_count, 1, 4096)), dim3(256), 0, s, r, long_list,
                                long_count, ukeys.get(), usums.get(), nu, key_bits);

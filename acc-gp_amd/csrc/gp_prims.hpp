@@ -66,78 +66,124 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-// Block-wide inclusive scan of one value per thread (Hillis-Steele in LDS).
-template <typename Op, typename T>
-__device__ inline T block_inclusive_scan(T x, T* lds) {
-    const int t = threadIdx.x;
-    lds[t] = x;
-    __syncthreads();
-    for (int d = 1; d < kScanThreads; d <<= 1) {
-        T y = t >= d ? Op::template op<T>(lds[t - d], lds[t]) : lds[t];
-        __syncthreads();
-        lds[t] = y;
-        __syncthreads();
-    }
-    T r = lds[t];
-    __syncthreads();
-    return r;
+// Lane shuffles of scan values.
+template <typename T>
+__device__ inline T shfl_up_val(T x, int d) {
+    return __shfl_up(x, d);
+}
+__device__ inline SegU64 shfl_up_val(SegU64 x, int d) {
+    return SegU64{__shfl_up(x.v, d), __shfl_up(x.head, d)};
 }
 
-template <typename Op, typename T, typename In>
-__global__ void k_scan_reduce(const In* in, std::int64_t n, T* partial) {
-    __shared__ T lds[kScanThreads];
-    const std::int64_t base = blockIdx.x * (std::int64_t)kScanTile;
-    T acc = Op::template id<T>();
-    for (int i = 0; i < kScanItems; ++i) {
-        const std::int64_t idx = base + threadIdx.x * kScanItems + i;
-        if (idx < n) acc = Op::template op<T>(acc, static_cast<T>(in[idx]));
+// Block-wide scan of one value per thread: wave scans with shuffles, then
+// the wave totals through LDS (two barriers).  wsum holds one T per wave.
+template <typename Op, typename T>
+__device__ inline void block_scan(T x, T* wsum, T& inclusive, T& exclusive, T& total) {
+    const int lane = lane_id(), w = threadIdx.x / kWave;
+    for (int d = 1; d < kWave; d <<= 1) {
+        const T y = shfl_up_val(x, d);
+        if (lane >= d) x = Op::template op<T>(y, x);
     }
-    T r = block_inclusive_scan<Op>(acc, lds);
-    if (threadIdx.x == kScanThreads - 1) partial[blockIdx.x] = r;
+    T wex = shfl_up_val(x, 1);
+    if (lane == kWave - 1) wsum[w] = x;
+    __syncthreads();
+    T pre = Op::template id<T>(), tot = Op::template id<T>();
+    for (int i = 0; i < kScanThreads / kWave; ++i) {
+        if (i < w) pre = Op::template op<T>(pre, wsum[i]);
+        tot = Op::template op<T>(tot, wsum[i]);
+    }
+    __syncthreads();
+    inclusive = Op::template op<T>(pre, x);
+    exclusive = lane ? Op::template op<T>(pre, wex) : pre;
+    total = tot;
+}
+
+// Inclusive scan across the 64 lanes of a wave.
+template <typename Op, typename T>
+__device__ inline T wave_inclusive_scan(T x) {
+    const int lane = lane_id();
+    for (int d = 1; d < kWave; d <<= 1) {
+        const T y = shfl_up_val(x, d);
+        if (lane >= d) x = Op::template op<T>(y, x);
+    }
+    return x;
+}
+template <typename T>
+__device__ inline T shfl_val(T x, int src) {
+    return __shfl(x, src);
+}
+__device__ inline SegU64 shfl_val(SegU64 x, int src) {
+    return SegU64{__shfl(x.v, src), __shfl(x.head, src)};
+}
+
+// Tile layout: wave w owns the contiguous kScanItems * 64 elements at
+// base + w * kScanItems * 64, read in rounds of 64 consecutive elements so
+// that every load and store instruction covers contiguous memory.
+template <typename Op, typename T, typename In>
+__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const In* in, std::int64_t n, T* partial) {
+    __shared__ T wsum[kScanThreads / kWave];
+    const int lane = lane_id(), w = threadIdx.x / kWave;
+    const std::int64_t base = blockIdx.x * (std::int64_t)kScanTile + w * (kScanItems * kWave);
+    T acc = Op::template id<T>();
+    for (int r = 0; r < kScanItems; ++r) {
+        const std::int64_t idx = base + r * kWave + lane;
+        const T x = idx < n ? static_cast<T>(in[idx]) : Op::template id<T>();
+        acc = Op::template op<T>(acc, shfl_val(wave_inclusive_scan<Op>(x), kWave - 1));
+    }
+    if (lane == 0) wsum[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T tot = wsum[0];
+        for (int i = 1; i < kScanThreads / kWave; ++i) tot = Op::template op<T>(tot, wsum[i]);
+        partial[blockIdx.x] = tot;
+    }
 }
 
 // Exclusive scan of the block partials by one block (loops over chunks).
 template <typename Op, typename T>
-__global__ void k_scan_partials(T* partial, std::int64_t nb, T* total) {
-    __shared__ T lds[kScanThreads];
+__global__ void __launch_bounds__(kScanThreads) k_scan_partials(T* partial, std::int64_t nb, T* total) {
+    __shared__ T wsum[kScanThreads / kWave];
     T carry = Op::template id<T>();
     for (std::int64_t base = 0; base < nb; base += kScanThreads) {
         const std::int64_t idx = base + threadIdx.x;
-        T x = idx < nb ? partial[idx] : Op::template id<T>();
-        T inc = block_inclusive_scan<Op>(x, lds);
-        lds[threadIdx.x] = inc;
-        __syncthreads();
-        // exclusive = carry (+) inclusive of the previous thread
-        T ex = threadIdx.x ? Op::template op<T>(carry, lds[threadIdx.x - 1]) : carry;
-        T block_total = lds[kScanThreads - 1];
-        __syncthreads();
-        if (idx < nb) partial[idx] = ex;
-        carry = Op::template op<T>(carry, block_total);
+        const T x = idx < nb ? partial[idx] : Op::template id<T>();
+        T inc, exc, tot;
+        block_scan<Op>(x, wsum, inc, exc, tot);
+        if (idx < nb) partial[idx] = Op::template op<T>(carry, exc);
+        carry = Op::template op<T>(carry, tot);
     }
     if (threadIdx.x == 0 && total) *total = carry;
 }
 
 // Final pass; exclusive=true writes out[i] = prefix before i.
 template <typename Op, typename T, typename In, typename Out>
-__global__ void k_scan_apply(const In* in, Out* out, std::int64_t n, const T* partial, bool exclusive) {
-    __shared__ T lds[kScanThreads];
-    const std::int64_t base = blockIdx.x * (std::int64_t)kScanTile;
-    T vals[kScanItems];
+__global__ void __launch_bounds__(kScanThreads) k_scan_apply(const In* in, Out* out, std::int64_t n, const T* partial,
+                                                             bool exclusive) {
+    __shared__ T wsum[kScanThreads / kWave];
+    const int lane = lane_id(), w = threadIdx.x / kWave;
+    const std::int64_t base = blockIdx.x * (std::int64_t)kScanTile + w * (kScanItems * kWave);
+    T inc[kScanItems], own[kScanItems];
     T acc = Op::template id<T>();
-    for (int i = 0; i < kScanItems; ++i) {
-        const std::int64_t idx = base + threadIdx.x * kScanItems + i;
-        vals[i] = idx < n ? static_cast<T>(in[idx]) : Op::template id<T>();
-        acc = Op::template op<T>(acc, vals[i]);
+#pragma unroll
+    for (int r = 0; r < kScanItems; ++r) {
+        const std::int64_t idx = base + r * kWave + lane;
+        own[r] = idx < n ? static_cast<T>(in[idx]) : Op::template id<T>();
+        inc[r] = wave_inclusive_scan<Op>(own[r]);
+        acc = Op::template op<T>(acc, shfl_val(inc[r], kWave - 1));
     }
-    T inc = block_inclusive_scan<Op>(acc, lds);
-    lds[threadIdx.x] = inc;
+    if (lane == 0) wsum[w] = acc;
     __syncthreads();
-    T run = Op::template op<T>(partial[blockIdx.x], threadIdx.x ? lds[threadIdx.x - 1] : Op::template id<T>());
-    for (int i = 0; i < kScanItems; ++i) {
-        const std::int64_t idx = base + threadIdx.x * kScanItems + i;
-        T next = Op::template op<T>(run, vals[i]);
-        if (idx < n) out[idx] = static_cast<Out>(exclusive ? run : next);
-        run = next;
+    T carry = partial[blockIdx.x];
+    for (int i = 0; i < w; ++i) carry = Op::template op<T>(carry, wsum[i]);
+#pragma unroll
+    for (int r = 0; r < kScanItems; ++r) {
+        const std::int64_t idx = base + r * kWave + lane;
+        const T full = Op::template op<T>(carry, inc[r]);
+        // exclusive: the inclusive value of the previous lane (or carry)
+        T prev = shfl_up_val(full, 1);
+        if (lane == 0) prev = carry;
+        if (idx < n) out[idx] = static_cast<Out>(exclusive ? prev : full);
+        carry = Op::template op<T>(carry, shfl_val(inc[r], kWave - 1));
     }
 }
 

@@ -79,14 +79,21 @@ __device__ inline std::uint64_t team_histogram(const Team& t, const RefArgs& a, 
     for (int p = t.rank(); p < a.k; p += Team::size) hist[p] = 0;
     t.sync();
     std::uint64_t total = 0;
-    const std::int64_t b = a.off[v], e = a.off[v + 1];
-    for (std::int64_t x = b + t.rank(); x < e; x += Team::size) {
-        const std::int32_t u = a.nbr[x];
-        if (u == v) continue;
-        const std::uint64_t w = ew_at(a.ew, x);
-        atomicAdd(&hist[a.labels[u]], (unsigned long long)w);
-        total += w;
-    }
+    struct Rec {
+        int label;  // -1 for a self loop
+        std::uint32_t w;
+    };
+    row_batches(
+        t, a.off[v], a.off[v + 1],
+        [&](std::int64_t x) {
+            const std::int32_t u = a.nbr[x];
+            return Rec{u == v ? -1 : a.labels[u], (std::uint32_t)ew_at(a.ew, x)};
+        },
+        [&](std::int64_t, const Rec& r) {
+            if (r.label < 0) return;
+            atomicAdd(&hist[r.label], (unsigned long long)r.w);
+            total += r.w;
+        });
     t.sync();
     return t.sum(total);
 }
@@ -193,16 +200,22 @@ struct AfterBody : RowBody {
         const int s = a.labels[v], tv = a.targets[v];
         const std::int64_t gv = a.gains[v];
         std::int64_t corr = 0;
-        for (std::int64_t x = a.off[v] + t.rank(); x < a.off[v + 1]; x += Team::size) {
-            const std::int32_t u = a.nbr[x];
-            if (u == v) continue;
-            const int tu = a.targets[u];
-            if (tu < 0) continue;
-            const std::int64_t gu = a.gains[u];
-            if (!(gu > gv || (gu == gv && u < v))) continue;
-            const int lu = a.labels[u];
-            corr += (std::int64_t)ew_at(a.ew, x) * ((tu == tv) - (tu == s) - (lu == tv) + (lu == s));
-        }
+        struct Rec {
+            std::int32_t u;
+            int tu, lu;
+            std::int64_t gu;
+        };
+        row_batches(
+            t, a.off[v], a.off[v + 1],
+            [&](std::int64_t x) {
+                const std::int32_t u = a.nbr[x];
+                return Rec{u, a.targets[u], a.labels[u], a.gains[u]};
+            },
+            [&](std::int64_t x, const Rec& r) {
+                if (r.u == v || r.tu < 0) return;
+                if (!(r.gu > gv || (r.gu == gv && r.u < v))) return;
+                corr += (std::int64_t)ew_at(a.ew, x) * ((r.tu == tv) - (r.tu == s) - (r.lu == tv) + (r.lu == s));
+            });
         corr = t.sum(corr);
         bool keep;
         if (a.prices) {
@@ -224,15 +237,22 @@ struct UpdateBody : RowBody {
     __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long*) const {
         const int s = a.labels[v], tv = a.targets[v];
         std::int64_t gain = 0;
-        for (std::int64_t x = a.off[v] + t.rank(); x < a.off[v + 1]; x += Team::size) {
-            const std::int32_t u = a.nbr[x];
-            if (u == v) continue;
-            const int old_u = a.labels[u];
-            const bool moves = a.flags[u] != 0;
-            const int new_u = moves ? a.targets[u] : old_u;
-            a.member[u] = 1;
-            if (!moves || u > v) gain += (std::int64_t)ew_at(a.ew, x) * ((s != old_u) - (tv != new_u));
-        }
+        struct Rec {
+            std::int32_t u;
+            int old_u, moves, tu;
+        };
+        row_batches(
+            t, a.off[v], a.off[v + 1],
+            [&](std::int64_t x) {
+                const std::int32_t u = a.nbr[x];
+                return Rec{u, a.labels[u], a.flags[u], a.targets[u]};
+            },
+            [&](std::int64_t x, const Rec& r) {
+                if (r.u == v) return;
+                a.member[r.u] = 1;
+                const int new_u = r.moves ? r.tu : r.old_u;
+                if (!r.moves || r.u > v) gain += (std::int64_t)ew_at(a.ew, x) * ((s != r.old_u) - (tv != new_u));
+            });
         gain = t.sum(gain);
         if (samples && a.samples && t.leader()) {
             atomicAdd(&a.samples[s].loss, (unsigned long long)(-gain));
@@ -248,10 +268,13 @@ struct BoundaryBody : RowBody {
     __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long*) const {
         const int s = a.labels[v];
         int any = 0;
-        for (std::int64_t x = a.off[v] + t.rank(); x < a.off[v + 1]; x += Team::size) {
-            const std::int32_t u = a.nbr[x];
-            if (u != v && a.labels[u] != s) any = 1;
-        }
+        row_batches(
+            t, a.off[v], a.off[v + 1],
+            [&](std::int64_t x) {
+                const std::int32_t u = a.nbr[x];
+                return u != v && a.labels[u] != s;
+            },
+            [&](std::int64_t, bool other) { any |= other; });
         any = t.sum(any);
         if (t.leader()) {
             a.member[v] = any != 0;
@@ -303,10 +326,18 @@ __global__ void k_compact_flags(std::int64_t n, const std::int32_t* flags, std::
 __device__ inline int gain_bucket(const RefArgs& a, double gain) {
     if (gain > 0) return 0;
     if (gain == 0) return 1;
+    // First j in -23..23 with x <= 1.5^j gives bucket 25 + j (at least 2);
+    // beyond the table, 49.  Equivalently 2 + #{table entries below x}.
     const double x = -gain;
-    for (int j = -23; j <= 23; ++j)
-        if (x <= a.pow15.p[j + 23]) return 25 + j < 2 ? 2 : 25 + j;
-    return 49;
+    int lo = 0, hi = 47;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a.pow15.p[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return 2 + lo;
 }
 
 // Donor flags, resets (flags, bucket) on frozen loads.
@@ -477,135 +508,255 @@ __global__ void k_conn_rows(RefArgs a, const std::int32_t* list, std::int64_t co
 
 // Receiver reservation (rebalance_graph.cuh:86-156) in canonical order:
 // requesters of one part are ranked by vertex id on top of the part's
-// request total at the start of the attempt.  Ranks come from a stable
-// one-pass radix sort by key followed by a segmented scan of weights.
-constexpr std::uint64_t kNoKey = 0xff;
+// request total at the start of the attempt.  A rank is an exclusive
+// weighted prefix among earlier items with the same key, computed without
+// sorting: per-tile key sums, a per-key scan over tiles, then in-tile
+// prefixes from wave scans per distinct key.
+constexpr int kKeyTile = 1024;
+constexpr int kKeyRounds = kKeyTile / kTierThreads;
 
-// Eligibility keys: part of a donor in its part's boundary bucket.
-__global__ void k_elig_keys(RefArgs a, const std::int32_t* L, std::int64_t nl, const std::int32_t* bstar,
-                            std::uint64_t* keys, std::uint32_t* vals) {
-    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < nl;
-         i += (std::int64_t)gridDim.x * blockDim.x) {
-        const int b = a.bucket[L[i]];
-        const int s = b / kBalanceWidth;
-        keys[i] = (b % kBalanceWidth) == bstar[s] ? (std::uint64_t)s : kNoKey;
-        vals[i] = (std::uint32_t)i;
-    }
-}
-// Segment items over keys sorted ascending; w(i) supplied by the caller list.
-__global__ void k_key_items(const std::uint64_t* keys, const std::uint32_t* vals, std::int64_t count,
-                            const std::int32_t* verts, const std::uint32_t* vw, SegU64* items) {
-    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
-         j += (std::int64_t)gridDim.x * blockDim.x) {
-        const bool head = j == 0 || keys[j] != keys[j - 1];
-        const std::uint64_t w = keys[j] == kNoKey ? 0 : (std::uint64_t)vw[verts[vals[j]]];
-        items[j] = SegU64{w, head ? 1u : 0u};
-    }
-}
-__global__ void k_eligible(RefArgs a, const std::int32_t* L, const std::uint64_t* keys, const std::uint32_t* vals,
-                           const SegU64* P, std::int64_t nl, const unsigned long long* prefix,
-                           const std::int32_t* bstar, const unsigned long long* excess, std::uint8_t* elig) {
-    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < nl;
-         j += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::uint32_t i = vals[j];
-        const std::int32_t v = L[i];
-        const int b = a.bucket[v];
-        const int s = b / kBalanceWidth;
-        bool e = (b % kBalanceWidth) < bstar[s];
-        if (keys[j] != kNoKey) {
-            const std::uint64_t rank = P[j].v - a.vw[v];
-            e = prefix[b] + rank < excess[s];
+// Exclusive weighted prefix among lanes with the same key (lower lanes
+// first); wt[key] receives the wave's total for every key present.  key < 0
+// marks a lane without a key.  All lanes of the wave call it.
+__device__ inline std::uint64_t wave_keyed_exclusive(int key, std::uint64_t w, unsigned long long* wt) {
+    const int lane = lane_id();
+    unsigned long long pending = __ballot(key >= 0);
+    std::uint64_t excl = 0;
+    while (pending) {
+        const int leader = __ffsll(pending) - 1;
+        const int kk = __shfl(key, leader);
+        const bool mine = key == kk;
+        const unsigned long long peers = __ballot(mine);
+        std::uint64_t x = mine ? w : 0;
+        for (int d = 1; d < kWave; d <<= 1) {
+            const std::uint64_t y = __shfl_up(x, d);
+            if (lane >= d) x += y;
         }
-        elig[i] = e;
+        if (mine) excl = x - w;
+        const std::uint64_t total = __shfl(x, kWave - 1);
+        if (lane == 0) wt[kk] = total;
+        pending &= ~peers;
+    }
+    return excl;
+}
+
+// Per-key exclusive scan over tiles (one wave per key); with requests,
+// ranks start at the request totals and the totals advance by this
+// attempt's requests.
+constexpr int kKeyScanThreads = 1024;
+
+__global__ void __launch_bounds__(kKeyScanThreads) k_keyed_scan(unsigned long long* S, int nb, int k,
+                                                               unsigned long long* requests) {
+    const int lane = lane_id();
+    for (int c = threadIdx.x / kWave; c < k; c += kKeyScanThreads / kWave) {
+        unsigned long long carry = requests ? requests[c] : 0;
+        for (int b0 = 0; b0 < nb; b0 += kWave) {
+            const int b = b0 + lane;
+            const unsigned long long own = b < nb ? S[(std::int64_t)c * nb + b] : 0;
+            unsigned long long x = own;
+            for (int d = 1; d < kWave; d <<= 1) {
+                const unsigned long long y = __shfl_up(x, d);
+                if (lane >= d) x += y;
+            }
+            if (b < nb) S[(std::int64_t)c * nb + b] = carry + x - own;
+            carry += __shfl(x, kWave - 1);
+        }
+        if (requests && lane == 0) requests[c] = carry;
     }
 }
-__global__ void k_gather_i32(const std::int32_t* src, const std::int32_t* idx, std::int64_t count, std::int32_t* dst) {
-    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
-         j += (std::int64_t)gridDim.x * blockDim.x)
-        dst[j] = src[idx[j]];
+
+// In-tile ranks in index order; f(i, key, weight, rank) returns whether i
+// is appended to the stage list.
+template <class F>
+__global__ void __launch_bounds__(kTierThreads) k_keyed_apply(const std::int32_t* keys, std::int64_t n,
+                                                             const unsigned long long* S, int nb, F f,
+                                                             std::int32_t* list, unsigned long long* count) {
+    __shared__ unsigned long long wt[kTierThreads / kWave][kMaxParts];
+    __shared__ unsigned long long run[kMaxParts];
+    __shared__ std::int32_t sb[kTierThreads / kWave][kStage];
+    const int w = threadIdx.x / kWave;
+    Stage st{sb[w], list, count, 0};
+    for (int c = threadIdx.x; c < kMaxParts; c += kTierThreads) {
+        run[c] = 0;
+        for (int i = 0; i < kTierThreads / kWave; ++i) wt[i][c] = 0;
+    }
+    __syncthreads();
+    const std::int64_t base = blockIdx.x * (std::int64_t)kKeyTile;
+    for (int r = 0; r < kKeyRounds; ++r) {
+        const std::int64_t i = base + r * kTierThreads + threadIdx.x;
+        const int key = i < n ? keys[i] : -1;
+        const std::uint64_t wgt = key >= 0 ? f.weight(i) : 0;
+        const std::uint64_t ex = wave_keyed_exclusive(key, wgt, wt[w]);
+        __syncthreads();
+        bool append = false;
+        if (key >= 0) {
+            std::uint64_t rank = S[(std::int64_t)key * nb + blockIdx.x] + run[key] + ex;
+            for (int j = 0; j < w; ++j) rank += wt[j][key];
+            append = f(i, key, wgt, rank);
+        }
+        st.push(append, append ? f.vertex(i) : 0);
+        __syncthreads();
+        for (int c = threadIdx.x; c < kMaxParts; c += kTierThreads) {
+            unsigned long long add = 0;
+            for (int j = 0; j < kTierThreads / kWave; ++j) {
+                add += wt[j][c];
+                wt[j][c] = 0;
+            }
+            run[c] += add;
+        }
+        __syncthreads();
+    }
+    st.flush();
+}
+
+// Per-tile key weight sums (S is key-major: S[key * nb + tile]).
+template <class F>
+__device__ inline void tile_key_sums(const std::int32_t* keys, std::int64_t n, int k, int nb, const F& f,
+                                     unsigned long long* acc, unsigned long long* S) {
+    __syncthreads();
+    const std::int64_t base = blockIdx.x * (std::int64_t)kKeyTile;
+    for (int r = 0; r < kKeyRounds; ++r) {
+        const std::int64_t i = base + r * kTierThreads + threadIdx.x;
+        if (i < n && keys[i] >= 0) atomicAdd(&acc[keys[i]], (unsigned long long)f.weight(i));
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < k; c += kTierThreads) S[(std::int64_t)c * nb + blockIdx.x] = acc[c];
+}
+
+// Eligibility (rebalance_graph.cuh:86-110): donors below the boundary
+// bucket are eligible; inside it, while their rank stays below the excess.
+struct EligFn {
+    RefArgs a;
+    const std::int32_t* L;
+    const unsigned long long* prefix;
+    const unsigned long long* excess;
+    std::uint8_t* elig;
+    __device__ std::uint64_t weight(std::int64_t i) const { return a.vw[L[i]]; }
+    __device__ std::int32_t vertex(std::int64_t i) const { return L[i]; }
+    __device__ bool operator()(std::int64_t i, int key, std::uint64_t, std::uint64_t rank) const {
+        elig[i] = prefix[a.bucket[L[i]]] + rank < excess[key];
+        return false;
+    }
+};
+
+__global__ void __launch_bounds__(kTierThreads) k_elig_keys(RefArgs a, const std::int32_t* L, std::int64_t nl,
+                                                           const std::int32_t* bstar, int nb, std::int32_t* keys,
+                                                           std::uint8_t* elig, unsigned long long* S) {
+    __shared__ unsigned long long acc[kMaxParts];
+    for (int c = threadIdx.x; c < kMaxParts; c += kTierThreads) acc[c] = 0;
+    const std::int64_t base = blockIdx.x * (std::int64_t)kKeyTile;
+    for (int r = 0; r < kKeyRounds; ++r) {
+        const std::int64_t i = base + r * kTierThreads + threadIdx.x;
+        if (i >= nl) break;
+        const int b = a.bucket[L[i]];
+        const int s = b / kBalanceWidth, lb = b % kBalanceWidth;
+        keys[i] = lb == bstar[s] ? s : -1;
+        elig[i] = lb < bstar[s];
+    }
+    EligFn f{a, L, nullptr, nullptr, nullptr};
+    tile_key_sums(keys, nl, a.k, nb, f, acc, S);
 }
 
 struct ReserveState {
     const std::int32_t* E;           // eligible donors in vertex order
     std::int64_t ne;
     const unsigned long long* conn;  // ne x k
-    unsigned long long* requests;    // k, start-of-attempt totals
+    unsigned long long* requests;    // k, request totals
     const unsigned long long* slack; // k
     std::uint8_t* done;
     unsigned long long* tried;       // ne x 2
 };
 
-__global__ void k_choose(RefArgs a, ReserveState r, int attempt, std::uint64_t* keys, std::uint32_t* vals,
-                         unsigned long long* active) {
+// Choices of one attempt, one 16-lane group per donor.  Attempt 0
+// requests the proposed target; later attempts the untried part with the
+// most connection, then the most room left at the start of the attempt,
+// then the lowest id.
+struct Choice {
+    int p;
+    unsigned long long conn, room;
+};
+__device__ inline bool choice_better(const Choice& x, const Choice& y) {
+    if (x.p < 0) return false;
+    if (y.p < 0) return true;
+    return x.conn > y.conn || (x.conn == y.conn && (x.room > y.room || (x.room == y.room && x.p < y.p)));
+}
+
+__global__ void __launch_bounds__(kTierThreads) k_choose(RefArgs a, ReserveState r, int attempt, std::int32_t* keys,
+                                                        unsigned long long* active) {
+    constexpr int W = 16;
+    const int lg = threadIdx.x % W;
+    const std::int64_t groups = (std::int64_t)gridDim.x * (kTierThreads / W);
     unsigned long long cnt = 0;
-    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane_id(); base < r.ne;
-         base += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int64_t e = base + lane_id();
-        if (e >= r.ne) continue;
-        int c = -1;
+    for (std::int64_t e = (blockIdx.x * (std::int64_t)kTierThreads + threadIdx.x) / W; e < r.ne; e += groups) {
+        Choice best{-1, 0, 0};
         if (!r.done[e]) {
             const std::int32_t v = r.E[e];
             if (attempt == 0)
-                c = a.targets[v];
+                best.p = a.targets[v];
             else {
                 const std::uint64_t w = a.vw[v];
                 const int lv = a.labels[v];
-                unsigned long long best = 0, room = 0;
-                for (int p = 0; p < a.k; ++p) {
+                for (int p = lg; p < a.k; p += W) {
                     if (p == lv || ((r.tried[2 * e + (p >> 6)] >> (p & 63)) & 1ULL)) continue;
                     const unsigned long long avail = r.requests[p] < r.slack[p] ? r.slack[p] - r.requests[p] : 0;
                     if (avail < w) continue;
-                    const unsigned long long cn = r.conn[e * a.k + p];
-                    if (c < 0 || cn > best || (cn == best && avail > room)) {
-                        c = p;
-                        best = cn;
-                        room = avail;
-                    }
+                    const Choice x{p, r.conn[e * a.k + p], avail};
+                    if (choice_better(x, best)) best = x;
+                }
+                for (int d = W / 2; d; d >>= 1) {
+                    const Choice o{__shfl_xor(best.p, d, W), __shfl_xor(best.conn, d, W), __shfl_xor(best.room, d, W)};
+                    if (choice_better(o, best)) best = o;
                 }
             }
         }
-        keys[e] = c >= 0 ? (std::uint64_t)c : kNoKey;
-        vals[e] = (std::uint32_t)e;
-        cnt += c >= 0;
+        if (lg == 0) {
+            keys[e] = best.p;
+            cnt += best.p >= 0;
+        }
     }
     cnt = wave_sum(cnt);
     if (lane_id() == 0 && cnt) atomicAdd(active, cnt);
 }
 
-__global__ void k_admit(RefArgs a, ReserveState r, const std::uint64_t* keys, const std::uint32_t* vals,
-                        const SegU64* P, std::int32_t* movers, unsigned long long* mcount) {
-    __shared__ std::int32_t sb[kTierThreads / kWave][kStage];
-    Stage st{sb[threadIdx.x / kWave], movers, mcount, 0};
-    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane_id(); base < r.ne;
-         base += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int64_t j = base + lane_id();
-        bool admit = false;
-        std::int32_t v = 0;
-        if (j < r.ne && keys[j] != kNoKey) {
-            const int c = (int)keys[j];
-            const std::uint32_t e = vals[j];
-            v = r.E[e];
-            const std::uint64_t w = a.vw[v];
-            const std::uint64_t begin = r.requests[c] + P[j].v - w;
-            r.tried[2 * e + (c >> 6)] |= 1ULL << (c & 63);
-            if (begin <= r.slack[c] && w <= r.slack[c] - begin) {
-                a.targets[v] = c;
-                a.flags[v] = 1;
-                r.done[e] = 1;
-                admit = true;
-            }
+struct DonorWeight {
+    RefArgs a;
+    const std::int32_t* E;
+    __device__ std::uint64_t weight(std::int64_t i) const { return a.vw[E[i]]; }
+};
+
+template <class F>
+__global__ void __launch_bounds__(kTierThreads) k_keyed_sums(const std::int32_t* keys, std::int64_t n, int k, int nb,
+                                                            F f, unsigned long long* S) {
+    __shared__ unsigned long long acc[kMaxParts];
+    for (int c = threadIdx.x; c < kMaxParts; c += kTierThreads) acc[c] = 0;
+    tile_key_sums(keys, n, k, nb, f, acc, S);
+}
+
+// Admission: the request fits when it ends within the receiver's slack.
+struct AdmitFn {
+    RefArgs a;
+    ReserveState r;
+    __device__ std::uint64_t weight(std::int64_t e) const { return a.vw[r.E[e]]; }
+    __device__ std::int32_t vertex(std::int64_t e) const { return r.E[e]; }
+    __device__ bool operator()(std::int64_t e, int c, std::uint64_t w, std::uint64_t begin) const {
+        r.tried[2 * e + (c >> 6)] |= 1ULL << (c & 63);
+        if (begin <= r.slack[c] && w <= r.slack[c] - begin) {
+            const std::int32_t v = r.E[e];
+            a.targets[v] = c;
+            a.flags[v] = 1;
+            r.done[e] = 1;
+            return true;
         }
-        st.push(admit, v);
+        return false;
     }
-    st.flush();
+};
+
+__global__ void k_gather_i32(const std::int32_t* src, const std::int32_t* idx, std::int64_t count, std::int32_t* dst) {
+    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
+         j += (std::int64_t)gridDim.x * blockDim.x)
+        dst[j] = src[idx[j]];
 }
-__global__ void k_requests_add(ReserveState r, const std::uint64_t* keys, const SegU64* P) {
-    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < r.ne;
-         j += (std::int64_t)gridDim.x * blockDim.x) {
-        if (keys[j] == kNoKey) continue;
-        if (j + 1 == r.ne || keys[j + 1] != keys[j]) r.requests[keys[j]] += P[j].v;
-    }
-}
+
 __global__ void k_slack(RefArgs a, unsigned long long* slack, unsigned long long* requests) {
     for (int p = threadIdx.x; p < a.k; p += blockDim.x) {
         slack[p] = a.loads0[p] < a.cap ? a.cap - a.loads0[p] : 0;
@@ -1330,31 +1481,22 @@ public:
         return true;
     }
 
-    // Ranks of keyed items: stable sort by key, then an inclusive segmented
-    // scan of weights; P[j] is the weight of items with the same key up to j.
-    void keyed_prefix(std::uint64_t* keys, std::uint32_t* vals, std::int64_t count, const std::int32_t* verts,
-                      SegU64* P) {
-        radix_sort_pairs(c_, keys, vals, count, 0, 8);
-        hipLaunchKernelGGL(k_key_items, dim3(grid_for(count)), dim3(256), 0, c_.stream(), keys, vals, count, verts,
-                           a_.vw, P);
-        GP_LAUNCH_CHECK();
-        scan<SegPlusOp, SegU64>(c_, P, P, count, false);
-    }
-
     // Receiver reservation over the flagged donors L (vertex order).
     void reserve(const std::int32_t* L, std::int64_t nl, const unsigned long long* prefix, const std::int32_t* bstar,
                  const unsigned long long* excess) {
         const hipStream_t s = c_.stream();
         const int k = a_.k;
-        DBuf<std::uint64_t> keys = c_.buf<std::uint64_t>(static_cast<std::size_t>(nl));
-        DBuf<std::uint32_t> vals = c_.buf<std::uint32_t>(static_cast<std::size_t>(nl));
-        DBuf<SegU64> P = c_.buf<SegU64>(static_cast<std::size_t>(nl));
+        const int nbl = static_cast<int>(ceil_div(nl, kKeyTile));
+        DBuf<std::int32_t> keys = c_.buf<std::int32_t>(static_cast<std::size_t>(nl));
+        DBuf<unsigned long long> S = c_.buf<unsigned long long>(static_cast<std::size_t>(nbl) * k);
         DBuf<std::uint8_t> elig = c_.buf<std::uint8_t>(static_cast<std::size_t>(nl));
-        hipLaunchKernelGGL(k_elig_keys, dim3(grid_for(nl)), dim3(256), 0, s, a_, L, nl, bstar, keys.get(), vals.get());
+        hipLaunchKernelGGL(k_elig_keys, dim3(nbl), dim3(kTierThreads), 0, s, a_, L, nl, bstar, nbl, keys.get(),
+                           elig.get(), S.get());
         GP_LAUNCH_CHECK();
-        keyed_prefix(keys.get(), vals.get(), nl, L, P.get());
-        hipLaunchKernelGGL(k_eligible, dim3(grid_for(nl)), dim3(256), 0, s, a_, L, keys.get(), vals.get(), P.get(), nl,
-                           prefix, bstar, excess, elig.get());
+        hipLaunchKernelGGL(k_keyed_scan, dim3(1), dim3(kKeyScanThreads), 0, s, S.get(), nbl, k, nullptr);
+        GP_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_keyed_apply<EligFn>), dim3(nbl), dim3(kTierThreads), 0, s, keys.get(), nl, S.get(), nbl,
+                           EligFn{a_, L, prefix, excess, elig.get()}, nullptr, nullptr);
         GP_LAUNCH_CHECK();
         DBuf<std::int32_t> idx = c_.buf<std::int32_t>(static_cast<std::size_t>(nl));
         DBuf<std::int64_t> cnt = c_.buf<std::int64_t>(1);
@@ -1376,18 +1518,22 @@ public:
         fill<unsigned long long>(c_, tried.get(), 2 * ne, 0ULL);
         hipLaunchKernelGGL(k_slack, dim3(1), dim3(kMaxParts), 0, s, a_, slack.get(), requests.get());
         GP_LAUNCH_CHECK();
+        const int nb = static_cast<int>(ceil_div(ne, kKeyTile));
         ReserveState r{E.get(), ne, conn.get(), requests.get(), slack.get(), done.get(), tried.get()};
         for (int attempt = 0; attempt < k; ++attempt) {
             fill<unsigned long long>(c_, ctr_.get() + kActive, 1, 0ULL);
-            hipLaunchKernelGGL(k_choose, dim3(grid_for(ne)), dim3(256), 0, s, a_, r, attempt, keys.get(), vals.get(),
-                               ctr_.get() + kActive);
+            hipLaunchKernelGGL(k_choose, dim3(grid_for(ne, kTierThreads / 16, 4096)), dim3(kTierThreads), 0, s, a_, r,
+                               attempt, keys.get(), ctr_.get() + kActive);
             GP_LAUNCH_CHECK();
             if (!c_.read_scalar(ctr_.get() + kActive)) break;
-            keyed_prefix(keys.get(), vals.get(), ne, E.get(), P.get());
-            hipLaunchKernelGGL(k_admit, dim3(grid_for(ne)), dim3(256), 0, s, a_, r, keys.get(), vals.get(), P.get(),
-                               movers_.get(), ctr_.get() + kMovers);
+            hipLaunchKernelGGL((k_keyed_sums<DonorWeight>), dim3(nb), dim3(kTierThreads), 0, s, keys.get(), ne, k, nb,
+                               DonorWeight{a_, E.get()}, S.get());
             GP_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_requests_add, dim3(grid_for(ne)), dim3(256), 0, s, r, keys.get(), P.get());
+            // Ranks start at the start-of-attempt totals; totals advance.
+            hipLaunchKernelGGL(k_keyed_scan, dim3(1), dim3(kKeyScanThreads), 0, s, S.get(), nb, k, requests.get());
+            GP_LAUNCH_CHECK();
+            hipLaunchKernelGGL((k_keyed_apply<AdmitFn>), dim3(nb), dim3(kTierThreads), 0, s, keys.get(), ne, S.get(),
+                               nb, AdmitFn{a_, r}, movers_.get(), ctr_.get() + kMovers);
             GP_LAUNCH_CHECK();
         }
     }

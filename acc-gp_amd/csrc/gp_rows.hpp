@@ -103,6 +103,29 @@ struct BlockTeam {
     }
 };
 
+// Visits entries [b, e) of a row with the team's stride and kBatch entries
+// per lane in flight: every load(x) of a batch is issued before the first
+// use(x, record), so dependent gathers overlap instead of serialising.
+constexpr int kBatch = 4;
+
+template <class Team, class Load, class Use>
+__device__ inline void row_batches(const Team& t, std::int64_t b, std::int64_t e, Load load, Use use) {
+    for (std::int64_t x0 = b + t.rank(); x0 < e; x0 += kBatch * Team::size) {
+        using Rec = decltype(load(x0));
+        Rec rec[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const std::int64_t x = x0 + j * Team::size;
+            rec[j] = load(x < e ? x : e - 1);
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const std::int64_t x = x0 + j * Team::size;
+            if (x < e) use(x, rec[j]);
+        }
+    }
+}
+
 struct RowOut {
     bool append;
     std::int64_t acc;

@@ -92,6 +92,7 @@ __device__ inline int group_reduce_runs(const std::int32_t (&k)[I], const std::u
 // last element of each run; returns the emission count to every thread.
 constexpr int kBlockSortThreads = 256;
 constexpr int kBlockSortMax = 4096;
+static_assert(kBlockSortThreads == kScanThreads, "block scans assume one block shape");
 
 struct BlockSortSmem {
     std::int32_t key[kBlockSortMax];
@@ -100,8 +101,14 @@ struct BlockSortSmem {
     int count[kBlockSortThreads];
 };
 
-template <typename Emit>
-__device__ inline int block_sort_reduce(BlockSortSmem& s, int n, Emit&& emit) {
+struct NoBegin {
+    __device__ void operator()(int) const {}
+};
+
+// begin(total) is called by every thread once the emission count is known,
+// before the first emit.
+template <typename Emit, typename Begin = NoBegin>
+__device__ inline int block_sort_reduce(BlockSortSmem& s, int n, Emit&& emit, Begin&& begin = Begin{}) {
     int N = 2;
     while (N < n) N <<= 1;
     if (N < kBlockSortThreads * 2) N = kBlockSortThreads * 2;
@@ -140,11 +147,8 @@ __device__ inline int block_sort_reduce(BlockSortSmem& s, int n, Emit&& emit) {
         head_seen |= head ? 1u : 0u;
     }
     SegU64 agg{run, head_seen};
-    SegU64 inc = block_inclusive_scan<SegPlusOp>(agg, s.scan);
-    s.scan[threadIdx.x] = inc;
-    __syncthreads();
-    SegU64 prev = threadIdx.x ? s.scan[threadIdx.x - 1] : SegU64{0, 0};
-    __syncthreads();
+    SegU64 inc, prev, all;
+    block_scan<SegPlusOp>(agg, s.scan, inc, prev, all);
     // Count run ends per thread for the emission index.
     int ends = 0;
     for (int i = 0; i < E; ++i) {
@@ -153,13 +157,9 @@ __device__ inline int block_sort_reduce(BlockSortSmem& s, int n, Emit&& emit) {
         const std::int32_t nxt = idx + 1 < N ? s.key[idx + 1] : kSentinel;
         ends += key != kSentinel && key != nxt;
     }
-    s.count[threadIdx.x] = ends;
-    __syncthreads();
-    int before = 0, total = 0;
-    for (int t = 0; t < kBlockSortThreads; ++t) {
-        if (t < static_cast<int>(threadIdx.x)) before += s.count[t];
-        total += s.count[t];
-    }
+    int ends_inc, before, total;
+    block_scan<PlusOp>(ends, s.count, ends_inc, before, total);
+    begin(total);
     std::uint64_t carry = prev.v;
     bool continuing = true;  // the thread's first run continues from previous threads
     std::uint64_t sum = 0;

@@ -143,13 +143,13 @@ struct TierLists {
 
 // Light tier: list (or the range [0, n) when list is null) of count rows,
 // count read from count_ptr when non-null.
-template <class Body>
+template <class Body, int W>
 __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const std::int32_t* list, std::int64_t n,
                                                              const unsigned long long* count_ptr, TierLists t,
                                                              int scratch_per_group) {
     extern __shared__ unsigned long long smem[];
     __shared__ std::int32_t stage_buf[kTierThreads / kWave][2][kStage];
-    constexpr int W = 16, G = kWave / W;
+    constexpr int G = kWave / W;
     const int lane = lane_id(), gw = lane / W, lg = lane % W;
     unsigned long long* scratch = smem + (threadIdx.x / W) * scratch_per_group;
     Stage defer_stage{stage_buf[threadIdx.x / kWave][0], t.medium, t.medium_count, 0};
@@ -239,10 +239,20 @@ void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int6
     const hipStream_t s = c.stream();
     fill<unsigned long long>(c, tier_counts, 2, 0ULL);
     TierLists t{medium, tier_counts, heavy, tier_counts + 1, out, out_count, acc};
-    const int light_blocks =
-        static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(max_rows, kTierThreads / 16), 4096)));
-    hipLaunchKernelGGL((k_tier_light<Body>), dim3(light_blocks), dim3(kTierThreads),
-                       (kTierThreads / 16) * scratch * sizeof(unsigned long long), s, body, list, n, count_ptr, t, scratch);
+    int lanes = c.light_lanes();
+    while (lanes < 16 && (kTierThreads / lanes) * scratch * sizeof(unsigned long long) > 48 * 1024) lanes *= 2;
+    const int light_blocks = static_cast<int>(
+        std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(max_rows, kTierThreads / lanes), 4096)));
+    const std::size_t light_lds = (kTierThreads / lanes) * scratch * sizeof(unsigned long long);
+    if (lanes == 4)
+        hipLaunchKernelGGL((k_tier_light<Body, 4>), dim3(light_blocks), dim3(kTierThreads), light_lds, s, body, list, n,
+                           count_ptr, t, scratch);
+    else if (lanes == 8)
+        hipLaunchKernelGGL((k_tier_light<Body, 8>), dim3(light_blocks), dim3(kTierThreads), light_lds, s, body, list, n,
+                           count_ptr, t, scratch);
+    else
+        hipLaunchKernelGGL((k_tier_light<Body, 16>), dim3(light_blocks), dim3(kTierThreads), light_lds, s, body, list, n,
+                           count_ptr, t, scratch);
     GP_LAUNCH_CHECK();
     const int medium_blocks =
         static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(max_rows, kTierThreads / kWave), 2048)));

@@ -5,7 +5,9 @@
 #pragma once
 #include "gp_common.hpp"
 
+#include <cstdlib>
 #include <cstring>
+#include <stdexcept>
 #include <map>
 #include <string>
 #include <mutex>
@@ -160,6 +162,7 @@ public:
         GP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         arena_ = new Arena(stream_);
         GP_HIP(hipHostMalloc(&pinned_, kPinnedBytes, hipHostMallocDefault));
+        if (const char* e = std::getenv("GP_LIGHT_LANES")) set_light_lanes(std::atoi(e));
     }
     ~Context() {
         delete arena_;
@@ -171,6 +174,13 @@ public:
 
     hipStream_t stream() const { return stream_; }
     Arena* arena() { return arena_; }
+
+    // Lanes per row team for short rows (4, 8 or 16; GP_LIGHT_LANES).
+    int light_lanes() const { return light_lanes_; }
+    void set_light_lanes(int lanes) {
+        if (lanes != 4 && lanes != 8 && lanes != 16) throw std::invalid_argument("light lanes must be 4, 8 or 16");
+        light_lanes_ = lanes;
+    }
     int device() const { return device_; }
     void activate() const { GP_HIP(hipSetDevice(device_)); }
 
@@ -264,6 +274,7 @@ private:
     hipStream_t stream_ = nullptr;
     Arena* arena_ = nullptr;
     void* pinned_ = nullptr;
+    int light_lanes_ = 16;
 };
 
 }  // namespace gp

@@ -163,13 +163,42 @@ static int cmp_cw_pair(const void* a, const void* b) {
 
 /* Admission record: target asc, key desc, tie desc, vertex asc.
  * include/detail/capacity_admission.hpp:14-33, src/coarsen.cu:666-718. */
-typedef struct { int32_t target; uint64_t key; uint32_t tie; int32_t v; } og_adm;
+typedef struct { int32_t target; uint64_t key; uint32_t tie; int32_t v; uint64_t w, base; int64_t idx; } og_adm;
 static int cmp_adm(const void* a, const void* b) {
     const og_adm* x = a; const og_adm* y = b;
     if (x->target != y->target) return x->target < y->target ? -1 : 1;
     if (x->key != y->key) return x->key > y->key ? -1 : 1;
     if (x->tie != y->tie) return x->tie > y->tie ? -1 : 1;
     return x->v < y->v ? -1 : x->v > y->v;
+}
+
+/* One admission batch (include/detail/capacity_admission.hpp:14-33,
+ * src/coarsen.cu:1031-1075): requests sorted by cmp_adm; request i is
+ * accepted iff the inclusive weight prefix of its target's requests up to i
+ * is at most cap - base, base being the target's weight frozen at the start
+ * of the batch.  A prefix, not bin packing: a rejected request still counts
+ * toward the prefix of later ones.  accept[] is indexed by adm[].idx. */
+static void og_admission(og_adm* adm, int64_t cnt, uint64_t cap, uint8_t* accept) {
+    qsort(adm, (size_t)cnt, sizeof(og_adm), cmp_adm);
+    uint64_t prefix = 0;
+    for (int64_t i = 0; i < cnt; ++i) {
+        if (i == 0 || adm[i].target != adm[i - 1].target) prefix = 0;
+        prefix += adm[i].w;
+        accept[adm[i].idx] = adm[i].base <= cap && prefix <= cap - adm[i].base;
+    }
+}
+
+int og_admission_batch(int64_t cnt, const int32_t* targets, const uint64_t* keys, const uint32_t* ties,
+                       const int32_t* vertices, const uint64_t* weights, const uint64_t* bases, uint64_t cap,
+                       uint8_t* accepted) {
+    og_adm* adm = og_alloc((size_t)(cnt > 0 ? cnt : 1), sizeof(og_adm));
+    for (int64_t i = 0; i < cnt; ++i) {
+        adm[i].target = targets[i]; adm[i].key = keys[i]; adm[i].tie = ties[i]; adm[i].v = vertices[i];
+        adm[i].w = weights[i]; adm[i].base = bases[i]; adm[i].idx = i;
+    }
+    og_admission(adm, cnt, cap, accepted);
+    free(adm);
+    return 0;
 }
 
 typedef struct {
@@ -363,24 +392,20 @@ static int og_aggregate(const og_graph* g, og_cstate* st, int parts, uint32_t se
             for (int64_t v = 0; v < n; ++v) {
                 if (A.proposals[v] == OG_INVALID) continue;
                 adm[cnt].target = A.proposals[v]; adm[cnt].key = A.regret[v];
-                adm[cnt].tie = A.ties[v]; adm[cnt].v = (int32_t)v; ++cnt;
+                adm[cnt].tie = A.ties[v]; adm[cnt].v = (int32_t)v;
+                adm[cnt].w = g->vw[v]; adm[cnt].base = A.cw[A.proposals[v]]; adm[cnt].idx = cnt;
+                ++cnt;
             }
-            qsort(adm, (size_t)cnt, sizeof(og_adm), cmp_adm);
-            /* Snapshot receiver weights before any commit of this batch. */
-            uint64_t prefix = 0;
-            uint64_t* base = og_alloc((size_t)cnt, sizeof(uint64_t));
-            for (int64_t i = 0; i < cnt; ++i) base[i] = A.cw[adm[i].target];
+            uint8_t* accept = og_alloc((size_t)(cnt > 0 ? cnt : 1), 1);
+            og_admission(adm, cnt, cap, accept);
             for (int64_t i = 0; i < cnt; ++i) {
-                if (i == 0 || adm[i].target != adm[i - 1].target) prefix = 0;
+                if (!accept[adm[i].idx]) continue;
                 const int32_t v = adm[i].v;
                 const uint64_t w = g->vw[v];
-                prefix += w;
-                if (base[i] <= cap && prefix <= cap - base[i]) {
-                    const int32_t s = A.clusters[v];
-                    A.cw[s] -= w; A.cw[adm[i].target] += w; A.clusters[v] = adm[i].target;
-                }
+                const int32_t s = A.clusters[v];
+                A.cw[s] -= w; A.cw[adm[i].target] += w; A.clusters[v] = adm[i].target;
             }
-            free(base);
+            free(accept);
         }
         cluster_count = og_count_clusters(A.cw, n);
         bound = next_bound;
@@ -1336,7 +1361,8 @@ static int og_refine_level(const og_graph* g, int32_t* labels, uint64_t* loads, 
         const double filter = maximum_edge <= 1 ? 0.25 : 0.75;              /* incremental.cuh:897-899 */
         const int graph_mode = n >= o->graph_loop_threshold;
         int stale = 0, empty = 0;
-        for (int phase = 0; phase < rounds && stale < o->stale_rounds; ++phase) {
+        const int lp_rounds = o->exchange_only ? 0 : rounds;
+        for (int phase = 0; phase < lp_rounds && stale < o->stale_rounds; ++phase) {
             st->rounds = phase + 1;
             if (R.samples) memset(R.samples, 0, (size_t)k * sizeof(og_sample));
             /* LP proposals: incremental.cuh:166-198,229-296 */

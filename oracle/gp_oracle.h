@@ -46,6 +46,7 @@ typedef struct og_refine_opts {
     int exchange_rounds;       /* 2 */
     int graph_loop_threshold;  /* 65536: levels with n >= this use the integer stale rule */
     int exchange_max_n;        /* 65536 */
+    int exchange_only;         /* checks only: skip LP rounds, run the exchange passes */
 } og_refine_opts;
 
 typedef struct og_level_stats {
@@ -99,6 +100,11 @@ uint64_t og_cut(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_
 int og_contract_graph(int64_t n, int64_t m, const int32_t* off, const int32_t* nbr,
                       const uint32_t* ew, const int32_t* map, int64_t nc,
                       int32_t* coff, int32_t* cnbr, uint32_t* cew, int64_t* cm);
+
+/* One coarsening admission batch on explicit requests (checks only). */
+int og_admission_batch(int64_t cnt, const int32_t* targets, const uint64_t* keys, const uint32_t* ties,
+                       const int32_t* vertices, const uint64_t* weights, const uint64_t* bases, uint64_t cap,
+                       uint8_t* accepted);
 
 const char* og_last_error(void);
 

@@ -28,7 +28,7 @@ class RefineOpts(ctypes.Structure):
         ("parts", ctypes.c_int), ("imbalance", ctypes.c_double), ("max_rounds", ctypes.c_int),
         ("stale_rounds", ctypes.c_int), ("growth_ratio", ctypes.c_double),
         ("exchange_rounds", ctypes.c_int), ("graph_loop_threshold", ctypes.c_int),
-        ("exchange_max_n", ctypes.c_int),
+        ("exchange_max_n", ctypes.c_int), ("exchange_only", ctypes.c_int),
     ]
 
 
@@ -70,6 +70,7 @@ def lib():
         L.og_last_error.restype = ctypes.c_char_p
         L.og_default_coarsen_opts.argtypes = [ctypes.c_int, ctypes.c_uint32, vp]
         L.og_default_refine_opts.argtypes = [ctypes.c_int, ctypes.c_double, vp]
+        L.og_admission_batch.argtypes = [ctypes.c_int64, vp, vp, vp, vp, vp, vp, ctypes.c_uint64, vp]
     return _LIB
 
 
@@ -192,3 +193,14 @@ def contract(off, nbr, ew, cmap, nc):
     if rc != 0:
         raise RuntimeError("oracle contract failed: " + _err())
     return coff, cnbr[:cm.value].copy(), cew[:cm.value].copy()
+
+
+def admission_batch(targets, keys, ties, vertices, weights, bases, cap):
+    """One coarsening admission batch: accepted flags in request order."""
+    n = len(targets)
+    t = np.ascontiguousarray(targets, np.int32); k = np.ascontiguousarray(keys, np.uint64)
+    ti = np.ascontiguousarray(ties, np.uint32); v = np.ascontiguousarray(vertices, np.int32)
+    w = np.ascontiguousarray(weights, np.uint64); b = np.ascontiguousarray(bases, np.uint64)
+    out = np.zeros(max(n, 1), np.uint8)
+    lib().og_admission_batch(n, _p(t), _p(k), _p(ti), _p(v), _p(w), _p(b), cap, _p(out))
+    return out[:n].astype(bool)

@@ -1,0 +1,178 @@
+"""Host-side checks that need no GPU: the C ABI library and its exports, the
+graph generators and the on-disk CSR format, the host initial partitioner,
+and the weighted CSR checker against the reference's own gpart_verify_csr
+(tools/verify_csr.cpp, built into oracle/_ref when the reference tree is
+present)."""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests import graphs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gpart_mi355x.h")
+VERIFY = os.path.join(ROOT, "acc-gp_amd", "bin", "gpart_verify_csr")
+REF_VERIFY = os.path.join(ROOT, "oracle", "_ref", "gpart_verify_csr_ref")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    return sorted(set(re.findall(r"\b(gp_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(gp):
+    lib = gp.lib()
+    names = _declared()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(gp.EXPORTS) == sorted(set(gp.EXPORTS))
+    assert set(gp.EXPORTS) <= set(names)
+
+
+def test_library_symbols_by_nm():
+    so = os.path.join(ROOT, "acc-gp_amd", "libgpart_mi355x.so")
+    out = subprocess.run(["nm", "-D", "--defined-only", so], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (gp_[a-z0-9_]+)", out))
+    assert set(_declared()) <= exported
+
+
+def test_capacity_through_the_abi(gp, og):
+    for total, parts, eps in [(65536, 8, 1.03), (4194304, 64, 1.03), (20, 2, 1.0), (12345678901, 64, 1.03)]:
+        assert gp.capacity(total, parts, eps) == og.capacity(total, parts, eps)
+    with pytest.raises(Exception):
+        gp.capacity(7, 3, 1.03)
+    with pytest.raises(Exception):
+        gp.capacity(100, 1, 1.03)
+
+
+def _check_sym_csr(off, nbr):
+    n = len(off) - 1
+    assert off[0] == 0 and off[-1] == len(nbr) and np.all(np.diff(off) >= 0)
+    src = np.repeat(np.arange(n), np.diff(off))
+    assert not np.any(src == nbr)
+    for v in range(0, n, max(1, n // 200)):
+        row = nbr[off[v]:off[v + 1]]
+        assert np.all(np.diff(row) > 0)
+    a = set(zip(src.tolist(), nbr.tolist()))
+    assert all((y, x) in a for x, y in a)
+
+
+def test_generate_grid_matches_reference_numbering(gp):
+    off, nbr = gp.generate_grid(256, 256)
+    assert len(off) - 1 == 65536 and len(nbr) == 2 * 130560
+    ref_off, ref_nbr = graphs.grid(256, 256)
+    assert np.array_equal(off, ref_off) and np.array_equal(nbr, ref_nbr)
+
+
+def test_generate_rmat_properties(gp):
+    off, nbr = gp.generate_rmat(12, 16, seed=1)
+    assert len(off) - 1 == 4096
+    _check_sym_csr(off, nbr)
+    again = gp.generate_rmat(12, 16, seed=1)
+    assert np.array_equal(off, again[0]) and np.array_equal(nbr, again[1])
+    other = gp.generate_rmat(12, 16, seed=2)
+    assert not np.array_equal(nbr[:1000], other[1][:1000])
+    deg = np.diff(off)
+    assert deg.max() > 20 * max(1, int(np.median(deg))), "skewed degrees"
+
+
+def test_generate_knn_and_rgg(gp):
+    off, nbr, ew, vw = gp.generate_knn3d(2000, 6, seed=2)
+    _check_sym_csr(off, nbr)
+    assert ew.min() >= 1 and ew.max() <= 100 and vw.min() >= 1 and vw.max() <= 4
+    src = np.repeat(np.arange(2000), np.diff(off))
+    w = dict(zip(zip(src.tolist(), nbr.tolist()), ew.tolist()))
+    assert all(w[(b, a)] == x for (a, b), x in w.items())
+    off, nbr = gp.generate_rgg2d(4096, (8.0 / (3.14159265 * 4096)) ** 0.5, seed=5)
+    _check_sym_csr(off, nbr)
+
+
+def test_csr_file_roundtrip(gp, tmp_path):
+    off, nbr = gp.generate_rmat(10, 8, seed=3)
+    ip, ix = str(tmp_path / "g.indptr"), str(tmp_path / "g.indices")
+    gp.save_csr(ip, ix, off, nbr)
+    assert os.path.getsize(ip) == 8 * len(off) and os.path.getsize(ix) == 8 * len(nbr)
+    assert np.array_equal(np.fromfile(ip, "<i8"), off) and np.array_equal(np.fromfile(ix, "<i8"), nbr)
+    o2, n2 = gp.load_csr(ip, ix)
+    assert np.array_equal(o2, off) and np.array_equal(n2, nbr)
+
+
+def test_csr_loader_rejects_bad_input(gp, tmp_path):
+    ip, ix = str(tmp_path / "b.indptr"), str(tmp_path / "b.indices")
+    np.array([0, 2, 3], "<i8").tofile(ip)
+    np.array([1, 5, 0], "<i8").tofile(ix)  # neighbour id out of range
+    with pytest.raises(Exception):
+        gp.load_csr(ip, ix)
+    np.array([0, 3, 2], "<i8").tofile(ip)  # not monotone
+    np.array([1, 0, 0], "<i8").tofile(ix)
+    with pytest.raises(Exception):
+        gp.load_csr(ip, ix)
+
+
+def test_initial_partition_host_valid_and_deterministic(gp, og):
+    for k in (2, 8, 64):
+        off, nbr = gp.generate_grid(24, 24)
+        n = len(off) - 1
+        a = gp.initial_partition_host(off, nbr, None, None, k)
+        b = gp.initial_partition_host(off, nbr, None, None, k)
+        assert np.array_equal(a, b)
+        assert a.min() >= 0 and a.max() < k
+    off, nbr, ew = graphs.random_graph(500, 2000, seed=3, max_weight=5)
+    vw = np.random.default_rng(1).integers(1, 5, 500).astype(np.uint32)
+    lab = gp.initial_partition_host(off, nbr, ew, vw, 8)
+    assert lab.min() >= 0 and lab.max() < 8
+
+
+def _write_partition(tmp_path, labels, fmt):
+    p = str(tmp_path / ("part." + fmt))
+    if fmt == "binary":
+        np.asarray(labels, "<i4").tofile(p)
+    else:
+        open(p, "w").write("\n".join(str(int(x)) for x in labels) + "\n")
+    return p
+
+
+@pytest.mark.parametrize("fmt", ["binary", "text"])
+def test_verify_tool_matches_reference_checker(gp, og, tmp_path, fmt):
+    off, nbr = gp.generate_rmat(11, 8, seed=6)
+    n, k = len(off) - 1, 8
+    ip, ix = str(tmp_path / "g.indptr"), str(tmp_path / "g.indices")
+    gp.save_csr(ip, ix, off, nbr)
+    lab = (np.arange(n) % k).astype(np.int32)
+    part = _write_partition(tmp_path, lab, fmt)
+    res = subprocess.run([VERIFY, ip, ix, str(k), part, fmt, "--symmetric"], capture_output=True, text=True)
+    got = json.loads(res.stdout)
+    assert got["cut"] == og.cut(off, nbr, None, lab)
+    assert got["undirected_edges"] == len(nbr) // 2
+    assert got["loads"] == np.bincount(lab, minlength=k).tolist()
+    assert got["capacity"] == n * 103 // (100 * k)
+    if os.path.exists(REF_VERIFY):
+        ref = subprocess.run([REF_VERIFY, ip, ix, str(k), part, fmt, "--symmetric"], capture_output=True, text=True)
+        want = json.loads(ref.stdout)
+        assert res.returncode == ref.returncode
+        for key, value in want.items():
+            assert got[key] == value, key
+
+
+def test_verify_tool_weighted_extension(gp, og, tmp_path):
+    off, nbr, ew, vw = gp.generate_knn3d(1500, 6, seed=2)
+    n, k = len(off) - 1, 4
+    ip, ix = str(tmp_path / "w.indptr"), str(tmp_path / "w.indices")
+    gp.save_csr(ip, ix, off, nbr)
+    ewf, vwf = str(tmp_path / "w.ew"), str(tmp_path / "w.vw")
+    ew.astype("<u4").tofile(ewf)
+    vw.astype("<u4").tofile(vwf)
+    lab = (np.arange(n) % k).astype(np.int32)
+    part = _write_partition(tmp_path, lab, "binary")
+    res = subprocess.run([VERIFY, ip, ix, str(k), part, "binary", "--edge-weights", ewf, "--vertex-weights", vwf],
+                         capture_output=True, text=True)
+    got = json.loads(res.stdout)
+    assert got["cut"] == og.cut(off, nbr, ew, lab)
+    assert got["loads"] == np.bincount(lab, weights=vw, minlength=k).astype(np.int64).tolist()

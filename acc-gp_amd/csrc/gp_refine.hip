@@ -32,7 +32,8 @@ enum Ctr {
     kDelta = 5,
     kDonors = 6,
     kActive = 7,
-    kCtrCount = 8
+    kRowSum = 8,  // entries of the rows scanned by the last LP proposal sweep
+    kCtrCount = 9
 };
 
 struct RepairSample {
@@ -172,7 +173,7 @@ struct LpProposeBody : RowBody {
         const std::uint64_t total = team_histogram(t, a, v, hist);
         const int s = a.labels[v];
         const std::uint64_t current = hist[s];
-        if (current == total) return RowOut{false, 0};
+        if (current == total) return RowOut{false, degree(v)};
         const std::uint64_t w = a.vw[v];
         const bool priced = a.prices != nullptr;
         Cand best{-1, 0, 0.0};
@@ -189,7 +190,7 @@ struct LpProposeBody : RowBody {
             a.targets[v] = best.p;
             a.gains[v] = best.conn >= current ? (std::int64_t)(best.conn - current) : -(std::int64_t)(current - best.conn);
         }
-        return RowOut{accept, 0};
+        return RowOut{accept, degree(v)};
     }
 };
 
@@ -1450,8 +1451,13 @@ public:
         hipLaunchKernelGGL(k_compact_flags, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, n, member_.get(),
                            frontier_.get(), ctr_.get() + kFrontier);
         GP_LAUNCH_CHECK();
-        tiers(LpProposeBody{{a_}}, frontier_.get(), 0, ctr_.get() + kFrontier, cand_.get(), ctr_.get() + kCand,
-              nullptr, a_.k);
+        // Algorithmic bytes per scanned row: offsets 8, label and weight 8,
+        // target and gain 12, and per entry neighbour 4, neighbour label 4
+        // and edge weight 4 when weighted; added once the row sums are read.
+        c_.timed("lp_propose", 0.0, [&] {
+            tiers(LpProposeBody{{a_}}, frontier_.get(), 0, ctr_.get() + kFrontier, cand_.get(), ctr_.get() + kCand,
+                  ctr_.get() + kRowSum, a_.k);
+        });
         tiers(AfterBody{{a_}}, cand_.get(), 0, ctr_.get() + kCand, movers_.get(), ctr_.get() + kMovers, nullptr, 0);
         return apply(true, false, delta);
     }
@@ -1469,6 +1475,8 @@ public:
         c_.read(h.data(), ctr_.get(), h.size() * sizeof(unsigned long long));
         for (int p = 0; p < a_.k; ++p) loads_host_[static_cast<std::size_t>(p)] = h[static_cast<std::size_t>(kCtrCount + p)];
         delta = (std::int64_t)h[kDelta];
+        if (lp && c_.profiling())
+            c_.add_bytes("lp_propose", 28.0 * (double)h[kFrontier] + (8.0 + (a_.ew ? 4.0 : 0.0)) * (double)h[kRowSum]);
         return (std::int64_t)h[kMovers];
     }
 

@@ -239,6 +239,10 @@ public:
         GP_HIP(hipEventRecord(b, stream_));
         pending_.push_back(Pending{name, bytes, a, b});
     }
+    // Algorithmic bytes known only after the work ran (device counters).
+    void add_bytes(const char* name, double bytes) {
+        if (profiling_) profile_[name].bytes += bytes;
+    }
     // Synchronizes and folds pending events into the totals.
     void resolve_profile() {
         if (pending_.empty()) return;

@@ -453,6 +453,41 @@ __global__ void __launch_bounds__(kBlockSortThreads) k_affinity_block(AffinityAr
     }
 }
 
+// Hash variant of the block kernel (row weights below 2^32): per-cluster
+// sums in an LDS hash table, then the order-independent top-2 over slots.
+__global__ void __launch_bounds__(kBlockSortThreads) k_affinity_block_hash(AffinityArgs a, const std::int32_t* list,
+                                                                           std::int64_t count) {
+    __shared__ BlockHashSmem s;
+    __shared__ Top2 part[kBlockSortThreads / kWave];
+    for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
+        const std::int32_t v = list[idx];
+        const std::int32_t src = a.clusters[v];
+        if (!a.roles[src]) continue;
+        const std::int64_t b = a.off[v];
+        const int d = (int)(a.off[v + 1] - b);
+        const int slots = hash_slots_for(d);
+        block_hash_clear(s, slots);
+        __syncthreads();
+        for (int x = threadIdx.x; x < d; x += kBlockSortThreads) {
+            const std::int32_t u = a.nbr[b + x];
+            if (u != v) block_hash_add(s, slots, a.clusters[u], (std::uint32_t)ew_at(a.ew, b + x));
+        }
+        __syncthreads();
+        Top2 t{0, 0, 0, 0};
+        for (int i = threadIdx.x; i < slots; i += kBlockSortThreads)
+            if (s.key[i] >= 0) top2_add(a, v, src, s.key[i], s.val[i], t);
+        t = top2_group_reduce<kWave>(t);
+        if (lane_id() == 0) part[threadIdx.x / kWave] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            Top2 r = part[0];
+            for (int i = 1; i < kBlockSortThreads / kWave; ++i) r = top2_merge(r, part[i]);
+            decode_proposal(a, v, src, r);
+        }
+        __syncthreads();
+    }
+}
+
 // Hub rows (> 4096 entries): records (row, cluster, w), radix sort, reduce.
 __global__ void k_hub_records(AffinityArgs a, const std::int32_t* list, std::int64_t count,
                               const std::int64_t* rec_off, int key_bits, std::uint64_t* keys, std::uint32_t* vals) {
@@ -520,6 +555,65 @@ __global__ void __launch_bounds__(kBlockSortThreads) k_chunk_records(Src src, co
                 if (threadIdx.x == 0) obase = total ? atomicAdd(out_count, (unsigned long long)total) : 0ULL;
                 __syncthreads();
             });
+        __syncthreads();
+    }
+}
+
+// Hash variant of k_chunk_records for row weights below 2^32: a chunk's
+// unique keys are gathered from an LDS hash table (output order is
+// arbitrary; the global sort makes it canonical).
+template <class Src>
+__global__ void __launch_bounds__(kBlockSortThreads) k_chunk_records_hash(Src src, const std::int64_t* chunk_off,
+                                                                          std::int64_t count, int key_bits,
+                                                                          std::uint64_t* keys, std::uint32_t* vals,
+                                                                          unsigned long long* out_count) {
+    __shared__ BlockHashSmem s;
+    __shared__ int wsum[kBlockSortThreads / kWave];
+    __shared__ unsigned long long obase;
+    __shared__ std::int64_t row_of;
+    const std::int64_t chunks = chunk_off[count];
+    for (std::int64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+        if (threadIdx.x == 0) {
+            std::int64_t lo = 0, hi = count;  // last i with chunk_off[i] <= ch
+            while (hi - lo > 1) {
+                const std::int64_t mid = lo + (hi - lo) / 2;
+                if (chunk_off[mid] <= ch)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            row_of = lo;
+        }
+        __syncthreads();
+        const std::int64_t li = row_of;
+        const std::int64_t b = src.begin(li), e = src.end(li);
+        const std::int64_t x0 = b + (ch - chunk_off[li]) * kBlockSortMax;
+        const int d = (int)((e - x0) < kBlockSortMax ? (e - x0) : kBlockSortMax);
+        const int slots = hash_slots_for(d);
+        block_hash_clear(s, slots);
+        __syncthreads();
+        for (int x = threadIdx.x; x < d; x += kBlockSortThreads) {
+            const std::int32_t key = src.key(li, x0 + x);
+            if (key != kSentinel) block_hash_add(s, slots, key, src.val(li, x0 + x));
+        }
+        __syncthreads();
+        // Slots are owned in contiguous runs per thread for a block scan.
+        const int per = slots / kBlockSortThreads;
+        const int first = threadIdx.x * per;
+        int mine = 0;
+        for (int i = 0; i < per; ++i) mine += s.key[first + i] >= 0;
+        int inc, before, total;
+        block_scan<PlusOp>(mine, wsum, inc, before, total);
+        if (threadIdx.x == 0) obase = total ? atomicAdd(out_count, (unsigned long long)total) : 0ULL;
+        __syncthreads();
+        unsigned long long o = obase + (unsigned long long)before;
+        for (int i = 0; i < per; ++i) {
+            const std::int32_t key = s.key[first + i];
+            if (key < 0) continue;
+            keys[o] = ((std::uint64_t)li << key_bits) | (std::uint64_t)key;
+            vals[o] = s.val[first + i];
+            ++o;
+        }
         __syncthreads();
     }
 }
@@ -1130,8 +1224,8 @@ static std::int64_t reduce_records(Context& c, DBuf<std::uint64_t>& keys, DBuf<s
 // raw records.
 template <class Src>
 static bool reduce_long_rows(Context& c, const Src& src, std::int64_t count, std::int64_t raw_records, int key_bits,
-                             int total_bits, DBuf<std::uint64_t>& ukeys, DBuf<unsigned long long>& usums,
-                             std::int64_t& nu) {
+                             int total_bits, std::uint64_t row_weight_bound, DBuf<std::uint64_t>& ukeys,
+                             DBuf<unsigned long long>& usums, std::int64_t& nu) {
     DBuf<std::int64_t> chunk_off = c.buf<std::int64_t>(static_cast<std::size_t>(count + 1));
     DBuf<std::int64_t> ctot = c.buf<std::int64_t>(1);
     hipLaunchKernelGGL((k_chunk_counts<Src>), dim3(grid_for(count)), dim3(256), 0, c.stream(), src, count,
@@ -1144,9 +1238,13 @@ static bool reduce_long_rows(Context& c, const Src& src, std::int64_t count, std
     DBuf<std::uint64_t> keys = c.buf<std::uint64_t>(static_cast<std::size_t>(std::max<std::int64_t>(raw_records, 1)));
     DBuf<std::uint32_t> vals = c.buf<std::uint32_t>(static_cast<std::size_t>(std::max<std::int64_t>(raw_records, 1)));
     const std::int64_t max_chunks = raw_records / kBlockSortMax + count;
-    hipLaunchKernelGGL((k_chunk_records<Src>), dim3(grid_for(max_chunks, 1, 4096)), dim3(kBlockSortThreads), 0,
-                       c.stream(), src, chunk_off.get(), count, key_bits, keys.get(), vals.get(), ctr.get(),
-                       ctr.get() + 1);
+    if (row_weight_bound <= 0xffffffffULL)
+        hipLaunchKernelGGL((k_chunk_records_hash<Src>), dim3(grid_for(max_chunks, 1, 4096)), dim3(kBlockSortThreads),
+                           0, c.stream(), src, chunk_off.get(), count, key_bits, keys.get(), vals.get(), ctr.get());
+    else
+        hipLaunchKernelGGL((k_chunk_records<Src>), dim3(grid_for(max_chunks, 1, 4096)), dim3(kBlockSortThreads), 0,
+                           c.stream(), src, chunk_off.get(), count, key_bits, keys.get(), vals.get(), ctr.get(),
+                           ctr.get() + 1);
     GP_LAUNCH_CHECK();
     unsigned long long h[2];
     c.read(h, ctr.get(), sizeof h);
@@ -1302,8 +1400,12 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
         };
         auto block = [&](const std::int32_t* list, std::int64_t count) {
             if (!count) return;
-            hipLaunchKernelGGL(k_affinity_block, dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, s, a, list,
-                               count);
+            if (st.max_row_weight <= 0xffffffffULL)
+                hipLaunchKernelGGL(k_affinity_block_hash, dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, s,
+                                   a, list, count);
+            else
+                hipLaunchKernelGGL(k_affinity_block, dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, s, a,
+                                   list, count);
             GP_LAUNCH_CHECK();
         };
         launch_classes(c, lists.get(), n, cc.c, group, block);
@@ -1312,7 +1414,7 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
             DBuf<unsigned long long> usums;
             std::int64_t nu = 0;
             if (!reduce_long_rows(c, HubSrc{a, hub_list}, hub_count, hub_records, cluster_bits, cluster_bits + hub_bits,
-                                  ukeys, usums, nu)) {
+                                  st.max_row_weight, ukeys, usums, nu)) {
                 DBuf<std::uint64_t> keys = c.buf<std::uint64_t>(static_cast<std::size_t>(hub_records));
                 DBuf<std::uint32_t> vals = c.buf<std::uint32_t>(static_cast<std::size_t>(hub_records));
                 hipLaunchKernelGGL(k_hub_records, dim3(grid_for(hub_count, 1, 4096)), dim3(256), 0, s, a, hub_list,
@@ -1610,8 +1712,8 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
         DBuf<std::uint64_t> ukeys;
         DBuf<unsigned long long> usums;
         std::int64_t nu = 0;
-        if (!reduce_long_rows(c, LongSrc{r, long_list}, long_count, records, key_bits, key_bits + row_bits, ukeys,
-                              usums, nu)) {
+        if (!reduce_long_rows(c, LongSrc{r, long_list}, long_count, records, key_bits, key_bits + row_bits, crb,
+                              ukeys, usums, nu)) {
             DBuf<std::uint64_t> keys = c.buf<std::uint64_t>(static_cast<std::size_t>(records));
             DBuf<std::uint32_t> vals = c.buf<std::uint32_t>(static_cast<std::size_t>(records));
             hipLaunchKernelGGL(k_long_records, dim3(grid_for(long_count, 1, 4096)), dim3(256), 0, s, r, long_list,

@@ -36,6 +36,10 @@ enum Ctr {
     kCtrCount = 9
 };
 
+// Part ids inside refinement: k <= 128 fits a byte, which keeps the label
+// array of a level small enough for the neighbour gathers to hit in cache.
+using Label = std::uint8_t;
+
 struct RepairSample {
     unsigned long long loss;    // modular sum of -gain
     unsigned long long weight;
@@ -54,14 +58,14 @@ struct RefArgs {
     int k;
     std::uint64_t cap;
     double filter;
-    std::int32_t* labels;
+    Label* labels;
     unsigned long long* loads;
     const double* prices;  // null unless vertex weights exceed n
-    std::int32_t* targets;
+    std::int8_t* targets;  // -1: none
     std::int64_t* gains;
-    std::int32_t* locks;
-    std::int32_t* flags;
-    std::int32_t* member;
+    std::uint8_t* locks;
+    std::uint8_t* flags;
+    std::uint8_t* member;
     unsigned long long* ctr;
     RepairSample* samples;
     // repair
@@ -210,7 +214,8 @@ struct AfterBody : RowBody {
             t, a.off[v], a.off[v + 1],
             [&](std::int64_t x) {
                 const std::int32_t u = a.nbr[x];
-                return Rec{u, a.targets[u], a.labels[u], a.gains[u]};
+                const int tu = a.targets[u];
+                return Rec{u, tu, a.labels[u], tu >= 0 ? a.gains[u] : 0};
             },
             [&](std::int64_t x, const Rec& r) {
                 if (r.u == v || r.tu < 0) return;
@@ -301,7 +306,7 @@ __global__ void k_commit(RefArgs a, const std::int32_t* list, const unsigned lon
         const unsigned long long w = a.vw[v];
         atomicAdd(&delta[s], 0ULL - w);
         atomicAdd(&delta[t], w);
-        a.labels[v] = t;
+        a.labels[v] = (Label)t;
         if (lp) a.locks[v] = 1;
         a.member[v] = 1;
     }
@@ -311,7 +316,8 @@ __global__ void k_commit(RefArgs a, const std::int32_t* list, const unsigned lon
 }
 
 // Frontier list from member flags (order irrelevant: per-vertex decisions).
-__global__ void k_compact_flags(std::int64_t n, const std::int32_t* flags, std::int32_t* out,
+template <typename F>
+__global__ void k_compact_flags(std::int64_t n, const F* flags, std::int32_t* out,
                                 unsigned long long* count) {
     __shared__ std::int32_t sb[kTierThreads / kWave][kStage];
     Stage st{sb[threadIdx.x / kWave], out, count, 0};
@@ -1258,13 +1264,15 @@ __global__ void k_joint_accept(RefArgs a, const JMove* m, int count, const std::
 }
 
 // ---------------------------------------------------------- basics
-__global__ void k_project(std::int64_t n, const std::int32_t* map, const std::int32_t* coarse, std::int32_t* fine) {
+template <typename L>
+__global__ void k_project(std::int64_t n, const std::int32_t* map, const L* coarse, L* fine) {
     for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
          v += (std::int64_t)gridDim.x * blockDim.x)
         fine[v] = coarse[map[v]];
 }
+template <typename L>
 __global__ void k_cut(std::int64_t n, const std::int32_t* off, const std::int32_t* nbr, const std::uint32_t* ew,
-                      const std::int32_t* labels, unsigned long long* out) {
+                      const L* labels, unsigned long long* out) {
     const int lane = lane_id();
     unsigned long long local = 0;
     for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane; base < n;
@@ -1294,7 +1302,8 @@ __global__ void k_cut(std::int64_t n, const std::int32_t* off, const std::int32_
     local = wave_sum(local);
     if (lane == 0 && local) atomicAdd(out, local);
 }
-__global__ void k_loads(std::int64_t n, const std::uint32_t* vw, const std::int32_t* labels, unsigned long long* loads) {
+template <typename L>
+__global__ void k_loads(std::int64_t n, const std::uint32_t* vw, const L* labels, unsigned long long* loads) {
     for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
          v += (std::int64_t)gridDim.x * blockDim.x)
         atomicAdd(&loads[labels[v]], (unsigned long long)vw[v]);
@@ -1308,8 +1317,45 @@ __global__ void k_max_u32(std::int64_t n, const std::uint32_t* x, unsigned long 
     if (lane_id() == 0 && m) atomicMax(out, m);
 }
 
+template <typename A, typename B>
+__global__ void k_convert(std::int64_t n, const A* in, B* out) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x)
+        out[v] = (B)in[v];
+}
+
 inline int grid_for(std::int64_t n, int per_block = 256, int cap = 8192) {
     return static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(n, per_block), cap)));
+}
+
+template <typename A, typename B>
+void convert(Context& c, std::int64_t n, const A* in, B* out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_convert<A, B>), dim3(grid_for(n, 256, 4096)), dim3(256), 0, c.stream(), n, in, out);
+    GP_LAUNCH_CHECK();
+}
+
+template <typename L>
+std::uint64_t cut_of(Context& c, const DGraph& g, const L* labels) {
+    DBuf<unsigned long long> out = c.buf<unsigned long long>(1);
+    fill<unsigned long long>(c, out.get(), 1, 0ULL);
+    if (g.n > 0) {
+        hipLaunchKernelGGL((k_cut<L>), dim3(grid_for(g.n)), dim3(256), 0, c.stream(), g.n, g.off.get(), g.nbr.get(),
+                           g.ew_ptr(), labels, out.get());
+        GP_LAUNCH_CHECK();
+    }
+    const unsigned long long d = c.read_scalar(out.get());
+    if (d & 1ULL) throw std::runtime_error("refinement cut is not symmetric");
+    return d / 2;
+}
+
+template <typename L>
+void loads_of(Context& c, const DGraph& g, const L* labels, int k, std::uint64_t* loads_host) {
+    DBuf<unsigned long long> out = c.buf<unsigned long long>(static_cast<std::size_t>(k));
+    fill<unsigned long long>(c, out.get(), k, 0ULL);
+    hipLaunchKernelGGL((k_loads<L>), dim3(grid_for(g.n)), dim3(256), 0, c.stream(), g.n, g.vw.get(), labels, out.get());
+    GP_LAUNCH_CHECK();
+    c.read(loads_host, out.get(), static_cast<std::size_t>(k) * sizeof(unsigned long long));
 }
 
 }  // namespace
@@ -1318,22 +1364,11 @@ inline int grid_for(std::int64_t n, int per_block = 256, int cap = 8192) {
 
 void project(Context& c, std::int64_t n, const std::int32_t* map, const std::int32_t* coarse, std::int32_t* fine) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_project, dim3(grid_for(n)), dim3(256), 0, c.stream(), n, map, coarse, fine);
+    hipLaunchKernelGGL((k_project<std::int32_t>), dim3(grid_for(n)), dim3(256), 0, c.stream(), n, map, coarse, fine);
     GP_LAUNCH_CHECK();
 }
 
-std::uint64_t device_cut(Context& c, const DGraph& g, const std::int32_t* labels) {
-    DBuf<unsigned long long> out = c.buf<unsigned long long>(1);
-    fill<unsigned long long>(c, out.get(), 1, 0ULL);
-    if (g.n > 0) {
-        hipLaunchKernelGGL(k_cut, dim3(grid_for(g.n)), dim3(256), 0, c.stream(), g.n, g.off.get(), g.nbr.get(),
-                           g.ew_ptr(), labels, out.get());
-        GP_LAUNCH_CHECK();
-    }
-    const unsigned long long d = c.read_scalar(out.get());
-    if (d & 1ULL) throw std::runtime_error("refinement cut is not symmetric");
-    return d / 2;
-}
+std::uint64_t device_cut(Context& c, const DGraph& g, const std::int32_t* labels) { return cut_of(c, g, labels); }
 
 std::uint64_t device_edge_weight_sum(Context& c, const DGraph& g) {
     if (!g.ew) return static_cast<std::uint64_t>(g.m);
@@ -1348,18 +1383,14 @@ std::uint64_t device_edge_weight_sum(Context& c, const DGraph& g) {
 }
 
 void device_loads(Context& c, const DGraph& g, const std::int32_t* labels, int k, std::uint64_t* loads_host) {
-    DBuf<unsigned long long> out = c.buf<unsigned long long>(static_cast<std::size_t>(k));
-    fill<unsigned long long>(c, out.get(), k, 0ULL);
-    hipLaunchKernelGGL(k_loads, dim3(grid_for(g.n)), dim3(256), 0, c.stream(), g.n, g.vw.get(), labels, out.get());
-    GP_LAUNCH_CHECK();
-    c.read(loads_host, out.get(), static_cast<std::size_t>(k) * sizeof(unsigned long long));
+    loads_of(c, g, labels, k, loads_host);
 }
 
 namespace {
 
 class LevelRefiner {
 public:
-    LevelRefiner(Context& c, const DGraph& g, std::int32_t* labels, std::vector<std::uint64_t>& loads,
+    LevelRefiner(Context& c, const DGraph& g, Label* labels, std::vector<std::uint64_t>& loads,
                  const RefineOptions& o, std::uint64_t cap, LevelStats& st)
         : c_(c), g_(g), loads_host_(loads), o_(o), st_(st) {
         const std::int64_t n = g.n;
@@ -1373,11 +1404,11 @@ public:
         a_.k = k;
         a_.cap = cap;
         a_.labels = labels;
-        targets_ = c.buf<std::int32_t>(n);
+        targets_ = c.buf<std::int8_t>(n);
         gains_ = c.buf<std::int64_t>(n);
-        locks_ = c.buf<std::int32_t>(n);
-        flags_ = c.buf<std::int32_t>(n);
-        member_ = c.buf<std::int32_t>(n);
+        locks_ = c.buf<std::uint8_t>(n);
+        flags_ = c.buf<std::uint8_t>(n);
+        member_ = c.buf<std::uint8_t>(n);
         frontier_ = c.buf<std::int32_t>(n);
         cand_ = c.buf<std::int32_t>(n);
         movers_ = c.buf<std::int32_t>(n);
@@ -1448,7 +1479,7 @@ public:
         const hipStream_t s = c_.stream();
         const std::int64_t n = a_.n;
         fill<unsigned long long>(c_, ctr_.get(), kCtrCount, 0ULL);
-        hipLaunchKernelGGL(k_compact_flags, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, n, member_.get(),
+        hipLaunchKernelGGL(k_compact_flags<std::uint8_t>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, n, member_.get(),
                            frontier_.get(), ctr_.get() + kFrontier);
         GP_LAUNCH_CHECK();
         // Algorithmic bytes per scanned row: offsets 8, label and weight 8,
@@ -1661,11 +1692,11 @@ public:
                                owners.get(), events.get());
             hipLaunchKernelGGL(k_joint_prefix, dim3(ceil_div(k, 128)), dim3(128), 0, s, a_, moves.get(), active,
                                events.get(), stops.get());
-            fill<std::int32_t>(c_, flags_.get(), n, 0);
+            fill<std::uint8_t>(c_, flags_.get(), n, 0);
             hipLaunchKernelGGL(k_joint_accept, dim3(ceil_div(active, 256)), dim3(256), 0, s, a_, moves.get(), active,
                                stops.get());
             fill<unsigned long long>(c_, ctr_.get() + kMovers, 1, 0ULL);
-            hipLaunchKernelGGL(k_compact_flags, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, n, flags_.get(),
+            hipLaunchKernelGGL(k_compact_flags<std::uint8_t>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, n, flags_.get(),
                                movers_.get(), ctr_.get() + kMovers);
             GP_LAUNCH_CHECK();
             const unsigned long long cnt = c_.read_scalar(ctr_.get() + kMovers);
@@ -1759,7 +1790,9 @@ public:
     const RefineOptions& o_;
     LevelStats& st_;
     RefArgs a_;
-    DBuf<std::int32_t> targets_, locks_, flags_, member_, frontier_, cand_, movers_, medium_, heavy_, bucket_;
+    DBuf<std::int8_t> targets_;
+    DBuf<std::uint8_t> locks_, flags_, member_;
+    DBuf<std::int32_t> frontier_, cand_, movers_, medium_, heavy_, bucket_;
     DBuf<std::int64_t> gains_;
     DBuf<unsigned long long> ctr_, loads0_, counts_;
     DBuf<double> prices_;
@@ -1768,16 +1801,18 @@ public:
 
 }  // namespace
 
-LevelStats refine_level(Context& c, const DGraph& g, std::int32_t* labels, std::vector<std::uint64_t>& loads,
-                        const RefineOptions& o, std::uint64_t cap, int rounds, bool has_known_cut,
-                        std::uint64_t known_cut) {
+namespace {
+
+LevelStats refine_level_impl(Context& c, const DGraph& g, Label* labels, std::vector<std::uint64_t>& loads,
+                             const RefineOptions& o, std::uint64_t cap, int rounds, bool has_known_cut,
+                             std::uint64_t known_cut) {
     LevelStats st;
     st.n = g.n;
     st.m = g.m;
     st.refined = 1;
     const int k = o.parts;
     LevelRefiner R(c, g, labels, loads, o, cap, st);
-    std::uint64_t cut = has_known_cut ? known_cut : device_cut(c, g, labels);
+    std::uint64_t cut = has_known_cut ? known_cut : cut_of(c, g, labels);
     st.initial_cut = cut;
     if (R.max_load() > cap) {
         R.reset_samples();
@@ -1789,9 +1824,9 @@ LevelStats refine_level(Context& c, const DGraph& g, std::int32_t* labels, std::
     }
     const std::uint64_t feasible_start = cut;
     std::uint64_t best_cut = cut;
-    DBuf<std::int32_t> best_labels = c.buf<std::int32_t>(static_cast<std::size_t>(g.n));
+    DBuf<Label> best_labels = c.buf<Label>(static_cast<std::size_t>(g.n));
     std::vector<std::uint64_t> best_loads = loads;
-    c.copy_d2d(best_labels.get(), labels, static_cast<std::size_t>(g.n) * sizeof(std::int32_t));
+    c.copy_d2d(best_labels.get(), labels, static_cast<std::size_t>(g.n) * sizeof(Label));
     const bool graph_mode = g.n >= o.graph_loop_threshold;
     int stale = 0, empty = 0;
     for (int phase = 0; phase < rounds && stale < o.stale_rounds; ++phase) {
@@ -1816,17 +1851,17 @@ LevelStats refine_level(Context& c, const DGraph& g, std::int32_t* labels, std::
             } else if ((long double)cut < 0.999L * (long double)best_cut)
                 stale = 0;
             best_cut = cut;
-            c.copy_d2d(best_labels.get(), labels, static_cast<std::size_t>(g.n) * sizeof(std::int32_t));
+            c.copy_d2d(best_labels.get(), labels, static_cast<std::size_t>(g.n) * sizeof(Label));
             best_loads = loads;
         }
     }
-    c.copy_d2d(labels, best_labels.get(), static_cast<std::size_t>(g.n) * sizeof(std::int32_t));
+    c.copy_d2d(labels, best_labels.get(), static_cast<std::size_t>(g.n) * sizeof(Label));
     loads = best_loads;
     c.upload(R.loads_dev(), loads.data(), static_cast<std::size_t>(k) * sizeof(std::uint64_t));
     cut = best_cut;
     if (rounds > 0 && o.exchange_rounds > 0 && g.n <= o.exchange_max_n) {
-        fill<std::int32_t>(c, R.flags_.get(), g.n, 0);
-        fill<std::int32_t>(c, R.targets_.get(), g.n, -1);
+        fill<std::uint8_t>(c, R.flags_.get(), g.n, 0);
+        fill<std::int8_t>(c, R.targets_.get(), g.n, (std::int8_t)-1);
         R.exchange(o.exchange_rounds, cut, best_cut);
     }
     st.final_cut = cut;
@@ -1837,28 +1872,44 @@ LevelStats refine_level(Context& c, const DGraph& g, std::int32_t* labels, std::
     return st;
 }
 
+}  // namespace
+
+LevelStats refine_level(Context& c, const DGraph& g, std::int32_t* labels, std::vector<std::uint64_t>& loads,
+                        const RefineOptions& o, std::uint64_t cap, int rounds, bool has_known_cut,
+                        std::uint64_t known_cut) {
+    DBuf<Label> lab = c.buf<Label>(static_cast<std::size_t>(g.n));
+    convert(c, g.n, labels, lab.get());
+    LevelStats st = refine_level_impl(c, g, lab.get(), loads, o, cap, rounds, has_known_cut, known_cut);
+    convert(c, g.n, lab.get(), labels);
+    return st;
+}
+
 UncoarsenResult uncoarsen(Context& c, const Hierarchy& h, const std::int32_t* coarsest_labels, const RefineOptions& o) {
     const int k = o.parts;
     UncoarsenResult res;
     const DGraph& cg = h.level(h.levels() - 1);
-    DBuf<std::int32_t> labels = c.buf<std::int32_t>(static_cast<std::size_t>(cg.n));
-    c.copy_d2d(labels.get(), coarsest_labels, static_cast<std::size_t>(cg.n) * sizeof(std::int32_t));
+    DBuf<Label> labels = c.buf<Label>(static_cast<std::size_t>(cg.n));
+    convert(c, cg.n, coarsest_labels, labels.get());
     std::vector<std::uint64_t> loads(static_cast<std::size_t>(k), 0);
-    device_loads(c, cg, labels.get(), k, loads.data());
+    loads_of(c, cg, labels.get(), k, loads.data());
     std::uint64_t total = 0;
     for (auto x : loads) total += x;
     const long double raw = (long double)total * o.imbalance / k;
     const std::uint64_t cap = (std::uint64_t)std::floor(raw);
     if (cap < total / (unsigned)k + (total % (unsigned)k != 0))
         throw std::invalid_argument("integer partition capacity cannot hold total weight");
-    LevelStats st = refine_level(c, cg, labels.get(), loads, o, cap, o.max_rounds, false, 0);
+    LevelStats st = refine_level_impl(c, cg, labels.get(), loads, o, cap, o.max_rounds, false, 0);
     res.levels.push_back(st);
     std::uint64_t last_cut = st.final_cut;
     std::int64_t last_refined = cg.n;
     for (int lvl = h.levels() - 2; lvl >= 0; --lvl) {
         const DGraph& fg = h.level(lvl);
-        DBuf<std::int32_t> fl = c.buf<std::int32_t>(static_cast<std::size_t>(fg.n));
-        project(c, fg.n, h.maps[static_cast<std::size_t>(lvl)].get(), labels.get(), fl.get());
+        DBuf<Label> fl = c.buf<Label>(static_cast<std::size_t>(fg.n));
+        if (fg.n > 0) {
+            hipLaunchKernelGGL((k_project<Label>), dim3(grid_for(fg.n)), dim3(256), 0, c.stream(), fg.n,
+                               h.maps[static_cast<std::size_t>(lvl)].get(), labels.get(), fl.get());
+            GP_LAUNCH_CHECK();
+        }
         labels = std::move(fl);
         std::uint64_t maxload = 0;
         for (auto x : loads) maxload = std::max(maxload, x);
@@ -1867,7 +1918,7 @@ UncoarsenResult uncoarsen(Context& c, const Hierarchy& h, const std::int32_t* co
         ls.n = fg.n;
         ls.m = fg.m;
         if (refine) {
-            ls = refine_level(c, fg, labels.get(), loads, o, cap, o.max_rounds, true, last_cut);
+            ls = refine_level_impl(c, fg, labels.get(), loads, o, cap, o.max_rounds, true, last_cut);
             if (!ls.initial_repair_failed) last_refined = fg.n;
         } else {
             ls.initial_cut = ls.final_cut = last_cut;
@@ -1878,7 +1929,9 @@ UncoarsenResult uncoarsen(Context& c, const Hierarchy& h, const std::int32_t* co
     std::uint64_t maxload = 0;
     for (auto x : loads) maxload = std::max(maxload, x);
     if (maxload > cap) throw std::runtime_error("final original partition violates capacity");
-    res.labels = std::move(labels);
+    const DGraph& g0 = h.level(0);
+    res.labels = c.buf<std::int32_t>(static_cast<std::size_t>(g0.n));
+    convert(c, g0.n, labels.get(), res.labels.get());
     res.loads = loads;
     res.cut = last_cut;
     return res;

@@ -259,8 +259,9 @@ void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int6
     hipLaunchKernelGGL((k_tier_medium<Body>), dim3(medium_blocks), dim3(kTierThreads),
                        (kTierThreads / kWave) * scratch * sizeof(unsigned long long), s, body, t, scratch);
     GP_LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_tier_heavy<Body>), dim3(256), dim3(kTierThreads), scratch * sizeof(unsigned long long), s, body,
-                       t);
+    // Heavy rows are few but long; blocks beyond the row count exit at once.
+    hipLaunchKernelGGL((k_tier_heavy<Body>), dim3(1024), dim3(kTierThreads), scratch * sizeof(unsigned long long), s,
+                       body, t);
     GP_LAUNCH_CHECK();
 }
 

@@ -175,7 +175,9 @@ public:
     hipStream_t stream() const { return stream_; }
     Arena* arena() { return arena_; }
 
-    // Lanes per row team for short rows (4, 8 or 16; GP_LIGHT_LANES).
+    // Lanes per row team for short rows (4, 8 or 16; GP_LIGHT_LANES).  Four
+    // keeps the most rows in flight per wave, which the dependent gathers of
+    // short rows need.
     int light_lanes() const { return light_lanes_; }
     void set_light_lanes(int lanes) {
         if (lanes != 4 && lanes != 8 && lanes != 16) throw std::invalid_argument("light lanes must be 4, 8 or 16");
@@ -278,7 +280,7 @@ private:
     hipStream_t stream_ = nullptr;
     Arena* arena_ = nullptr;
     void* pinned_ = nullptr;
-    int light_lanes_ = 16;
+    int light_lanes_ = 4;
 };
 
 }  // namespace gp

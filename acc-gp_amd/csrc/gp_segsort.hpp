@@ -177,4 +177,41 @@ __device__ inline int block_sort_reduce(BlockSortSmem& s, int n, Emit&& emit, Be
     return total;
 }
 
+// Block-wide aggregation of (key >= 0, u32 weight) pairs in an LDS hash
+// table with linear probing.  Sums are exact while every row's total weight
+// stays below 2^32 (callers check the level's row-weight bound).  Slot order
+// is arbitrary: callers use it only where the result is order-independent.
+constexpr int kHashSlots = 8192;
+
+struct BlockHashSmem {
+    std::int32_t key[kHashSlots];
+    std::uint32_t val[kHashSlots];
+};
+
+// Table size for n entries: a power of two >= 2n (at least 512).
+__device__ inline int hash_slots_for(int n) {
+    int h = 512;
+    while (h < 2 * n) h <<= 1;
+    return h;
+}
+
+__device__ inline void block_hash_clear(BlockHashSmem& s, int slots) {
+    for (int i = threadIdx.x; i < slots; i += blockDim.x) {
+        s.key[i] = -1;
+        s.val[i] = 0;
+    }
+}
+
+__device__ inline void block_hash_add(BlockHashSmem& s, int slots, std::int32_t key, std::uint32_t w) {
+    unsigned slot = mix32((std::uint32_t)key) & (unsigned)(slots - 1);
+    while (true) {
+        const std::int32_t prev = atomicCAS(&s.key[slot], -1, key);
+        if (prev == -1 || prev == key) {
+            atomicAdd(&s.val[slot], w);
+            return;
+        }
+        slot = (slot + 1) & (unsigned)(slots - 1);
+    }
+}
+
 }  // namespace gp

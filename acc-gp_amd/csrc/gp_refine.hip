@@ -12,6 +12,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <type_traits>
 
 namespace gp {
 namespace {
@@ -103,6 +104,54 @@ __device__ inline std::uint64_t team_histogram(const Team& t, const RefArgs& a, 
     return t.sum(total);
 }
 
+// Short rows (light teams of at most 16 lanes, at most kTierLight entries):
+// the row's labels stay in registers and only their bins are touched, so a
+// table that starts zero is left zero and no per-row clear of all k bins is
+// needed.  The light kernel zeroes the tables once at start.
+template <int W>
+struct LightRow {
+    static constexpr int kPer = kTierLight / W;
+    int lab[kPer];  // -1: no entry or a self loop
+    std::uint32_t w[kPer];
+};
+
+template <int W>
+__device__ inline std::uint64_t light_histogram(const GroupTeam<W>& t, const RefArgs& a, std::int64_t v,
+                                                unsigned long long* hist, LightRow<W>& r) {
+    const std::int64_t b = a.off[v], e = a.off[v + 1];
+#pragma unroll
+    for (int j = 0; j < LightRow<W>::kPer; ++j) {
+        const std::int64_t x = b + t.rank() + j * W;
+        const bool in = x < e;
+        const std::int32_t u = in ? a.nbr[x] : (std::int32_t)v;
+        r.lab[j] = u != v ? (int)a.labels[u] : -1;
+        r.w[j] = in ? (std::uint32_t)ew_at(a.ew, x) : 0u;
+    }
+    std::uint64_t total = 0;
+    t.sync();  // the previous row's clean-up is done
+#pragma unroll
+    for (int j = 0; j < LightRow<W>::kPer; ++j)
+        if (r.lab[j] >= 0) {
+            atomicAdd(&hist[r.lab[j]], (unsigned long long)r.w[j]);
+            total += r.w[j];
+        }
+    t.sync();
+    return t.sum(total);
+}
+
+template <int W>
+__device__ inline void light_clear(const GroupTeam<W>& t, unsigned long long* hist, const LightRow<W>& r) {
+    t.sync();  // all reads of the row's bins are done
+#pragma unroll
+    for (int j = 0; j < LightRow<W>::kPer; ++j)
+        if (r.lab[j] >= 0) hist[r.lab[j]] = 0;
+}
+
+template <class Team>
+struct IsLight : std::false_type {};
+template <int W>
+struct IsLight<GroupTeam<W>> : std::integral_constant<bool, (W <= 16)> {};
+
 // estimated_repair_cost (src/refinement/incremental.cuh:156-159).
 __device__ inline double repair_cost(const RefArgs& a, int part, std::uint64_t weight) {
     const std::uint64_t slack = a.cap - a.loads[part];
@@ -174,20 +223,34 @@ struct LpProposeBody : RowBody {
             a.locks[v] = 0;
         }
         if (locked) return RowOut{false, 0};
-        const std::uint64_t total = team_histogram(t, a, v, hist);
-        const int s = a.labels[v];
-        const std::uint64_t current = hist[s];
-        if (current == total) return RowOut{false, degree(v)};
-        const std::uint64_t w = a.vw[v];
         const bool priced = a.prices != nullptr;
+        const std::uint64_t w = a.vw[v];
+        const int s = a.labels[v];
         Cand best{-1, 0, 0.0};
-        for (int p = t.rank(); p < a.k; p += Team::size) {
-            const std::uint64_t c = hist[p];
-            if (p == s || !c) continue;
+        std::uint64_t total, current;
+        auto consider = [&](int p, std::uint64_t c) {
+            if (p == s || !c) return;
             Cand x{p, c, priced ? (double)c - repair_cost(a, p, w) : 0.0};
             if (cand_better(x, best, priced)) best = x;
+        };
+        if constexpr (IsLight<Team>::value) {
+            LightRow<Team::size> r;
+            total = light_histogram(t, a, v, hist, r);
+            current = hist[s];
+            if (current != total)
+#pragma unroll
+                for (int j = 0; j < LightRow<Team::size>::kPer; ++j)
+                    if (r.lab[j] >= 0) consider(r.lab[j], hist[r.lab[j]]);
+            best = team_reduce(t, best, priced);
+            light_clear(t, hist, r);
+        } else {
+            total = team_histogram(t, a, v, hist);
+            current = hist[s];
+            if (current != total)
+                for (int p = t.rank(); p < a.k; p += Team::size) consider(p, hist[p]);
+            best = team_reduce(t, best, priced);
         }
-        best = team_reduce(t, best, priced);
+        if (current == total) return RowOut{false, degree(v)};
         const bool accept = best.p >= 0 && (best.conn >= current ||
                                             current - best.conn < (std::uint64_t)(a.filter * (double)current));
         if (accept && t.leader()) {
@@ -449,20 +512,32 @@ __device__ inline void repair_finish(const RefArgs& a, std::int64_t v, int s, st
 struct RepairProposeBody : RowBody {
     template <class Team>
     __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long* hist) const {
-        const std::uint64_t total = team_histogram(t, a, v, hist);
         const int s = a.labels[v];
         const std::uint64_t w = a.vw[v];
-        const std::uint64_t current = hist[s];
         RepairPick best{-1, 0, 0};
-        if (current != total)
-            for (int p = t.rank(); p < a.k; p += Team::size) {
-                if (p == s || a.loads0[p] >= a.cap || w > a.cap - a.loads0[p]) continue;
-                const std::uint64_t c = hist[p];
-                if (!c) continue;
-                RepairPick x{p, c, a.cap - a.loads0[p]};
-                if (pick_better(x, best)) best = x;
-            }
-        best = team_reduce(t, best);
+        auto consider = [&](int p, std::uint64_t c) {
+            if (p == s || !c || a.loads0[p] >= a.cap || w > a.cap - a.loads0[p]) return;
+            RepairPick x{p, c, a.cap - a.loads0[p]};
+            if (pick_better(x, best)) best = x;
+        };
+        std::uint64_t total, current;
+        if constexpr (IsLight<Team>::value) {
+            LightRow<Team::size> r;
+            total = light_histogram(t, a, v, hist, r);
+            current = hist[s];
+            if (current != total)
+#pragma unroll
+                for (int j = 0; j < LightRow<Team::size>::kPer; ++j)
+                    if (r.lab[j] >= 0) consider(r.lab[j], hist[r.lab[j]]);
+            best = team_reduce(t, best);
+            light_clear(t, hist, r);
+        } else {
+            total = team_histogram(t, a, v, hist);
+            current = hist[s];
+            if (current != total)
+                for (int p = t.rank(); p < a.k; p += Team::size) consider(p, hist[p]);
+            best = team_reduce(t, best);
+        }
         if (t.leader()) repair_finish(a, v, s, w, current, best);
         return RowOut{false, 0};
     }
@@ -470,25 +545,41 @@ struct RepairProposeBody : RowBody {
 
 // Per overloaded part: exclusive prefix over the 800 buckets and the bucket
 // that contains the excess threshold (balance_scan_buckets, incremental.cuh:628-652).
-__global__ void k_bucket_scan(RefArgs a, unsigned long long* prefix, std::int32_t* bstar, unsigned long long* excess) {
+// One block per part; the first bucket whose range contains the excess is
+// the minimum over buckets that satisfy the test.
+__global__ void __launch_bounds__(kScanThreads) k_bucket_scan(RefArgs a, unsigned long long* prefix,
+                                                             std::int32_t* bstar, unsigned long long* excess) {
+    __shared__ unsigned long long wsum[kScanThreads / kWave];
+    __shared__ int star;
     const int p = blockIdx.x;
-    if (threadIdx.x != 0) return;
     if (!(a.loads0[p] > a.cap)) {
-        bstar[p] = -1;
-        excess[p] = 0;
+        if (threadIdx.x == 0) {
+            bstar[p] = -1;
+            excess[p] = 0;
+        }
         return;
     }
     const unsigned long long ex = a.loads0[p] - a.cap;
-    unsigned long long sum = 0;
-    int star = kBalanceWidth;
-    for (int b = 0; b < kBalanceWidth; ++b) {
-        prefix[p * kBalanceWidth + b] = sum;
-        const unsigned long long cnt = a.counts[p * kBalanceWidth + b];
-        if (star == kBalanceWidth && cnt && sum < ex && ex <= sum + cnt) star = b;
-        sum += cnt;
+    if (threadIdx.x == 0) star = kBalanceWidth;
+    __syncthreads();
+    unsigned long long carry = 0;
+    for (int b0 = 0; b0 < kBalanceWidth; b0 += kScanThreads) {
+        const int b = b0 + threadIdx.x;
+        const unsigned long long cnt = b < kBalanceWidth ? a.counts[p * kBalanceWidth + b] : 0;
+        unsigned long long inc, exc, tot;
+        block_scan<PlusOp>(cnt, wsum, inc, exc, tot);
+        const unsigned long long sum = carry + exc;
+        if (b < kBalanceWidth) {
+            prefix[p * kBalanceWidth + b] = sum;
+            if (cnt && sum < ex && ex <= sum + cnt) atomicMin(&star, b);
+        }
+        carry += tot;
     }
-    bstar[p] = star;
-    excess[p] = ex;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bstar[p] = star;
+        excess[p] = ex;
+    }
 }
 
 __global__ void k_repair_flag(RefArgs a, const std::int32_t* bstar, std::uint8_t* f) {
@@ -500,12 +591,14 @@ __global__ void k_repair_flag(RefArgs a, const std::int32_t* bstar, std::uint8_t
 }
 
 template <int W>
-__global__ void k_conn_rows(RefArgs a, const std::int32_t* list, std::int64_t count, unsigned long long* rows) {
+__global__ void k_conn_rows(RefArgs a, const std::int32_t* list, std::int64_t count, const std::uint8_t* skip,
+                            unsigned long long* rows) {
     extern __shared__ unsigned long long smem[];
     const int lg = threadIdx.x % W, gi = threadIdx.x / W;
     unsigned long long* hist = smem + gi * a.k;
     const std::int64_t gpb = blockDim.x / W;
     for (std::int64_t idx = blockIdx.x * gpb + gi; idx < count; idx += (std::int64_t)gridDim.x * gpb) {
+        if (skip && skip[idx]) continue;
         const std::int64_t v = list[idx];
         team_histogram(GroupTeam<W>{lg}, a, v, hist);
         for (int p = lg; p < a.k; p += W) rows[idx * a.k + p] = hist[p];
@@ -1545,10 +1638,9 @@ public:
         DBuf<std::int32_t> E = c_.buf<std::int32_t>(static_cast<std::size_t>(ne));
         hipLaunchKernelGGL(k_gather_i32, dim3(grid_for(ne)), dim3(256), 0, s, L, idx.get(), ne, E.get());
         GP_LAUNCH_CHECK();
+        // Connection rows are read from the second attempt on, and only for
+        // donors still waiting; they are built once attempt 0 is done.
         DBuf<unsigned long long> conn = c_.buf<unsigned long long>(static_cast<std::size_t>(ne) * k);
-        hipLaunchKernelGGL((k_conn_rows<kWave>), dim3(grid_for(ne, 4, 2048)), dim3(256), 4 * k * sizeof(unsigned long long),
-                           s, a_, E.get(), ne, conn.get());
-        GP_LAUNCH_CHECK();
         DBuf<unsigned long long> requests = c_.buf<unsigned long long>(static_cast<std::size_t>(k));
         DBuf<unsigned long long> slack = c_.buf<unsigned long long>(static_cast<std::size_t>(k));
         DBuf<std::uint8_t> done = c_.buf<std::uint8_t>(static_cast<std::size_t>(ne));
@@ -1560,6 +1652,11 @@ public:
         const int nb = static_cast<int>(ceil_div(ne, kKeyTile));
         ReserveState r{E.get(), ne, conn.get(), requests.get(), slack.get(), done.get(), tried.get()};
         for (int attempt = 0; attempt < k; ++attempt) {
+            if (attempt == 1) {
+                hipLaunchKernelGGL((k_conn_rows<kWave>), dim3(grid_for(ne, 4, 2048)), dim3(256),
+                                   4 * k * sizeof(unsigned long long), s, a_, E.get(), ne, done.get(), conn.get());
+                GP_LAUNCH_CHECK();
+            }
             fill<unsigned long long>(c_, ctr_.get() + kActive, 1, 0ULL);
             hipLaunchKernelGGL(k_choose, dim3(grid_for(ne, kTierThreads / 16, 4096)), dim3(kTierThreads), 0, s, a_, r,
                                attempt, keys.get(), ctr_.get() + kActive);
@@ -1599,7 +1696,8 @@ public:
                                ctr_.get() + kDonors);
             GP_LAUNCH_CHECK();
             tiers(RepairProposeBody{{a_}}, cand_.get(), 0, ctr_.get() + kDonors, nullptr, nullptr, nullptr, a_.k);
-            hipLaunchKernelGGL(k_bucket_scan, dim3(k), dim3(64), 0, s, a_, prefix.get(), bstar.get(), excess.get());
+            hipLaunchKernelGGL(k_bucket_scan, dim3(k), dim3(kScanThreads), 0, s, a_, prefix.get(), bstar.get(),
+                               excess.get());
             GP_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_repair_flag, dim3(grid_for(n)), dim3(256), 0, s, a_, bstar.get(), f.get());
             GP_LAUNCH_CHECK();

@@ -154,6 +154,9 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const st
     unsigned long long* scratch = smem + (threadIdx.x / W) * scratch_per_group;
     Stage defer_stage{stage_buf[threadIdx.x / kWave][0], t.medium, t.medium_count, 0};
     Stage out_stage{stage_buf[threadIdx.x / kWave][1], t.out, t.out_count, 0};
+    // Team tables start zero; bodies that use them leave them zero.
+    for (int i = threadIdx.x; i < (kTierThreads / W) * scratch_per_group; i += kTierThreads) smem[i] = 0;
+    __syncthreads();
     const std::int64_t count = count_ptr ? (std::int64_t)*count_ptr : n;
     const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
     const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;

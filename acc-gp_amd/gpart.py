@@ -52,6 +52,16 @@ class PartitionStats(ctypes.Structure):
     ]
 
 
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int64), ("m", ctypes.c_int64), ("offsets", ctypes.c_void_p), ("neighbors", ctypes.c_void_p),
+        ("edge_weights", ctypes.c_void_p), ("vertex_weights", ctypes.c_void_p), ("parts", ctypes.c_int),
+        ("imbalance", ctypes.c_double), ("seed", ctypes.c_uint32), ("warm_labels", ctypes.c_void_p),
+        ("labels", ctypes.c_void_p), ("stats", PartitionStats), ("status", ctypes.c_int),
+        ("queue_seconds", ctypes.c_double), ("device_seconds", ctypes.c_double), ("error", ctypes.c_char * 256),
+    ]
+
+
 # Every symbol declared in include/gpart_mi355x.h.
 EXPORTS = [
     "gp_last_error", "gp_context_create", "gp_context_destroy", "gp_context_reserved_bytes",
@@ -62,7 +72,8 @@ EXPORTS = [
     "gp_result_labels", "gp_result_cut", "gp_result_loads", "gp_result_levels", "gp_result_level_stats",
     "gp_result_destroy", "gp_refine_partition", "gp_partition", "gp_generate_grid", "gp_generate_rmat",
     "gp_generate_rgg2d", "gp_generate_knn3d", "gp_free_host", "gp_load_csr", "gp_save_csr",
-    "gp_profile_enable", "gp_profile_query", "gp_profile_reset",
+    "gp_profile_enable", "gp_profile_query", "gp_profile_reset", "gp_server_create", "gp_server_submit",
+    "gp_server_wait", "gp_server_destroy", "gp_partition_batch",
 ]
 
 
@@ -116,6 +127,11 @@ def lib():
         L.gp_profile_enable.argtypes = [vp, i32]
         L.gp_profile_query.argtypes = [vp, ctypes.c_char_p, vp, vp, vp]
         L.gp_profile_reset.argtypes = [vp]
+        L.gp_server_create.argtypes = [i32, i32, u64, pp]
+        L.gp_server_submit.argtypes = [vp, vp, pp]
+        L.gp_server_wait.argtypes = [vp, vp]
+        L.gp_server_destroy.argtypes = [vp]
+        L.gp_partition_batch.argtypes = [i32, vp, i64, i32, u64]
         _LIB = L
     return _LIB
 
@@ -337,3 +353,69 @@ class Hierarchy:
             return out, lib().gp_result_cut(r), loads, stats
         finally:
             lib().gp_result_destroy(r)
+
+
+class BatchItem:
+    """One request of a batch or of the server: a host CSR and its options."""
+
+    def __init__(self, off, nbr, parts, imbalance=1.03, seed=0, ew=None, vw=None, warm=None):
+        self.off = np.ascontiguousarray(off, np.int32)
+        self.nbr = np.ascontiguousarray(nbr, np.int32)
+        self.ew = None if ew is None else np.ascontiguousarray(ew, np.uint32)
+        self.vw = None if vw is None else np.ascontiguousarray(vw, np.uint32)
+        self.warm = None if warm is None else np.ascontiguousarray(warm, np.int32)
+        self.labels = np.zeros(len(self.off) - 1, np.int32)
+        self.parts, self.imbalance, self.seed = parts, imbalance, seed
+
+    def fill(self, req):
+        req.n, req.m = len(self.off) - 1, len(self.nbr)
+        req.offsets, req.neighbors = _p(self.off), _p(self.nbr)
+        req.edge_weights, req.vertex_weights = _p(self.ew), _p(self.vw)
+        req.parts, req.imbalance, req.seed = self.parts, self.imbalance, self.seed
+        req.warm_labels, req.labels = _p(self.warm), _p(self.labels)
+
+
+def _stats_dict(req):
+    return dict((f, getattr(req.stats, f)) for f, _ in PartitionStats._fields_)
+
+
+def partition_batch(items, workers=4, hbm_budget=0, device=0):
+    """Partitions BatchItems concurrently; returns per-item stats dicts."""
+    reqs = (Request * len(items))()
+    for it, r in zip(items, reqs):
+        it.fill(r)
+    rc = lib().gp_partition_batch(device, ctypes.cast(reqs, ctypes.c_void_p), len(items), workers, hbm_budget)
+    for r in reqs:
+        if r.status != GP_OK:
+            raise RuntimeError("gpart batch item error %d: %s" % (r.status, r.error.decode()))
+    check(rc)
+    return [dict(_stats_dict(r), queue_seconds=r.queue_seconds, device_seconds=r.device_seconds) for r in reqs]
+
+
+class Server:
+    """Continuous-batching request loop (gp_server_*)."""
+
+    def __init__(self, workers=4, hbm_budget=0, device=0):
+        self.h = ctypes.c_void_p()
+        check(lib().gp_server_create(device, workers, hbm_budget, ctypes.byref(self.h)))
+        self._live = {}
+
+    def submit(self, item):
+        req = Request()
+        item.fill(req)
+        t = ctypes.c_void_p()
+        check(lib().gp_server_submit(self.h, ctypes.byref(req), ctypes.byref(t)))
+        self._live[t.value] = (req, item)
+        return t.value
+
+    def wait(self, ticket):
+        req, item = self._live.pop(ticket)
+        rc = lib().gp_server_wait(self.h, ctypes.c_void_p(ticket))
+        if rc != GP_OK:
+            raise RuntimeError("gpart request error %d: %s" % (rc, req.error.decode()))
+        return dict(_stats_dict(req), queue_seconds=req.queue_seconds, device_seconds=req.device_seconds)
+
+    def close(self):
+        if self.h:
+            lib().gp_server_destroy(self.h)
+            self.h = None

@@ -31,6 +31,9 @@ def main():
     ap.add_argument("--parts", type=int, default=64)
     ap.add_argument("--cpu-scale", type=int, default=19, help="R-MAT scale of the bounded CPU-oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-graphs", type=int, default=256, help="graphs in the batched mix (0: skip)")
+    ap.add_argument("--batch-workers", type=int, default=8)
+    ap.add_argument("--serve-requests", type=int, default=128, help="requests of the serving leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; multi-GPU sharding is a design note, not a run.",
@@ -121,12 +124,159 @@ def main():
         out[key] = {"bound": "hbm", "kernel": kernels, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(name), "avg_launch_us": avg_s * 1e6,
                     "algorithmic_bytes_per_launch": byts / launches, "launches": launches}
+    if args.batch_graphs > 0:
+        out["batched"], mix, items = batched_leg(gp, ctx, args.batch_graphs, args.batch_workers)
+        if args.serve_requests > 0:
+            out["serving"] = serving_leg(gp, ctx, mix, items, args.batch_workers, args.serve_requests,
+                                         0.7 * out["batched"]["graphs"] / out["batched"]["seconds"])
     if not args.no_cpu:
         from oracle import oracle as og
         out["cpu_baseline"] = cpu_baseline(gp, og, args.cpu_scale, args.parts)
     g.close()
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def batch_mix(gp, count, seed=5):
+    """Mixed-size workload: 1/3 grids (side 32-1024), 1/3 R-MAT (scale 10-18,
+    edge factor 8-16), 1/3 random geometric 2-D graphs (2^10-2^20 points,
+    mean degree 8); k drawn from {4, 8, 16, 32, 64}; eps 1.03."""
+    rng = np.random.default_rng(seed)
+    mix = []
+    for i in range(count):
+        k = int(rng.choice([4, 8, 16, 32, 64]))
+        kind = i % 3
+        if kind == 0:
+            side = int(rng.integers(32, 1025))
+            off, nbr = gp.generate_grid(side, side)
+        elif kind == 1:
+            off, nbr = gp.generate_rmat(int(rng.integers(10, 19)), int(rng.integers(8, 17)), 0.57, 0.19, 0.19,
+                                        seed=int(rng.integers(1, 1 << 30)))
+        else:
+            n = int(round(2.0 ** rng.uniform(10, 20)))
+            off, nbr = gp.generate_rgg2d(n, (8.0 / (np.pi * n)) ** 0.5, seed=int(rng.integers(1, 1 << 30)))
+        if len(off) - 1 >= 2 * k:
+            mix.append((off, nbr, k))
+    return mix
+
+
+def batched_leg(gp, ctx, count, workers):
+    """The same mix partitioned by the plain per-graph loop (one context,
+    upload + partition + download per graph) and by the batched form
+    (concurrent requests); results must be identical."""
+    t = time.time()
+    mix = batch_mix(gp, count)
+    edges = sum(len(nbr) // 2 for _, nbr, _ in mix)
+    print("bench: batched mix of %d graphs, %d undirected edges, generated in %.1fs" % (len(mix), edges, time.time() - t),
+          file=sys.stderr, flush=True)
+    # Warm both paths once on a small graph.
+    small = [gp.BatchItem(mix[0][0], mix[0][1], mix[0][2])]
+    gp.partition_batch(small, workers=workers)
+    plain_labels, plain_cuts, plain_lat = [], [], []
+    t0 = time.perf_counter()
+    for off, nbr, k in mix:
+        s0 = time.perf_counter()
+        g = gp.Graph(ctx, off, nbr)
+        lab, st = g.partition(k)
+        g.close()
+        plain_lat.append(time.perf_counter() - s0)
+        plain_labels.append(lab)
+        plain_cuts.append(st["final_cut"])
+    plain_s = time.perf_counter() - t0
+    items = [gp.BatchItem(off, nbr, k) for off, nbr, k in mix]
+    t0 = time.perf_counter()
+    stats = gp.partition_batch(items, workers=workers)
+    batch_s = time.perf_counter() - t0
+    same = all(np.array_equal(it.labels, lab) for it, lab in zip(items, plain_labels))
+    cut_ratio = [(st["final_cut"] + 1) / (c + 1) for st, c in zip(stats, plain_cuts)]
+    lat = [st["device_seconds"] for st in stats]
+    print("bench: plain %.2fs batched %.2fs identical=%s" % (plain_s, batch_s, same), file=sys.stderr, flush=True)
+    if not same:
+        raise RuntimeError("bench: batched results differ from the plain loop")
+    return {"workload": "mix%d-seed5" % len(mix), "graphs": len(mix), "undirected_edges": edges,
+            "seconds": batch_s, "plain_seconds": plain_s,
+            "value": edges / batch_s, "plain_value": edges / plain_s, "speedup": plain_s / batch_s,
+            "workers": workers, "identical_to_plain": same,
+            "cut_geomean_ratio": float(np.exp(np.mean(np.log(cut_ratio)))),
+            "p50_request_ms": 1000.0 * float(np.percentile(lat, 50)),
+            "p99_request_ms": 1000.0 * float(np.percentile(lat, 99)),
+            "plain_p99_request_ms": 1000.0 * float(np.percentile(plain_lat, 99))}, mix, items
+
+
+def perturb(off, nbr, frac, rng):
+    """Applies frac*m/2 random edge deletions and as many insertions."""
+    n = len(off) - 1
+    src = np.repeat(np.arange(n, dtype=np.int64), np.diff(off))
+    keep = src < nbr
+    a, b = src[keep], nbr[keep].astype(np.int64)
+    k = max(1, int(frac * len(a)))
+    drop = rng.choice(len(a), size=k, replace=False)
+    mask = np.ones(len(a), bool)
+    mask[drop] = False
+    a, b = a[mask], b[mask]
+    u, v = rng.integers(0, n, k), rng.integers(0, n, k)
+    ok = u != v
+    a = np.concatenate([a, np.minimum(u, v)[ok]])
+    b = np.concatenate([b, np.maximum(u, v)[ok]])
+    key = np.unique(a * n + b)
+    a, b = key // n, key % n
+    s2 = np.concatenate([a, b])
+    d2 = np.concatenate([b, a])
+    order = np.lexsort((d2, s2))
+    s2, d2 = s2[order], d2[order]
+    off2 = np.zeros(n + 1, np.int64)
+    np.add.at(off2, s2 + 1, 1)
+    return np.cumsum(off2).astype(np.int32), d2.astype(np.int32)
+
+
+def serving_leg(gp, ctx, mix, done_items, workers, count, rate, seed=6):
+    """Poisson arrivals at `rate` requests/s (70% of the batched capacity):
+    cold requests from the mix with k in [2, 128], and 10% repartitions of
+    resident graphs with 1% edge insertions and deletions, warm-started from
+    their labels.  Latency = queueing + upload + partition + download."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        j = int(rng.integers(0, len(mix)))
+        off, nbr, k0 = mix[j]
+        if i % 10 == 9:
+            off2, nbr2 = perturb(off, nbr, 0.01, rng)
+            reqs.append(("warm", gp.BatchItem(off2, nbr2, k0, warm=done_items[j].labels), (off2, nbr2, k0)))
+        else:
+            k = int(rng.integers(2, 129))
+            k = min(k, 128, max(2, (len(off) - 1) // 2))
+            reqs.append(("cold", gp.BatchItem(off, nbr, k), None))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    srv = gp.Server(workers=workers)
+    try:
+        tickets = []
+        t0 = time.perf_counter()
+        for (kind, item, _), at in zip(reqs, arrivals):
+            delay = t0 + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(item))
+        stats = [srv.wait(t) for t in tickets]
+        wall = time.perf_counter() - t0
+    finally:
+        srv.close()
+    lat = [st["queue_seconds"] + st["device_seconds"] for st in stats]
+    edges = sum(len(item.nbr) // 2 for _, item, _ in reqs)
+    ratios = []
+    for (kind, item, extra), st in zip(reqs, stats):
+        if kind != "warm":
+            continue
+        off2, nbr2, k0 = extra
+        g = gp.Graph(ctx, off2, nbr2)
+        _, cold = g.partition(k0)
+        g.close()
+        ratios.append((st["final_cut"] + 1) / (cold["final_cut"] + 1))
+    print("bench: served %d requests in %.2fs, p99 %.1f ms" % (count, wall, 1000 * np.percentile(lat, 99)),
+          file=sys.stderr, flush=True)
+    return {"requests": count, "rate_per_s": rate, "workers": workers, "edges_per_s": edges / wall,
+            "p50_ms": 1000.0 * float(np.percentile(lat, 50)), "p99_ms": 1000.0 * float(np.percentile(lat, 99)),
+            "repartitions": len(ratios),
+            "warm_vs_cold_cut_geomean": float(np.exp(np.mean(np.log(ratios)))) if ratios else None}
 
 
 def traffic_of(name):

@@ -165,6 +165,45 @@ int gp_load_csr(const char* indptr_path, const char* indices_path, int64_t* n, i
 int gp_save_csr(const char* indptr_path, const char* indices_path, int64_t n, int64_t m, const int32_t* offsets,
                 const int32_t* neighbors);
 
+/* ---- batched and served partitioning ----------------------------------
+ * No counterpart in the reference (its CLI partitions one graph per process).
+ * Independent requests run concurrently on one GPU, each in its own context;
+ * a request is admitted while the estimated HBM footprint of the requests in
+ * flight stays within the budget.  Each result equals the same request run
+ * alone through gp_partition (cold) or gp_refine_partition (warm start from
+ * warm_labels).  The caller keeps every pointer valid until the request is
+ * waited for. */
+typedef struct gp_request {
+    int64_t n, m;
+    const int32_t* offsets;
+    const int32_t* neighbors;
+    const uint32_t* edge_weights;     /* NULL: unit */
+    const uint32_t* vertex_weights;   /* NULL: unit */
+    int parts;
+    double imbalance;
+    uint32_t seed;
+    const int32_t* warm_labels;       /* NULL: cold partition; else refine from these labels */
+    int32_t* labels;                  /* out: n labels */
+    gp_partition_stats stats;         /* out */
+    int status;                       /* out: gp_status (-1 while pending) */
+    double queue_seconds;             /* out: submission to admission */
+    double device_seconds;            /* out: upload + partition + download */
+    char error[256];                  /* out */
+} gp_request;
+
+typedef struct gp_server gp_server;
+typedef struct gp_ticket gp_ticket;
+
+/* workers: concurrent requests (contexts); hbm_budget_bytes 0 = 64 GiB. */
+int gp_server_create(int device, int workers, uint64_t hbm_budget_bytes, gp_server** out);
+int gp_server_submit(gp_server* s, gp_request* r, gp_ticket** ticket);
+/* Blocks until the request is done; releases the ticket; returns its status. */
+int gp_server_wait(gp_server* s, gp_ticket* ticket);
+/* Finishes queued requests, then stops the workers. */
+void gp_server_destroy(gp_server* s);
+/* Whole batch through a temporary server, largest request first. */
+int gp_partition_batch(int device, gp_request* items, int64_t count, int workers, uint64_t hbm_budget_bytes);
+
 /* Kernel-time accounting (HIP events on the context stream) for roofline reports. */
 int gp_profile_enable(gp_context* ctx, int enable);
 /* name: kernel class; returns accumulated ms, algorithmic bytes and launches. */

@@ -1,0 +1,86 @@
+"""Batched and served forms: every result equals the same graph partitioned
+alone (bit-exact; deterministic mode is the only mode)."""
+import numpy as np
+import pytest
+
+from tests import graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix(gp):
+    items = []
+    rng = np.random.default_rng(5)
+    for i in range(10):
+        kind = i % 3
+        k = int(rng.choice([4, 8, 16, 32, 64]))
+        if kind == 0:
+            side = int(rng.integers(32, 160))
+            off, nbr = gp.generate_grid(side, side)
+            items.append(("grid%d_k%d" % (side, k), off, nbr, None, None, k))
+        elif kind == 1:
+            scale = int(rng.integers(10, 15))
+            off, nbr = gp.generate_rmat(scale, int(rng.integers(8, 17)), seed=int(rng.integers(1, 1000)))
+            items.append(("rmat%d_k%d" % (scale, k), off, nbr, None, None, k))
+        else:
+            n = 1 << int(rng.integers(10, 15))
+            off, nbr = gp.generate_rgg2d(n, (8.0 / (np.pi * n)) ** 0.5, seed=int(rng.integers(1, 1000)))
+            items.append(("rgg%d_k%d" % (n, k), off, nbr, None, None, k))
+    off, nbr, ew, vw = gp.generate_knn3d(4000, 6, seed=2)
+    items.append(("knn4000_k16_weighted", off, nbr, ew, vw, 16))
+    return items
+
+
+def test_batch_equals_alone(gp, ctx):
+    mix = _mix(gp)
+    alone = []
+    for name, off, nbr, ew, vw, k in mix:
+        g = gp.Graph(ctx, off, nbr, ew, vw)
+        lab, st = g.partition(k)
+        g.close()
+        alone.append((lab, st["final_cut"]))
+    print("batch of %d graphs" % len(mix), flush=True)
+    items = [gp.BatchItem(off, nbr, k, ew=ew, vw=vw) for _, off, nbr, ew, vw, k in mix]
+    stats = gp.partition_batch(items, workers=4)
+    for (name, off, nbr, ew, vw, k), it, st, (lab, cut) in zip(mix, items, stats, alone):
+        assert np.array_equal(it.labels, lab), name
+        assert st["final_cut"] == cut, name
+
+
+def test_server_mixed_order_and_warm_start(gp, ctx, og):
+    mix = _mix(gp)[:6]
+    srv = gp.Server(workers=3, hbm_budget=1 << 30)
+    try:
+        items = [gp.BatchItem(off, nbr, k, ew=ew, vw=vw) for _, off, nbr, ew, vw, k in mix]
+        tickets = [srv.submit(it) for it in reversed(items)]
+        for t in tickets:
+            srv.wait(t)
+        for (name, off, nbr, ew, vw, k), it in zip(mix, items):
+            g = gp.Graph(ctx, off, nbr, ew, vw)
+            lab, _ = g.partition(k)
+            g.close()
+            assert np.array_equal(it.labels, lab), name
+        # Warm start: refine prior labels on a perturbed graph, as
+        # refine_partition would alone.
+        name, off, nbr, ew, vw, k = mix[0]
+        rng = np.random.default_rng(3)
+        n = len(off) - 1
+        extra = set()
+        while len(extra) < max(1, len(nbr) // 200):
+            u, v = (int(x) for x in rng.integers(0, n, 2))
+            if u != v:
+                extra.add((min(u, v), max(u, v)))
+        src = np.repeat(np.arange(n), np.diff(off))
+        edges = set(zip(src.tolist(), nbr.tolist()))
+        pairs = sorted({(a, b) for a, b in edges if a < b} | extra)
+        off2, nbr2, _ = graphs.csr_from_edges(n, pairs)
+        warm = items[0].labels.copy()
+        it = gp.BatchItem(off2, nbr2, k, warm=warm)
+        srv.wait(srv.submit(it))
+        g = gp.Graph(ctx, off2, nbr2)
+        ref, st = g.refine(warm, gp.refine_options(k))
+        g.close()
+        assert np.array_equal(it.labels, ref)
+        assert og.cut(off2, nbr2, None, it.labels) == st["final_cut"]
+    finally:
+        srv.close()

@@ -176,3 +176,16 @@ def test_verify_tool_weighted_extension(gp, og, tmp_path):
     got = json.loads(res.stdout)
     assert got["cut"] == og.cut(off, nbr, ew, lab)
     assert got["loads"] == np.bincount(lab, weights=vw, minlength=k).astype(np.int64).tolist()
+
+
+def test_server_fails_cleanly_without_a_device(gp):
+    # Failure handling runs on the CPU: with no usable device every worker
+    # fails to create its context and requests report an error (no hang).
+    try:
+        gp.Context(0).close()
+        pytest.skip("a GPU is present")
+    except RuntimeError:
+        pass
+    off, nbr = gp.generate_grid(8, 8)
+    with pytest.raises(RuntimeError):
+        gp.partition_batch([gp.BatchItem(off, nbr, 2)], workers=2)

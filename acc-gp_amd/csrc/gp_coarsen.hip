@@ -826,6 +826,48 @@ __global__ void k_valid_flags(std::int64_t n, const std::int32_t* targets, std::
         flags[v] = targets[v] != kInvalid;
 }
 
+// Admission needs the ordered weight prefix only where the requests of a
+// target can exceed its room: with base = frozen weight of the target and
+// total = weight of all its requests, base <= cap and total <= cap - base
+// admits every request (whatever the order), base > cap admits none.
+// flags: 1 contended (ordered admission), 2 admitted without ordering.
+__global__ void k_target_totals(std::int64_t n, const std::int32_t* targets, const std::uint32_t* vw,
+                                unsigned long long* tot) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x)
+        if (targets[v] != kInvalid) atomicAdd(&tot[targets[v]], (unsigned long long)vw[v]);
+}
+__global__ void k_contended_flags(std::int64_t n, const std::int32_t* targets, const unsigned long long* cw,
+                                  const unsigned long long* tot, std::uint64_t cap, std::uint8_t* flags) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t t = targets[v];
+        std::uint8_t f = 0;
+        if (t != kInvalid) {
+            const std::uint64_t base = cw[t];
+            if (base <= cap) f = tot[t] > cap - base ? 1 : 2;
+        }
+        flags[v] = f;
+    }
+}
+__global__ void k_commit_uncontended(std::int64_t n, const std::int32_t* targets, const std::uint8_t* flags,
+                                     const std::uint32_t* vw, std::int32_t* clusters, unsigned long long* cw) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x) {
+        if (flags[v] != 2) continue;
+        const std::int32_t s = clusters[v], t = targets[v];
+        const unsigned long long w = vw[v];
+        atomicAdd(&cw[s], 0ULL - w);
+        atomicAdd(&cw[t], w);
+        clusters[v] = t;
+    }
+}
+__global__ void k_flag_is(std::int64_t n, const std::uint8_t* in, std::uint8_t value, std::uint8_t* out) {
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x)
+        out[v] = in[v] == value;
+}
+
 // Composite key, least significant first: (~tie) | (~score masked) << 32 | target << (32+sb).
 __global__ void k_admission_keys(std::int64_t count, const std::int32_t* order, const std::int32_t* targets,
                                  const std::uint64_t* score, const std::uint32_t* ties, int score_bits, bool with_target,
@@ -1443,12 +1485,16 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
     const int target_bits = std::max(1, bit_width_u64(static_cast<std::uint64_t>(n - 1)));
     // Stable selection of valid proposals then LSD sort by
     // (target asc, score desc, tie desc); vertex order breaks ties.
-    auto admission_order = [&](const std::uint64_t* score, DBuf<std::uint64_t>& keys,
-                               DBuf<std::uint32_t>& vals) -> std::int64_t {
+    // select: vertices to order (null: every valid proposal).
+    auto admission_order = [&](const std::uint64_t* score, DBuf<std::uint64_t>& keys, DBuf<std::uint32_t>& vals,
+                               const std::uint8_t* select) -> std::int64_t {
         DBuf<std::uint8_t> flags = c.buf<std::uint8_t>(n);
         DBuf<std::int32_t> order = c.buf<std::int32_t>(n);
         DBuf<std::int64_t> cnt = c.buf<std::int64_t>(1);
-        hipLaunchKernelGGL(k_valid_flags, dim3(vb), dim3(256), 0, s, n, proposals.get(), flags.get());
+        if (select)
+            c.copy_d2d(flags.get(), select, static_cast<std::size_t>(n));
+        else
+            hipLaunchKernelGGL(k_valid_flags, dim3(vb), dim3(256), 0, s, n, proposals.get(), flags.get());
         GP_LAUNCH_CHECK();
         stable_select(c, flags.get(), n, order.get(), cnt.get());
         const std::int64_t count = c.read_scalar(cnt.get());
@@ -1498,7 +1544,17 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
         } else {
             DBuf<std::uint64_t> keys;
             DBuf<std::uint32_t> vals;
-            const std::int64_t count = admission_order(regret.get(), keys, vals);
+            DBuf<unsigned long long> tot = c.buf<unsigned long long>(n);
+            DBuf<std::uint8_t> aflags = c.buf<std::uint8_t>(n);
+            DBuf<std::uint8_t> contended = c.buf<std::uint8_t>(n);
+            fill<unsigned long long>(c, tot.get(), n, 0ULL);
+            hipLaunchKernelGGL(k_target_totals, dim3(vb), dim3(256), 0, s, n, proposals.get(), g.vw.get(), tot.get());
+            hipLaunchKernelGGL(k_contended_flags, dim3(vb), dim3(256), 0, s, n, proposals.get(), cw.get(), tot.get(),
+                               cap, aflags.get());
+            hipLaunchKernelGGL(k_flag_is, dim3(vb), dim3(256), 0, s, n, aflags.get(), (std::uint8_t)1,
+                               contended.get());
+            GP_LAUNCH_CHECK();
+            const std::int64_t count = admission_order(regret.get(), keys, vals, contended.get());
             if (count) {
                 DBuf<SegU64> items = c.buf<SegU64>(static_cast<std::size_t>(count));
                 DBuf<std::uint64_t> base = c.buf<std::uint64_t>(static_cast<std::size_t>(count));
@@ -1514,6 +1570,11 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
                                    cw.get());
                 GP_LAUNCH_CHECK();
             }
+            // Sources (movers) and targets (non-movers) are disjoint, so the
+            // frozen decisions of uncontended targets stay valid here.
+            hipLaunchKernelGGL(k_commit_uncontended, dim3(vb), dim3(256), 0, s, n, proposals.get(), aflags.get(),
+                               g.vw.get(), clusters.get(), cw.get());
+            GP_LAUNCH_CHECK();
         }
         cluster_count = count_clusters();
         bound = next_bound;
@@ -1557,7 +1618,7 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
         GP_LAUNCH_CHECK();
         DBuf<std::uint64_t> keys;
         DBuf<std::uint32_t> vals;
-        const std::int64_t count = admission_order(gains.get(), keys, vals);
+        const std::int64_t count = admission_order(gains.get(), keys, vals, nullptr);
         if (count) {
             DBuf<SegU64> ranks = c.buf<SegU64>(static_cast<std::size_t>(count));
             hipLaunchKernelGGL(k_seg_items, dim3(grid_for(count)), dim3(256), 0, s, count, vals.get(), proposals.get(),

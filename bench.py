@@ -155,9 +155,21 @@ def batch_mix(gp, count, seed=5):
         else:
             n = int(round(2.0 ** rng.uniform(10, 20)))
             off, nbr = gp.generate_rgg2d(n, (8.0 / (np.pi * n)) ** 0.5, seed=int(rng.integers(1, 1 << 30)))
-        if len(off) - 1 >= 2 * k:
+        if feasible(gp, len(off) - 1, k):
             mix.append((off, nbr, k))
     return mix
+
+
+def feasible(gp, n, k):
+    """The integer capacity floor(1.03 n / k) must hold ceil(n / k) (the
+    reference rejects the request otherwise)."""
+    if n < 2 * k:
+        return False
+    try:
+        gp.capacity(n, k, 1.03)
+        return True
+    except RuntimeError:
+        return False
 
 
 def batched_leg(gp, ctx, count, workers):
@@ -244,7 +256,8 @@ def serving_leg(gp, ctx, mix, done_items, workers, count, rate, seed=6):
             reqs.append(("warm", gp.BatchItem(off2, nbr2, k0, warm=done_items[j].labels), (off2, nbr2, k0)))
         else:
             k = int(rng.integers(2, 129))
-            k = min(k, 128, max(2, (len(off) - 1) // 2))
+            while k > 2 and not feasible(gp, len(off) - 1, k):
+                k -= 1
             reqs.append(("cold", gp.BatchItem(off, nbr, k), None))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
     srv = gp.Server(workers=workers)

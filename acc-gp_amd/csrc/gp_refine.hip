@@ -212,6 +212,23 @@ struct RowBody {
 
 // LP proposal (incremental.cuh:166-198, 229-296).
 struct LpProposeBody : RowBody {
+    // With prices, the scored candidates read (cap - load, price) of their
+    // part from an LDS copy of the part table.
+    const unsigned long long* part = nullptr;
+    __host__ __device__ int part_words() const { return a.prices ? 2 * a.k : 0; }
+    __device__ void stage(unsigned long long* t) const {
+        for (int p = threadIdx.x; p < a.k; p += blockDim.x) {
+            t[2 * p] = a.cap - a.loads[p];
+            t[2 * p + 1] = (unsigned long long)__double_as_longlong(a.prices[p]);
+        }
+    }
+    __device__ void bind(const unsigned long long* t) { part = t; }
+    // estimated_repair_cost (src/refinement/incremental.cuh:156-159).
+    __device__ double cost(int p, std::uint64_t w) const {
+        const std::uint64_t slack = part ? part[2 * p] : a.cap - a.loads[p];
+        const double price = part ? __longlong_as_double((long long)part[2 * p + 1]) : a.prices[p];
+        return w > slack ? price * (double)(w - slack) : 0.0;
+    }
     template <class Team>
     __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long* hist) const {
         const int locked = a.locks[v];
@@ -230,7 +247,7 @@ struct LpProposeBody : RowBody {
         std::uint64_t total, current;
         auto consider = [&](int p, std::uint64_t c) {
             if (p == s || !c) return;
-            Cand x{p, c, priced ? (double)c - repair_cost(a, p, w) : 0.0};
+            Cand x{p, c, priced ? (double)c - cost(p, w) : 0.0};
             if (cand_better(x, best, priced)) best = x;
         };
         if constexpr (IsLight<Team>::value) {
@@ -510,14 +527,22 @@ __device__ inline void repair_finish(const RefArgs& a, std::int64_t v, int s, st
 }
 
 struct RepairProposeBody : RowBody {
+    // Frozen part loads in LDS for the destination scan.
+    const unsigned long long* part = nullptr;
+    __host__ __device__ int part_words() const { return a.k; }
+    __device__ void stage(unsigned long long* t) const {
+        for (int p = threadIdx.x; p < a.k; p += blockDim.x) t[p] = a.loads0[p];
+    }
+    __device__ void bind(const unsigned long long* t) { part = t; }
     template <class Team>
     __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long* hist) const {
         const int s = a.labels[v];
         const std::uint64_t w = a.vw[v];
         RepairPick best{-1, 0, 0};
+        const unsigned long long* load0 = part ? part : a.loads0;
         auto consider = [&](int p, std::uint64_t c) {
-            if (p == s || !c || a.loads0[p] >= a.cap || w > a.cap - a.loads0[p]) return;
-            RepairPick x{p, c, a.cap - a.loads0[p]};
+            if (p == s || !c || load0[p] >= a.cap || w > a.cap - load0[p]) return;
+            RepairPick x{p, c, a.cap - load0[p]};
             if (pick_better(x, best)) best = x;
         };
         std::uint64_t total, current;

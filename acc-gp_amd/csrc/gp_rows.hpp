@@ -9,6 +9,9 @@
 #pragma once
 #include "gp_prims.hpp"
 
+#include <type_traits>
+#include <utility>
+
 namespace gp {
 
 constexpr int kTierLight = 64;
@@ -126,6 +129,35 @@ __device__ inline void row_batches(const Team& t, std::int64_t b, std::int64_t e
     }
 }
 
+// Bodies may stage a block-shared, read-only table at kernel start:
+// part_words() u64 words, filled cooperatively by stage(); bind() hands the
+// LDS copy to the body.  Bodies without part_words() stage nothing.
+template <class B, class = void>
+struct HasParts : std::false_type {};
+template <class B>
+struct HasParts<B, std::void_t<decltype(std::declval<const B&>().part_words())>> : std::true_type {};
+
+template <class Body>
+__host__ __device__ inline int part_words_of(const Body& body) {
+    if constexpr (HasParts<Body>::value)
+        return body.part_words();
+    else
+        return 0;
+}
+
+template <class Body>
+__device__ inline Body staged_body(const Body& body, unsigned long long* part) {
+    Body b = body;
+    if constexpr (HasParts<Body>::value) {
+        if (body.part_words()) {
+            body.stage(part);
+            __syncthreads();
+            b.bind(part);
+        }
+    }
+    return b;
+}
+
 struct RowOut {
     bool append;
     std::int64_t acc;
@@ -151,11 +183,13 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const st
     __shared__ std::int32_t stage_buf[kTierThreads / kWave][2][kStage];
     constexpr int G = kWave / W;
     const int lane = lane_id(), gw = lane / W, lg = lane % W;
-    unsigned long long* scratch = smem + (threadIdx.x / W) * scratch_per_group;
+    const Body b = staged_body(body, smem);
+    unsigned long long* team_smem = smem + part_words_of(body);
+    unsigned long long* scratch = team_smem + (threadIdx.x / W) * scratch_per_group;
     Stage defer_stage{stage_buf[threadIdx.x / kWave][0], t.medium, t.medium_count, 0};
     Stage out_stage{stage_buf[threadIdx.x / kWave][1], t.out, t.out_count, 0};
     // Team tables start zero; bodies that use them leave them zero.
-    for (int i = threadIdx.x; i < (kTierThreads / W) * scratch_per_group; i += kTierThreads) smem[i] = 0;
+    for (int i = threadIdx.x; i < (kTierThreads / W) * scratch_per_group; i += kTierThreads) team_smem[i] = 0;
     __syncthreads();
     const std::int64_t count = count_ptr ? (std::int64_t)*count_ptr : n;
     const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
@@ -168,11 +202,11 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const st
         RowOut r{false, 0};
         if (idx < count) {
             v = list ? list[idx] : (std::int32_t)idx;
-            const std::int64_t deg = body.degree(v);
+            const std::int64_t deg = b.degree(v);
             if (deg > kTierLight)
                 defer = true;
             else
-                r = body(GroupTeam<W>{lg}, v, scratch);
+                r = b(GroupTeam<W>{lg}, v, scratch);
         }
         if (lg == 0) acc += r.acc;
         defer_stage.push(lg == 0 && defer, v);
@@ -191,7 +225,8 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLis
     extern __shared__ unsigned long long smem[];
     __shared__ std::int32_t stage_buf[kTierThreads / kWave][2][kStage];
     const int lane = lane_id();
-    unsigned long long* scratch = smem + (threadIdx.x / kWave) * scratch_per_group;
+    const Body b = staged_body(body, smem);
+    unsigned long long* scratch = smem + part_words_of(body) + (threadIdx.x / kWave) * scratch_per_group;
     Stage defer_stage{stage_buf[threadIdx.x / kWave][0], t.heavy, t.heavy_count, 0};
     Stage out_stage{stage_buf[threadIdx.x / kWave][1], t.out, t.out_count, 0};
     const std::int64_t count = (std::int64_t)*t.medium_count;
@@ -202,10 +237,10 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLis
         const std::int32_t v = t.medium[idx];
         bool defer = false;
         RowOut r{false, 0};
-        if (body.degree(v) > kTierMedium)
+        if (b.degree(v) > kTierMedium)
             defer = true;
         else
-            r = body(GroupTeam<kWave>{lane}, v, scratch);
+            r = b(GroupTeam<kWave>{lane}, v, scratch);
         if (lane == 0) acc += r.acc;
         defer_stage.push(lane == 0 && defer, v);
         out_stage.push(lane == 0 && r.append, v);
@@ -220,10 +255,13 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_heavy(Body body, TierList
     extern __shared__ unsigned long long smem[];
     __shared__ unsigned long long red[kBlockRed];
     const std::int64_t count = (std::int64_t)*t.heavy_count;
+    if ((std::int64_t)blockIdx.x >= count) return;
+    const Body b = staged_body(body, smem);
+    unsigned long long* scratch = smem + part_words_of(body);
     std::int64_t acc = 0;
     for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
         const std::int32_t v = t.heavy[idx];
-        RowOut r = body(BlockTeam{red}, v, smem);
+        RowOut r = b(BlockTeam{red}, v, scratch);
         if (threadIdx.x == 0) {
             acc += r.acc;
             if (t.out && r.append) t.out[atomicAdd(t.out_count, 1ULL)] = v;
@@ -246,7 +284,8 @@ void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int6
     while (lanes < 16 && (kTierThreads / lanes) * scratch * sizeof(unsigned long long) > 48 * 1024) lanes *= 2;
     const int light_blocks = static_cast<int>(
         std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(max_rows, kTierThreads / lanes), 4096)));
-    const std::size_t light_lds = (kTierThreads / lanes) * scratch * sizeof(unsigned long long);
+    const std::size_t part = static_cast<std::size_t>(part_words_of(body)) * sizeof(unsigned long long);
+    const std::size_t light_lds = part + (kTierThreads / lanes) * scratch * sizeof(unsigned long long);
     if (lanes == 4)
         hipLaunchKernelGGL((k_tier_light<Body, 4>), dim3(light_blocks), dim3(kTierThreads), light_lds, s, body, list, n,
                            count_ptr, t, scratch);
@@ -260,11 +299,11 @@ void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int6
     const int medium_blocks =
         static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(max_rows, kTierThreads / kWave), 2048)));
     hipLaunchKernelGGL((k_tier_medium<Body>), dim3(medium_blocks), dim3(kTierThreads),
-                       (kTierThreads / kWave) * scratch * sizeof(unsigned long long), s, body, t, scratch);
+                       part + (kTierThreads / kWave) * scratch * sizeof(unsigned long long), s, body, t, scratch);
     GP_LAUNCH_CHECK();
     // Heavy rows are few but long; blocks beyond the row count exit at once.
-    hipLaunchKernelGGL((k_tier_heavy<Body>), dim3(1024), dim3(kTierThreads), scratch * sizeof(unsigned long long), s,
-                       body, t);
+    hipLaunchKernelGGL((k_tier_heavy<Body>), dim3(1024), dim3(kTierThreads),
+                       part + scratch * sizeof(unsigned long long), s, body, t);
     GP_LAUNCH_CHECK();
 }
 

@@ -34,7 +34,12 @@ def main():
     ap.add_argument("--batch-graphs", type=int, default=256, help="graphs in the batched mix (0: skip)")
     ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--serve-requests", type=int, default=128, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 1 GiB device-to-device copies (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; multi-GPU sharding is a design note, not a run.",
               file=sys.stderr)
@@ -290,6 +295,19 @@ def serving_leg(gp, ctx, mix, done_items, workers, count, rate, seed=6):
             "p50_ms": 1000.0 * float(np.percentile(lat, 50)), "p99_ms": 1000.0 * float(np.percentile(lat, 99)),
             "repartitions": len(ratios),
             "warm_vs_cold_cut_geomean": float(np.exp(np.mean(np.log(ratios)))) if ratios else None}
+
+
+def copy_calibration():
+    """Three plain 1 GiB device copies: what FETCH_SIZE / WRITE_SIZE read
+    for a known byte count scales the counters of the partitioner."""
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("copy calibration: 3 x %d bytes" % (4 << 28), flush=True)
 
 
 def traffic_of(name):

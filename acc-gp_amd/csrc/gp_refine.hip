@@ -46,9 +46,19 @@ struct RepairSample {
     unsigned long long weight;
 };
 
+// 1.5^j for j = -23..24 at index j + 23, by repeated exact-IEEE products
+// and quotients (compile-time evaluation rounds like the run-time ops).
 struct Pow15 {
-    double p[48];  // 1.5^j for j=-23..24 at index j+23
+    double p[48];
 };
+constexpr Pow15 make_pow15() {
+    Pow15 t{};
+    t.p[23] = 1.0;
+    for (int j = 1; j <= 24; ++j) t.p[23 + j] = t.p[22 + j] * 1.5;
+    for (int j = 1; j <= 23; ++j) t.p[23 - j] = t.p[24 - j] / 1.5;
+    return t;
+}
+__constant__ Pow15 kPow15 = make_pow15();
 
 struct RefArgs {
     std::int64_t n;
@@ -73,7 +83,6 @@ struct RefArgs {
     const unsigned long long* loads0;
     std::int32_t* bucket;
     unsigned long long* counts;
-    Pow15 pow15;
 };
 
 // Part-connectivity table of one vertex in team-private LDS; returns the
@@ -92,8 +101,8 @@ __device__ inline std::uint64_t team_histogram(const Team& t, const RefArgs& a, 
     row_batches(
         t, a.off[v], a.off[v + 1],
         [&](std::int64_t x) {
-            const std::int32_t u = a.nbr[x];
-            return Rec{u == v ? -1 : a.labels[u], (std::uint32_t)ew_at(a.ew, x)};
+            const std::int32_t u = a.nbr[(std::uint32_t)x];
+            return Rec{u == v ? -1 : a.labels[(std::uint32_t)u], (std::uint32_t)ew_at(a.ew, x)};
         },
         [&](std::int64_t, const Rec& r) {
             if (r.label < 0) return;
@@ -111,30 +120,45 @@ __device__ inline std::uint64_t team_histogram(const Team& t, const RefArgs& a, 
 template <int W>
 struct LightRow {
     static constexpr int kPer = kTierLight / W;
-    int lab[kPer];  // -1: no entry or a self loop
-    std::uint32_t w[kPer];
+    static constexpr int kWords = (kPer + 3) / 4;
+    unsigned packed[kWords];  // four byte labels per word; 0xff: no entry or a self loop
+    __device__ int lab(int j) const { return (int)((packed[j >> 2] >> (8 * (j & 3))) & 0xffu); }
 };
 
 template <int W>
 __device__ inline std::uint64_t light_histogram(const GroupTeam<W>& t, const RefArgs& a, std::int64_t v,
                                                 unsigned long long* hist, LightRow<W>& r) {
     const std::int64_t b = a.off[v], e = a.off[v + 1];
-#pragma unroll
-    for (int j = 0; j < LightRow<W>::kPer; ++j) {
-        const std::int64_t x = b + t.rank() + j * W;
-        const bool in = x < e;
-        const std::int32_t u = in ? a.nbr[x] : (std::int32_t)v;
-        r.lab[j] = u != v ? (int)a.labels[u] : -1;
-        r.w[j] = in ? (std::uint32_t)ew_at(a.ew, x) : 0u;
-    }
     std::uint64_t total = 0;
-    t.sync();  // the previous row's clean-up is done
 #pragma unroll
-    for (int j = 0; j < LightRow<W>::kPer; ++j)
-        if (r.lab[j] >= 0) {
-            atomicAdd(&hist[r.lab[j]], (unsigned long long)r.w[j]);
-            total += r.w[j];
+    for (int q = 0; q < LightRow<W>::kWords; ++q) r.packed[q] = 0xffffffffu;
+    t.sync();  // the previous row's clean-up is done
+    // Four entries per lane in flight; iterations beyond the longest row of
+    // the wave are skipped as a whole (t.wave_rows_max is wave-uniform).
+#pragma unroll
+    for (int q = 0; q < LightRow<W>::kWords; ++q) {
+        if (q * 4 * W >= t.wave_rows_max) break;
+        int lab[4];
+        std::uint32_t w[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const std::int64_t x = b + t.rank() + (q * 4 + i) * W;
+            const bool in = x < e;
+            const std::int32_t u = in ? a.nbr[(std::uint32_t)x] : (std::int32_t)v;
+            lab[i] = u != v ? (int)a.labels[(std::uint32_t)u] : 0xff;
+            w[i] = in ? (std::uint32_t)ew_at(a.ew, x) : 0u;
         }
+        unsigned word = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (lab[i] != 0xff) {
+                atomicAdd(&hist[lab[i]], (unsigned long long)w[i]);
+                total += w[i];
+            }
+            word |= (unsigned)lab[i] << (8 * i);
+        }
+        r.packed[q] = word;
+    }
     t.sync();
     return t.sum(total);
 }
@@ -143,8 +167,10 @@ template <int W>
 __device__ inline void light_clear(const GroupTeam<W>& t, unsigned long long* hist, const LightRow<W>& r) {
     t.sync();  // all reads of the row's bins are done
 #pragma unroll
-    for (int j = 0; j < LightRow<W>::kPer; ++j)
-        if (r.lab[j] >= 0) hist[r.lab[j]] = 0;
+    for (int j = 0; j < LightRow<W>::kPer; ++j) {
+        const int l = r.lab(j);
+        if (l != 0xff) hist[l] = 0;
+    }
 }
 
 template <class Team>
@@ -214,7 +240,6 @@ struct RowBody {
 struct LpProposeBody : RowBody {
     // With prices, the scored candidates read (cap - load, price) of their
     // part from an LDS copy of the part table.
-    const unsigned long long* part = nullptr;
     __host__ __device__ int part_words() const { return a.prices ? 2 * a.k : 0; }
     __device__ void stage(unsigned long long* t) const {
         for (int p = threadIdx.x; p < a.k; p += blockDim.x) {
@@ -222,9 +247,8 @@ struct LpProposeBody : RowBody {
             t[2 * p + 1] = (unsigned long long)__double_as_longlong(a.prices[p]);
         }
     }
-    __device__ void bind(const unsigned long long* t) { part = t; }
     // estimated_repair_cost (src/refinement/incremental.cuh:156-159).
-    __device__ double cost(int p, std::uint64_t w) const {
+    __device__ double cost(const unsigned long long* part, int p, std::uint64_t w) const {
         const std::uint64_t slack = part ? part[2 * p] : a.cap - a.loads[p];
         const double price = part ? __longlong_as_double((long long)part[2 * p + 1]) : a.prices[p];
         return w > slack ? price * (double)(w - slack) : 0.0;
@@ -247,7 +271,7 @@ struct LpProposeBody : RowBody {
         std::uint64_t total, current;
         auto consider = [&](int p, std::uint64_t c) {
             if (p == s || !c) return;
-            Cand x{p, c, priced ? (double)c - cost(p, w) : 0.0};
+            Cand x{p, c, priced ? (double)c - cost(t.part, p, w) : 0.0};
             if (cand_better(x, best, priced)) best = x;
         };
         if constexpr (IsLight<Team>::value) {
@@ -256,8 +280,10 @@ struct LpProposeBody : RowBody {
             current = hist[s];
             if (current != total)
 #pragma unroll
-                for (int j = 0; j < LightRow<Team::size>::kPer; ++j)
-                    if (r.lab[j] >= 0) consider(r.lab[j], hist[r.lab[j]]);
+                for (int j = 0; j < LightRow<Team::size>::kPer; ++j) {
+                    const int l = r.lab(j);
+                    if (l != 0xff) consider(l, hist[l]);
+                }
             best = team_reduce(t, best, priced);
             light_clear(t, hist, r);
         } else {
@@ -285,22 +311,26 @@ struct AfterBody : RowBody {
         const int s = a.labels[v], tv = a.targets[v];
         const std::int64_t gv = a.gains[v];
         std::int64_t corr = 0;
+        // Neighbour state packed in one word: target (signed byte, -1 none)
+        // in bits 0-7, label in bits 8-15.
         struct Rec {
             std::int32_t u;
-            int tu, lu;
+            int tl;
             std::int64_t gu;
         };
         row_batches(
             t, a.off[v], a.off[v + 1],
             [&](std::int64_t x) {
-                const std::int32_t u = a.nbr[x];
-                const int tu = a.targets[u];
-                return Rec{u, tu, a.labels[u], tu >= 0 ? a.gains[u] : 0};
+                const std::int32_t u = a.nbr[(std::uint32_t)x];
+                const std::uint32_t uu = (std::uint32_t)u;
+                const int tu = a.targets[uu];
+                return Rec{u, (tu & 0xff) | ((int)a.labels[uu] << 8), tu >= 0 ? a.gains[uu] : 0};
             },
             [&](std::int64_t x, const Rec& r) {
-                if (r.u == v || r.tu < 0) return;
+                const int tu = (int)(std::int8_t)(r.tl & 0xff), lu = r.tl >> 8;
+                if (r.u == v || tu < 0) return;
                 if (!(r.gu > gv || (r.gu == gv && r.u < v))) return;
-                corr += (std::int64_t)ew_at(a.ew, x) * ((r.tu == tv) - (r.tu == s) - (r.lu == tv) + (r.lu == s));
+                corr += (std::int64_t)ew_at(a.ew, x) * ((tu == tv) - (tu == s) - (lu == tv) + (lu == s));
             });
         corr = t.sum(corr);
         bool keep;
@@ -323,21 +353,29 @@ struct UpdateBody : RowBody {
     __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long*) const {
         const int s = a.labels[v], tv = a.targets[v];
         std::int64_t gain = 0;
+        // Neighbour state packed in one word: label, moving flag, target
+        // (signed byte) and frontier membership, one byte each.
         struct Rec {
             std::int32_t u;
-            int old_u, moves, tu;
+            unsigned st;
         };
         row_batches(
             t, a.off[v], a.off[v + 1],
             [&](std::int64_t x) {
-                const std::int32_t u = a.nbr[x];
-                return Rec{u, a.labels[u], a.flags[u], a.targets[u]};
+                const std::int32_t u = a.nbr[(std::uint32_t)x];
+                const std::uint32_t uu = (std::uint32_t)u;
+                return Rec{u, (unsigned)a.labels[uu] | ((unsigned)a.flags[uu] << 8) |
+                                  ((unsigned)(std::uint8_t)a.targets[uu] << 16) | ((unsigned)a.member[uu] << 24)};
             },
             [&](std::int64_t x, const Rec& r) {
                 if (r.u == v) return;
-                a.member[r.u] = 1;
-                const int new_u = r.moves ? r.tu : r.old_u;
-                if (!r.moves || r.u > v) gain += (std::int64_t)ew_at(a.ew, x) * ((s != r.old_u) - (tv != new_u));
+                const int old_u = (int)(r.st & 0xff), tu = (int)(std::int8_t)((r.st >> 16) & 0xff);
+                const bool moves = ((r.st >> 8) & 0xff) != 0;
+                // Most neighbours are already in the frontier: skip the
+                // scattered byte store for them.
+                if (!(r.st >> 24)) a.member[r.u] = 1;
+                const int new_u = moves ? tu : old_u;
+                if (!moves || r.u > v) gain += (std::int64_t)ew_at(a.ew, x) * ((s != old_u) - (tv != new_u));
             });
         gain = t.sum(gain);
         if (samples && a.samples && t.leader()) {
@@ -357,8 +395,8 @@ struct BoundaryBody : RowBody {
         row_batches(
             t, a.off[v], a.off[v + 1],
             [&](std::int64_t x) {
-                const std::int32_t u = a.nbr[x];
-                return u != v && a.labels[u] != s;
+                const std::int32_t u = a.nbr[(std::uint32_t)x];
+                return u != v && a.labels[(std::uint32_t)u] != s;
             },
             [&](std::int64_t, bool other) { any |= other; });
         any = t.sum(any);
@@ -419,7 +457,7 @@ __device__ inline int gain_bucket(const RefArgs& a, double gain) {
     int lo = 0, hi = 47;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (a.pow15.p[mid] < x)
+        if (kPow15.p[mid] < x)
             lo = mid + 1;
         else
             hi = mid;
@@ -528,18 +566,16 @@ __device__ inline void repair_finish(const RefArgs& a, std::int64_t v, int s, st
 
 struct RepairProposeBody : RowBody {
     // Frozen part loads in LDS for the destination scan.
-    const unsigned long long* part = nullptr;
     __host__ __device__ int part_words() const { return a.k; }
     __device__ void stage(unsigned long long* t) const {
         for (int p = threadIdx.x; p < a.k; p += blockDim.x) t[p] = a.loads0[p];
     }
-    __device__ void bind(const unsigned long long* t) { part = t; }
     template <class Team>
     __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long* hist) const {
         const int s = a.labels[v];
         const std::uint64_t w = a.vw[v];
         RepairPick best{-1, 0, 0};
-        const unsigned long long* load0 = part ? part : a.loads0;
+        const unsigned long long* load0 = t.part ? t.part : a.loads0;
         auto consider = [&](int p, std::uint64_t c) {
             if (p == s || !c || load0[p] >= a.cap || w > a.cap - load0[p]) return;
             RepairPick x{p, c, a.cap - load0[p]};
@@ -552,8 +588,10 @@ struct RepairProposeBody : RowBody {
             current = hist[s];
             if (current != total)
 #pragma unroll
-                for (int j = 0; j < LightRow<Team::size>::kPer; ++j)
-                    if (r.lab[j] >= 0) consider(r.lab[j], hist[r.lab[j]]);
+                for (int j = 0; j < LightRow<Team::size>::kPer; ++j) {
+                    const int l = r.lab(j);
+                    if (l != 0xff) consider(l, hist[l]);
+                }
             best = team_reduce(t, best);
             light_clear(t, hist, r);
         } else {
@@ -1547,9 +1585,6 @@ public:
         a_.loads0 = loads0_.get();
         a_.bucket = bucket_.get();
         a_.counts = counts_.get();
-        a_.pow15.p[23] = 1.0;
-        for (int j = 1; j <= 24; ++j) a_.pow15.p[23 + j] = a_.pow15.p[22 + j] * 1.5;
-        for (int j = 1; j <= 23; ++j) a_.pow15.p[23 - j] = a_.pow15.p[24 - j] / 1.5;
         c.upload(a_.loads, loads.data(), static_cast<std::size_t>(k) * sizeof(std::uint64_t));
         fill<unsigned long long>(c, ctr_.get(), kCtrCount, 0ULL);
         std::uint64_t total = 0;

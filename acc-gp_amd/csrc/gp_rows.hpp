@@ -67,6 +67,8 @@ struct Stage {
 template <int W>
 struct GroupTeam {
     int lg;
+    int wave_rows_max = 1 << 30;           // longest row any team of this wave holds (wave-uniform)
+    const unsigned long long* part = nullptr;  // block-shared staged table (or null)
     static constexpr int size = W;
     __device__ int rank() const { return lg; }
     template <typename T>
@@ -79,6 +81,7 @@ struct GroupTeam {
 
 struct BlockTeam {
     unsigned long long* red;  // LDS scratch of kBlockRed entries
+    const unsigned long long* part = nullptr;  // block-shared staged table (or null)
     static constexpr int size = kTierThreads;
     __device__ int rank() const { return threadIdx.x; }
     template <typename T>
@@ -130,8 +133,8 @@ __device__ inline void row_batches(const Team& t, std::int64_t b, std::int64_t e
 }
 
 // Bodies may stage a block-shared, read-only table at kernel start:
-// part_words() u64 words, filled cooperatively by stage(); bind() hands the
-// LDS copy to the body.  Bodies without part_words() stage nothing.
+// part_words() u64 words, filled cooperatively by stage(); teams carry a
+// pointer to it.  Bodies without part_words() stage nothing.
 template <class B, class = void>
 struct HasParts : std::false_type {};
 template <class B>
@@ -146,16 +149,15 @@ __host__ __device__ inline int part_words_of(const Body& body) {
 }
 
 template <class Body>
-__device__ inline Body staged_body(const Body& body, unsigned long long* part) {
-    Body b = body;
+__device__ inline const unsigned long long* stage_parts(const Body& body, unsigned long long* part) {
     if constexpr (HasParts<Body>::value) {
         if (body.part_words()) {
             body.stage(part);
             __syncthreads();
-            b.bind(part);
+            return part;
         }
     }
-    return b;
+    return nullptr;
 }
 
 struct RowOut {
@@ -183,7 +185,7 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const st
     __shared__ std::int32_t stage_buf[kTierThreads / kWave][2][kStage];
     constexpr int G = kWave / W;
     const int lane = lane_id(), gw = lane / W, lg = lane % W;
-    const Body b = staged_body(body, smem);
+    const unsigned long long* part = stage_parts(body, smem);
     unsigned long long* team_smem = smem + part_words_of(body);
     unsigned long long* scratch = team_smem + (threadIdx.x / W) * scratch_per_group;
     Stage defer_stage{stage_buf[threadIdx.x / kWave][0], t.medium, t.medium_count, 0};
@@ -200,13 +202,55 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const st
         std::int32_t v = 0;
         bool defer = false;
         RowOut r{false, 0};
+        std::int64_t deg = 0;
         if (idx < count) {
             v = list ? list[idx] : (std::int32_t)idx;
-            const std::int64_t deg = b.degree(v);
-            if (deg > kTierLight)
+            deg = body.degree(v);
+            defer = deg > kTierLight;
+        }
+        // Wave-uniform bound on the entries of the rows handled together,
+        // so that bodies with unrolled row loops skip empty iterations.
+        const int wmax = (int)wave_max((std::uint64_t)(idx < count && !defer ? deg : 0));
+        if (idx < count && !defer) r = body(GroupTeam<W>{lg, wmax, part}, v, scratch);
+        if (lg == 0) acc += r.acc;
+        defer_stage.push(lg == 0 && defer, v);
+        out_stage.push(lg == 0 && r.append, v);
+    }
+    defer_stage.flush();
+    out_stage.flush();
+    if (t.acc) {
+        acc = wave_sum(acc);
+        if (lane == 0 && acc) atomicAdd(t.acc, (unsigned long long)acc);
+    }
+}
+
+// Medium tier: teams of W (32 or 64) lanes, G = 64 / W rows per wave step.
+template <class Body, int W>
+__global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLists t, int scratch_per_group) {
+    static_assert(W == 32 || W == 64, "medium teams are half or whole waves");
+    extern __shared__ unsigned long long smem[];
+    __shared__ std::int32_t stage_buf[kTierThreads / kWave][2][kStage];
+    constexpr int G = kWave / W;
+    const int lane = lane_id(), gw = lane / W, lg = lane % W;
+    const unsigned long long* part = stage_parts(body, smem);
+    unsigned long long* scratch = smem + part_words_of(body) + (threadIdx.x / W) * scratch_per_group;
+    Stage defer_stage{stage_buf[threadIdx.x / kWave][0], t.heavy, t.heavy_count, 0};
+    Stage out_stage{stage_buf[threadIdx.x / kWave][1], t.out, t.out_count, 0};
+    const std::int64_t count = (std::int64_t)*t.medium_count;
+    const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
+    const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;
+    std::int64_t acc = 0;
+    for (std::int64_t base = wave * G; base < count; base += waves * G) {
+        const std::int64_t idx = base + gw;
+        std::int32_t v = 0;
+        bool defer = false;
+        RowOut r{false, 0};
+        if (idx < count) {
+            v = t.medium[idx];
+            if (body.degree(v) > kTierMedium)
                 defer = true;
             else
-                r = b(GroupTeam<W>{lg}, v, scratch);
+                r = body(GroupTeam<W>{lg, 1 << 30, part}, v, scratch);
         }
         if (lg == 0) acc += r.acc;
         defer_stage.push(lg == 0 && defer, v);
@@ -221,47 +265,17 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const st
 }
 
 template <class Body>
-__global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLists t, int scratch_per_group) {
-    extern __shared__ unsigned long long smem[];
-    __shared__ std::int32_t stage_buf[kTierThreads / kWave][2][kStage];
-    const int lane = lane_id();
-    const Body b = staged_body(body, smem);
-    unsigned long long* scratch = smem + part_words_of(body) + (threadIdx.x / kWave) * scratch_per_group;
-    Stage defer_stage{stage_buf[threadIdx.x / kWave][0], t.heavy, t.heavy_count, 0};
-    Stage out_stage{stage_buf[threadIdx.x / kWave][1], t.out, t.out_count, 0};
-    const std::int64_t count = (std::int64_t)*t.medium_count;
-    const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
-    const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;
-    std::int64_t acc = 0;
-    for (std::int64_t idx = wave; idx < count; idx += waves) {
-        const std::int32_t v = t.medium[idx];
-        bool defer = false;
-        RowOut r{false, 0};
-        if (b.degree(v) > kTierMedium)
-            defer = true;
-        else
-            r = b(GroupTeam<kWave>{lane}, v, scratch);
-        if (lane == 0) acc += r.acc;
-        defer_stage.push(lane == 0 && defer, v);
-        out_stage.push(lane == 0 && r.append, v);
-    }
-    defer_stage.flush();
-    out_stage.flush();
-    if (t.acc && lane == 0 && acc) atomicAdd(t.acc, (unsigned long long)acc);
-}
-
-template <class Body>
 __global__ void __launch_bounds__(kTierThreads) k_tier_heavy(Body body, TierLists t) {
     extern __shared__ unsigned long long smem[];
     __shared__ unsigned long long red[kBlockRed];
     const std::int64_t count = (std::int64_t)*t.heavy_count;
     if ((std::int64_t)blockIdx.x >= count) return;
-    const Body b = staged_body(body, smem);
+    const unsigned long long* part = stage_parts(body, smem);
     unsigned long long* scratch = smem + part_words_of(body);
     std::int64_t acc = 0;
     for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
         const std::int32_t v = t.heavy[idx];
-        RowOut r = b(BlockTeam{red}, v, scratch);
+        RowOut r = body(BlockTeam{red, part}, v, scratch);
         if (threadIdx.x == 0) {
             acc += r.acc;
             if (t.out && r.append) t.out[atomicAdd(t.out_count, 1ULL)] = v;
@@ -296,10 +310,16 @@ void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int6
         hipLaunchKernelGGL((k_tier_light<Body, 16>), dim3(light_blocks), dim3(kTierThreads), light_lds, s, body, list, n,
                            count_ptr, t, scratch);
     GP_LAUNCH_CHECK();
-    const int medium_blocks =
-        static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(max_rows, kTierThreads / kWave), 2048)));
-    hipLaunchKernelGGL((k_tier_medium<Body>), dim3(medium_blocks), dim3(kTierThreads),
-                       part + (kTierThreads / kWave) * scratch * sizeof(unsigned long long), s, body, t, scratch);
+    const int mlanes = c.medium_lanes();
+    const int medium_blocks = static_cast<int>(
+        std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(max_rows, kTierThreads / mlanes), 2048)));
+    const std::size_t medium_lds = part + (kTierThreads / mlanes) * scratch * sizeof(unsigned long long);
+    if (mlanes == 32)
+        hipLaunchKernelGGL((k_tier_medium<Body, 32>), dim3(medium_blocks), dim3(kTierThreads), medium_lds, s, body, t,
+                           scratch);
+    else
+        hipLaunchKernelGGL((k_tier_medium<Body, 64>), dim3(medium_blocks), dim3(kTierThreads), medium_lds, s, body, t,
+                           scratch);
     GP_LAUNCH_CHECK();
     // Heavy rows are few but long; blocks beyond the row count exit at once.
     hipLaunchKernelGGL((k_tier_heavy<Body>), dim3(1024), dim3(kTierThreads),

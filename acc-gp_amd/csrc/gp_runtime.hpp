@@ -163,6 +163,7 @@ public:
         arena_ = new Arena(stream_);
         GP_HIP(hipHostMalloc(&pinned_, kPinnedBytes, hipHostMallocDefault));
         if (const char* e = std::getenv("GP_LIGHT_LANES")) set_light_lanes(std::atoi(e));
+        if (const char* e = std::getenv("GP_MEDIUM_LANES")) set_medium_lanes(std::atoi(e));
     }
     ~Context() {
         delete arena_;
@@ -179,6 +180,12 @@ public:
     // keeps the most rows in flight per wave, which the dependent gathers of
     // short rows need.
     int light_lanes() const { return light_lanes_; }
+    // Lanes per row team for medium rows (32 or 64; GP_MEDIUM_LANES).
+    int medium_lanes() const { return medium_lanes_; }
+    void set_medium_lanes(int lanes) {
+        if (lanes != 32 && lanes != 64) throw std::invalid_argument("medium lanes must be 32 or 64");
+        medium_lanes_ = lanes;
+    }
     void set_light_lanes(int lanes) {
         if (lanes != 4 && lanes != 8 && lanes != 16) throw std::invalid_argument("light lanes must be 4, 8 or 16");
         light_lanes_ = lanes;
@@ -281,6 +288,7 @@ private:
     Arena* arena_ = nullptr;
     void* pinned_ = nullptr;
     int light_lanes_ = 4;
+    int medium_lanes_ = 64;
 };
 
 }  // namespace gp

@@ -847,7 +847,13 @@ __device__ inline bool choice_better(const Choice& x, const Choice& y) {
 __global__ void __launch_bounds__(kTierThreads) k_choose(RefArgs a, ReserveState r, int attempt, std::int32_t* keys,
                                                         unsigned long long* active) {
     constexpr int W = 16;
+    __shared__ unsigned long long room[kMaxParts];  // start-of-attempt room per part
     const int lg = threadIdx.x % W;
+    if (attempt > 0) {
+        for (int p = threadIdx.x; p < a.k; p += kTierThreads)
+            room[p] = r.requests[p] < r.slack[p] ? r.slack[p] - r.requests[p] : 0;
+        __syncthreads();
+    }
     const std::int64_t groups = (std::int64_t)gridDim.x * (kTierThreads / W);
     unsigned long long cnt = 0;
     for (std::int64_t e = (blockIdx.x * (std::int64_t)kTierThreads + threadIdx.x) / W; e < r.ne; e += groups) {
@@ -859,9 +865,10 @@ __global__ void __launch_bounds__(kTierThreads) k_choose(RefArgs a, ReserveState
             else {
                 const std::uint64_t w = a.vw[v];
                 const int lv = a.labels[v];
+                const unsigned long long tried0 = r.tried[2 * e], tried1 = r.tried[2 * e + 1];
                 for (int p = lg; p < a.k; p += W) {
-                    if (p == lv || ((r.tried[2 * e + (p >> 6)] >> (p & 63)) & 1ULL)) continue;
-                    const unsigned long long avail = r.requests[p] < r.slack[p] ? r.slack[p] - r.requests[p] : 0;
+                    if (p == lv || (((p >> 6) ? tried1 : tried0) >> (p & 63)) & 1ULL) continue;
+                    const unsigned long long avail = room[p];
                     if (avail < w) continue;
                     const Choice x{p, r.conn[e * a.k + p], avail};
                     if (choice_better(x, best)) best = x;
@@ -1711,26 +1718,34 @@ public:
         GP_LAUNCH_CHECK();
         const int nb = static_cast<int>(ceil_div(ne, kKeyTile));
         ReserveState r{E.get(), ne, conn.get(), requests.get(), slack.get(), done.get(), tried.get()};
-        for (int attempt = 0; attempt < k; ++attempt) {
-            if (attempt == 1) {
-                hipLaunchKernelGGL((k_conn_rows<kWave>), dim3(grid_for(ne, 4, 2048)), dim3(256),
-                                   4 * k * sizeof(unsigned long long), s, a_, E.get(), ne, done.get(), conn.get());
+        // Attempts run in groups between host checks: an attempt without a
+        // chooser changes nothing (no tried bits, no requests, no admissions),
+        // so the attempts after the last active one are no-ops.
+        constexpr int kAttemptGroup = 4;
+        DBuf<unsigned long long> active = c_.buf<unsigned long long>(kAttemptGroup);
+        for (int attempt = 0; attempt < k;) {
+            const int group = std::min(kAttemptGroup, k - attempt);
+            fill<unsigned long long>(c_, active.get(), kAttemptGroup, 0ULL);
+            for (int j = 0; j < group; ++j, ++attempt) {
+                if (attempt == 1) {
+                    hipLaunchKernelGGL((k_conn_rows<kWave>), dim3(grid_for(ne, 4, 2048)), dim3(256),
+                                       4 * k * sizeof(unsigned long long), s, a_, E.get(), ne, done.get(), conn.get());
+                    GP_LAUNCH_CHECK();
+                }
+                hipLaunchKernelGGL(k_choose, dim3(grid_for(ne, kTierThreads / 16, 4096)), dim3(kTierThreads), 0, s, a_,
+                                   r, attempt, keys.get(), active.get() + j);
+                GP_LAUNCH_CHECK();
+                hipLaunchKernelGGL((k_keyed_sums<DonorWeight>), dim3(nb), dim3(kTierThreads), 0, s, keys.get(), ne, k,
+                                   nb, DonorWeight{a_, E.get()}, S.get());
+                GP_LAUNCH_CHECK();
+                // Ranks start at the start-of-attempt totals; totals advance.
+                hipLaunchKernelGGL(k_keyed_scan, dim3(1), dim3(kKeyScanThreads), 0, s, S.get(), nb, k, requests.get());
+                GP_LAUNCH_CHECK();
+                hipLaunchKernelGGL((k_keyed_apply<AdmitFn>), dim3(nb), dim3(kTierThreads), 0, s, keys.get(), ne,
+                                   S.get(), nb, AdmitFn{a_, r}, movers_.get(), ctr_.get() + kMovers);
                 GP_LAUNCH_CHECK();
             }
-            fill<unsigned long long>(c_, ctr_.get() + kActive, 1, 0ULL);
-            hipLaunchKernelGGL(k_choose, dim3(grid_for(ne, kTierThreads / 16, 4096)), dim3(kTierThreads), 0, s, a_, r,
-                               attempt, keys.get(), ctr_.get() + kActive);
-            GP_LAUNCH_CHECK();
-            if (!c_.read_scalar(ctr_.get() + kActive)) break;
-            hipLaunchKernelGGL((k_keyed_sums<DonorWeight>), dim3(nb), dim3(kTierThreads), 0, s, keys.get(), ne, k, nb,
-                               DonorWeight{a_, E.get()}, S.get());
-            GP_LAUNCH_CHECK();
-            // Ranks start at the start-of-attempt totals; totals advance.
-            hipLaunchKernelGGL(k_keyed_scan, dim3(1), dim3(kKeyScanThreads), 0, s, S.get(), nb, k, requests.get());
-            GP_LAUNCH_CHECK();
-            hipLaunchKernelGGL((k_keyed_apply<AdmitFn>), dim3(nb), dim3(kTierThreads), 0, s, keys.get(), ne, S.get(),
-                               nb, AdmitFn{a_, r}, movers_.get(), ctr_.get() + kMovers);
-            GP_LAUNCH_CHECK();
+            if (!c_.read_scalar(active.get() + group - 1)) break;
         }
     }
 

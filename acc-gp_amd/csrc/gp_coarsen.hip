@@ -1242,7 +1242,7 @@ void launch_classes(Context& c, const std::int32_t* lists, std::int64_t stride, 
 // Global reduce of (row,key) records: sort, unique, per-run sums.
 static std::int64_t reduce_records(Context& c, DBuf<std::uint64_t>& keys, DBuf<std::uint32_t>& vals, std::int64_t count,
                                    int total_bits, DBuf<std::uint64_t>& ukeys, DBuf<unsigned long long>& usums) {
-    radix_sort_pairs(c, keys.get(), vals.get(), count, 0, total_bits);
+    radix_sort_pairs(c, keys, vals, count, 0, total_bits);
     DBuf<std::uint8_t> heads = c.buf<std::uint8_t>(static_cast<std::size_t>(count));
     DBuf<std::int64_t> pos = c.buf<std::int64_t>(static_cast<std::size_t>(count));
     DBuf<std::int64_t> total = c.buf<std::int64_t>(1);
@@ -1508,19 +1508,19 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
                            vals.get());
         GP_LAUNCH_CHECK();
         if (fits) {
-            radix_sort_pairs(c, keys.get(), vals.get(), count, 0, 32 + score_bits + target_bits);
+            radix_sort_pairs(c, keys, vals, count, 0, 32 + score_bits + target_bits);
         } else {
-            radix_sort_pairs(c, keys.get(), vals.get(), count, 0, 32 + low_bits);
+            radix_sort_pairs(c, keys, vals, count, 0, 32 + low_bits);
             if (score_bits > low_bits) {
                 hipLaunchKernelGGL(k_key_field, dim3(grid_for(count)), dim3(256), 0, s, count, vals.get(),
                                    proposals.get(), score, 1, low_bits, keys.get());
                 GP_LAUNCH_CHECK();
-                radix_sort_pairs(c, keys.get(), vals.get(), count, 0, score_bits - low_bits);
+                radix_sort_pairs(c, keys, vals, count, 0, score_bits - low_bits);
             }
             hipLaunchKernelGGL(k_key_field, dim3(grid_for(count)), dim3(256), 0, s, count, vals.get(),
                                proposals.get(), score, 0, 0, keys.get());
             GP_LAUNCH_CHECK();
-            radix_sort_pairs(c, keys.get(), vals.get(), count, 0, target_bits);
+            radix_sort_pairs(c, keys, vals, count, 0, target_bits);
         }
         return count;
     };

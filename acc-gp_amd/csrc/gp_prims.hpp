@@ -256,82 +256,218 @@ static __global__ void k_radix_hist(const std::uint64_t* keys, std::int64_t n, i
     hist[(std::int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter.  Elements of a tile are ranked in index order: rounds of
-// 256 consecutive keys, wave-level peers via 8 ballots, per-wave digit counts.
-static __global__ void k_radix_scatter(const std::uint64_t* keys_in, const std::uint32_t* vals_in,
-                                       std::uint64_t* keys_out, std::uint32_t* vals_out, std::int64_t n, int shift,
-                                       const std::uint32_t* offsets, int nblocks) {
+// Stable scatter of one tile.  Each wave ranks a contiguous quarter of the
+// tile (16 rounds of 64 keys, peers by 8 ballots) against wave-private digit
+// counters, so ranking needs no block barrier; the tile is then reordered by
+// digit in LDS and written out in index order, which makes the stores of
+// one digit run consecutive.
+static __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
+    const std::uint64_t* keys_in, const std::uint32_t* vals_in, std::uint64_t* keys_out, std::uint32_t* vals_out,
+    std::int64_t n, int shift, const std::uint32_t* offsets, int nblocks) {
     constexpr int kWaves = kRadixThreads / kWave;
-    __shared__ std::uint32_t running[kRadixBins];
+    constexpr int kPerWave = kRadixTile / kWaves;
+    constexpr int kRounds = kPerWave / kWave;
     __shared__ std::uint32_t wcount[kWaves][kRadixBins];
+    __shared__ std::uint32_t start[kRadixBins];
+    __shared__ std::uint32_t gbase[kRadixBins];
+    __shared__ std::uint32_t wsum[kWaves];
+    __shared__ std::uint64_t skeys[kRadixTile];
+    __shared__ std::uint32_t svals[kRadixTile];
     const int t = threadIdx.x, w = t / kWave, lane = t % kWave;
-    running[t] = offsets[(std::int64_t)t * nblocks + blockIdx.x];
     for (int i = 0; i < kWaves; ++i) wcount[i][t] = 0;
     __syncthreads();
-    const std::int64_t base = blockIdx.x * (std::int64_t)kRadixTile;
+    const std::int64_t tile = blockIdx.x * (std::int64_t)kRadixTile;
+    const std::int64_t wbase = tile + (std::int64_t)w * kPerWave;
     const unsigned long long lt = (1ULL << lane) - 1ULL;
-    for (int r = 0; r < kRadixItems; ++r) {
-        const std::int64_t i = base + r * kRadixThreads + t;
-        const bool valid = i < n;
-        std::uint64_t key = valid ? keys_in[i] : 0;
-        std::uint32_t val = valid && vals_in ? vals_in[i] : 0;
-        const std::uint32_t d = valid ? static_cast<std::uint32_t>((key >> shift) & 0xff) : 0;
+    std::uint64_t key[kRounds];
+    std::uint32_t val[kRounds];
+    std::uint32_t off[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const std::int64_t i = wbase + r * kWave + lane;
+        key[r] = i < n ? keys_in[i] : 0;
+        val[r] = i < n && vals_in ? vals_in[i] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const bool valid = wbase + r * kWave + lane < n;
+        const std::uint32_t d = static_cast<std::uint32_t>((key[r] >> shift) & 0xff);
         unsigned long long peers = __ballot(valid);
         for (int b = 0; b < 8; ++b) {
             const bool bit = (d >> b) & 1u;
             const unsigned long long bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
         }
-        const std::uint32_t rank = static_cast<std::uint32_t>(__popcll(peers & lt));
-        if (valid && rank == 0) wcount[w][d] = static_cast<std::uint32_t>(__popcll(peers));
-        __syncthreads();
-        if (valid) {
-            std::uint32_t off = running[d];
-            for (int j = 0; j < w; ++j) off += wcount[j][d];
-            off += rank;
-            keys_out[off] = key;
-            if (vals_out) vals_out[off] = val;
+        // Program order of one wave's LDS accesses keeps the read of the
+        // running count ahead of the leader's update.
+        const std::uint32_t before = valid ? wcount[w][d] : 0;
+        off[r] = before + static_cast<std::uint32_t>(__popcll(peers & lt));
+        if (valid && (peers & lt) == 0) wcount[w][d] = before + static_cast<std::uint32_t>(__popcll(peers));
+    }
+    __syncthreads();
+    std::uint32_t run = 0;
+    for (int j = 0; j < kWaves; ++j) {
+        const std::uint32_t c = wcount[j][t];
+        wcount[j][t] = run;
+        run += c;
+    }
+    std::uint32_t inc, exc, tot;
+    block_scan<PlusOp>(run, wsum, inc, exc, tot);
+    start[t] = exc;
+    gbase[t] = offsets[(std::int64_t)t * nblocks + blockIdx.x] - exc;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        if (wbase + r * kWave + lane < n) {
+            const std::uint32_t d = static_cast<std::uint32_t>((key[r] >> shift) & 0xff);
+            const std::uint32_t p = start[d] + wcount[w][d] + off[r];
+            skeys[p] = key[r];
+            svals[p] = val[r];
         }
-        __syncthreads();
-        std::uint32_t add = 0;
-        for (int j = 0; j < kWaves; ++j) {
-            add += wcount[j][t];
-            wcount[j][t] = 0;
-        }
-        running[t] += add;
-        __syncthreads();
+    }
+    __syncthreads();
+    const int count = static_cast<int>(std::min<std::int64_t>(kRadixTile, n - tile));
+    for (int i = t; i < count; i += kRadixThreads) {
+        const std::uint64_t k = skeys[i];
+        const std::uint32_t g = gbase[(k >> shift) & 0xff] + static_cast<std::uint32_t>(i);
+        keys_out[g] = k;
+        if (vals_out) vals_out[g] = svals[i];
     }
 }
 
-// Sorts (keys, vals) by bits [begin_bit, end_bit).  Uses alt buffers; the
-// result is copied back into keys/vals.  vals may be null.
-inline void radix_sort_pairs(Context& c, std::uint64_t* keys, std::uint32_t* vals, std::int64_t n, int begin_bit,
-                             int end_bit) {
+// Whole sort of at most kSortTileMax keys in one block: the keys stay in
+// registers and LDS across all digit passes (same ranking as the scatter
+// pass), so a short sort costs one launch instead of three per digit.  A
+// pass whose digit is equal for every key is skipped (stable identity).
+constexpr int kSortTileThreads = 1024;
+constexpr int kSortTileRounds = 8;
+constexpr int kSortTileMax = kSortTileThreads * kSortTileRounds;
+
+static __global__ __launch_bounds__(kSortTileThreads) void k_radix_sort_tile(std::uint64_t* keys, std::uint32_t* vals,
+                                                                             int n, int begin_bit, int end_bit) {
+    constexpr int kWaves = kSortTileThreads / kWave;
+    constexpr int kPerWave = kSortTileRounds * kWave;
+    constexpr int kScanWaves = kRadixBins / kWave;
+    __shared__ std::uint32_t wcount[kWaves][kRadixBins];
+    __shared__ std::uint32_t start[kRadixBins];
+    __shared__ std::uint32_t wsum[kScanWaves];
+    __shared__ int same;
+    __shared__ std::uint64_t skeys[kSortTileMax];
+    __shared__ std::uint32_t svals[kSortTileMax];
+    const int t = threadIdx.x, w = t / kWave, lane = t % kWave;
+    const int wbase = w * kPerWave;
+    const unsigned long long lt = (1ULL << lane) - 1ULL;
+    std::uint64_t key[kSortTileRounds];
+    std::uint32_t val[kSortTileRounds];
+    std::uint32_t off[kSortTileRounds];
+#pragma unroll
+    for (int r = 0; r < kSortTileRounds; ++r) {
+        const int i = wbase + r * kWave + lane;
+        key[r] = i < n ? keys[i] : 0;
+        val[r] = i < n && vals ? vals[i] : 0;
+    }
+    // same holds shift + 1 of a pass whose digit is constant; it is written
+    // after the first barrier of a pass, which every thread passes only
+    // after reading it in the previous pass, so it needs no reset.
+    if (t == 0) same = -1;
+    for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        for (int i = t; i < kWaves * kRadixBins; i += kSortTileThreads) wcount[i / kRadixBins][i % kRadixBins] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSortTileRounds; ++r) {
+            const bool valid = wbase + r * kWave + lane < n;
+            const std::uint32_t d = static_cast<std::uint32_t>((key[r] >> shift) & 0xff);
+            unsigned long long peers = __ballot(valid);
+            for (int b = 0; b < 8; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const unsigned long long bal = __ballot(bit);
+                peers &= bit ? bal : ~bal;
+            }
+            const std::uint32_t before = valid ? wcount[w][d] : 0;
+            off[r] = before + static_cast<std::uint32_t>(__popcll(peers & lt));
+            if (valid && (peers & lt) == 0) wcount[w][d] = before + static_cast<std::uint32_t>(__popcll(peers));
+        }
+        __syncthreads();
+        std::uint32_t run = 0, x = 0;
+        if (t < kRadixBins) {
+            for (int j = 0; j < kWaves; ++j) {
+                const std::uint32_t c = wcount[j][t];
+                wcount[j][t] = run;
+                run += c;
+            }
+            if (run == static_cast<std::uint32_t>(n)) same = shift + 1;
+            x = run;
+            for (int d = 1; d < kWave; d <<= 1) {
+                const std::uint32_t y = __shfl_up(x, d);
+                if (lane >= d) x += y;
+            }
+            if (lane == kWave - 1) wsum[w] = x;
+        }
+        __syncthreads();
+        if (same == shift + 1) continue;
+        if (t < kRadixBins) {
+            std::uint32_t pre = 0;
+            for (int j = 0; j < w; ++j) pre += wsum[j];
+            start[t] = pre + x - run;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSortTileRounds; ++r) {
+            if (wbase + r * kWave + lane < n) {
+                const std::uint32_t d = static_cast<std::uint32_t>((key[r] >> shift) & 0xff);
+                const std::uint32_t p = start[d] + wcount[w][d] + off[r];
+                skeys[p] = key[r];
+                svals[p] = val[r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSortTileRounds; ++r) {
+            const int i = wbase + r * kWave + lane;
+            if (i < n) {
+                key[r] = skeys[i];
+                val[r] = svals[i];
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kSortTileRounds; ++r) {
+        const int i = wbase + r * kWave + lane;
+        if (i < n) {
+            keys[i] = key[r];
+            if (vals) vals[i] = val[r];
+        }
+    }
+}
+
+// Sorts (keys, vals) by bits [begin_bit, end_bit), stably.  The sorted
+// data ends in whichever of the two buffer pairs the last pass wrote; the
+// DBufs are exchanged rather than copied back.  vals may be empty.
+inline void radix_sort_pairs(Context& c, DBuf<std::uint64_t>& keys, DBuf<std::uint32_t>& vals, std::int64_t n,
+                             int begin_bit, int end_bit) {
     if (n <= 1 || end_bit <= begin_bit) return;
     if (n > 0xffffffffLL) throw std::invalid_argument("radix sort size exceeds 32-bit offsets");
+    if (n <= kSortTileMax) {
+        hipLaunchKernelGGL(k_radix_sort_tile, dim3(1), dim3(kSortTileThreads), 0, c.stream(), keys.get(), vals.get(),
+                           static_cast<int>(n), begin_bit, end_bit);
+        GP_LAUNCH_CHECK();
+        return;
+    }
     const int nblocks = ceil_div(n, kRadixTile);
     DBuf<std::uint64_t> k2 = c.buf<std::uint64_t>(static_cast<std::size_t>(n));
     DBuf<std::uint32_t> v2;
-    if (vals) v2 = c.buf<std::uint32_t>(static_cast<std::size_t>(n));
+    if (vals.get()) v2 = c.buf<std::uint32_t>(static_cast<std::size_t>(n));
     DBuf<std::uint32_t> hist = c.buf<std::uint32_t>(static_cast<std::size_t>(kRadixBins) * nblocks);
-    std::uint64_t* kin = keys;
-    std::uint64_t* kout = k2.get();
-    std::uint32_t* vin = vals;
-    std::uint32_t* vout = v2.get();
     for (int shift = begin_bit; shift < end_bit; shift += 8) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kRadixThreads), 0, c.stream(), kin, n, shift,
+        hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kRadixThreads), 0, c.stream(), keys.get(), n, shift,
                            hist.get(), nblocks);
         GP_LAUNCH_CHECK();
         exclusive_sum<std::uint32_t>(c, hist.get(), hist.get(), static_cast<std::int64_t>(kRadixBins) * nblocks);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kRadixThreads), 0, c.stream(), kin, vin, kout,
-                           vout, n, shift, hist.get(), nblocks);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kRadixThreads), 0, c.stream(), keys.get(), vals.get(),
+                           k2.get(), v2.get(), n, shift, hist.get(), nblocks);
         GP_LAUNCH_CHECK();
-        std::swap(kin, kout);
-        std::swap(vin, vout);
-    }
-    if (kin != keys) {
-        c.copy_d2d(keys, kin, static_cast<std::size_t>(n) * sizeof(std::uint64_t));
-        if (vals) c.copy_d2d(vals, vin, static_cast<std::size_t>(n) * sizeof(std::uint32_t));
+        keys.swap(k2);
+        vals.swap(v2);
     }
 }
 

@@ -31,6 +31,10 @@ def _cases(gp):
     cases.append(("knn3000_k16_weighted", off, nbr, ew, vw, 16))
     off, nbr = gp.generate_rmat(15, 16, seed=1)
     cases.append(("rmat15_k64", off, nbr, None, None, 64))
+    off, nbr, ew, vw = gp.generate_knn3d(20000, 10, seed=4)
+    cases.append(("knn20000_k128_weighted", off, nbr, ew, vw, 128))
+    off, nbr = gp.generate_rmat(11, 8, seed=9)
+    cases.append(("rmat11_k2", off, nbr, None, None, 2))
     return cases
 
 

@@ -1036,15 +1036,21 @@ __global__ void k_narrow_weights(std::int64_t n, const unsigned long long* sums,
 
 // ----------------------------------------------------------- contraction
 // contract_record_bounds (src/coarsen.cu:101-119) plus raw coarse-row sizes.
+// plain: the host has shown that no sum can saturate (n * max bound fits
+// 64 bits), so the saturating compare-and-swap loop reduces to an add.
 __global__ void k_contract_bounds(std::int64_t n, const std::int32_t* map, const std::int32_t* off,
                                   const std::uint64_t* fine_bounds, unsigned long long* coarse_bounds,
-                                  unsigned long long* raw_counts) {
+                                  unsigned long long* raw_counts, bool plain) {
     for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
          v += (std::int64_t)gridDim.x * blockDim.x) {
         const std::int32_t c = map[v];
         atomicAdd(&raw_counts[c], (unsigned long long)(off[v + 1] - off[v]));
         unsigned long long* target = &coarse_bounds[c];
         const unsigned long long amount = fine_bounds[v];
+        if (plain) {
+            atomicAdd(target, amount);
+            continue;
+        }
         unsigned long long prev = *target;
         while (true) {
             const unsigned long long sum = ~0ULL - prev < amount ? ~0ULL : prev + amount;
@@ -1135,19 +1141,24 @@ __global__ void k_rows_group(RowArgs r, const std::int32_t* list, std::int64_t c
     }
 }
 
+// Coarse keys are below 2^key_bits, so an LDS radix sort needs
+// ceil((key_bits + 1) / 8) passes.
 __global__ void __launch_bounds__(kBlockSortThreads) k_rows_block(RowArgs r, const std::int32_t* list,
-                                                                  std::int64_t count) {
+                                                                  std::int64_t count, int key_bits) {
     __shared__ BlockSortSmem s;
+    __shared__ BlockRadixSmem x;
     for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
         const std::int32_t row = list[idx];
         const std::int64_t b = r.raw_off[row];
         const int d = (int)(r.raw_off[row + 1] - b);
-        for (int x = threadIdx.x; x < d; x += kBlockSortThreads) {
-            s.key[x] = r.rkey[b + x];
-            s.val[x] = r.rval[b + x];
+        for (int i = threadIdx.x; i < d; i += kBlockSortThreads) {
+            s.key[i] = r.rkey[b + i];
+            s.val[i] = r.rval[b + i];
         }
         __syncthreads();
-        const int u = block_sort_reduce(s, d, [&](std::int32_t key, std::uint64_t sum, int i) {
+        block_radix_sort(s, x, d, key_bits);
+        const int N = block_pad(s, d);
+        const int u = block_reduce_sorted(s, N, [&](std::int32_t key, std::uint64_t sum, int i) {
             emit_row_entry(r, b, key, sum, i);
         });
         if (threadIdx.x == 0) r.ucount[row] = (unsigned long long)u;
@@ -1476,7 +1487,7 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
 
     auto count_clusters = [&]() {
         fill<unsigned long long>(c, scalar.get(), 1, 0ULL);
-        hipLaunchKernelGGL(k_count_nonzero, dim3(vb), dim3(256), 0, s, n, cw.get(), scalar.get());
+        hipLaunchKernelGGL(k_count_nonzero, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, n, cw.get(), scalar.get());
         GP_LAUNCH_CHECK();
         return (std::int64_t)c.read_scalar(scalar.get());
     };
@@ -1680,8 +1691,9 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
     DBuf<unsigned long long> rawc = c.buf<unsigned long long>(static_cast<std::size_t>(nc));
     fill<unsigned long long>(c, next_rb.get(), nc, 0ULL);
     fill<unsigned long long>(c, rawc.get(), nc, 0ULL);
+    const bool plain_bounds = st.max_row_weight == 0 || (std::uint64_t)n <= ~0ULL / st.max_row_weight;
     hipLaunchKernelGGL(k_contract_bounds, dim3(grid_for(n)), dim3(256), 0, s, n, map, fine.off.get(), st.row_bound.get(),
-                       next_rb.get(), rawc.get());
+                       next_rb.get(), rawc.get(), plain_bounds);
     GP_LAUNCH_CHECK();
     DBuf<unsigned long long> red = c.buf<unsigned long long>(2);
     fill<unsigned long long>(c, red.get(), 2, 0ULL);
@@ -1754,7 +1766,8 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
     };
     auto block = [&](const std::int32_t* list, std::int64_t count) {
         if (!count) return;
-        hipLaunchKernelGGL(k_rows_block, dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, s, r, list, count);
+        hipLaunchKernelGGL(k_rows_block, dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, s, r, list, count,
+                           std::max(1, bit_width_u64(static_cast<std::uint64_t>(nc))));
         GP_LAUNCH_CHECK();
     };
     launch_classes(c, lists.get(), nc, cc.c, group, block);

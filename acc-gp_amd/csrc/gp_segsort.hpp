@@ -105,10 +105,8 @@ struct NoBegin {
     __device__ void operator()(int) const {}
 };
 
-// begin(total) is called by every thread once the emission count is known,
-// before the first emit.
-template <typename Emit, typename Begin = NoBegin>
-__device__ inline int block_sort_reduce(BlockSortSmem& s, int n, Emit&& emit, Begin&& begin = Begin{}) {
+// Pads [n, N) with sentinels; N is a power of two >= max(n, 512).
+__device__ inline int block_pad(BlockSortSmem& s, int n) {
     int N = 2;
     while (N < n) N <<= 1;
     if (N < kBlockSortThreads * 2) N = kBlockSortThreads * 2;
@@ -117,6 +115,10 @@ __device__ inline int block_sort_reduce(BlockSortSmem& s, int n, Emit&& emit, Be
         s.val[i] = 0;
     }
     __syncthreads();
+    return N;
+}
+
+__device__ inline void block_bitonic(BlockSortSmem& s, int N) {
     for (int size = 2; size <= N; size <<= 1) {
         for (int dist = size >> 1; dist > 0; dist >>= 1) {
             for (int p = threadIdx.x; p < N / 2; p += kBlockSortThreads) {
@@ -135,6 +137,110 @@ __device__ inline int block_sort_reduce(BlockSortSmem& s, int n, Emit&& emit, Be
             __syncthreads();
         }
     }
+}
+
+// Scratch of block_radix_sort.
+struct BlockRadixSmem {
+    std::uint32_t wcount[kBlockSortThreads / kWave][256];
+    std::uint32_t wsum[kBlockSortThreads / kWave];
+    std::uint32_t start[256];
+    int same;
+};
+
+// Stable LSD sort of s.key[0, n) (values along) for keys in [0, 2^key_bits)
+// or kSentinel, which orders last (ranked as 2^key_bits): ceil((key_bits +
+// 1) / 8) passes of 8 bits instead of the log^2 stages of a bitonic network.
+// Each wave ranks a contiguous slice in registers against wave-private
+// digit counters; a pass whose digit is equal for all keys is skipped.
+__device__ inline void block_radix_sort(BlockSortSmem& s, BlockRadixSmem& x, int n, int key_bits) {
+    constexpr int kRounds = kBlockSortMax / kBlockSortThreads;
+    const int t = threadIdx.x, w = t / kWave, lane = t % kWave;
+    const int rounds = (n + kBlockSortThreads - 1) / kBlockSortThreads;
+    const int wbase = w * rounds * kWave;
+    const unsigned long long lt = (1ULL << lane) - 1ULL;
+    const std::uint32_t top = 1u << key_bits;
+    std::uint32_t key[kRounds], val[kRounds], off[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int i = wbase + r * kWave + lane;
+        key[r] = 0;
+        val[r] = 0;
+        if (r < rounds && i < n) {
+            const std::int32_t k = s.key[i];
+            key[r] = k == kSentinel ? top : static_cast<std::uint32_t>(k);
+            val[r] = s.val[i];
+        }
+    }
+    // same holds shift + 1 of a skipped pass; it is written after the first
+    // barrier of a pass, which every thread passes only after reading it.
+    if (t == 0) x.same = -1;
+    bool moved = false;
+    for (int shift = 0; shift <= key_bits; shift += 8) {
+        for (int i = t; i < (kBlockSortThreads / kWave) * 256; i += kBlockSortThreads) x.wcount[i / 256][i % 256] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            const bool valid = r < rounds && wbase + r * kWave + lane < n;
+            const std::uint32_t d = (key[r] >> shift) & 0xffu;
+            unsigned long long peers = __ballot(valid);
+            for (int b = 0; b < 8; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const unsigned long long bal = __ballot(bit);
+                peers &= bit ? bal : ~bal;
+            }
+            const std::uint32_t before = valid ? x.wcount[w][d] : 0;
+            off[r] = before + static_cast<std::uint32_t>(__popcll(peers & lt));
+            if (valid && (peers & lt) == 0) x.wcount[w][d] = before + static_cast<std::uint32_t>(__popcll(peers));
+        }
+        __syncthreads();
+        std::uint32_t run = 0;
+        for (int j = 0; j < kBlockSortThreads / kWave; ++j) {
+            const std::uint32_t c = x.wcount[j][t];
+            x.wcount[j][t] = run;
+            run += c;
+        }
+        if (run == static_cast<std::uint32_t>(n)) x.same = shift + 1;
+        std::uint32_t inc, exc, tot;
+        block_scan<PlusOp>(run, x.wsum, inc, exc, tot);
+        if (x.same == shift + 1) continue;
+        x.start[t] = exc;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            if (r < rounds && wbase + r * kWave + lane < n) {
+                const std::uint32_t d = (key[r] >> shift) & 0xffu;
+                const std::uint32_t p = x.start[d] + x.wcount[w][d] + off[r];
+                s.key[p] = static_cast<std::int32_t>(key[r]);
+                s.val[p] = val[r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            const int i = wbase + r * kWave + lane;
+            if (r < rounds && i < n) {
+                key[r] = static_cast<std::uint32_t>(s.key[i]);
+                val[r] = s.val[i];
+            }
+        }
+        moved = true;
+    }
+    if (moved) {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            const int i = wbase + r * kWave + lane;
+            if (r < rounds && i < n) s.key[i] = key[r] == top ? kSentinel : static_cast<std::int32_t>(key[r]);
+        }
+    }
+    __syncthreads();
+}
+
+// Run reduction of sorted s.key[0, N) (N from block_pad).
+// begin(total) is called by every thread once the emission count is known,
+// before the first emit.
+template <typename Emit, typename Begin = NoBegin>
+__device__ inline int block_reduce_sorted(BlockSortSmem& s, int N, Emit&& emit, Begin&& begin = Begin{}) {
     // Segmented scan: each thread owns E consecutive elements.
     const int E = N / kBlockSortThreads;
     const int b0 = threadIdx.x * E;
@@ -175,6 +281,13 @@ __device__ inline int block_sort_reduce(BlockSortSmem& s, int n, Emit&& emit, Be
     }
     __syncthreads();
     return total;
+}
+
+template <typename Emit, typename Begin = NoBegin>
+__device__ inline int block_sort_reduce(BlockSortSmem& s, int n, Emit&& emit, Begin&& begin = Begin{}) {
+    const int N = block_pad(s, n);
+    block_bitonic(s, N);
+    return block_reduce_sorted(s, N, static_cast<Emit&&>(emit), static_cast<Begin&&>(begin));
 }
 
 // Block-wide aggregation of (key >= 0, u32 weight) pairs in an LDS hash

@@ -826,14 +826,18 @@ struct ReserveState {
     const unsigned long long* conn;  // ne x k
     unsigned long long* requests;    // k, request totals
     const unsigned long long* slack; // k
-    std::uint8_t* done;
+    std::uint8_t* done;              // 1 admitted, 2 without a part left
     unsigned long long* tried;       // ne x 2
+    const std::int32_t* live;        // donors still choosing, ascending
+    std::int64_t nlive;
 };
 
-// Choices of one attempt, one 16-lane group per donor.  Attempt 0
+// Choices of one attempt, one 16-lane group per live donor.  Attempt 0
 // requests the proposed target; later attempts the untried part with the
 // most connection, then the most room left at the start of the attempt,
-// then the lowest id.
+// then the lowest id.  Room only shrinks (request totals only grow) and
+// tried parts stay tried, so a donor without a choice in some attempt has
+// none in any later one: it is marked done (2) and leaves the live list.
 struct Choice {
     int p;
     unsigned long long conn, room;
@@ -856,9 +860,11 @@ __global__ void __launch_bounds__(kTierThreads) k_choose(RefArgs a, ReserveState
     }
     const std::int64_t groups = (std::int64_t)gridDim.x * (kTierThreads / W);
     unsigned long long cnt = 0;
-    for (std::int64_t e = (blockIdx.x * (std::int64_t)kTierThreads + threadIdx.x) / W; e < r.ne; e += groups) {
+    for (std::int64_t i = (blockIdx.x * (std::int64_t)kTierThreads + threadIdx.x) / W; i < r.nlive; i += groups) {
+        const std::int64_t e = r.live[i];
         Choice best{-1, 0, 0};
-        if (!r.done[e]) {
+        bool open = !r.done[e];
+        if (open) {
             const std::int32_t v = r.E[e];
             if (attempt == 0)
                 best.p = a.targets[v];
@@ -880,8 +886,9 @@ __global__ void __launch_bounds__(kTierThreads) k_choose(RefArgs a, ReserveState
             }
         }
         if (lg == 0) {
-            keys[e] = best.p;
+            keys[i] = best.p;
             cnt += best.p >= 0;
+            if (open && attempt > 0 && best.p < 0) r.done[e] = 2;
         }
     }
     cnt = wave_sum(cnt);
@@ -891,7 +898,8 @@ __global__ void __launch_bounds__(kTierThreads) k_choose(RefArgs a, ReserveState
 struct DonorWeight {
     RefArgs a;
     const std::int32_t* E;
-    __device__ std::uint64_t weight(std::int64_t i) const { return a.vw[E[i]]; }
+    const std::int32_t* live;
+    __device__ std::uint64_t weight(std::int64_t i) const { return a.vw[E[live[i]]]; }
 };
 
 template <class F>
@@ -906,9 +914,10 @@ __global__ void __launch_bounds__(kTierThreads) k_keyed_sums(const std::int32_t*
 struct AdmitFn {
     RefArgs a;
     ReserveState r;
-    __device__ std::uint64_t weight(std::int64_t e) const { return a.vw[r.E[e]]; }
-    __device__ std::int32_t vertex(std::int64_t e) const { return r.E[e]; }
-    __device__ bool operator()(std::int64_t e, int c, std::uint64_t w, std::uint64_t begin) const {
+    __device__ std::uint64_t weight(std::int64_t i) const { return a.vw[r.E[r.live[i]]]; }
+    __device__ std::int32_t vertex(std::int64_t i) const { return r.E[r.live[i]]; }
+    __device__ bool operator()(std::int64_t i, int c, std::uint64_t w, std::uint64_t begin) const {
+        const std::int64_t e = r.live[i];
         r.tried[2 * e + (c >> 6)] |= 1ULL << (c & 63);
         if (begin <= r.slack[c] && w <= r.slack[c] - begin) {
             const std::int32_t v = r.E[e];
@@ -925,6 +934,19 @@ __global__ void k_gather_i32(const std::int32_t* src, const std::int32_t* idx, s
     for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
          j += (std::int64_t)gridDim.x * blockDim.x)
         dst[j] = src[idx[j]];
+}
+
+__global__ void k_iota_i32(std::int32_t* out, std::int64_t count) {
+    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
+         j += (std::int64_t)gridDim.x * blockDim.x)
+        out[j] = static_cast<std::int32_t>(j);
+}
+
+__global__ void k_live_flags(const std::int32_t* live, std::int64_t count, const std::uint8_t* done,
+                             std::uint8_t* flags) {
+    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
+         j += (std::int64_t)gridDim.x * blockDim.x)
+        flags[j] = done[live[j]] == 0;
 }
 
 __global__ void k_slack(RefArgs a, unsigned long long* slack, unsigned long long* requests) {
@@ -1471,15 +1493,6 @@ __global__ void k_loads(std::int64_t n, const std::uint32_t* vw, const L* labels
          v += (std::int64_t)gridDim.x * blockDim.x)
         atomicAdd(&loads[labels[v]], (unsigned long long)vw[v]);
 }
-__global__ void k_max_u32(std::int64_t n, const std::uint32_t* x, unsigned long long* out) {
-    unsigned long long m = 0;
-    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (std::int64_t)gridDim.x * blockDim.x)
-        m = m > x[i] ? m : x[i];
-    m = wave_max(m);
-    if (lane_id() == 0 && m) atomicMax(out, m);
-}
-
 template <typename A, typename B>
 __global__ void k_convert(std::int64_t n, const A* in, B* out) {
     for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
@@ -1611,7 +1624,8 @@ public:
             DBuf<unsigned long long> mx = c.buf<unsigned long long>(1);
             fill<unsigned long long>(c, mx.get(), 1, 0ULL);
             if (g.m > 0) {
-                hipLaunchKernelGGL(k_max_u32, dim3(grid_for(g.m)), dim3(256), 0, c.stream(), g.m, g.ew.get(), mx.get());
+                hipLaunchKernelGGL((k_reduce_max<std::uint32_t>), dim3(grid_for(g.m, 256, 2048)), dim3(256), 0, c.stream(),
+                                   g.ew.get(), g.m, mx.get());
                 GP_LAUNCH_CHECK();
             }
             max_edge = c.read_scalar(mx.get());
@@ -1716,15 +1730,22 @@ public:
         fill<unsigned long long>(c_, tried.get(), 2 * ne, 0ULL);
         hipLaunchKernelGGL(k_slack, dim3(1), dim3(kMaxParts), 0, s, a_, slack.get(), requests.get());
         GP_LAUNCH_CHECK();
-        const int nb = static_cast<int>(ceil_div(ne, kKeyTile));
-        ReserveState r{E.get(), ne, conn.get(), requests.get(), slack.get(), done.get(), tried.get()};
+        DBuf<std::int32_t> live = c_.buf<std::int32_t>(static_cast<std::size_t>(ne));
+        hipLaunchKernelGGL(k_iota_i32, dim3(grid_for(ne)), dim3(256), 0, s, live.get(), ne);
+        GP_LAUNCH_CHECK();
+        ReserveState r{E.get(), ne, conn.get(), requests.get(), slack.get(), done.get(), tried.get(), live.get(), ne};
         // Attempts run in groups between host checks: an attempt without a
         // chooser changes nothing (no tried bits, no requests, no admissions),
-        // so the attempts after the last active one are no-ops.
+        // so the attempts after the last active one are no-ops.  Between
+        // groups the live list drops donors that are admitted or have no
+        // part left; it stays in ascending order, so ranks are unchanged.
         constexpr int kAttemptGroup = 4;
+        constexpr std::int64_t kCompactMin = 4096;
         DBuf<unsigned long long> active = c_.buf<unsigned long long>(kAttemptGroup);
         for (int attempt = 0; attempt < k;) {
             const int group = std::min(kAttemptGroup, k - attempt);
+            const std::int64_t nlv = r.nlive;
+            const int nb = static_cast<int>(ceil_div(nlv, kKeyTile));
             fill<unsigned long long>(c_, active.get(), kAttemptGroup, 0ULL);
             for (int j = 0; j < group; ++j, ++attempt) {
                 if (attempt == 1) {
@@ -1732,20 +1753,37 @@ public:
                                        4 * k * sizeof(unsigned long long), s, a_, E.get(), ne, done.get(), conn.get());
                     GP_LAUNCH_CHECK();
                 }
-                hipLaunchKernelGGL(k_choose, dim3(grid_for(ne, kTierThreads / 16, 4096)), dim3(kTierThreads), 0, s, a_,
+                hipLaunchKernelGGL(k_choose, dim3(grid_for(nlv, kTierThreads / 16, 4096)), dim3(kTierThreads), 0, s, a_,
                                    r, attempt, keys.get(), active.get() + j);
                 GP_LAUNCH_CHECK();
-                hipLaunchKernelGGL((k_keyed_sums<DonorWeight>), dim3(nb), dim3(kTierThreads), 0, s, keys.get(), ne, k,
-                                   nb, DonorWeight{a_, E.get()}, S.get());
+                hipLaunchKernelGGL((k_keyed_sums<DonorWeight>), dim3(nb), dim3(kTierThreads), 0, s, keys.get(), nlv, k,
+                                   nb, DonorWeight{a_, E.get(), r.live}, S.get());
                 GP_LAUNCH_CHECK();
                 // Ranks start at the start-of-attempt totals; totals advance.
                 hipLaunchKernelGGL(k_keyed_scan, dim3(1), dim3(kKeyScanThreads), 0, s, S.get(), nb, k, requests.get());
                 GP_LAUNCH_CHECK();
-                hipLaunchKernelGGL((k_keyed_apply<AdmitFn>), dim3(nb), dim3(kTierThreads), 0, s, keys.get(), ne,
+                hipLaunchKernelGGL((k_keyed_apply<AdmitFn>), dim3(nb), dim3(kTierThreads), 0, s, keys.get(), nlv,
                                    S.get(), nb, AdmitFn{a_, r}, movers_.get(), ctr_.get() + kMovers);
                 GP_LAUNCH_CHECK();
             }
             if (!c_.read_scalar(active.get() + group - 1)) break;
+            if (nlv >= kCompactMin && attempt < k) {
+                DBuf<std::uint8_t> lf = c_.buf<std::uint8_t>(static_cast<std::size_t>(nlv));
+                DBuf<std::int32_t> li = c_.buf<std::int32_t>(static_cast<std::size_t>(nlv));
+                DBuf<std::int64_t> lc = c_.buf<std::int64_t>(1);
+                hipLaunchKernelGGL(k_live_flags, dim3(grid_for(nlv)), dim3(256), 0, s, r.live, nlv, done.get(), lf.get());
+                GP_LAUNCH_CHECK();
+                stable_select(c_, lf.get(), nlv, li.get(), lc.get());
+                const std::int64_t next = c_.read_scalar(lc.get());
+                if (next == 0) break;
+                DBuf<std::int32_t> live2 = c_.buf<std::int32_t>(static_cast<std::size_t>(next));
+                hipLaunchKernelGGL(k_gather_i32, dim3(grid_for(next)), dim3(256), 0, s, r.live, li.get(), next,
+                                   live2.get());
+                GP_LAUNCH_CHECK();
+                live.swap(live2);
+                r.live = live.get();
+                r.nlive = next;
+            }
         }
     }
 

@@ -152,6 +152,11 @@ void gp_context_destroy(gp_context* ctx) { delete ctx; }
 uint64_t gp_context_reserved_bytes(const gp_context* ctx) {
     return ctx ? const_cast<gp_context*>(ctx)->ctx.arena()->reserved_bytes() : 0;
 }
+int gp_context_set_overlap(gp_context* ctx, int on) {
+    if (!ctx) return fail(GP_ERR_INVALID, "null context");
+    ctx->ctx.set_overlap(on != 0);
+    return GP_OK;
+}
 
 int gp_graph_create(gp_context* ctx, int64_t n, int64_t m, const int32_t* offsets, const int32_t* neighbors,
                     const uint32_t* edge_weights, const uint32_t* vertex_weights, gp_graph** out) {

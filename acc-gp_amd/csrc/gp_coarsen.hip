@@ -15,7 +15,8 @@ namespace {
 
 constexpr int kLowDegree = 16;     // group of 16 lanes
 constexpr int kWaveDegree = 512;   // one wave, up to 8 items per lane
-constexpr int kBlockDegree = 4096; // one block, LDS bitonic
+constexpr int kBlockDegree = 4096; // one block, row held in LDS
+static_assert(kBlockDegree <= kBlockSortMax, "block-class rows fit one LDS tile");
 
 __device__ inline bool sclp_better(std::uint64_t a, std::uint64_t t, std::uint64_t oa, std::uint64_t ot) {
     return a > oa || (a == oa && a != 0 && t > ot);  // aggregation.cuh:79-84
@@ -468,10 +469,20 @@ __global__ void __launch_bounds__(kBlockSortThreads) k_affinity_block_hash(Affin
         const int slots = hash_slots_for(d);
         block_hash_clear(s, slots);
         __syncthreads();
-        for (int x = threadIdx.x; x < d; x += kBlockSortThreads) {
-            const std::int32_t u = a.nbr[b + x];
-            if (u != v) block_hash_add(s, slots, a.clusters[u], (std::uint32_t)ew_at(a.ew, b + x));
+        // The row's gathers are all issued before the inserts (d <= 4096).
+        constexpr int kPer = kBlockSortMax / kBlockSortThreads;
+        std::int32_t kk[kPer];
+        std::uint32_t vv[kPer];
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const int x = r * kBlockSortThreads + threadIdx.x;
+            const std::int32_t u = x < d ? a.nbr[b + x] : v;
+            kk[r] = u != v ? a.clusters[u] : -1;
+            vv[r] = x < d ? (std::uint32_t)ew_at(a.ew, b + x) : 0u;
         }
+#pragma unroll
+        for (int r = 0; r < kPer; ++r)
+            if (kk[r] >= 0) block_hash_add(s, slots, kk[r], vv[r]);
         __syncthreads();
         Top2 t{0, 0, 0, 0};
         for (int i = threadIdx.x; i < slots; i += kBlockSortThreads)
@@ -592,27 +603,45 @@ __global__ void __launch_bounds__(kBlockSortThreads) k_chunk_records_hash(Src sr
         const int slots = hash_slots_for(d);
         block_hash_clear(s, slots);
         __syncthreads();
-        for (int x = threadIdx.x; x < d; x += kBlockSortThreads) {
-            const std::int32_t key = src.key(li, x0 + x);
-            if (key != kSentinel) block_hash_add(s, slots, key, src.val(li, x0 + x));
+        // The chunk's loads are all issued before the inserts.
+        constexpr int kPer = kBlockSortMax / kBlockSortThreads;
+        std::int32_t kk[kPer];
+        std::uint32_t vv[kPer];
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const int x = r * kBlockSortThreads + threadIdx.x;
+            kk[r] = kSentinel;
+            vv[r] = 0;
+            if (x < d) {
+                kk[r] = src.key(li, x0 + x);
+                vv[r] = src.val(li, x0 + x);
+            }
         }
+#pragma unroll
+        for (int r = 0; r < kPer; ++r)
+            if (kk[r] != kSentinel) block_hash_add(s, slots, kk[r], vv[r]);
         __syncthreads();
-        // Slots are owned in contiguous runs per thread for a block scan.
-        const int per = slots / kBlockSortThreads;
-        const int first = threadIdx.x * per;
-        int mine = 0;
-        for (int i = 0; i < per; ++i) mine += s.key[first + i] >= 0;
+        // Each wave compacts a contiguous quarter of the slots with ballots,
+        // so consecutive lanes write consecutive records.
+        const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+        const int q = slots / (kBlockSortThreads / kWave);
+        int cnt = 0;
+        for (int i0 = w * q; i0 < (w + 1) * q; i0 += kWave) cnt += __popcll(__ballot(s.key[i0 + lane] >= 0));
         int inc, before, total;
-        block_scan<PlusOp>(mine, wsum, inc, before, total);
+        block_scan<PlusOp>(lane == 0 ? cnt : 0, wsum, inc, before, total);
         if (threadIdx.x == 0) obase = total ? atomicAdd(out_count, (unsigned long long)total) : 0ULL;
         __syncthreads();
-        unsigned long long o = obase + (unsigned long long)before;
-        for (int i = 0; i < per; ++i) {
-            const std::int32_t key = s.key[first + i];
-            if (key < 0) continue;
-            keys[o] = ((std::uint64_t)li << key_bits) | (std::uint64_t)key;
-            vals[o] = s.val[first + i];
-            ++o;
+        unsigned long long o = obase + (unsigned long long)__shfl(before, 0);
+        const unsigned long long lt = (1ULL << lane) - 1ULL;
+        for (int i0 = w * q; i0 < (w + 1) * q; i0 += kWave) {
+            const std::int32_t key = s.key[i0 + lane];
+            const unsigned long long mask = __ballot(key >= 0);
+            if (key >= 0) {
+                const unsigned long long j = o + __popcll(mask & lt);
+                keys[j] = ((std::uint64_t)li << key_bits) | (std::uint64_t)key;
+                vals[j] = s.val[i0 + lane];
+            }
+            o += __popcll(mask);
         }
         __syncthreads();
     }
@@ -1729,7 +1758,7 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
         const ScatterBody body{fine.off.get(), fine.nbr.get(), fine.ew_ptr(), map, cursor.get(), rkey.get(), rval.get()};
         c.timed("contract_scatter", bytes,
                 [&] { run_tiers(c, body, nullptr, n, nullptr, tier_a.get(), tier_b.get(), tier_n.get(), nullptr, nullptr,
-                                nullptr, 0, n); });
+                                nullptr, 0, n, fine.max_degree > static_cast<std::uint64_t>(kTierMedium)); });
     }
     cursor.reset();
     // Row classes by raw length.
@@ -1815,7 +1844,9 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
     out->ew = c.buf<std::uint32_t>(static_cast<std::size_t>(std::max<std::int64_t>(mc, 1)));
     const CopyBody copy{raw_off.get(), rkey.get(), rval.get(), coff64.get(), out->off.get(), out->nbr.get(),
                         out->ew.get(), nc};
-    run_tiers(c, copy, nullptr, nc, nullptr, tier_a.get(), tier_b.get(), tier_n.get(), nullptr, nullptr, nullptr, 0, nc);
+    // Unique coarse rows are no longer than the raw ones (max rb[1]).
+    run_tiers(c, copy, nullptr, nc, nullptr, tier_a.get(), tier_b.get(), tier_n.get(), nullptr, nullptr, nullptr, 0, nc,
+              rb[1] > static_cast<unsigned long long>(kTierMedium));
     out->vw = std::move(coarse_vw);
     out->incident_bound = crb;
     if (st.positive_ew) {

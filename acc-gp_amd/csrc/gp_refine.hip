@@ -60,6 +60,20 @@ constexpr Pow15 make_pow15() {
 }
 __constant__ Pow15 kPow15 = make_pow15();
 
+// One byte lane of the packed per-vertex state word (indexing like an
+// array of T; the four lanes of a vertex share one 32-bit word).
+template <typename T>
+struct StateLane {
+    T* base;
+    __host__ __device__ T& operator[](std::int64_t v) const { return base[4 * v]; }
+};
+
+// Per-vertex refinement state, one 32-bit word per vertex: target part
+// (signed byte, -1 none) in byte 0, moving flag in byte 1, frontier
+// membership in byte 2, lock in byte 3.  The update sweep reads a
+// neighbour's whole state with one gather.
+constexpr int kStateTarget = 0, kStateFlag = 1, kStateMember = 2, kStateLock = 3;
+
 struct RefArgs {
     std::int64_t n;
     const std::int32_t* off;
@@ -72,11 +86,12 @@ struct RefArgs {
     Label* labels;
     unsigned long long* loads;
     const double* prices;  // null unless vertex weights exceed n
-    std::int8_t* targets;  // -1: none
+    std::uint32_t* state;  // packed lanes below
+    StateLane<std::int8_t> targets;  // -1: none
     std::int64_t* gains;
-    std::uint8_t* locks;
-    std::uint8_t* flags;
-    std::uint8_t* member;
+    StateLane<std::uint8_t> locks;
+    StateLane<std::uint8_t> flags;
+    StateLane<std::uint8_t> member;
     unsigned long long* ctr;
     RepairSample* samples;
     // repair
@@ -125,9 +140,9 @@ struct LightRow {
     __device__ int lab(int j) const { return (int)((packed[j >> 2] >> (8 * (j & 3))) & 0xffu); }
 };
 
-template <int W>
-__device__ inline std::uint64_t light_histogram(const GroupTeam<W>& t, const RefArgs& a, std::int64_t v,
-                                                unsigned long long* hist, LightRow<W>& r) {
+template <int W, class Bin>
+__device__ inline std::uint64_t light_histogram(const GroupTeam<W>& t, const RefArgs& a, std::int64_t v, Bin* hist,
+                                                LightRow<W>& r) {
     const std::int64_t b = a.off[v], e = a.off[v + 1];
     std::uint64_t total = 0;
 #pragma unroll
@@ -152,7 +167,7 @@ __device__ inline std::uint64_t light_histogram(const GroupTeam<W>& t, const Ref
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             if (lab[i] != 0xff) {
-                atomicAdd(&hist[lab[i]], (unsigned long long)w[i]);
+                atomicAdd(&hist[lab[i]], static_cast<Bin>(w[i]));
                 total += w[i];
             }
             word |= (unsigned)lab[i] << (8 * i);
@@ -163,8 +178,8 @@ __device__ inline std::uint64_t light_histogram(const GroupTeam<W>& t, const Ref
     return t.sum(total);
 }
 
-template <int W>
-__device__ inline void light_clear(const GroupTeam<W>& t, unsigned long long* hist, const LightRow<W>& r) {
+template <int W, class Bin>
+__device__ inline void light_clear(const GroupTeam<W>& t, Bin* hist, const LightRow<W>& r) {
     t.sync();  // all reads of the row's bins are done
 #pragma unroll
     for (int j = 0; j < LightRow<W>::kPer; ++j) {
@@ -236,8 +251,12 @@ struct RowBody {
     __device__ std::int64_t degree(std::int32_t v) const { return a.off[v + 1] - a.off[v]; }
 };
 
-// LP proposal (incremental.cuh:166-198, 229-296).
+// LP proposal (incremental.cuh:166-198, 229-296).  Bin is the bin type of
+// the short-row tables: 32 bits when no row of the level weighs 2^32 or
+// more (half the LDS per team, twice the resident teams), else 64.
+template <class Bin>
 struct LpProposeBody : RowBody {
+    using LightBin = Bin;
     // With prices, the scored candidates read (cap - load, price) of their
     // part from an LDS copy of the part table.
     __host__ __device__ int part_words() const { return a.prices ? 2 * a.k : 0; }
@@ -276,16 +295,17 @@ struct LpProposeBody : RowBody {
         };
         if constexpr (IsLight<Team>::value) {
             LightRow<Team::size> r;
-            total = light_histogram(t, a, v, hist, r);
-            current = hist[s];
+            Bin* lh = reinterpret_cast<Bin*>(hist);
+            total = light_histogram(t, a, v, lh, r);
+            current = lh[s];
             if (current != total)
 #pragma unroll
                 for (int j = 0; j < LightRow<Team::size>::kPer; ++j) {
                     const int l = r.lab(j);
-                    if (l != 0xff) consider(l, hist[l]);
+                    if (l != 0xff) consider(l, lh[l]);
                 }
             best = team_reduce(t, best, priced);
-            light_clear(t, hist, r);
+            light_clear(t, lh, r);
         } else {
             total = team_histogram(t, a, v, hist);
             current = hist[s];
@@ -353,27 +373,26 @@ struct UpdateBody : RowBody {
     __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long*) const {
         const int s = a.labels[v], tv = a.targets[v];
         std::int64_t gain = 0;
-        // Neighbour state packed in one word: label, moving flag, target
-        // (signed byte) and frontier membership, one byte each.
+        // Neighbour label and state word (two gathers per entry).
         struct Rec {
             std::int32_t u;
-            unsigned st;
+            int label;
+            std::uint32_t st;
         };
         row_batches(
             t, a.off[v], a.off[v + 1],
             [&](std::int64_t x) {
                 const std::int32_t u = a.nbr[(std::uint32_t)x];
                 const std::uint32_t uu = (std::uint32_t)u;
-                return Rec{u, (unsigned)a.labels[uu] | ((unsigned)a.flags[uu] << 8) |
-                                  ((unsigned)(std::uint8_t)a.targets[uu] << 16) | ((unsigned)a.member[uu] << 24)};
+                return Rec{u, (int)a.labels[uu], a.state[uu]};
             },
             [&](std::int64_t x, const Rec& r) {
                 if (r.u == v) return;
-                const int old_u = (int)(r.st & 0xff), tu = (int)(std::int8_t)((r.st >> 16) & 0xff);
-                const bool moves = ((r.st >> 8) & 0xff) != 0;
+                const int old_u = r.label, tu = (int)(std::int8_t)((r.st >> (8 * kStateTarget)) & 0xff);
+                const bool moves = ((r.st >> (8 * kStateFlag)) & 0xff) != 0;
                 // Most neighbours are already in the frontier: skip the
                 // scattered byte store for them.
-                if (!(r.st >> 24)) a.member[r.u] = 1;
+                if (!((r.st >> (8 * kStateMember)) & 0xff)) a.member[r.u] = 1;
                 const int new_u = moves ? tu : old_u;
                 if (!moves || r.u > v) gain += (std::int64_t)ew_at(a.ew, x) * ((s != old_u) - (tv != new_u));
             });
@@ -433,10 +452,16 @@ __global__ void k_commit(RefArgs a, const std::int32_t* list, const unsigned lon
         if (delta[p]) atomicAdd(&a.loads[p], delta[p]);
 }
 
+__global__ void k_set_lane(std::uint32_t* state, std::int64_t n, int lane, std::uint8_t value) {
+    const std::uint32_t keep = ~(0xffu << (8 * lane)), put = (std::uint32_t)value << (8 * lane);
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x)
+        state[v] = (state[v] & keep) | put;
+}
+
 // Frontier list from member flags (order irrelevant: per-vertex decisions).
 template <typename F>
-__global__ void k_compact_flags(std::int64_t n, const F* flags, std::int32_t* out,
-                                unsigned long long* count) {
+__global__ void k_compact_flags(std::int64_t n, F flags, std::int32_t* out, unsigned long long* count) {
     __shared__ std::int32_t sb[kTierThreads / kWave][kStage];
     Stage st{sb[threadIdx.x / kWave], out, count, 0};
     for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane_id(); base < n;
@@ -564,7 +589,9 @@ __device__ inline void repair_finish(const RefArgs& a, std::int64_t v, int s, st
     atomicAdd(&a.counts[b], (unsigned long long)w);
 }
 
+template <class Bin>
 struct RepairProposeBody : RowBody {
+    using LightBin = Bin;
     // Frozen part loads in LDS for the destination scan.
     __host__ __device__ int part_words() const { return a.k; }
     __device__ void stage(unsigned long long* t) const {
@@ -584,16 +611,17 @@ struct RepairProposeBody : RowBody {
         std::uint64_t total, current;
         if constexpr (IsLight<Team>::value) {
             LightRow<Team::size> r;
-            total = light_histogram(t, a, v, hist, r);
-            current = hist[s];
+            Bin* lh = reinterpret_cast<Bin*>(hist);
+            total = light_histogram(t, a, v, lh, r);
+            current = lh[s];
             if (current != total)
 #pragma unroll
                 for (int j = 0; j < LightRow<Team::size>::kPer; ++j) {
                     const int l = r.lab(j);
-                    if (l != 0xff) consider(l, hist[l]);
+                    if (l != 0xff) consider(l, lh[l]);
                 }
             best = team_reduce(t, best);
-            light_clear(t, hist, r);
+            light_clear(t, lh, r);
         } else {
             total = team_histogram(t, a, v, hist);
             current = hist[s];
@@ -653,18 +681,34 @@ __global__ void k_repair_flag(RefArgs a, const std::int32_t* bstar, std::uint8_t
     }
 }
 
-template <int W>
+// Sparse connection rows: the parts a listed vertex connects to, in part
+// order, as (weight, part) pairs in slots [0, nnz) of the vertex's k slots.
+// One wave per vertex (donor rows can be long); ballots give the slot of
+// each nonzero part.
 __global__ void k_conn_rows(RefArgs a, const std::int32_t* list, std::int64_t count, const std::uint8_t* skip,
-                            unsigned long long* rows) {
+                            unsigned long long* vals, std::uint8_t* parts, std::uint8_t* nnz) {
     extern __shared__ unsigned long long smem[];
-    const int lg = threadIdx.x % W, gi = threadIdx.x / W;
+    const int lane = lane_id(), gi = threadIdx.x / kWave;
+    const unsigned long long lt = (1ULL << lane) - 1ULL;
     unsigned long long* hist = smem + gi * a.k;
-    const std::int64_t gpb = blockDim.x / W;
+    const std::int64_t gpb = blockDim.x / kWave;
     for (std::int64_t idx = blockIdx.x * gpb + gi; idx < count; idx += (std::int64_t)gridDim.x * gpb) {
         if (skip && skip[idx]) continue;
         const std::int64_t v = list[idx];
-        team_histogram(GroupTeam<W>{lg}, a, v, hist);
-        for (int p = lg; p < a.k; p += W) rows[idx * a.k + p] = hist[p];
+        team_histogram(GroupTeam<kWave>{lane}, a, v, hist);
+        int base = 0;
+        for (int p0 = 0; p0 < a.k; p0 += kWave) {
+            const int p = p0 + lane;
+            const unsigned long long c = p < a.k ? hist[p] : 0;
+            const unsigned long long mask = __ballot(c != 0);
+            if (c) {
+                const std::int64_t j = idx * a.k + base + __popcll(mask & lt);
+                vals[j] = c;
+                parts[j] = static_cast<std::uint8_t>(p);
+            }
+            base += __popcll(mask);
+        }
+        if (lane == 0) nnz[idx] = static_cast<std::uint8_t>(base);
         wave_sync();
     }
 }
@@ -823,7 +867,9 @@ __global__ void __launch_bounds__(kTierThreads) k_elig_keys(RefArgs a, const std
 struct ReserveState {
     const std::int32_t* E;           // eligible donors in vertex order
     std::int64_t ne;
-    const unsigned long long* conn;  // ne x k
+    const unsigned long long* conn;  // ne x k sparse rows: weights
+    const std::uint8_t* conn_part;   // ne x k: their parts
+    const std::uint8_t* conn_nnz;    // ne: entries per row
     unsigned long long* requests;    // k, request totals
     const unsigned long long* slack; // k
     std::uint8_t* done;              // 1 admitted, 2 without a part left
@@ -832,7 +878,8 @@ struct ReserveState {
     std::int64_t nlive;
 };
 
-// Choices of one attempt, one 16-lane group per live donor.  Attempt 0
+// Choices of one attempt, one 4-lane group per live donor (many donors in
+// flight per wave: the per-donor loads form a dependent chain).  Attempt 0
 // requests the proposed target; later attempts the untried part with the
 // most connection, then the most room left at the start of the attempt,
 // then the lowest id.  Room only shrinks (request totals only grow) and
@@ -848,14 +895,28 @@ __device__ inline bool choice_better(const Choice& x, const Choice& y) {
     return x.conn > y.conn || (x.conn == y.conn && (x.room > y.room || (x.room == y.room && x.p < y.p)));
 }
 
+// A part without connection to the donor (conn 0) loses to every connected
+// one, and among unconnected parts the order is (room desc, id asc): the
+// scan walks the connected parts of the sparse row, and only when none is
+// open falls back to the first untried part in that block-wide order (open
+// when its room admits the donor; rooms after it are no larger).
+constexpr int kChooseLanes = 4;
 __global__ void __launch_bounds__(kTierThreads) k_choose(RefArgs a, ReserveState r, int attempt, std::int32_t* keys,
                                                         unsigned long long* active) {
-    constexpr int W = 16;
+    constexpr int W = kChooseLanes;
     __shared__ unsigned long long room[kMaxParts];  // start-of-attempt room per part
+    __shared__ std::uint8_t order[kMaxParts];       // parts by (room desc, id asc)
     const int lg = threadIdx.x % W;
+    const int gbase = (threadIdx.x % kWave) / W * W;
     if (attempt > 0) {
         for (int p = threadIdx.x; p < a.k; p += kTierThreads)
             room[p] = r.requests[p] < r.slack[p] ? r.slack[p] - r.requests[p] : 0;
+        __syncthreads();
+        for (int p = threadIdx.x; p < a.k; p += kTierThreads) {
+            int rank = 0;
+            for (int q = 0; q < a.k; ++q) rank += room[q] > room[p] || (room[q] == room[p] && q < p);
+            order[rank] = static_cast<std::uint8_t>(p);
+        }
         __syncthreads();
     }
     const std::int64_t groups = (std::int64_t)gridDim.x * (kTierThreads / W);
@@ -872,16 +933,31 @@ __global__ void __launch_bounds__(kTierThreads) k_choose(RefArgs a, ReserveState
                 const std::uint64_t w = a.vw[v];
                 const int lv = a.labels[v];
                 const unsigned long long tried0 = r.tried[2 * e], tried1 = r.tried[2 * e + 1];
-                for (int p = lg; p < a.k; p += W) {
-                    if (p == lv || (((p >> 6) ? tried1 : tried0) >> (p & 63)) & 1ULL) continue;
+                auto closed = [&](int p) { return p == lv || ((((p >> 6) ? tried1 : tried0) >> (p & 63)) & 1ULL); };
+                const int cnt = r.conn_nnz[e];
+                for (int j = lg; j < cnt; j += W) {
+                    const int p = r.conn_part[e * a.k + j];
+                    if (closed(p)) continue;
                     const unsigned long long avail = room[p];
                     if (avail < w) continue;
-                    const Choice x{p, r.conn[e * a.k + p], avail};
+                    const Choice x{p, r.conn[e * a.k + j], avail};
                     if (choice_better(x, best)) best = x;
                 }
                 for (int d = W / 2; d; d >>= 1) {
                     const Choice o{__shfl_xor(best.p, d, W), __shfl_xor(best.conn, d, W), __shfl_xor(best.room, d, W)};
                     if (choice_better(o, best)) best = o;
+                }
+                if (best.p < 0) {
+                    for (int q0 = 0; q0 < a.k; q0 += W) {
+                        const int q = q0 + lg;
+                        const bool open = q < a.k && !closed(order[q]);
+                        const unsigned sub = static_cast<unsigned>((__ballot(open) >> gbase) & ((1ULL << W) - 1));
+                        if (sub) {
+                            const int p = order[q0 + __ffs(sub) - 1];
+                            if (room[p] >= w) best = Choice{p, 0, room[p]};
+                            break;
+                        }
+                    }
                 }
             }
         }
@@ -1580,11 +1656,8 @@ public:
         a_.k = k;
         a_.cap = cap;
         a_.labels = labels;
-        targets_ = c.buf<std::int8_t>(n);
+        state_ = c.buf<std::uint32_t>(n);
         gains_ = c.buf<std::int64_t>(n);
-        locks_ = c.buf<std::uint8_t>(n);
-        flags_ = c.buf<std::uint8_t>(n);
-        member_ = c.buf<std::uint8_t>(n);
         frontier_ = c.buf<std::int32_t>(n);
         cand_ = c.buf<std::int32_t>(n);
         movers_ = c.buf<std::int32_t>(n);
@@ -1595,11 +1668,13 @@ public:
         loads0_ = c.buf<unsigned long long>(static_cast<std::size_t>(k));
         bucket_ = c.buf<std::int32_t>(n);
         counts_ = c.buf<unsigned long long>(static_cast<std::size_t>(k) * kBalanceWidth);
-        a_.targets = targets_.get();
+        std::uint8_t* lanes = reinterpret_cast<std::uint8_t*>(state_.get());
+        a_.state = state_.get();
+        a_.targets = StateLane<std::int8_t>{reinterpret_cast<std::int8_t*>(lanes + kStateTarget)};
+        a_.flags = StateLane<std::uint8_t>{lanes + kStateFlag};
+        a_.member = StateLane<std::uint8_t>{lanes + kStateMember};
+        a_.locks = StateLane<std::uint8_t>{lanes + kStateLock};
         a_.gains = gains_.get();
-        a_.locks = locks_.get();
-        a_.flags = flags_.get();
-        a_.member = member_.get();
         a_.ctr = ctr_.get();
         a_.loads = ctr_.get() + kCtrCount;
         a_.loads0 = loads0_.get();
@@ -1641,11 +1716,22 @@ public:
 
     unsigned long long* loads_dev() const { return a_.loads; }
 
+    // Sets one byte lane of every vertex's state word.
+    void set_lane(Context& c, int lane, std::uint8_t value) {
+        hipLaunchKernelGGL(k_set_lane, dim3(grid_for(a_.n, 256, 2048)), dim3(256), 0, c.stream(), state_.get(), a_.n,
+                           lane, value);
+        GP_LAUNCH_CHECK();
+    }
+
+    // Short-row bins fit 32 bits when every row of the level weighs less
+    // than 2^32 (the level's incident-weight bound).
+    bool narrow_bins() const { return g_.incident_bound <= 0xffffffffULL; }
+
     template <class Body>
     void tiers(const Body& body, const std::int32_t* list, std::int64_t n, const unsigned long long* count,
                std::int32_t* out, unsigned long long* out_count, unsigned long long* acc, int scratch) {
         run_tiers(c_, body, list, n, count, medium_.get(), heavy_.get(), ctr_.get() + kTier, out, out_count, acc,
-                  scratch, a_.n);
+                  scratch, a_.n, g_.max_degree > static_cast<std::uint64_t>(kTierMedium));
     }
 
     // One LP round over the frontier; returns the number of movers and the cut delta.
@@ -1653,15 +1739,20 @@ public:
         const hipStream_t s = c_.stream();
         const std::int64_t n = a_.n;
         fill<unsigned long long>(c_, ctr_.get(), kCtrCount, 0ULL);
-        hipLaunchKernelGGL(k_compact_flags<std::uint8_t>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, n, member_.get(),
+        hipLaunchKernelGGL(k_compact_flags<StateLane<std::uint8_t>>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, n,
+                           a_.member,
                            frontier_.get(), ctr_.get() + kFrontier);
         GP_LAUNCH_CHECK();
         // Algorithmic bytes per scanned row: offsets 8, label and weight 8,
         // target and gain 12, and per entry neighbour 4, neighbour label 4
         // and edge weight 4 when weighted; added once the row sums are read.
         c_.timed("lp_propose", 0.0, [&] {
-            tiers(LpProposeBody{{a_}}, frontier_.get(), 0, ctr_.get() + kFrontier, cand_.get(), ctr_.get() + kCand,
-                  ctr_.get() + kRowSum, a_.k);
+            if (narrow_bins())
+                tiers(LpProposeBody<unsigned>{{a_}}, frontier_.get(), 0, ctr_.get() + kFrontier, cand_.get(),
+                      ctr_.get() + kCand, ctr_.get() + kRowSum, a_.k);
+            else
+                tiers(LpProposeBody<unsigned long long>{{a_}}, frontier_.get(), 0, ctr_.get() + kFrontier, cand_.get(),
+                      ctr_.get() + kCand, ctr_.get() + kRowSum, a_.k);
         });
         tiers(AfterBody{{a_}}, cand_.get(), 0, ctr_.get() + kCand, movers_.get(), ctr_.get() + kMovers, nullptr, 0);
         return apply(true, false, delta);
@@ -1722,6 +1813,8 @@ public:
         // Connection rows are read from the second attempt on, and only for
         // donors still waiting; they are built once attempt 0 is done.
         DBuf<unsigned long long> conn = c_.buf<unsigned long long>(static_cast<std::size_t>(ne) * k);
+        DBuf<std::uint8_t> conn_part = c_.buf<std::uint8_t>(static_cast<std::size_t>(ne) * k);
+        DBuf<std::uint8_t> conn_nnz = c_.buf<std::uint8_t>(static_cast<std::size_t>(ne));
         DBuf<unsigned long long> requests = c_.buf<unsigned long long>(static_cast<std::size_t>(k));
         DBuf<unsigned long long> slack = c_.buf<unsigned long long>(static_cast<std::size_t>(k));
         DBuf<std::uint8_t> done = c_.buf<std::uint8_t>(static_cast<std::size_t>(ne));
@@ -1733,7 +1826,8 @@ public:
         DBuf<std::int32_t> live = c_.buf<std::int32_t>(static_cast<std::size_t>(ne));
         hipLaunchKernelGGL(k_iota_i32, dim3(grid_for(ne)), dim3(256), 0, s, live.get(), ne);
         GP_LAUNCH_CHECK();
-        ReserveState r{E.get(), ne, conn.get(), requests.get(), slack.get(), done.get(), tried.get(), live.get(), ne};
+        ReserveState r{E.get(),     ne,         conn.get(), conn_part.get(), conn_nnz.get(), requests.get(),
+                       slack.get(), done.get(), tried.get(), live.get(),     ne};
         // Attempts run in groups between host checks: an attempt without a
         // chooser changes nothing (no tried bits, no requests, no admissions),
         // so the attempts after the last active one are no-ops.  Between
@@ -1749,11 +1843,12 @@ public:
             fill<unsigned long long>(c_, active.get(), kAttemptGroup, 0ULL);
             for (int j = 0; j < group; ++j, ++attempt) {
                 if (attempt == 1) {
-                    hipLaunchKernelGGL((k_conn_rows<kWave>), dim3(grid_for(ne, 4, 2048)), dim3(256),
-                                       4 * k * sizeof(unsigned long long), s, a_, E.get(), ne, done.get(), conn.get());
+                    hipLaunchKernelGGL(k_conn_rows, dim3(grid_for(ne, 256 / kWave, 2048)), dim3(256),
+                                       (256 / kWave) * k * sizeof(unsigned long long), s, a_, E.get(), ne, done.get(),
+                                       conn.get(), conn_part.get(), conn_nnz.get());
                     GP_LAUNCH_CHECK();
                 }
-                hipLaunchKernelGGL(k_choose, dim3(grid_for(nlv, kTierThreads / 16, 4096)), dim3(kTierThreads), 0, s, a_,
+                hipLaunchKernelGGL(k_choose, dim3(grid_for(nlv, kTierThreads / kChooseLanes, 8192)), dim3(kTierThreads), 0, s, a_,
                                    r, attempt, keys.get(), active.get() + j);
                 GP_LAUNCH_CHECK();
                 hipLaunchKernelGGL((k_keyed_sums<DonorWeight>), dim3(nb), dim3(kTierThreads), 0, s, keys.get(), nlv, k,
@@ -1808,7 +1903,12 @@ public:
             hipLaunchKernelGGL(k_repair_donors, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, a_, cand_.get(),
                                ctr_.get() + kDonors);
             GP_LAUNCH_CHECK();
-            tiers(RepairProposeBody{{a_}}, cand_.get(), 0, ctr_.get() + kDonors, nullptr, nullptr, nullptr, a_.k);
+            if (narrow_bins())
+                tiers(RepairProposeBody<unsigned>{{a_}}, cand_.get(), 0, ctr_.get() + kDonors, nullptr, nullptr, nullptr,
+                      a_.k);
+            else
+                tiers(RepairProposeBody<unsigned long long>{{a_}}, cand_.get(), 0, ctr_.get() + kDonors, nullptr,
+                      nullptr, nullptr, a_.k);
             hipLaunchKernelGGL(k_bucket_scan, dim3(k), dim3(kScanThreads), 0, s, a_, prefix.get(), bstar.get(),
                                excess.get());
             GP_LAUNCH_CHECK();
@@ -1903,11 +2003,12 @@ public:
                                owners.get(), events.get());
             hipLaunchKernelGGL(k_joint_prefix, dim3(ceil_div(k, 128)), dim3(128), 0, s, a_, moves.get(), active,
                                events.get(), stops.get());
-            fill<std::uint8_t>(c_, flags_.get(), n, 0);
+            set_lane(c_, kStateFlag, 0);
             hipLaunchKernelGGL(k_joint_accept, dim3(ceil_div(active, 256)), dim3(256), 0, s, a_, moves.get(), active,
                                stops.get());
             fill<unsigned long long>(c_, ctr_.get() + kMovers, 1, 0ULL);
-            hipLaunchKernelGGL(k_compact_flags<std::uint8_t>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, n, flags_.get(),
+            hipLaunchKernelGGL(k_compact_flags<StateLane<std::uint8_t>>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s,
+                               n, a_.flags,
                                movers_.get(), ctr_.get() + kMovers);
             GP_LAUNCH_CHECK();
             const unsigned long long cnt = c_.read_scalar(ctr_.get() + kMovers);
@@ -2001,8 +2102,7 @@ public:
     const RefineOptions& o_;
     LevelStats& st_;
     RefArgs a_;
-    DBuf<std::int8_t> targets_;
-    DBuf<std::uint8_t> locks_, flags_, member_;
+    DBuf<std::uint32_t> state_;
     DBuf<std::int32_t> frontier_, cand_, movers_, medium_, heavy_, bucket_;
     DBuf<std::int64_t> gains_;
     DBuf<unsigned long long> ctr_, loads0_, counts_;
@@ -2071,8 +2171,8 @@ LevelStats refine_level_impl(Context& c, const DGraph& g, Label* labels, std::ve
     c.upload(R.loads_dev(), loads.data(), static_cast<std::size_t>(k) * sizeof(std::uint64_t));
     cut = best_cut;
     if (rounds > 0 && o.exchange_rounds > 0 && g.n <= o.exchange_max_n) {
-        fill<std::uint8_t>(c, R.flags_.get(), g.n, 0);
-        fill<std::int8_t>(c, R.targets_.get(), g.n, (std::int8_t)-1);
+        R.set_lane(c, kStateFlag, 0);
+        R.set_lane(c, kStateTarget, 0xff);
         R.exchange(o.exchange_rounds, cut, best_cut);
     }
     st.final_cut = cut;

@@ -165,6 +165,21 @@ struct RowOut {
     std::int64_t acc;
 };
 
+// Short-row tables of bodies that declare LightBin use that bin width; the
+// scratch argument of run_tiers counts 64-bit bins.
+template <class B, class = void>
+struct LightBinOf {
+    using type = unsigned long long;
+};
+template <class B>
+struct LightBinOf<B, std::void_t<typename B::LightBin>> {
+    using type = typename B::LightBin;
+};
+template <class Body>
+constexpr int light_scratch_words(int scratch) {
+    return static_cast<int>((static_cast<std::size_t>(scratch) * sizeof(typename LightBinOf<Body>::type) + 7) / 8);
+}
+
 struct TierLists {
     std::int32_t* medium;
     unsigned long long* medium_count;
@@ -180,7 +195,7 @@ struct TierLists {
 template <class Body, int W>
 __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const std::int32_t* list, std::int64_t n,
                                                              const unsigned long long* count_ptr, TierLists t,
-                                                             int scratch_per_group) {
+                                                             int scratch_per_group, bool route_heavy) {
     extern __shared__ unsigned long long smem[];
     __shared__ std::int32_t stage_buf[kTierThreads / kWave][2][kStage];
     constexpr int G = kWave / W;
@@ -203,17 +218,23 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const st
         bool defer = false;
         RowOut r{false, 0};
         std::int64_t deg = 0;
+        bool heavy = false;
         if (idx < count) {
             v = list ? list[idx] : (std::int32_t)idx;
             deg = body.degree(v);
             defer = deg > kTierLight;
+            heavy = route_heavy && deg > kTierMedium;
         }
+        // Heavy rows (rare) go straight to the heavy list, so the heavy
+        // launch does not wait for the medium one; without route_heavy the
+        // medium launch takes every longer row.
+        if (lg == 0 && heavy) t.heavy[atomicAdd(t.heavy_count, 1ULL)] = v;
         // Wave-uniform bound on the entries of the rows handled together,
         // so that bodies with unrolled row loops skip empty iterations.
         const int wmax = (int)wave_max((std::uint64_t)(idx < count && !defer ? deg : 0));
         if (idx < count && !defer) r = body(GroupTeam<W>{lg, wmax, part}, v, scratch);
         if (lg == 0) acc += r.acc;
-        defer_stage.push(lg == 0 && defer, v);
+        defer_stage.push(lg == 0 && defer && !heavy, v);
         out_stage.push(lg == 0 && r.append, v);
     }
     defer_stage.flush();
@@ -229,13 +250,12 @@ template <class Body, int W>
 __global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLists t, int scratch_per_group) {
     static_assert(W == 32 || W == 64, "medium teams are half or whole waves");
     extern __shared__ unsigned long long smem[];
-    __shared__ std::int32_t stage_buf[kTierThreads / kWave][2][kStage];
+    __shared__ std::int32_t stage_buf[kTierThreads / kWave][kStage];
     constexpr int G = kWave / W;
     const int lane = lane_id(), gw = lane / W, lg = lane % W;
     const unsigned long long* part = stage_parts(body, smem);
     unsigned long long* scratch = smem + part_words_of(body) + (threadIdx.x / W) * scratch_per_group;
-    Stage defer_stage{stage_buf[threadIdx.x / kWave][0], t.heavy, t.heavy_count, 0};
-    Stage out_stage{stage_buf[threadIdx.x / kWave][1], t.out, t.out_count, 0};
+    Stage out_stage{stage_buf[threadIdx.x / kWave], t.out, t.out_count, 0};
     const std::int64_t count = (std::int64_t)*t.medium_count;
     const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
     const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;
@@ -243,20 +263,16 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLis
     for (std::int64_t base = wave * G; base < count; base += waves * G) {
         const std::int64_t idx = base + gw;
         std::int32_t v = 0;
-        bool defer = false;
         RowOut r{false, 0};
+        // The light launch routes rows longer than kTierMedium to the heavy
+        // list; every row listed here is handled here.
         if (idx < count) {
             v = t.medium[idx];
-            if (body.degree(v) > kTierMedium)
-                defer = true;
-            else
-                r = body(GroupTeam<W>{lg, 1 << 30, part}, v, scratch);
+            r = body(GroupTeam<W>{lg, 1 << 30, part}, v, scratch);
         }
         if (lg == 0) acc += r.acc;
-        defer_stage.push(lg == 0 && defer, v);
         out_stage.push(lg == 0 && r.append, v);
     }
-    defer_stage.flush();
     out_stage.flush();
     if (t.acc) {
         acc = wave_sum(acc);
@@ -287,29 +303,44 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_heavy(Body body, TierList
 
 // Host launcher: three launches, counters in dev[0] (medium) and dev[1]
 // (heavy) are reset here.  scratch is per-team LDS in u64 units.
+// heavy_rows: the rows may include some longer than kTierMedium (a bound
+// the caller knows); when false every longer row runs in the medium launch
+// (correct for any length) and no heavy launch is made.
 template <class Body>
 void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int64_t n, const unsigned long long* count_ptr,
                std::int32_t* medium, std::int32_t* heavy, unsigned long long* tier_counts, std::int32_t* out,
-               unsigned long long* out_count, unsigned long long* acc, int scratch, std::int64_t max_rows) {
+               unsigned long long* out_count, unsigned long long* acc, int scratch, std::int64_t max_rows,
+               bool heavy_rows = true) {
     const hipStream_t s = c.stream();
     fill<unsigned long long>(c, tier_counts, 2, 0ULL);
     TierLists t{medium, tier_counts, heavy, tier_counts + 1, out, out_count, acc};
+    const int lscratch = light_scratch_words<Body>(scratch);
     int lanes = c.light_lanes();
-    while (lanes < 16 && (kTierThreads / lanes) * scratch * sizeof(unsigned long long) > 48 * 1024) lanes *= 2;
+    while (lanes < 16 && (kTierThreads / lanes) * lscratch * sizeof(unsigned long long) > 48 * 1024) lanes *= 2;
     const int light_blocks = static_cast<int>(
         std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(max_rows, kTierThreads / lanes), 4096)));
     const std::size_t part = static_cast<std::size_t>(part_words_of(body)) * sizeof(unsigned long long);
-    const std::size_t light_lds = part + (kTierThreads / lanes) * scratch * sizeof(unsigned long long);
+    const std::size_t light_lds = part + (kTierThreads / lanes) * lscratch * sizeof(unsigned long long);
     if (lanes == 4)
         hipLaunchKernelGGL((k_tier_light<Body, 4>), dim3(light_blocks), dim3(kTierThreads), light_lds, s, body, list, n,
-                           count_ptr, t, scratch);
+                           count_ptr, t, lscratch, heavy_rows);
     else if (lanes == 8)
         hipLaunchKernelGGL((k_tier_light<Body, 8>), dim3(light_blocks), dim3(kTierThreads), light_lds, s, body, list, n,
-                           count_ptr, t, scratch);
+                           count_ptr, t, lscratch, heavy_rows);
     else
         hipLaunchKernelGGL((k_tier_light<Body, 16>), dim3(light_blocks), dim3(kTierThreads), light_lds, s, body, list, n,
-                           count_ptr, t, scratch);
+                           count_ptr, t, lscratch, heavy_rows);
     GP_LAUNCH_CHECK();
+    // The light launch fills the heavy list: with overlap on, the heavy
+    // launch (few long rows, few blocks) runs on the side stream beside the
+    // medium launch.
+    const std::size_t heavy_lds = part + scratch * sizeof(unsigned long long);
+    const bool side = heavy_rows && c.overlap();
+    if (side) {
+        c.fork();
+        hipLaunchKernelGGL((k_tier_heavy<Body>), dim3(1024), dim3(kTierThreads), heavy_lds, c.aux(), body, t);
+        GP_LAUNCH_CHECK();
+    }
     const int mlanes = c.medium_lanes();
     const int medium_blocks = static_cast<int>(
         std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(max_rows, kTierThreads / mlanes), 2048)));
@@ -322,9 +353,12 @@ void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int6
                            scratch);
     GP_LAUNCH_CHECK();
     // Heavy rows are few but long; blocks beyond the row count exit at once.
-    hipLaunchKernelGGL((k_tier_heavy<Body>), dim3(1024), dim3(kTierThreads),
-                       part + scratch * sizeof(unsigned long long), s, body, t);
-    GP_LAUNCH_CHECK();
+    if (side) {
+        c.join();
+    } else if (heavy_rows) {
+        hipLaunchKernelGGL((k_tier_heavy<Body>), dim3(1024), dim3(kTierThreads), heavy_lds, s, body, t);
+        GP_LAUNCH_CHECK();
+    }
 }
 
 }  // namespace gp

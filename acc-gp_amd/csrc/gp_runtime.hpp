@@ -164,10 +164,14 @@ public:
         GP_HIP(hipHostMalloc(&pinned_, kPinnedBytes, hipHostMallocDefault));
         if (const char* e = std::getenv("GP_LIGHT_LANES")) set_light_lanes(std::atoi(e));
         if (const char* e = std::getenv("GP_MEDIUM_LANES")) set_medium_lanes(std::atoi(e));
+        if (const char* e = std::getenv("GP_TIER_OVERLAP")) overlap_ = std::atoi(e) != 0;
     }
     ~Context() {
         delete arena_;
         if (pinned_) (void)hipHostFree(pinned_);
+        if (fork_) (void)hipEventDestroy(fork_);
+        if (join_) (void)hipEventDestroy(join_);
+        if (aux_) (void)hipStreamDestroy(aux_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
     Context(const Context&) = delete;
@@ -189,6 +193,27 @@ public:
     void set_light_lanes(int lanes) {
         if (lanes != 4 && lanes != 8 && lanes != 16) throw std::invalid_argument("light lanes must be 4, 8 or 16");
         light_lanes_ = lanes;
+    }
+    // Side stream for work that overlaps the main stream between fork() and
+    // join(); after join() the main stream is ordered after the side work,
+    // so buffers freed later (main-stream order) are no longer in use.  The
+    // side stream is created on first use: a context that never overlaps
+    // holds one stream (one hardware queue).
+    bool overlap() const { return overlap_; }
+    void set_overlap(bool on) { overlap_ = on; }
+    hipStream_t aux() const { return aux_; }
+    void fork() {
+        if (!aux_) {
+            GP_HIP(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
+            GP_HIP(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+            GP_HIP(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
+        }
+        GP_HIP(hipEventRecord(fork_, stream_));
+        GP_HIP(hipStreamWaitEvent(aux_, fork_, 0));
+    }
+    void join() {
+        GP_HIP(hipEventRecord(join_, aux_));
+        GP_HIP(hipStreamWaitEvent(stream_, join_, 0));
     }
     int device() const { return device_; }
     void activate() const { GP_HIP(hipSetDevice(device_)); }
@@ -285,6 +310,9 @@ private:
     std::map<std::string, ProfileEntry> profile_;
     int device_;
     hipStream_t stream_ = nullptr;
+    hipStream_t aux_ = nullptr;
+    hipEvent_t fork_ = nullptr, join_ = nullptr;
+    bool overlap_ = true;
     Arena* arena_ = nullptr;
     void* pinned_ = nullptr;
     int light_lanes_ = 4;

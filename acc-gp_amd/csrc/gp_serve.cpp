@@ -62,6 +62,8 @@ struct gp_server {
             cv_done.notify_all();
             return;
         }
+        // Requests overlap one another; each context keeps a single stream.
+        gp_context_set_overlap(ctx, 0);
         for (;;) {
             std::shared_ptr<Job> job;
             {

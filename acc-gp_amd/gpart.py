@@ -64,7 +64,7 @@ class Request(ctypes.Structure):
 
 # Every symbol declared in include/gpart_mi355x.h.
 EXPORTS = [
-    "gp_last_error", "gp_context_create", "gp_context_destroy", "gp_context_reserved_bytes",
+    "gp_last_error", "gp_context_create", "gp_context_destroy", "gp_context_reserved_bytes", "gp_context_set_overlap",
     "gp_graph_create", "gp_graph_info", "gp_graph_download", "gp_graph_destroy", "gp_capacity",
     "gp_default_coarsen_options", "gp_coarsen", "gp_hierarchy_levels", "gp_hierarchy_level",
     "gp_hierarchy_map_download", "gp_hierarchy_level_info", "gp_hierarchy_destroy",
@@ -91,6 +91,7 @@ def lib():
         L.gp_context_destroy.argtypes = [vp]
         L.gp_context_reserved_bytes.argtypes = [vp]
         L.gp_context_reserved_bytes.restype = u64
+        L.gp_context_set_overlap.argtypes = [vp, i32]
         L.gp_graph_create.argtypes = [vp, i64, i64, vp, vp, vp, vp, pp]
         L.gp_graph_info.argtypes = [vp, vp, vp]
         L.gp_graph_download.argtypes = [vp, vp, vp, vp, vp, vp]
@@ -254,6 +255,9 @@ class Context:
 
     def profile(self, enable=True):
         check(lib().gp_profile_enable(self.h, int(enable)))
+
+    def set_overlap(self, on=True):
+        check(lib().gp_context_set_overlap(self.h, int(on)))
 
     def profile_query(self, name):
         ms, by, ln = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()

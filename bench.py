@@ -70,6 +70,9 @@ def main():
         s0 = time.perf_counter()
         _, st = g.partition(args.parts, 1.03, 0, download=False)
         per_step.append(time.perf_counter() - s0)
+        phases = {"coarsen_ms": 1000.0 * st["coarsen_seconds"],
+                  "initial_partition_ms": 1000.0 * st["initial_partition_seconds"],
+                  "uncoarsen_ms": 1000.0 * st["uncoarsen_seconds"]}
         print("bench: step %d %.3fs (algorithm %.3fs) cut=%d" % (i, per_step[-1], st["algorithm_total_seconds"],
                                                                    st["final_cut"]), file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
@@ -113,6 +116,7 @@ def main():
             "levels": st["levels"],
             "p50_step_ms": 1000.0 * float(np.percentile(per_step, 50)),
             "p99_step_ms": 1000.0 * float(np.percentile(per_step, 99)),
+            "last_step_phases": phases,
         },
     }
     # Dominant kernel class: the LP proposal sweep (three tier launches per

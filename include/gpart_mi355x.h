@@ -88,6 +88,12 @@ int gp_context_create(int device, gp_context** out);
 void gp_context_destroy(gp_context* ctx);
 /* Cached HBM bytes held by the context arena. */
 uint64_t gp_context_reserved_bytes(const gp_context* ctx);
+/* Tuning, no counterpart in the reference: on (default) runs the launch of
+ * the longest rows of each row sweep on a second stream beside the launch of
+ * medium rows; off keeps every launch on the context stream (one hardware
+ * queue per context, which the concurrent contexts of gp_server use).
+ * GP_TIER_OVERLAP=0 in the environment sets off at context creation. */
+int gp_context_set_overlap(gp_context* ctx, int on);
 
 /* ---- graphs (include/graph.hpp:59-76) -------------------------------- */
 /* make_device_weighted: uploads a host CSR.  ew==NULL or all-ones edge weights

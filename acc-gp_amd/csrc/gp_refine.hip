@@ -238,7 +238,7 @@ __device__ inline Cand team_reduce(const BlockTeam& t, Cand c, bool priced) {
     }
     __syncthreads();
     Cand r{(int)(long long)t.red[0], t.red[1], __longlong_as_double((long long)t.red[2])};
-    for (int i = 1; i < kTierThreads / kWave; ++i) {
+    for (int i = 1; i < BlockTeam::size / kWave; ++i) {
         Cand o{(int)(long long)t.red[3 * i], t.red[3 * i + 1], __longlong_as_double((long long)t.red[3 * i + 2])};
         if (cand_better(o, r, priced)) r = o;
     }
@@ -543,7 +543,7 @@ __device__ inline RepairPick team_reduce(const BlockTeam& t, RepairPick c) {
     }
     __syncthreads();
     RepairPick r{(int)(long long)t.red[0], t.red[1], t.red[2]};
-    for (int i = 1; i < kTierThreads / kWave; ++i) {
+    for (int i = 1; i < BlockTeam::size / kWave; ++i) {
         RepairPick o{(int)(long long)t.red[3 * i], t.red[3 * i + 1], t.red[3 * i + 2]};
         if (pick_better(o, r)) r = o;
     }

@@ -17,7 +17,8 @@ namespace gp {
 constexpr int kTierLight = 64;
 constexpr int kTierMedium = 4096;
 constexpr int kTierThreads = 256;
-constexpr int kBlockRed = 32;  // u64 words of block-team reduction scratch
+constexpr int kHeavyThreads = 1024;  // one block per heavy row: many entries in flight
+constexpr int kBlockRed = 3 * (kHeavyThreads / kWave);  // u64 words of block-team reduction scratch
 
 // Appends v for every lane with pred set; all lanes of the wave must call it.
 __device__ inline void wave_append(bool pred, std::int32_t v, std::int32_t* list, unsigned long long* count) {
@@ -82,7 +83,7 @@ struct GroupTeam {
 struct BlockTeam {
     unsigned long long* red;  // LDS scratch of kBlockRed entries
     const unsigned long long* part = nullptr;  // block-shared staged table (or null)
-    static constexpr int size = kTierThreads;
+    static constexpr int size = kHeavyThreads;
     __device__ int rank() const { return threadIdx.x; }
     template <typename T>
     __device__ T sum(T x) const {
@@ -91,7 +92,7 @@ struct BlockTeam {
         if (lane_id() == 0) red[threadIdx.x / kWave] = (unsigned long long)x;
         __syncthreads();
         T s = 0;
-        for (int i = 0; i < kTierThreads / kWave; ++i) s += (T)red[i];
+        for (int i = 0; i < size / kWave; ++i) s += (T)red[i];
         __syncthreads();
         return s;
     }
@@ -280,8 +281,10 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLis
     }
 }
 
+constexpr int kHeavyGrid = 512;  // blocks beyond the heavy row count exit at once
+
 template <class Body>
-__global__ void __launch_bounds__(kTierThreads) k_tier_heavy(Body body, TierLists t) {
+__global__ void __launch_bounds__(kHeavyThreads) k_tier_heavy(Body body, TierLists t) {
     extern __shared__ unsigned long long smem[];
     __shared__ unsigned long long red[kBlockRed];
     const std::int64_t count = (std::int64_t)*t.heavy_count;
@@ -338,7 +341,7 @@ void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int6
     const bool side = heavy_rows && c.overlap();
     if (side) {
         c.fork();
-        hipLaunchKernelGGL((k_tier_heavy<Body>), dim3(1024), dim3(kTierThreads), heavy_lds, c.aux(), body, t);
+        hipLaunchKernelGGL((k_tier_heavy<Body>), dim3(kHeavyGrid), dim3(kHeavyThreads), heavy_lds, c.aux(), body, t);
         GP_LAUNCH_CHECK();
     }
     const int mlanes = c.medium_lanes();
@@ -356,7 +359,7 @@ void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int6
     if (side) {
         c.join();
     } else if (heavy_rows) {
-        hipLaunchKernelGGL((k_tier_heavy<Body>), dim3(1024), dim3(kTierThreads), heavy_lds, s, body, t);
+        hipLaunchKernelGGL((k_tier_heavy<Body>), dim3(kHeavyGrid), dim3(kHeavyThreads), heavy_lds, s, body, t);
         GP_LAUNCH_CHECK();
     }
 }

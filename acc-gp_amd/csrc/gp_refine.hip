@@ -343,8 +343,10 @@ struct AfterBody : RowBody {
             [&](std::int64_t x) {
                 const std::int32_t u = a.nbr[(std::uint32_t)x];
                 const std::uint32_t uu = (std::uint32_t)u;
+                // The gain is gathered beside the target rather than after
+                // it: one round trip per batch instead of two.
                 const int tu = a.targets[uu];
-                return Rec{u, (tu & 0xff) | ((int)a.labels[uu] << 8), tu >= 0 ? a.gains[uu] : 0};
+                return Rec{u, (tu & 0xff) | ((int)a.labels[uu] << 8), a.gains[uu]};
             },
             [&](std::int64_t x, const Rec& r) {
                 const int tu = (int)(std::int8_t)(r.tl & 0xff), lu = r.tl >> 8;

@@ -866,6 +866,15 @@ __global__ void __launch_bounds__(kTierThreads) k_elig_keys(RefArgs a, const std
     tile_key_sums(keys, nl, a.k, nb, f, acc, S);
 }
 
+// A live donor as the attempts read it: index into the eligible list, the
+// vertex, its weight and label (one 16-byte load per donor).
+struct DonorInfo {
+    std::int32_t e;
+    std::int32_t v;
+    std::uint32_t w;
+    std::int32_t label;
+};
+
 struct ReserveState {
     const std::int32_t* E;           // eligible donors in vertex order
     std::int64_t ne;
@@ -876,7 +885,7 @@ struct ReserveState {
     const unsigned long long* slack; // k
     std::uint8_t* done;              // 1 admitted, 2 without a part left
     unsigned long long* tried;       // ne x 2
-    const std::int32_t* live;        // donors still choosing, ascending
+    const DonorInfo* live;           // donors still choosing, ascending
     std::int64_t nlive;
 };
 
@@ -924,16 +933,17 @@ __global__ void __launch_bounds__(kTierThreads) k_choose(RefArgs a, ReserveState
     const std::int64_t groups = (std::int64_t)gridDim.x * (kTierThreads / W);
     unsigned long long cnt = 0;
     for (std::int64_t i = (blockIdx.x * (std::int64_t)kTierThreads + threadIdx.x) / W; i < r.nlive; i += groups) {
-        const std::int64_t e = r.live[i];
+        const DonorInfo d = r.live[i];
+        const std::int64_t e = d.e;
         Choice best{-1, 0, 0};
         bool open = !r.done[e];
         if (open) {
-            const std::int32_t v = r.E[e];
+            const std::int32_t v = d.v;
             if (attempt == 0)
                 best.p = a.targets[v];
             else {
-                const std::uint64_t w = a.vw[v];
-                const int lv = a.labels[v];
+                const std::uint64_t w = d.w;
+                const int lv = d.label;
                 const unsigned long long tried0 = r.tried[2 * e], tried1 = r.tried[2 * e + 1];
                 auto closed = [&](int p) { return p == lv || ((((p >> 6) ? tried1 : tried0) >> (p & 63)) & 1ULL); };
                 const int cnt = r.conn_nnz[e];
@@ -974,10 +984,8 @@ __global__ void __launch_bounds__(kTierThreads) k_choose(RefArgs a, ReserveState
 }
 
 struct DonorWeight {
-    RefArgs a;
-    const std::int32_t* E;
-    const std::int32_t* live;
-    __device__ std::uint64_t weight(std::int64_t i) const { return a.vw[E[live[i]]]; }
+    const DonorInfo* live;
+    __device__ std::uint64_t weight(std::int64_t i) const { return live[i].w; }
 };
 
 template <class F>
@@ -992,13 +1000,13 @@ __global__ void __launch_bounds__(kTierThreads) k_keyed_sums(const std::int32_t*
 struct AdmitFn {
     RefArgs a;
     ReserveState r;
-    __device__ std::uint64_t weight(std::int64_t i) const { return a.vw[r.E[r.live[i]]]; }
-    __device__ std::int32_t vertex(std::int64_t i) const { return r.E[r.live[i]]; }
+    __device__ std::uint64_t weight(std::int64_t i) const { return r.live[i].w; }
+    __device__ std::int32_t vertex(std::int64_t i) const { return r.live[i].v; }
     __device__ bool operator()(std::int64_t i, int c, std::uint64_t w, std::uint64_t begin) const {
-        const std::int64_t e = r.live[i];
+        const std::int64_t e = r.live[i].e;
         r.tried[2 * e + (c >> 6)] |= 1ULL << (c & 63);
         if (begin <= r.slack[c] && w <= r.slack[c] - begin) {
-            const std::int32_t v = r.E[e];
+            const std::int32_t v = r.live[i].v;
             a.targets[v] = c;
             a.flags[v] = 1;
             r.done[e] = 1;
@@ -1014,17 +1022,24 @@ __global__ void k_gather_i32(const std::int32_t* src, const std::int32_t* idx, s
         dst[j] = src[idx[j]];
 }
 
-__global__ void k_iota_i32(std::int32_t* out, std::int64_t count) {
+__global__ void k_donor_info(RefArgs a, const std::int32_t* E, std::int64_t count, DonorInfo* out) {
     for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
-         j += (std::int64_t)gridDim.x * blockDim.x)
-        out[j] = static_cast<std::int32_t>(j);
+         j += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t v = E[j];
+        out[j] = DonorInfo{static_cast<std::int32_t>(j), v, a.vw[v], (std::int32_t)a.labels[v]};
+    }
 }
 
-__global__ void k_live_flags(const std::int32_t* live, std::int64_t count, const std::uint8_t* done,
-                             std::uint8_t* flags) {
+__global__ void k_live_flags(const DonorInfo* live, std::int64_t count, const std::uint8_t* done, std::uint8_t* flags) {
     for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
          j += (std::int64_t)gridDim.x * blockDim.x)
-        flags[j] = done[live[j]] == 0;
+        flags[j] = done[live[j].e] == 0;
+}
+
+__global__ void k_gather_info(const DonorInfo* src, const std::int32_t* idx, std::int64_t count, DonorInfo* dst) {
+    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
+         j += (std::int64_t)gridDim.x * blockDim.x)
+        dst[j] = src[idx[j]];
 }
 
 __global__ void k_slack(RefArgs a, unsigned long long* slack, unsigned long long* requests) {
@@ -1319,6 +1334,40 @@ __global__ void k_bitonic_global(int* idx, int N, int j, int kk, const JMove* m,
         idx[l] = x;
     }
 }
+// The stages of a bitonic network whose partners lie in one aligned chunk
+// of kBitonicChunk indices, run in LDS by one block per chunk: every stage
+// of the merges up to the chunk size (kk == 0), or the stages j < chunk of
+// the merge of size kk.  Directions use the global index.
+constexpr int kBitonicChunk = 2048;
+template <typename Cmp>
+__global__ void __launch_bounds__(1024) k_bitonic_chunk(int* idx, int kk_fixed, const JMove* m, Cmp cmp) {
+    __shared__ int s[kBitonicChunk];
+    const int base = blockIdx.x * kBitonicChunk;
+    for (int i = threadIdx.x; i < kBitonicChunk; i += blockDim.x) s[i] = idx[base + i];
+    __syncthreads();
+    auto stage = [&](int kk, int j) {
+        for (int t = threadIdx.x; t < kBitonicChunk; t += blockDim.x) {
+            const int l = t ^ j;
+            if (l > t) {
+                const bool asc = ((base + t) & kk) == 0;
+                const int x = s[t], y = s[l];
+                if (asc ? idx_after(x, y, m, cmp) : idx_after(y, x, m, cmp)) {
+                    s[t] = y;
+                    s[l] = x;
+                }
+            }
+        }
+        __syncthreads();
+    };
+    if (kk_fixed == 0) {
+        for (int kk = 2; kk <= kBitonicChunk; kk <<= 1)
+            for (int j = kk >> 1; j > 0; j >>= 1) stage(kk, j);
+    } else {
+        for (int j = kBitonicChunk >> 1; j > 0; j >>= 1) stage(kk_fixed, j);
+    }
+    for (int i = threadIdx.x; i < kBitonicChunk; i += blockDim.x) idx[base + i] = s[i];
+}
+
 template <typename Cmp>
 __global__ void k_bitonic_local(int* idx, int N, const JMove* m, Cmp cmp) {
     __shared__ int s[1024];
@@ -1825,8 +1874,8 @@ public:
         fill<unsigned long long>(c_, tried.get(), 2 * ne, 0ULL);
         hipLaunchKernelGGL(k_slack, dim3(1), dim3(kMaxParts), 0, s, a_, slack.get(), requests.get());
         GP_LAUNCH_CHECK();
-        DBuf<std::int32_t> live = c_.buf<std::int32_t>(static_cast<std::size_t>(ne));
-        hipLaunchKernelGGL(k_iota_i32, dim3(grid_for(ne)), dim3(256), 0, s, live.get(), ne);
+        DBuf<DonorInfo> live = c_.buf<DonorInfo>(static_cast<std::size_t>(ne));
+        hipLaunchKernelGGL(k_donor_info, dim3(grid_for(ne)), dim3(256), 0, s, a_, E.get(), ne, live.get());
         GP_LAUNCH_CHECK();
         ReserveState r{E.get(),     ne,         conn.get(), conn_part.get(), conn_nnz.get(), requests.get(),
                        slack.get(), done.get(), tried.get(), live.get(),     ne};
@@ -1854,7 +1903,7 @@ public:
                                    r, attempt, keys.get(), active.get() + j);
                 GP_LAUNCH_CHECK();
                 hipLaunchKernelGGL((k_keyed_sums<DonorWeight>), dim3(nb), dim3(kTierThreads), 0, s, keys.get(), nlv, k,
-                                   nb, DonorWeight{a_, E.get(), r.live}, S.get());
+                                   nb, DonorWeight{r.live}, S.get());
                 GP_LAUNCH_CHECK();
                 // Ranks start at the start-of-attempt totals; totals advance.
                 hipLaunchKernelGGL(k_keyed_scan, dim3(1), dim3(kKeyScanThreads), 0, s, S.get(), nb, k, requests.get());
@@ -1873,8 +1922,8 @@ public:
                 stable_select(c_, lf.get(), nlv, li.get(), lc.get());
                 const std::int64_t next = c_.read_scalar(lc.get());
                 if (next == 0) break;
-                DBuf<std::int32_t> live2 = c_.buf<std::int32_t>(static_cast<std::size_t>(next));
-                hipLaunchKernelGGL(k_gather_i32, dim3(grid_for(next)), dim3(256), 0, s, r.live, li.get(), next,
+                DBuf<DonorInfo> live2 = c_.buf<DonorInfo>(static_cast<std::size_t>(next));
+                hipLaunchKernelGGL(k_gather_info, dim3(grid_for(next)), dim3(256), 0, s, r.live, li.get(), next,
                                    live2.get());
                 GP_LAUNCH_CHECK();
                 live.swap(live2);
@@ -2052,12 +2101,20 @@ public:
             hipLaunchKernelGGL((k_bitonic_local<Cmp>), dim3(1), dim3(1024), 0, s, sidx.get(), N, src, cmp);
             GP_LAUNCH_CHECK();
         } else {
-            for (int kk = 2; kk <= N; kk <<= 1)
-                for (int j = kk >> 1; j > 0; j >>= 1) {
+            // N is a power of two above 1024: chunk-local stages in LDS, one
+            // launch per cross-chunk stage.
+            const int chunks = N / kBitonicChunk;
+            hipLaunchKernelGGL((k_bitonic_chunk<Cmp>), dim3(chunks), dim3(1024), 0, s, sidx.get(), 0, src, cmp);
+            GP_LAUNCH_CHECK();
+            for (int kk = 2 * kBitonicChunk; kk <= N; kk <<= 1) {
+                for (int j = kk >> 1; j >= kBitonicChunk; j >>= 1) {
                     hipLaunchKernelGGL((k_bitonic_global<Cmp>), dim3(ceil_div(N, 256)), dim3(256), 0, s, sidx.get(), N,
                                        j, kk, src, cmp);
                     GP_LAUNCH_CHECK();
                 }
+                hipLaunchKernelGGL((k_bitonic_chunk<Cmp>), dim3(chunks), dim3(1024), 0, s, sidx.get(), kk, src, cmp);
+                GP_LAUNCH_CHECK();
+            }
         }
         DBuf<JMove> sorted = c_.buf<JMove>(static_cast<std::size_t>(m));
         hipLaunchKernelGGL(k_gather_moves, dim3(ceil_div(m, 256)), dim3(256), 0, s, src, sidx.get(), m, sorted.get());

@@ -1899,7 +1899,9 @@ public:
                                        conn.get(), conn_part.get(), conn_nnz.get());
                     GP_LAUNCH_CHECK();
                 }
-                hipLaunchKernelGGL(k_choose, dim3(grid_for(nlv, kTierThreads / kChooseLanes, 8192)), dim3(kTierThreads), 0, s, a_,
+                // Blocks stride over the donors: the per-block room table and
+                // part order are set up once per block.
+                hipLaunchKernelGGL(k_choose, dim3(grid_for(nlv, kTierThreads / kChooseLanes, 1024)), dim3(kTierThreads), 0, s, a_,
                                    r, attempt, keys.get(), active.get() + j);
                 GP_LAUNCH_CHECK();
                 hipLaunchKernelGGL((k_keyed_sums<DonorWeight>), dim3(nb), dim3(kTierThreads), 0, s, keys.get(), nlv, k,

@@ -128,6 +128,44 @@ __device__ inline std::uint64_t team_histogram(const Team& t, const RefArgs& a, 
     return t.sum(total);
 }
 
+// Heavy rows: one block, so each wave adds into its own table (hub rows
+// put most of their entries into a few parts, and one shared table would
+// serialise the whole block on those bins); the wave tables are then summed
+// into hist[0, k).  The block's scratch holds 1 + waves tables.
+constexpr int kHeavyTables = 1 + BlockTeam::size / kWave;
+__device__ inline std::uint64_t team_histogram(const BlockTeam& t, const RefArgs& a, std::int64_t v,
+                                               unsigned long long* hist) {
+    constexpr int kWaves = BlockTeam::size / kWave;
+    unsigned long long* mine = hist + (threadIdx.x / kWave + 1) * a.k;
+    t.sync();  // earlier readers of the table are done
+    for (int i = t.rank(); i < kHeavyTables * a.k; i += BlockTeam::size) hist[i] = 0;
+    t.sync();
+    std::uint64_t total = 0;
+    struct Rec {
+        int label;  // -1 for a self loop
+        std::uint32_t w;
+    };
+    row_batches(
+        t, a.off[v], a.off[v + 1],
+        [&](std::int64_t x) {
+            const std::int32_t u = a.nbr[(std::uint32_t)x];
+            return Rec{u == v ? -1 : a.labels[(std::uint32_t)u], (std::uint32_t)ew_at(a.ew, x)};
+        },
+        [&](std::int64_t, const Rec& r) {
+            if (r.label < 0) return;
+            atomicAdd(&mine[r.label], (unsigned long long)r.w);
+            total += r.w;
+        });
+    t.sync();
+    for (int p = t.rank(); p < a.k; p += BlockTeam::size) {
+        unsigned long long sum = 0;
+        for (int w = 1; w <= kWaves; ++w) sum += hist[w * a.k + p];
+        hist[p] = sum;
+    }
+    t.sync();
+    return t.sum(total);
+}
+
 // Short rows (light teams of at most 16 lanes, at most kTierLight entries):
 // the row's labels stay in registers and only their bins are touched, so a
 // table that starts zero is left zero and no per-row clear of all k bins is

@@ -337,7 +337,9 @@ void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int6
     // The light launch fills the heavy list: with overlap on, the heavy
     // launch (few long rows, few blocks) runs on the side stream beside the
     // medium launch.
-    const std::size_t heavy_lds = part + scratch * sizeof(unsigned long long);
+    // Heavy teams keep one table per wave plus the merged one.
+    const std::size_t heavy_lds = part + static_cast<std::size_t>(scratch) * (1 + kHeavyThreads / kWave) *
+                                             sizeof(unsigned long long);
     const bool side = heavy_rows && c.overlap();
     if (side) {
         c.fork();

@@ -960,9 +960,15 @@ __global__ void k_admission_commit(std::int64_t count, const std::uint32_t* vals
 
 __global__ void k_count_nonzero(std::int64_t n, const unsigned long long* cw, unsigned long long* out) {
     unsigned long long c = 0;
-    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
-         v += (std::int64_t)gridDim.x * blockDim.x)
-        c += cw[v] != 0;
+    const std::int64_t stride = (std::int64_t)gridDim.x * blockDim.x;
+    // Four independent loads per step.
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n; v += 4 * stride) {
+        unsigned long long x[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) x[j] = v + j * stride < n ? cw[v + j * stride] : 0ULL;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c += x[j] != 0;
+    }
     c = wave_sum(c);
     if (lane_id() == 0 && c) atomicAdd(out, c);
 }

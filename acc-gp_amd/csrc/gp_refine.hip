@@ -500,14 +500,23 @@ __global__ void k_set_lane(std::uint32_t* state, std::int64_t n, int lane, std::
 }
 
 // Frontier list from member flags (order irrelevant: per-vertex decisions).
+// Each wave reads kScanRounds x 64 consecutive flags before appending them.
+constexpr int kScanRounds = 8;
 template <typename F>
 __global__ void k_compact_flags(std::int64_t n, F flags, std::int32_t* out, unsigned long long* count) {
     __shared__ std::int32_t sb[kTierThreads / kWave][kStage];
     Stage st{sb[threadIdx.x / kWave], out, count, 0};
-    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane_id(); base < n;
-         base += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int64_t v = base + lane_id();
-        st.push(v < n && flags[v] != 0, (std::int32_t)v);
+    const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
+    const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;
+    for (std::int64_t base = wave * (kWave * kScanRounds); base < n; base += waves * (kWave * kScanRounds)) {
+        bool f[kScanRounds];
+#pragma unroll
+        for (int r = 0; r < kScanRounds; ++r) {
+            const std::int64_t v = base + r * kWave + lane_id();
+            f[r] = v < n && flags[v] != 0;
+        }
+#pragma unroll
+        for (int r = 0; r < kScanRounds; ++r) st.push(f[r], (std::int32_t)(base + r * kWave + lane_id()));
     }
     st.flush();
 }
@@ -534,16 +543,26 @@ __device__ inline int gain_bucket(const RefArgs& a, double gain) {
 __global__ void k_repair_donors(RefArgs a, std::int32_t* donors, unsigned long long* count) {
     __shared__ std::int32_t sb[kTierThreads / kWave][kStage];
     Stage st{sb[threadIdx.x / kWave], donors, count, 0};
-    for (std::int64_t base = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) - lane_id(); base < a.n;
-         base += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int64_t v = base + lane_id();
-        bool d = false;
-        if (v < a.n) {
-            a.flags[v] = 0;
-            a.bucket[v] = -1;
-            d = a.loads0[a.labels[v]] > a.cap;
+    const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
+    const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;
+    for (std::int64_t base = wave * (kWave * kScanRounds); base < a.n; base += waves * (kWave * kScanRounds)) {
+        int lab[kScanRounds];
+#pragma unroll
+        for (int r = 0; r < kScanRounds; ++r) {
+            const std::int64_t v = base + r * kWave + lane_id();
+            lab[r] = v < a.n ? (int)a.labels[v] : -1;
         }
-        st.push(d, (std::int32_t)v);
+#pragma unroll
+        for (int r = 0; r < kScanRounds; ++r) {
+            const std::int64_t v = base + r * kWave + lane_id();
+            bool d = false;
+            if (lab[r] >= 0) {
+                a.flags[v] = 0;
+                a.bucket[v] = -1;
+                d = a.loads0[lab[r]] > a.cap;
+            }
+            st.push(d, (std::int32_t)v);
+        }
     }
     st.flush();
 }
@@ -1828,7 +1847,7 @@ public:
         const hipStream_t s = c_.stream();
         const std::int64_t n = a_.n;
         fill<unsigned long long>(c_, ctr_.get(), kCtrCount, 0ULL);
-        hipLaunchKernelGGL(k_compact_flags<StateLane<std::uint8_t>>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, n,
+        hipLaunchKernelGGL(k_compact_flags<StateLane<std::uint8_t>>, dim3(grid_for(n, 256 * kScanRounds, 2048)), dim3(256), 0, s, n,
                            a_.member,
                            frontier_.get(), ctr_.get() + kFrontier);
         GP_LAUNCH_CHECK();
@@ -1991,7 +2010,7 @@ public:
             c_.copy_d2d(loads0_.get(), a_.loads, static_cast<std::size_t>(k) * sizeof(unsigned long long));
             fill<unsigned long long>(c_, counts_.get(), (std::int64_t)k * kBalanceWidth, 0ULL);
             fill<unsigned long long>(c_, ctr_.get(), kCtrCount, 0ULL);
-            hipLaunchKernelGGL(k_repair_donors, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, a_, cand_.get(),
+            hipLaunchKernelGGL(k_repair_donors, dim3(grid_for(n, 256 * kScanRounds, 2048)), dim3(256), 0, s, a_, cand_.get(),
                                ctr_.get() + kDonors);
             GP_LAUNCH_CHECK();
             if (narrow_bins())
@@ -2098,7 +2117,7 @@ public:
             hipLaunchKernelGGL(k_joint_accept, dim3(ceil_div(active, 256)), dim3(256), 0, s, a_, moves.get(), active,
                                stops.get());
             fill<unsigned long long>(c_, ctr_.get() + kMovers, 1, 0ULL);
-            hipLaunchKernelGGL(k_compact_flags<StateLane<std::uint8_t>>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s,
+            hipLaunchKernelGGL(k_compact_flags<StateLane<std::uint8_t>>, dim3(grid_for(n, 256 * kScanRounds, 2048)), dim3(256), 0, s,
                                n, a_.flags,
                                movers_.get(), ctr_.get() + kMovers);
             GP_LAUNCH_CHECK();

@@ -225,8 +225,43 @@ static __global__ void k_select_scatter(const std::uint8_t* flags, const std::in
          i += (std::int64_t)gridDim.x * blockDim.x)
         if (flags[i]) out[pos[i]] = static_cast<std::int32_t>(i);
 }
+// Short inputs: one block walks the flags in rounds of 1024 (ballot ranks
+// inside waves, wave counts through LDS), one launch instead of four.
+constexpr int kSelectSmallMax = 8192;
+static __global__ void __launch_bounds__(1024) k_select_small(const std::uint8_t* flags, std::int64_t n,
+                                                              std::int32_t* out, std::int64_t* count_dev) {
+    __shared__ int wsum[1024 / kWave];
+    __shared__ int base;
+    const int t = threadIdx.x, w = t / kWave, lane = t % kWave;
+    const unsigned long long lt = (1ULL << lane) - 1ULL;
+    if (t == 0) base = 0;
+    __syncthreads();
+    for (std::int64_t r = 0; r < n; r += 1024) {
+        const std::int64_t i = r + t;
+        const bool f = i < n && flags[i] != 0;
+        const unsigned long long mask = __ballot(f);
+        if (lane == 0) wsum[w] = __popcll(mask);
+        __syncthreads();
+        int pre = base, total = 0;
+        for (int j = 0; j < 1024 / kWave; ++j) {
+            pre += j < w ? wsum[j] : 0;
+            total += wsum[j];
+        }
+        if (f) out[pre + __popcll(mask & lt)] = static_cast<std::int32_t>(i);
+        __syncthreads();
+        if (t == 0) base += total;
+        __syncthreads();
+    }
+    if (t == 0) *count_dev = base;
+}
+
 inline void stable_select(Context& c, const std::uint8_t* flags, std::int64_t n, std::int32_t* out,
                           std::int64_t* count_dev) {
+    if (n > 0 && n <= kSelectSmallMax) {
+        hipLaunchKernelGGL(k_select_small, dim3(1), dim3(1024), 0, c.stream(), flags, n, out, count_dev);
+        GP_LAUNCH_CHECK();
+        return;
+    }
     DBuf<std::int64_t> pos = c.buf<std::int64_t>(static_cast<std::size_t>(n > 0 ? n : 1));
     exclusive_sum<std::int64_t>(c, flags, pos.get(), n, count_dev);
     if (n > 0) {

@@ -165,10 +165,12 @@ public:
         if (const char* e = std::getenv("GP_LIGHT_LANES")) set_light_lanes(std::atoi(e));
         if (const char* e = std::getenv("GP_MEDIUM_LANES")) set_medium_lanes(std::atoi(e));
         if (const char* e = std::getenv("GP_TIER_OVERLAP")) overlap_ = std::atoi(e) != 0;
+        if (const char* e = std::getenv("GP_SPIN_SYNC")) spin_ = std::atoi(e) != 0;
     }
     ~Context() {
         delete arena_;
         if (pinned_) (void)hipHostFree(pinned_);
+        if (done_) (void)hipEventDestroy(done_);
         if (fork_) (void)hipEventDestroy(fork_);
         if (join_) (void)hipEventDestroy(join_);
         if (aux_) (void)hipStreamDestroy(aux_);
@@ -218,7 +220,24 @@ public:
     int device() const { return device_; }
     void activate() const { GP_HIP(hipSetDevice(device_)); }
 
-    void sync() { GP_HIP(hipStreamSynchronize(stream_)); }
+    // Waits for the stream.  With spin waits on (default; GP_SPIN_SYNC=0 turns
+    // them off) the host polls an event recorded at the tail of the stream
+    // instead of sleeping in hipStreamSynchronize: the loop control reads
+    // counters back about a thousand times per partition, and each wake-up
+    // is GPU idle time.
+    void sync() {
+        if (!spin_) {
+            GP_HIP(hipStreamSynchronize(stream_));
+            return;
+        }
+        if (!done_) GP_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+        GP_HIP(hipEventRecord(done_, stream_));
+        for (;;) {
+            const hipError_t e = hipEventQuery(done_);
+            if (e == hipSuccess) return;
+            if (e != hipErrorNotReady) GP_HIP(e);
+        }
+    }
 
     // D2H of a small region through pinned memory; synchronizes the stream.
     void read(void* host, const void* dev, std::size_t bytes) {
@@ -311,8 +330,9 @@ private:
     int device_;
     hipStream_t stream_ = nullptr;
     hipStream_t aux_ = nullptr;
-    hipEvent_t fork_ = nullptr, join_ = nullptr;
+    hipEvent_t fork_ = nullptr, join_ = nullptr, done_ = nullptr;
     bool overlap_ = true;
+    bool spin_ = true;
     Arena* arena_ = nullptr;
     void* pinned_ = nullptr;
     int light_lanes_ = 4;

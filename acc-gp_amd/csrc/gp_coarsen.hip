@@ -670,48 +670,91 @@ struct HubSrc {
 };
 
 // Unique (row, key) runs of sorted records: heads, scan, atomic sums.
-__global__ void k_run_heads(const std::uint64_t* keys, std::int64_t n, std::uint8_t* heads) {
-    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (std::int64_t)gridDim.x * blockDim.x)
-        heads[i] = keys[i] != ~0ULL && (i == 0 || keys[i] != keys[i - 1]);
+// Run heads of sorted records: a record whose key differs from its
+// predecessor's, padding keys (all ones) excluded.
+__device__ inline bool run_head(const std::uint64_t* keys, std::int64_t i) {
+    const std::uint64_t k = keys[i];
+    return k != ~0ULL && (i == 0 || k != keys[i - 1]);
 }
+
+// Heads per tile of kScanTile records (one block per tile).
+__global__ void __launch_bounds__(kScanThreads) k_run_tile_heads(const std::uint64_t* keys, std::int64_t n,
+                                                                std::int64_t* tile_heads) {
+    __shared__ std::int64_t wsum[kScanThreads / kWave];
+    const std::int64_t tile = blockIdx.x * (std::int64_t)kScanTile;
+    std::int64_t c = 0;
+#pragma unroll
+    for (int r = 0; r < kScanItems; ++r) {
+        const std::int64_t i = tile + r * kScanThreads + threadIdx.x;
+        c += i < n && run_head(keys, i);
+    }
+    c = wave_sum(c);
+    if (lane_id() == 0) wsum[threadIdx.x / kWave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        std::int64_t t = 0;
+        for (int w = 0; w < kScanThreads / kWave; ++w) t += wsum[w];
+        tile_heads[blockIdx.x] = t;
+    }
+}
+
 // Sums of runs of equal keys in sorted records.  A block reduces one tile;
 // runs inside the tile are stored, runs crossing a tile edge add their
-// partial sums atomically into the zero-filled usums.
+// partial sums atomically into the zero-filled usums.  A run's index is the
+// tile's head offset (exclusive scan of k_run_tile_heads) plus the heads
+// before it inside the tile.
 __global__ void __launch_bounds__(kScanThreads) k_run_reduce(const std::uint64_t* keys, const std::uint32_t* vals,
-                                                            std::int64_t n, const std::int64_t* pos,
+                                                            std::int64_t n, const std::int64_t* tile_base,
                                                             std::uint64_t* ukeys, unsigned long long* usums) {
     __shared__ SegU64 wsum[kScanThreads / kWave];
+    __shared__ unsigned hsum[kScanThreads / kWave];
     const int lane = lane_id(), w = threadIdx.x / kWave;
     const std::int64_t tile = blockIdx.x * (std::int64_t)kScanTile;
     const std::int64_t end = tile + kScanTile < n ? tile + kScanTile : n;
     const std::int64_t base = tile + w * (kScanItems * kWave);
     SegU64 inc[kScanItems];
+    unsigned hinc[kScanItems];
     SegU64 acc{0, 0};
+    unsigned hacc = 0;
 #pragma unroll
     for (int r = 0; r < kScanItems; ++r) {
         const std::int64_t idx = base + r * kWave + lane;
         SegU64 x{0, 0};
-        if (idx < end) x = SegU64{vals[idx], (idx == tile || keys[idx] != keys[idx - 1]) ? 1u : 0u};
+        unsigned h = 0;
+        if (idx < end) {
+            x = SegU64{vals[idx], (idx == tile || keys[idx] != keys[idx - 1]) ? 1u : 0u};
+            h = run_head(keys, idx) ? 1u : 0u;
+        }
         inc[r] = wave_inclusive_scan<SegPlusOp>(x);
+        hinc[r] = wave_inclusive_scan<PlusOp>(h);
         acc = SegPlusOp::op<SegU64>(acc, shfl_val(inc[r], kWave - 1));
+        hacc += __shfl(hinc[r], kWave - 1);
     }
-    if (lane == 0) wsum[w] = acc;
+    if (lane == 0) {
+        wsum[w] = acc;
+        hsum[w] = hacc;
+    }
     __syncthreads();
     SegU64 carry{0, 0};
-    for (int i = 0; i < w; ++i) carry = SegPlusOp::op<SegU64>(carry, wsum[i]);
+    std::int64_t hcarry = tile_base[blockIdx.x];
+    for (int i = 0; i < w; ++i) {
+        carry = SegPlusOp::op<SegU64>(carry, wsum[i]);
+        hcarry += hsum[i];
+    }
     const std::uint64_t left = tile > 0 ? keys[tile - 1] : ~0ULL;
 #pragma unroll
     for (int r = 0; r < kScanItems; ++r) {
         const std::int64_t idx = base + r * kWave + lane;
         const SegU64 full = SegPlusOp::op<SegU64>(carry, inc[r]);
+        const std::int64_t heads = hcarry + hinc[r];
         carry = SegPlusOp::op<SegU64>(carry, shfl_val(inc[r], kWave - 1));
+        hcarry += __shfl(hinc[r], kWave - 1);
         if (idx >= end) continue;
         const std::uint64_t key = keys[idx];
         if (key == ~0ULL) continue;
         const bool tail = idx + 1 == n || keys[idx + 1] != key;
         if (!tail && idx + 1 != end) continue;
-        const std::int64_t u = pos[idx] - 1;  // inclusive count of heads -> run id
+        const std::int64_t u = heads - 1;  // heads up to idx -> run id
         if (tail) ukeys[u] = key;
         if (tail && key != left)
             usums[u] = full.v;
@@ -766,19 +809,41 @@ __global__ void __launch_bounds__(256) k_hub_reduce(AffinityArgs a, const std::i
             return !a.roles[c] && base != 0 && base <= a.cap && a.cw[src] <= a.cap - base;
         };
         unsigned long long cur = 0, b1 = 0, b2 = 0;
-        for (std::int64_t j = first + threadIdx.x; j < last; j += blockDim.x) {
-            const std::int32_t c = (std::int32_t)(ukeys[j] & mask);
-            const unsigned long long val = usums[j];
-            if (c == src) {
-                cur = val;
-                continue;
+        // Four records per thread in flight: the records, then their
+        // clusters' weights and roles, then the order statistics.
+        const std::uint64_t srcw = a.cw[src];
+        constexpr int kHubBatch = 4;
+        for (std::int64_t j0 = first + threadIdx.x; j0 < last; j0 += kHubBatch * blockDim.x) {
+            std::int32_t c[kHubBatch];
+            unsigned long long val[kHubBatch], cwt[kHubBatch];
+            std::uint8_t role[kHubBatch];
+#pragma unroll
+            for (int q = 0; q < kHubBatch; ++q) {
+                const std::int64_t j = j0 + q * (std::int64_t)blockDim.x;
+                c[q] = j < last ? (std::int32_t)(ukeys[j] & mask) : -1;
+                val[q] = j < last ? usums[j] : 0ULL;
             }
-            if (!admissible(c)) continue;
-            if (val > b1) {
-                b2 = b1;
-                b1 = val;
-            } else if (val > b2)
-                b2 = val;
+#pragma unroll
+            for (int q = 0; q < kHubBatch; ++q) {
+                const bool look = c[q] >= 0 && c[q] != src && a.filter_roles;
+                cwt[q] = look ? a.cw[c[q]] : 0ULL;
+                role[q] = look ? a.roles[c[q]] : 0;
+            }
+#pragma unroll
+            for (int q = 0; q < kHubBatch; ++q) {
+                if (c[q] < 0) continue;
+                if (c[q] == src) {
+                    cur = val[q];
+                    continue;
+                }
+                if (a.filter_roles && !(!role[q] && cwt[q] != 0 && cwt[q] <= a.cap && srcw <= a.cap - cwt[q]))
+                    continue;
+                if (val[q] > b1) {
+                    b2 = b1;
+                    b1 = val[q];
+                } else if (val[q] > b2)
+                    b2 = val[q];
+            }
         }
         for (int d = kWave / 2; d; d >>= 1) {
             const unsigned long long o1 = __shfl_xor(b1, d), o2 = __shfl_xor(b2, d), oc = __shfl_xor(cur, d);
@@ -1289,19 +1354,24 @@ void launch_classes(Context& c, const std::int32_t* lists, std::int64_t stride, 
 static std::int64_t reduce_records(Context& c, DBuf<std::uint64_t>& keys, DBuf<std::uint32_t>& vals, std::int64_t count,
                                    int total_bits, DBuf<std::uint64_t>& ukeys, DBuf<unsigned long long>& usums) {
     radix_sort_pairs(c, keys, vals, count, 0, total_bits);
-    DBuf<std::uint8_t> heads = c.buf<std::uint8_t>(static_cast<std::size_t>(count));
-    DBuf<std::int64_t> pos = c.buf<std::int64_t>(static_cast<std::size_t>(count));
+    const std::int64_t tiles = std::max<std::int64_t>(1, ceil_div(count, kScanTile));
+    DBuf<std::int64_t> tile_base = c.buf<std::int64_t>(static_cast<std::size_t>(tiles));
     DBuf<std::int64_t> total = c.buf<std::int64_t>(1);
-    hipLaunchKernelGGL(k_run_heads, dim3(grid_for(count)), dim3(256), 0, c.stream(), keys.get(), count, heads.get());
-    GP_LAUNCH_CHECK();
-    inclusive_sum<std::int64_t>(c, heads.get(), pos.get(), count, total.get());
+    if (count > 0) {
+        hipLaunchKernelGGL(k_run_tile_heads, dim3(static_cast<unsigned>(tiles)), dim3(kScanThreads), 0, c.stream(),
+                           keys.get(), count, tile_base.get());
+        GP_LAUNCH_CHECK();
+    } else {
+        fill<std::int64_t>(c, tile_base.get(), 1, 0);
+    }
+    exclusive_sum<std::int64_t>(c, tile_base.get(), tile_base.get(), tiles, total.get());
     const std::int64_t nu = c.read_scalar(total.get());
     ukeys = c.buf<std::uint64_t>(static_cast<std::size_t>(std::max<std::int64_t>(nu, 1)));
     usums = c.buf<unsigned long long>(static_cast<std::size_t>(std::max<std::int64_t>(nu, 1)));
     fill<unsigned long long>(c, usums.get(), nu, 0ULL);
     if (count > 0 && nu > 0)
-        hipLaunchKernelGGL(k_run_reduce, dim3(static_cast<unsigned>(ceil_div(count, kScanTile))), dim3(kScanThreads), 0,
-                       c.stream(), keys.get(), vals.get(), count, pos.get(), ukeys.get(), usums.get());
+        hipLaunchKernelGGL(k_run_reduce, dim3(static_cast<unsigned>(tiles)), dim3(kScanThreads), 0, c.stream(),
+                           keys.get(), vals.get(), count, tile_base.get(), ukeys.get(), usums.get());
     GP_LAUNCH_CHECK();
     return nu;
 }

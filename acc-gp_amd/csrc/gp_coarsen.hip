@@ -763,25 +763,17 @@ __global__ void __launch_bounds__(kScanThreads) k_run_reduce(const std::uint64_t
     }
 }
 
-__device__ inline std::int64_t lower_bound_u64(const std::uint64_t* a, std::int64_t n, std::uint64_t key) {
-    std::int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        const std::int64_t mid = lo + (hi - lo) / 2;
-        if (a[mid] < key)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 // Record range [first, last) of list row li in keys sorted by (li, key).
-__device__ inline void block_record_range(const std::uint64_t* ukeys, std::int64_t nu, std::int64_t li, int key_bits,
-                                          std::int64_t* range) {
-    __syncthreads();
-    if (threadIdx.x == 0) range[0] = lower_bound_u64(ukeys, nu, (std::uint64_t)li << key_bits);
-    if (threadIdx.x == kWave) range[1] = lower_bound_u64(ukeys, nu, (std::uint64_t)(li + 1) << key_bits);
-    __syncthreads();
+// Record range [ranges[2r], ranges[2r+1]) of every row r of sorted unique
+// records keyed (row << key_bits | key): one pass marks where rows start
+// and end (rows without records keep the zero-filled empty range).
+__global__ void k_row_ranges(const std::uint64_t* ukeys, std::int64_t nu, int key_bits, std::int64_t* ranges) {
+    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < nu;
+         j += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::uint64_t row = ukeys[j] >> key_bits;
+        if (j == 0 || (ukeys[j - 1] >> key_bits) != row) ranges[2 * row] = j;
+        if (j + 1 == nu || (ukeys[j + 1] >> key_bits) != row) ranges[2 * row + 1] = j + 1;
+    }
 }
 
 struct HubTop2 {
@@ -793,14 +785,13 @@ struct HubTop2 {
 // depend on visiting order) and the hashed tie key among the best.
 __global__ void __launch_bounds__(256) k_hub_reduce(AffinityArgs a, const std::int32_t* list, std::int64_t count,
                                                    const std::uint64_t* ukeys, const unsigned long long* usums,
-                                                   std::int64_t nu, int key_bits, HubTop2* acc) {
-    __shared__ std::int64_t range[2];
+                                                   const std::int64_t* ranges, int key_bits, HubTop2* acc) {
     __shared__ unsigned long long part[4][3];
     const std::uint64_t mask = (1ULL << key_bits) - 1ULL;
     const int w = threadIdx.x / kWave;
     for (std::int64_t r = blockIdx.x; r < count; r += gridDim.x) {
-        block_record_range(ukeys, nu, r, key_bits, range);
-        const std::int64_t first = range[0], last = range[1];
+        __syncthreads();
+        const std::int64_t first = ranges[2 * r], last = ranges[2 * r + 1];
         const std::int32_t v = list[r];
         const std::int32_t src = a.clusters[v];
         auto admissible = [&](std::int32_t c) {
@@ -1282,11 +1273,9 @@ __global__ void k_long_records(RowArgs r, const std::int32_t* list, std::int64_t
 }
 // Write reduced long-row entries back at their row start; one block per row.
 __global__ void k_long_writeback(RowArgs r, const std::int32_t* list, std::int64_t count, const std::uint64_t* ukeys,
-                                 const unsigned long long* usums, std::int64_t nu, int key_bits) {
-    __shared__ std::int64_t range[2];
+                                 const unsigned long long* usums, const std::int64_t* ranges, int key_bits) {
     for (std::int64_t li = blockIdx.x; li < count; li += gridDim.x) {
-        block_record_range(ukeys, nu, li, key_bits, range);
-        const std::int64_t first = range[0], last = range[1];
+        const std::int64_t first = ranges[2 * li], last = ranges[2 * li + 1];
         const std::int32_t row = list[li];
         const std::int64_t start = r.raw_off[row];
         if (threadIdx.x == 0) r.ucount[row] = (unsigned long long)(last - first);
@@ -1581,8 +1570,15 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
                 nu = reduce_records(c, keys, vals, hub_records, cluster_bits + hub_bits, ukeys, usums);
             }
             DBuf<HubTop2> acc = c.buf<HubTop2>(static_cast<std::size_t>(hub_count));
+            DBuf<std::int64_t> ranges = c.buf<std::int64_t>(static_cast<std::size_t>(2 * hub_count));
+            fill<std::int64_t>(c, ranges.get(), 2 * hub_count, 0);
+            if (nu > 0) {
+                hipLaunchKernelGGL(k_row_ranges, dim3(grid_for(nu)), dim3(256), 0, s, ukeys.get(), nu, cluster_bits,
+                                   ranges.get());
+                GP_LAUNCH_CHECK();
+            }
             hipLaunchKernelGGL(k_hub_reduce, dim3(grid_for(hub_count, 1, 4096)), dim3(256), 0, s, a, hub_list,
-                               hub_count, ukeys.get(), usums.get(), nu, cluster_bits, acc.get());
+                               hub_count, ukeys.get(), usums.get(), ranges.get(), cluster_bits, acc.get());
             GP_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_hub_decode, dim3(grid_for(hub_count)), dim3(256), 0, s, a, hub_list, hub_count,
                                acc.get());
@@ -1901,8 +1897,13 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
             nu = reduce_records(c, keys, vals, records, key_bits + row_bits, ukeys, usums);
         }
         if (nu) {
+            DBuf<std::int64_t> ranges = c.buf<std::int64_t>(static_cast<std::size_t>(2 * long_count));
+            fill<std::int64_t>(c, ranges.get(), 2 * long_count, 0);
+            hipLaunchKernelGGL(k_row_ranges, dim3(grid_for(nu)), dim3(256), 0, s, ukeys.get(), nu, key_bits,
+                               ranges.get());
+            GP_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_long_writeback, dim3(grid_for(long_count, 1, 4096)), dim3(256), 0, s, r, long_list,
-                               long_count, ukeys.get(), usums.get(), nu, key_bits);
+                               long_count, ukeys.get(), usums.get(), ranges.get(), key_bits);
             GP_LAUNCH_CHECK();
         }
     }

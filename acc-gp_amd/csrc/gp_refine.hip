@@ -1073,6 +1073,95 @@ struct AdmitFn {
     }
 };
 
+// The remaining attempts of a reservation whose live donors fit one block
+// (at most kTailDonors, one thread each), looped inside one launch: per
+// attempt the same start-of-attempt room and part order, the same choice
+// per donor, the same ranks (exclusive weight prefix per requested part in
+// donor order on top of the part's request total) and admissions as the
+// per-attempt launches, and the loop stops after an attempt with no chooser.
+constexpr int kTailDonors = 1024;
+__global__ void __launch_bounds__(kTailDonors) k_reserve_tail(RefArgs a, ReserveState r, int attempt0,
+                                                              std::int32_t* movers, unsigned long long* mcount) {
+    constexpr int kWaves = kTailDonors / kWave;
+    __shared__ unsigned long long room[kMaxParts];
+    __shared__ std::uint8_t order[kMaxParts];
+    __shared__ unsigned long long wt[kWaves][kMaxParts];
+    __shared__ std::int32_t sb[kWaves][kStage];
+    __shared__ int active;
+    const int t = threadIdx.x, w = t / kWave;
+    Stage st{sb[w], movers, mcount, 0};
+    for (int attempt = attempt0; attempt < a.k; ++attempt) {
+        for (int p = t; p < a.k; p += kTailDonors)
+            room[p] = r.requests[p] < r.slack[p] ? r.slack[p] - r.requests[p] : 0;
+        for (int i = t; i < kWaves * kMaxParts; i += kTailDonors) wt[i / kMaxParts][i % kMaxParts] = 0;
+        if (t == 0) active = 0;
+        __syncthreads();
+        for (int p = t; p < a.k; p += kTailDonors) {
+            int rank = 0;
+            for (int q = 0; q < a.k; ++q) rank += room[q] > room[p] || (room[q] == room[p] && q < p);
+            order[rank] = static_cast<std::uint8_t>(p);
+        }
+        __syncthreads();
+        int key = -1;
+        DonorInfo d{-1, 0, 0, 0};
+        if (t < r.nlive) {
+            d = r.live[t];
+            const std::int64_t e = d.e;
+            if (!r.done[e]) {
+                const std::uint64_t wd = d.w;
+                const int lv = d.label;
+                const unsigned long long tried0 = r.tried[2 * e], tried1 = r.tried[2 * e + 1];
+                auto closed = [&](int p) { return p == lv || ((((p >> 6) ? tried1 : tried0) >> (p & 63)) & 1ULL); };
+                Choice best{-1, 0, 0};
+                const int cnt = r.conn_nnz[e];
+                for (int j = 0; j < cnt; ++j) {
+                    const int p = r.conn_part[e * a.k + j];
+                    if (closed(p) || room[p] < wd) continue;
+                    const Choice x{p, r.conn[e * a.k + j], room[p]};
+                    if (choice_better(x, best)) best = x;
+                }
+                if (best.p < 0)
+                    for (int q = 0; q < a.k; ++q) {
+                        const int p = order[q];
+                        if (closed(p)) continue;
+                        if (room[p] >= wd) best = Choice{p, 0, room[p]};
+                        break;
+                    }
+                if (best.p < 0) r.done[e] = 2;
+                key = best.p;
+            }
+        }
+        const std::uint64_t wgt = key >= 0 ? d.w : 0;
+        const std::uint64_t ex = wave_keyed_exclusive(key, wgt, wt[w]);
+        __syncthreads();
+        bool admit = false;
+        if (key >= 0) {
+            atomicAdd(&active, 1);
+            std::uint64_t rank = r.requests[key] + ex;
+            for (int j = 0; j < w; ++j) rank += wt[j][key];
+            const std::int64_t e = d.e;
+            r.tried[2 * e + (key >> 6)] |= 1ULL << (key & 63);
+            if (rank <= r.slack[key] && wgt <= r.slack[key] - rank) {
+                a.targets[d.v] = key;
+                a.flags[d.v] = 1;
+                r.done[e] = 1;
+                admit = true;
+            }
+        }
+        st.push(admit, d.v);
+        __syncthreads();
+        const int act = active;
+        for (int p = t; p < a.k; p += kTailDonors) {
+            unsigned long long add = 0;
+            for (int j = 0; j < kWaves; ++j) add += wt[j][p];
+            r.requests[p] += add;
+        }
+        __syncthreads();
+        if (act == 0) break;
+    }
+    st.flush();
+}
+
 __global__ void k_gather_i32(const std::int32_t* src, const std::int32_t* idx, std::int64_t count, std::int32_t* dst) {
     for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
          j += (std::int64_t)gridDim.x * blockDim.x)
@@ -1988,6 +2077,13 @@ public:
                 live.swap(live2);
                 r.live = live.get();
                 r.nlive = next;
+            }
+            // A donor list that fits one block finishes in one launch.
+            if (r.nlive <= kTailDonors && attempt < k) {
+                hipLaunchKernelGGL(k_reserve_tail, dim3(1), dim3(kTailDonors), 0, s, a_, r, attempt, movers_.get(),
+                                   ctr_.get() + kMovers);
+                GP_LAUNCH_CHECK();
+                break;
             }
         }
     }

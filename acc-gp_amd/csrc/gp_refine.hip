@@ -1076,9 +1076,11 @@ struct AdmitFn {
 // The remaining attempts of a reservation whose live donors fit one block
 // (at most kTailDonors, one thread each), looped inside one launch: per
 // attempt the same start-of-attempt room and part order, the same choice
-// per donor, the same ranks (exclusive weight prefix per requested part in
-// donor order on top of the part's request total) and admissions as the
-// per-attempt launches, and the loop stops after an attempt with no chooser.
+// per donor (attempt 0: the proposed target), the same ranks (exclusive
+// weight prefix per requested part in donor order on top of the part's
+// request total) and admissions as the per-attempt launches, and the loop
+// stops after an attempt with no chooser.  Connection rows must be built
+// before the launch.
 constexpr int kTailDonors = 1024;
 __global__ void __launch_bounds__(kTailDonors) k_reserve_tail(RefArgs a, ReserveState r, int attempt0,
                                                               std::int32_t* movers, unsigned long long* mcount) {
@@ -1107,7 +1109,9 @@ __global__ void __launch_bounds__(kTailDonors) k_reserve_tail(RefArgs a, Reserve
         if (t < r.nlive) {
             d = r.live[t];
             const std::int64_t e = d.e;
-            if (!r.done[e]) {
+            if (!r.done[e] && attempt == 0) {
+                key = a.targets[d.v];
+            } else if (!r.done[e]) {
                 const std::uint64_t wd = d.w;
                 const int lv = d.label;
                 const unsigned long long tried0 = r.tried[2 * e], tried1 = r.tried[2 * e + 1];
@@ -2033,6 +2037,18 @@ public:
         constexpr int kAttemptGroup = 4;
         constexpr std::int64_t kCompactMin = 4096;
         DBuf<unsigned long long> active = c_.buf<unsigned long long>(kAttemptGroup);
+        if (ne <= kTailDonors) {
+            // Every donor's connection row first (the rows of donors admitted
+            // in attempt 0 go unused), then all attempts in one launch.
+            hipLaunchKernelGGL(k_conn_rows, dim3(grid_for(ne, 256 / kWave, 2048)), dim3(256),
+                               (256 / kWave) * k * sizeof(unsigned long long), s, a_, E.get(), ne, nullptr, conn.get(),
+                               conn_part.get(), conn_nnz.get());
+            GP_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_reserve_tail, dim3(1), dim3(kTailDonors), 0, s, a_, r, 0, movers_.get(),
+                               ctr_.get() + kMovers);
+            GP_LAUNCH_CHECK();
+            return;
+        }
         for (int attempt = 0; attempt < k;) {
             const int group = std::min(kAttemptGroup, k - attempt);
             const std::int64_t nlv = r.nlive;

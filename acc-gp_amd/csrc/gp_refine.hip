@@ -1074,92 +1074,106 @@ struct AdmitFn {
 };
 
 // The remaining attempts of a reservation whose live donors fit one block
-// (at most kTailDonors, one thread each), looped inside one launch: per
+// (at most kTailDonors, in rounds of one donor per thread), looped inside one launch: per
 // attempt the same start-of-attempt room and part order, the same choice
 // per donor (attempt 0: the proposed target), the same ranks (exclusive
 // weight prefix per requested part in donor order on top of the part's
 // request total) and admissions as the per-attempt launches, and the loop
 // stops after an attempt with no chooser.  Connection rows must be built
 // before the launch.
-constexpr int kTailDonors = 1024;
-__global__ void __launch_bounds__(kTailDonors) k_reserve_tail(RefArgs a, ReserveState r, int attempt0,
-                                                              std::int32_t* movers, unsigned long long* mcount) {
-    constexpr int kWaves = kTailDonors / kWave;
+constexpr int kTailThreads = 1024;
+constexpr int kTailDonors = 4096;  // donors per launch: rounds of kTailThreads
+__global__ void __launch_bounds__(kTailThreads) k_reserve_tail(RefArgs a, ReserveState r, int attempt0,
+                                                               std::int32_t* movers, unsigned long long* mcount) {
+    constexpr int kWaves = kTailThreads / kWave;
     __shared__ unsigned long long room[kMaxParts];
+    __shared__ unsigned long long run[kMaxParts];  // this attempt's requests of earlier rounds
     __shared__ std::uint8_t order[kMaxParts];
     __shared__ unsigned long long wt[kWaves][kMaxParts];
     __shared__ std::int32_t sb[kWaves][kStage];
     __shared__ int active;
     const int t = threadIdx.x, w = t / kWave;
     Stage st{sb[w], movers, mcount, 0};
+    for (int i = t; i < kWaves * kMaxParts; i += kTailThreads) wt[i / kMaxParts][i % kMaxParts] = 0;
     for (int attempt = attempt0; attempt < a.k; ++attempt) {
-        for (int p = t; p < a.k; p += kTailDonors)
+        for (int p = t; p < a.k; p += kTailThreads) {
             room[p] = r.requests[p] < r.slack[p] ? r.slack[p] - r.requests[p] : 0;
-        for (int i = t; i < kWaves * kMaxParts; i += kTailDonors) wt[i / kMaxParts][i % kMaxParts] = 0;
+            run[p] = 0;
+        }
         if (t == 0) active = 0;
         __syncthreads();
-        for (int p = t; p < a.k; p += kTailDonors) {
+        for (int p = t; p < a.k; p += kTailThreads) {
             int rank = 0;
             for (int q = 0; q < a.k; ++q) rank += room[q] > room[p] || (room[q] == room[p] && q < p);
             order[rank] = static_cast<std::uint8_t>(p);
         }
         __syncthreads();
-        int key = -1;
-        DonorInfo d{-1, 0, 0, 0};
-        if (t < r.nlive) {
-            d = r.live[t];
-            const std::int64_t e = d.e;
-            if (!r.done[e] && attempt == 0) {
-                key = a.targets[d.v];
-            } else if (!r.done[e]) {
-                const std::uint64_t wd = d.w;
-                const int lv = d.label;
-                const unsigned long long tried0 = r.tried[2 * e], tried1 = r.tried[2 * e + 1];
-                auto closed = [&](int p) { return p == lv || ((((p >> 6) ? tried1 : tried0) >> (p & 63)) & 1ULL); };
-                Choice best{-1, 0, 0};
-                const int cnt = r.conn_nnz[e];
-                for (int j = 0; j < cnt; ++j) {
-                    const int p = r.conn_part[e * a.k + j];
-                    if (closed(p) || room[p] < wd) continue;
-                    const Choice x{p, r.conn[e * a.k + j], room[p]};
-                    if (choice_better(x, best)) best = x;
-                }
-                if (best.p < 0)
-                    for (int q = 0; q < a.k; ++q) {
-                        const int p = order[q];
-                        if (closed(p)) continue;
-                        if (room[p] >= wd) best = Choice{p, 0, room[p]};
-                        break;
+        for (std::int64_t base = 0; base < r.nlive; base += kTailThreads) {
+            const std::int64_t i = base + t;
+            int key = -1;
+            DonorInfo d{-1, 0, 0, 0};
+            if (i < r.nlive) {
+                d = r.live[i];
+                const std::int64_t e = d.e;
+                if (!r.done[e] && attempt == 0) {
+                    key = a.targets[d.v];
+                } else if (!r.done[e]) {
+                    const std::uint64_t wd = d.w;
+                    const int lv = d.label;
+                    const unsigned long long tried0 = r.tried[2 * e], tried1 = r.tried[2 * e + 1];
+                    auto closed = [&](int p) {
+                        return p == lv || ((((p >> 6) ? tried1 : tried0) >> (p & 63)) & 1ULL);
+                    };
+                    Choice best{-1, 0, 0};
+                    const int cnt = r.conn_nnz[e];
+                    for (int j = 0; j < cnt; ++j) {
+                        const int p = r.conn_part[e * a.k + j];
+                        if (closed(p) || room[p] < wd) continue;
+                        const Choice x{p, r.conn[e * a.k + j], room[p]};
+                        if (choice_better(x, best)) best = x;
                     }
-                if (best.p < 0) r.done[e] = 2;
-                key = best.p;
+                    if (best.p < 0)
+                        for (int q = 0; q < a.k; ++q) {
+                            const int p = order[q];
+                            if (closed(p)) continue;
+                            if (room[p] >= wd) best = Choice{p, 0, room[p]};
+                            break;
+                        }
+                    if (best.p < 0) r.done[e] = 2;
+                    key = best.p;
+                }
             }
-        }
-        const std::uint64_t wgt = key >= 0 ? d.w : 0;
-        const std::uint64_t ex = wave_keyed_exclusive(key, wgt, wt[w]);
-        __syncthreads();
-        bool admit = false;
-        if (key >= 0) {
-            atomicAdd(&active, 1);
-            std::uint64_t rank = r.requests[key] + ex;
-            for (int j = 0; j < w; ++j) rank += wt[j][key];
-            const std::int64_t e = d.e;
-            r.tried[2 * e + (key >> 6)] |= 1ULL << (key & 63);
-            if (rank <= r.slack[key] && wgt <= r.slack[key] - rank) {
-                a.targets[d.v] = key;
-                a.flags[d.v] = 1;
-                r.done[e] = 1;
-                admit = true;
+            const std::uint64_t wgt = key >= 0 ? d.w : 0;
+            const std::uint64_t ex = wave_keyed_exclusive(key, wgt, wt[w]);
+            __syncthreads();
+            bool admit = false;
+            if (key >= 0) {
+                atomicAdd(&active, 1);
+                std::uint64_t rank = r.requests[key] + run[key] + ex;
+                for (int j = 0; j < w; ++j) rank += wt[j][key];
+                const std::int64_t e = d.e;
+                r.tried[2 * e + (key >> 6)] |= 1ULL << (key & 63);
+                if (rank <= r.slack[key] && wgt <= r.slack[key] - rank) {
+                    a.targets[d.v] = key;
+                    a.flags[d.v] = 1;
+                    r.done[e] = 1;
+                    admit = true;
+                }
             }
+            st.push(admit, d.v);
+            __syncthreads();
+            for (int p = t; p < a.k; p += kTailThreads) {
+                unsigned long long add = 0;
+                for (int j = 0; j < kWaves; ++j) {
+                    add += wt[j][p];
+                    wt[j][p] = 0;
+                }
+                run[p] += add;
+            }
+            __syncthreads();
         }
-        st.push(admit, d.v);
-        __syncthreads();
         const int act = active;
-        for (int p = t; p < a.k; p += kTailDonors) {
-            unsigned long long add = 0;
-            for (int j = 0; j < kWaves; ++j) add += wt[j][p];
-            r.requests[p] += add;
-        }
+        for (int p = t; p < a.k; p += kTailThreads) r.requests[p] += run[p];
         __syncthreads();
         if (act == 0) break;
     }
@@ -2044,7 +2058,7 @@ public:
                                (256 / kWave) * k * sizeof(unsigned long long), s, a_, E.get(), ne, nullptr, conn.get(),
                                conn_part.get(), conn_nnz.get());
             GP_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_reserve_tail, dim3(1), dim3(kTailDonors), 0, s, a_, r, 0, movers_.get(),
+            hipLaunchKernelGGL(k_reserve_tail, dim3(1), dim3(kTailThreads), 0, s, a_, r, 0, movers_.get(),
                                ctr_.get() + kMovers);
             GP_LAUNCH_CHECK();
             return;
@@ -2096,7 +2110,7 @@ public:
             }
             // A donor list that fits one block finishes in one launch.
             if (r.nlive <= kTailDonors && attempt < k) {
-                hipLaunchKernelGGL(k_reserve_tail, dim3(1), dim3(kTailDonors), 0, s, a_, r, attempt, movers_.get(),
+                hipLaunchKernelGGL(k_reserve_tail, dim3(1), dim3(kTailThreads), 0, s, a_, r, attempt, movers_.get(),
                                    ctr_.get() + kMovers);
                 GP_LAUNCH_CHECK();
                 break;

@@ -45,3 +45,23 @@ def random_graph(n, m, seed, max_weight=1):
         edges.append((u, v))
         weights.append(int(rng.integers(1, max_weight + 1)))
     return csr_from_edges(n, edges, weights)
+
+
+def hub_ring(n, extra, seed):
+    """Vertex 0 joined to every other vertex, a ring over 1..n-1 and `extra`
+    random chords: one row far above a block's share (n - 1 entries) beside
+    short rows.  Unit edge weights; vertex weights skewed (1..1000,
+    log-uniform) so capacity repair has work."""
+    rng = np.random.default_rng(seed)
+    edges = set()
+    for v in range(1, n):
+        edges.add((0, v))
+        w = v + 1 if v + 1 < n else 1
+        edges.add((min(v, w), max(v, w)))
+    while len(edges) < (n - 1) * 2 + extra:
+        u, v = (int(x) for x in rng.integers(1, n, 2))
+        if u != v:
+            edges.add((min(u, v), max(u, v)))
+    off, nbr, ew = csr_from_edges(n, sorted(edges))
+    vw = np.exp(rng.uniform(0.0, np.log(1000.0), n)).astype(np.uint32).clip(1, 1000)
+    return off, nbr, ew, vw

@@ -35,6 +35,12 @@ def _cases(gp):
     cases.append(("knn20000_k128_weighted", off, nbr, ew, vw, 128))
     off, nbr = gp.generate_rmat(11, 8, seed=9)
     cases.append(("rmat11_k2", off, nbr, None, None, 2))
+    # Rows longer than 4096 entries (17 of them, up to 9647) run in the
+    # heavy tier on the device.
+    off, nbr = gp.generate_rmat(16, 16, seed=1)
+    cases.append(("rmat16_k64_heavy_rows", off, nbr, None, None, 64))
+    off, nbr, ew, vw = graphs.hub_ring(12000, 6000, seed=4)
+    cases.append(("hub12000_k8_skewed_weights", off, nbr, ew, vw, 8))
     return cases
 
 

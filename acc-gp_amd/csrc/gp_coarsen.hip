@@ -456,46 +456,72 @@ __global__ void __launch_bounds__(kBlockSortThreads) k_affinity_block(AffinityAr
 
 // Hash variant of the block kernel (row weights below 2^32): per-cluster
 // sums in an LDS hash table, then the order-independent top-2 over slots.
-__global__ void __launch_bounds__(kBlockSortThreads) k_affinity_block_hash(AffinityArgs a, const std::int32_t* list,
-                                                                           std::int64_t count) {
-    __shared__ BlockHashSmem s;
-    __shared__ Top2 part[kBlockSortThreads / kWave];
+// Rows of at most MaxRow entries, one block each, summed per cluster in an
+// LDS hash of H::slots slots (exact while every row weighs less than 2^32).
+template <class H, int MaxRow, int Threads>
+__global__ void __launch_bounds__(Threads) k_affinity_block_hash(AffinityArgs a, const std::int32_t* list,
+                                                                 std::int64_t count) {
+    static_assert(4 * MaxRow <= 3 * H::slots, "a row fills at most three quarters of the table");
+    __shared__ H s;
+    __shared__ Top2 part[Threads / kWave];
     for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
         const std::int32_t v = list[idx];
         const std::int32_t src = a.clusters[v];
         if (!a.roles[src]) continue;
         const std::int64_t b = a.off[v];
         const int d = (int)(a.off[v + 1] - b);
-        const int slots = hash_slots_for(d);
+        const int slots = hash_slots_for<H>(d);
         block_hash_clear(s, slots);
         __syncthreads();
-        // The row's gathers are all issued before the inserts (d <= 4096).
-        constexpr int kPer = kBlockSortMax / kBlockSortThreads;
-        std::int32_t kk[kPer];
-        std::uint32_t vv[kPer];
+        // Sixteen gathers per thread are issued before their inserts.
+        constexpr int kPer = 16;
+        for (int x0 = 0; x0 < d; x0 += kPer * Threads) {
+            std::int32_t kk[kPer];
+            std::uint32_t vv[kPer];
 #pragma unroll
-        for (int r = 0; r < kPer; ++r) {
-            const int x = r * kBlockSortThreads + threadIdx.x;
-            const std::int32_t u = x < d ? a.nbr[b + x] : v;
-            kk[r] = u != v ? a.clusters[u] : -1;
-            vv[r] = x < d ? (std::uint32_t)ew_at(a.ew, b + x) : 0u;
+            for (int r = 0; r < kPer; ++r) {
+                const int x = x0 + r * Threads + threadIdx.x;
+                const std::int32_t u = x < d ? a.nbr[b + x] : v;
+                kk[r] = u != v ? a.clusters[u] : -1;
+                vv[r] = x < d ? (std::uint32_t)ew_at(a.ew, b + x) : 0u;
+            }
+#pragma unroll
+            for (int r = 0; r < kPer; ++r)
+                if (kk[r] >= 0) block_hash_add(s, slots, kk[r], vv[r]);
         }
-#pragma unroll
-        for (int r = 0; r < kPer; ++r)
-            if (kk[r] >= 0) block_hash_add(s, slots, kk[r], vv[r]);
         __syncthreads();
         Top2 t{0, 0, 0, 0};
-        for (int i = threadIdx.x; i < slots; i += kBlockSortThreads)
+        for (int i = threadIdx.x; i < slots; i += Threads)
             if (s.key[i] >= 0) top2_add(a, v, src, s.key[i], s.val[i], t);
         t = top2_group_reduce<kWave>(t);
         if (lane_id() == 0) part[threadIdx.x / kWave] = t;
         __syncthreads();
         if (threadIdx.x == 0) {
             Top2 r = part[0];
-            for (int i = 1; i < kBlockSortThreads / kWave; ++i) r = top2_merge(r, part[i]);
+            for (int i = 1; i < Threads / kWave; ++i) r = top2_merge(r, part[i]);
             decode_proposal(a, v, src, r);
         }
         __syncthreads();
+    }
+}
+
+// Hub rows of at most kBigRowDegree entries take one block each with a
+// larger table (one block fills a CU's LDS, so it brings 16 waves);
+// longer rows go through records and the global sort.
+constexpr int kBigRowDegree = 12288;
+using BigHashSmem = BlockHashT<16384>;
+
+// Splits the hub list by row length (order inside each part is arbitrary:
+// each row's proposal is its own).
+__global__ void k_split_hubs(const std::int32_t* off, const std::int32_t* list, std::int64_t count, std::int32_t* big,
+                             std::int32_t* rest, unsigned long long* counts) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t v = list[i];
+        if (off[v + 1] - off[v] <= kBigRowDegree)
+            big[atomicAdd(&counts[0], 1ULL)] = v;
+        else
+            rest[atomicAdd(&counts[1], 1ULL)] = v;
     }
 }
 
@@ -1491,9 +1517,27 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
     res.stats_max_degree = stats_maxdeg;
     res.stats_has_heavy = stats_heavy;
 
-    // Hub record layout (rows > 4096) prepared once per level.
-    const std::int64_t hub_count = (std::int64_t)cc.c[6];
+    // Hub record layout (rows > 4096) prepared once per level; with row
+    // weights below 2^32 the rows of at most kBigRowDegree entries are split
+    // off for the one-block path.
+    std::int64_t hub_count = (std::int64_t)cc.c[6];
     const std::int32_t* hub_list = lists.get() + 6 * n;
+    DBuf<std::int32_t> big_hubs, rest_hubs;
+    std::int64_t big_count = 0;
+    if (hub_count && st.max_row_weight <= 0xffffffffULL) {
+        big_hubs = c.buf<std::int32_t>(static_cast<std::size_t>(hub_count));
+        rest_hubs = c.buf<std::int32_t>(static_cast<std::size_t>(hub_count));
+        DBuf<unsigned long long> hc = c.buf<unsigned long long>(2);
+        fill<unsigned long long>(c, hc.get(), 2, 0ULL);
+        hipLaunchKernelGGL(k_split_hubs, dim3(grid_for(hub_count)), dim3(256), 0, s, g.off.get(), hub_list, hub_count,
+                           big_hubs.get(), rest_hubs.get(), hc.get());
+        GP_LAUNCH_CHECK();
+        unsigned long long h2[2];
+        c.read(h2, hc.get(), sizeof h2);
+        big_count = (std::int64_t)h2[0];
+        hub_count = (std::int64_t)h2[1];
+        hub_list = rest_hubs.get();
+    }
     DBuf<std::int64_t> hub_off;
     std::int64_t hub_records = 0;
     if (hub_count) {
@@ -1548,14 +1592,19 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
         auto block = [&](const std::int32_t* list, std::int64_t count) {
             if (!count) return;
             if (st.max_row_weight <= 0xffffffffULL)
-                hipLaunchKernelGGL(k_affinity_block_hash, dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, s,
-                                   a, list, count);
+                hipLaunchKernelGGL((k_affinity_block_hash<BlockHashSmem, kBlockDegree, kBlockSortThreads>),
+                                   dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, s, a, list, count);
             else
                 hipLaunchKernelGGL(k_affinity_block, dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, s, a,
                                    list, count);
             GP_LAUNCH_CHECK();
         };
         launch_classes(c, lists.get(), n, cc.c, group, block);
+        if (big_count) {
+            hipLaunchKernelGGL((k_affinity_block_hash<BigHashSmem, kBigRowDegree, 1024>),
+                               dim3(grid_for(big_count, 1, 4096)), dim3(1024), 0, s, a, big_hubs.get(), big_count);
+            GP_LAUNCH_CHECK();
+        }
         if (hub_count) {
             DBuf<std::uint64_t> ukeys;
             DBuf<unsigned long long> usums;

@@ -296,26 +296,33 @@ __device__ inline int block_sort_reduce(BlockSortSmem& s, int n, Emit&& emit, Be
 // is arbitrary: callers use it only where the result is order-independent.
 constexpr int kHashSlots = 8192;
 
-struct BlockHashSmem {
-    std::int32_t key[kHashSlots];
-    std::uint32_t val[kHashSlots];
+template <int Slots>
+struct BlockHashT {
+    static constexpr int slots = Slots;
+    std::int32_t key[Slots];
+    std::uint32_t val[Slots];
 };
+using BlockHashSmem = BlockHashT<kHashSlots>;
 
-// Table size for n entries: a power of two >= 2n (at least 512).
+// Table size for n entries: a power of two >= 2n (at least 512), at most
+// the table's capacity (rows of a class fill at most three quarters).
+template <class H = BlockHashSmem>
 __device__ inline int hash_slots_for(int n) {
     int h = 512;
-    while (h < 2 * n) h <<= 1;
+    while (h < 2 * n && h < H::slots) h <<= 1;
     return h;
 }
 
-__device__ inline void block_hash_clear(BlockHashSmem& s, int slots) {
+template <class H>
+__device__ inline void block_hash_clear(H& s, int slots) {
     for (int i = threadIdx.x; i < slots; i += blockDim.x) {
         s.key[i] = -1;
         s.val[i] = 0;
     }
 }
 
-__device__ inline void block_hash_add(BlockHashSmem& s, int slots, std::int32_t key, std::uint32_t w) {
+template <class H>
+__device__ inline void block_hash_add(H& s, int slots, std::int32_t key, std::uint32_t w) {
     unsigned slot = mix32((std::uint32_t)key) & (unsigned)(slots - 1);
     while (true) {
         const std::int32_t prev = atomicCAS(&s.key[slot], -1, key);

@@ -473,8 +473,9 @@ __global__ void __launch_bounds__(Threads) k_affinity_block_hash(AffinityArgs a,
         const int slots = hash_slots_for<H>(d);
         block_hash_clear(s, slots);
         __syncthreads();
-        // Sixteen gathers per thread are issued before their inserts.
-        constexpr int kPer = 16;
+        // Sixteen (eight in 1024-thread blocks, within their register
+        // budget) gathers per thread are issued before their inserts.
+        constexpr int kPer = Threads >= 1024 ? 8 : 16;
         for (int x0 = 0; x0 < d; x0 += kPer * Threads) {
             std::int32_t kk[kPer];
             std::uint32_t vv[kPer];

@@ -200,6 +200,10 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const st
     extern __shared__ unsigned long long smem[];
     __shared__ std::int32_t stage_buf[kTierThreads / kWave][2][kStage];
     constexpr int G = kWave / W;
+    const std::int64_t count = count_ptr ? (std::int64_t)*count_ptr : n;
+    // Blocks past the listed rows leave before their set-up (the grid is
+    // sized for the longest list the caller may pass).
+    if ((std::int64_t)blockIdx.x * (kTierThreads / kWave) * G >= count) return;
     const int lane = lane_id(), gw = lane / W, lg = lane % W;
     const unsigned long long* part = stage_parts(body, smem);
     unsigned long long* team_smem = smem + part_words_of(body);
@@ -209,7 +213,6 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const st
     // Team tables start zero; bodies that use them leave them zero.
     for (int i = threadIdx.x; i < (kTierThreads / W) * scratch_per_group; i += kTierThreads) team_smem[i] = 0;
     __syncthreads();
-    const std::int64_t count = count_ptr ? (std::int64_t)*count_ptr : n;
     const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
     const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;
     std::int64_t acc = 0;
@@ -253,11 +256,12 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLis
     extern __shared__ unsigned long long smem[];
     __shared__ std::int32_t stage_buf[kTierThreads / kWave][kStage];
     constexpr int G = kWave / W;
+    const std::int64_t count = (std::int64_t)*t.medium_count;
+    if ((std::int64_t)blockIdx.x * (kTierThreads / kWave) * G >= count) return;
     const int lane = lane_id(), gw = lane / W, lg = lane % W;
     const unsigned long long* part = stage_parts(body, smem);
     unsigned long long* scratch = smem + part_words_of(body) + (threadIdx.x / W) * scratch_per_group;
     Stage out_stage{stage_buf[threadIdx.x / kWave], t.out, t.out_count, 0};
-    const std::int64_t count = (std::int64_t)*t.medium_count;
     const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
     const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;
     std::int64_t acc = 0;

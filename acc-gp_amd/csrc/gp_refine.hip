@@ -1270,33 +1270,58 @@ __global__ void k_quotient(RefArgs a, const unsigned long long* A, unsigned long
     }
 }
 
-// joint_match (exchange.cuh:55-83): one block, one thread per part.
-__global__ void k_joint_match(int k, const unsigned long long* Q, std::int32_t* partner, std::int32_t* visits) {
+// joint_match (exchange.cuh:55-83): greedy mutual-best pairing of parts in
+// rounds, one block.  The pair scores Q / (1 + visits)^2 are computed once
+// into LDS (a pair's visit count changes only when the pair is matched, and
+// matched parts are not scored again); each part's best partner (largest
+// score, then smallest part) is found by a group of W lanes over
+// q = sub, sub + W, ...
+constexpr int kMatchThreads = 1024;
+__global__ void __launch_bounds__(kMatchThreads) k_joint_match(int k, const unsigned long long* Q,
+                                                               std::int32_t* partner, std::int32_t* visits) {
+    __shared__ double score_of[kMaxParts * kMaxParts];
     __shared__ int choice[kMaxParts];
+    __shared__ int mate[kMaxParts];
     __shared__ int progress;
-    const int p = threadIdx.x;
-    if (p < k) partner[p] = -1;
+    for (int x = threadIdx.x; x < k * k; x += blockDim.x) {
+        const int p = x / k, q = x % k;
+        double v = 0.0;
+        if (p < q && Q[x]) {
+            const double penalty = 1.0 + visits[x];
+            v = (double)Q[x] / (penalty * penalty);
+        }
+        score_of[x] = v;
+    }
+    for (int p = threadIdx.x; p < k; p += blockDim.x) mate[p] = -1;
     __syncthreads();
+    int W = kWave;
+    while (W > 1 && W * k > kMatchThreads) W >>= 1;
+    const int p = threadIdx.x / W, sub = threadIdx.x % W;
     for (int round = 0; round < k / 2; ++round) {
-        if (p == 0) progress = 0;
-        int best = -1;
+        if (threadIdx.x == 0) progress = 0;
         double score = -1;
-        if (p < k && partner[p] < 0)
-            for (int q = 0; q < k; ++q) {
-                if (q == p || partner[q] >= 0) continue;
-                const int pair = (p < q ? p : q) * k + (p < q ? q : p);
-                if (!Q[pair]) continue;
-                const double penalty = 1.0 + visits[pair];
-                const double value = (double)Q[pair] / (penalty * penalty);
-                if (value > score || (value == score && q < best)) {
-                    score = value;
+        int best = -1;
+        if (p < k && mate[p] < 0)
+            for (int q = sub; q < k; q += W) {
+                if (q == p || mate[q] >= 0) continue;
+                const double v = score_of[p < q ? p * k + q : q * k + p];
+                if (v > 0.0 && v > score) {
+                    score = v;
                     best = q;
                 }
             }
-        if (p < k) choice[p] = best;
+        for (int d = W / 2; d; d >>= 1) {
+            const double os = __shfl_xor(score, d, W);
+            const int ob = __shfl_xor(best, d, W);
+            if (os > score || (os == score && ob < best)) {
+                score = os;
+                best = ob;
+            }
+        }
+        if (p < k && sub == 0) choice[p] = best;
         __syncthreads();
-        if (p < k && best >= 0 && choice[best] == p) {
-            partner[p] = best;
+        if (p < k && sub == 0 && best >= 0 && choice[best] == p) {
+            mate[p] = best;
             if (p < best) {
                 ++visits[p * k + best];
                 atomicAdd(&progress, 1);
@@ -1307,6 +1332,7 @@ __global__ void k_joint_match(int k, const unsigned long long* Q, std::int32_t* 
         __syncthreads();
         if (!gained) break;
     }
+    for (int q = threadIdx.x; q < k; q += blockDim.x) partner[q] = mate[q];
 }
 
 __global__ void k_joint_seeds(RefArgs a, const unsigned long long* A, const std::int32_t* partner,
@@ -1498,61 +1524,91 @@ __global__ void k_bitonic_global(int* idx, int N, int j, int kk, const JMove* m,
         idx[l] = x;
     }
 }
-// The stages of a bitonic network whose partners lie in one aligned chunk
-// of kBitonicChunk indices, run in LDS by one block per chunk: every stage
-// of the merges up to the chunk size (kk == 0), or the stages j < chunk of
-// the merge of size kk.  Directions use the global index.
-constexpr int kBitonicChunk = 2048;
+// x sorts after y, for local slots of moves staged in LDS (g holds each
+// slot's global index, -1 for padding, which sorts last).
 template <typename Cmp>
-__global__ void __launch_bounds__(1024) k_bitonic_chunk(int* idx, int kk_fixed, const JMove* m, Cmp cmp) {
-    __shared__ int s[kBitonicChunk];
-    const int base = blockIdx.x * kBitonicChunk;
-    for (int i = threadIdx.x; i < kBitonicChunk; i += blockDim.x) s[i] = idx[base + i];
-    __syncthreads();
-    auto stage = [&](int kk, int j) {
-        for (int t = threadIdx.x; t < kBitonicChunk; t += blockDim.x) {
-            const int l = t ^ j;
-            if (l > t) {
-                const bool asc = ((base + t) & kk) == 0;
-                const int x = s[t], y = s[l];
-                if (asc ? idx_after(x, y, m, cmp) : idx_after(y, x, m, cmp)) {
-                    s[t] = y;
-                    s[l] = x;
-                }
-            }
-        }
-        __syncthreads();
-    };
-    if (kk_fixed == 0) {
-        for (int kk = 2; kk <= kBitonicChunk; kk <<= 1)
-            for (int j = kk >> 1; j > 0; j >>= 1) stage(kk, j);
-    } else {
-        for (int j = kBitonicChunk >> 1; j > 0; j >>= 1) stage(kk_fixed, j);
-    }
-    for (int i = threadIdx.x; i < kBitonicChunk; i += blockDim.x) idx[base + i] = s[i];
+__device__ inline bool slot_after(int x, int y, const int* g, const JMove* mv, Cmp cmp) {
+    if (g[x] < 0) return g[y] >= 0;
+    if (g[y] < 0) return false;
+    return cmp(mv[y], mv[x]);
 }
 
-template <typename Cmp>
-__global__ void k_bitonic_local(int* idx, int N, const JMove* m, Cmp cmp) {
-    __shared__ int s[1024];
-    for (int i = threadIdx.x; i < N; i += blockDim.x) s[i] = idx[i];
+// Bitonic stages over C indices held by one block, with the moves they
+// name copied into LDS once so that every comparison reads LDS: stages
+// (kk, j) for kk from kk_first to kk_last and j from min(kk, C) / 2 down to
+// 1.  Directions use the global position base + t.
+template <int C, typename Cmp>
+__device__ inline void bitonic_lds(int* idx, int base, int kk_first, int kk_last, const JMove* m, Cmp cmp,
+                                   JMove* mv, int* g, short* s) {
+    for (int i = threadIdx.x; i < C; i += blockDim.x) {
+        const int x = idx[base + i];
+        g[i] = x;
+        s[i] = (short)i;
+        if (x >= 0) mv[i] = m[x];
+    }
     __syncthreads();
-    for (int kk = 2; kk <= N; kk <<= 1)
-        for (int j = kk >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < N; i += blockDim.x) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const int x = s[i], y = s[l];
-                    const bool asc = (i & kk) == 0;
-                    if (asc ? idx_after(x, y, m, cmp) : idx_after(y, x, m, cmp)) {
-                        s[i] = y;
-                        s[l] = x;
-                    }
+    for (int kk = kk_first; kk <= kk_last; kk <<= 1)
+        for (int j = (kk < C ? kk : C) >> 1; j > 0; j >>= 1) {
+            // One compare-exchange pair (t, t + j) per thread.
+            for (int q = threadIdx.x; q < C / 2; q += blockDim.x) {
+                const int t = ((q & ~(j - 1)) << 1) | (q & (j - 1)), l = t + j;
+                const bool asc = ((base + t) & kk) == 0;
+                const int x = s[t], y = s[l];
+                if (asc ? slot_after(x, y, g, mv, cmp) : slot_after(y, x, g, mv, cmp)) {
+                    s[t] = (short)y;
+                    s[l] = (short)x;
                 }
             }
             __syncthreads();
         }
-    for (int i = threadIdx.x; i < N; i += blockDim.x) idx[i] = s[i];
+    for (int i = threadIdx.x; i < C; i += blockDim.x) idx[base + i] = g[s[i]];
+}
+
+// The stages of a bitonic network whose partners lie in one aligned chunk
+// of kBitonicChunk indices, run in LDS by one block per chunk: every stage
+// of the merges up to the chunk size (kk == 0), or the stages j < chunk of
+// the merge of size kk.  The chunk's moves (64 bytes each) take 128 KiB of
+// LDS.
+constexpr int kBitonicChunk = 2048;
+template <typename Cmp>
+__global__ void __launch_bounds__(1024) k_bitonic_chunk(int* idx, int kk_fixed, const JMove* m, Cmp cmp) {
+    __shared__ JMove mv[kBitonicChunk];
+    __shared__ int g[kBitonicChunk];
+    __shared__ short s[kBitonicChunk];
+    const int base = blockIdx.x * kBitonicChunk;
+    if (kk_fixed == 0)
+        bitonic_lds<kBitonicChunk>(idx, base, 2, kBitonicChunk, m, cmp, mv, g, s);
+    else
+        bitonic_lds<kBitonicChunk>(idx, base, kk_fixed, kk_fixed, m, cmp, mv, g, s);
+}
+
+// Whole sort of N <= 1024 indices (N a power of two) in one block.
+template <typename Cmp>
+__global__ void __launch_bounds__(1024) k_bitonic_local(int* idx, int N, const JMove* m, Cmp cmp) {
+    __shared__ JMove mv[1024];
+    __shared__ int g[1024];
+    __shared__ short s[1024];
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const int x = idx[i];
+        g[i] = x;
+        s[i] = (short)i;
+        if (x >= 0) mv[i] = m[x];
+    }
+    __syncthreads();
+    for (int kk = 2; kk <= N; kk <<= 1)
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int q = threadIdx.x; q < N / 2; q += blockDim.x) {
+                const int t = ((q & ~(j - 1)) << 1) | (q & (j - 1)), l = t + j;
+                const bool asc = (t & kk) == 0;
+                const int x = s[t], y = s[l];
+                if (asc ? slot_after(x, y, g, mv, cmp) : slot_after(y, x, g, mv, cmp)) {
+                    s[t] = (short)y;
+                    s[l] = (short)x;
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < N; i += blockDim.x) idx[i] = g[s[i]];
 }
 
 __global__ void k_nonempty_moves(const JMove* m, int count, std::uint8_t* f) {
@@ -1598,6 +1654,8 @@ __global__ void k_joint_combine(RefArgs a, int count, const JMove* archive, int 
     const int lane = lane_id();
     const int i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (i >= count) return;
+    // Member loops run to the constant bound 8 with a size test, so the
+    // members stay in registers.
     const JMove x = archive[i];
     const std::int64_t lo = (std::int64_t)a.loads[x.a] - (std::int64_t)a.cap;
     const std::int64_t hi = (std::int64_t)a.cap - (std::int64_t)a.loads[x.b];
@@ -1610,40 +1668,76 @@ __global__ void k_joint_combine(RefArgs a, int count, const JMove* archive, int 
     }
     const int begin = jnet_bound(archive, count, x.a, lo - x.net, false);
     const int end = jnet_bound(archive, count, x.a, hi - x.net, true);
-    std::int64_t best_gain = 0;
-    int best = -1, previous = -1;
-    for (int r = 0; r < 4 && begin < end; ++r) {
+    // Up to four candidates (quartiles of the feasible range), checked
+    // first; the member rows of x are then scanned once for all of them.
+    int cand[4] = {-1, -1, -1, -1};
+    std::int32_t cv[4][8];
+    int nc = 0, previous = -1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) cv[r][t] = -1;
+        if (begin >= end) continue;
         const int j = begin + (int)(((std::int64_t)(end - begin - 1) * r) / 3);
         if (j == previous) continue;
         previous = j;
         if (j == i) continue;
-        const JMove& y = archive[j];
+        const JMove y = archive[j];
         if (y.a != x.a || y.b != x.b || x.size + y.size > 8) continue;
         bool overlap = false;
-        for (int u = 0; u < x.size; ++u)
-            for (int t = 0; t < y.size; ++t) overlap |= x.v[u] == y.v[t];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int t = 0; t < 8; ++t) overlap |= u < x.size && t < y.size && x.v[u] == y.v[t];
         if (overlap) continue;
-        std::int64_t corr = 0;
-        for (int u = 0; u < x.size; ++u) {
+        cand[r] = j;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) cv[r][t] = t < y.size ? y.v[t] : -1;
+        ++nc;
+    }
+    std::int64_t corr[4] = {0, 0, 0, 0};
+    if (nc)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (u >= x.size) break;
             const std::int32_t v = x.v[u];
             const int lv = a.labels[v];
             for (std::int64_t e = a.off[v] + lane; e < a.off[v + 1]; e += kWave) {
                 const std::int32_t nb = a.nbr[e];
-                for (int t = 0; t < y.size; ++t)
-                    if (nb == y.v[t]) corr += (lv == a.labels[nb] ? 2 : -2) * (std::int64_t)ew_at(a.ew, e);
+                bool seen = false;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) seen |= nb == cv[c][t];
+                if (!seen) continue;
+                const std::int64_t w = (lv == a.labels[nb] ? 2 : -2) * (std::int64_t)ew_at(a.ew, e);
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int t = 0; t < 8; ++t)
+                        if (nb == cv[c][t]) corr[c] += w;
             }
         }
-        corr = wave_sum(corr);
-        const std::int64_t gain = x.gain + y.gain + corr;
-        if (gain > best_gain || (gain == best_gain && gain > 0 && j < best)) {
+    std::int64_t best_gain = 0;
+    int best = -1;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        if (cand[c] < 0) continue;
+        const std::int64_t gain = x.gain + archive[cand[c]].gain + wave_sum(corr[c]);
+        if (gain > best_gain || (gain == best_gain && gain > 0 && cand[c] < best)) {
             best_gain = gain;
-            best = j;
+            best = cand[c];
         }
     }
     if (lane == 0 && best >= 0) {
         JMove res = x;
         const JMove y = archive[best];
-        for (int j = 0; j < y.size; ++j) res.v[res.size++] = y.v[j];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if (t < y.size && q == x.size + t) res.v[q] = y.v[t];
+        res.size = x.size + y.size;
         res.gain = best_gain;
         res.net += y.net;
         res.id = id_base + count + i;
@@ -1707,31 +1801,59 @@ __global__ void k_joint_events(RefArgs a, const JMove* m, int count, const std::
     own = wave_sum(own);
     if (lane == 0 && own) atomicAdd(&events[i], own);
 }
-// joint_prefix / joint_accept (exchange.cuh:263-289).
+// joint_prefix / joint_accept (exchange.cuh:263-289).  One wave per part
+// pa over its moves (the list is sorted by part): gain and net prefixes by
+// wave scans, 64 moves per step.  The serial rule (walk the moves, keep the
+// move where a feasible prefix gain first exceeds the best so far, starting
+// from 0) keeps the first move of the largest feasible positive prefix
+// gain, which is what the per-step maximum and first-lane ballot take.
 __global__ void k_joint_prefix(RefArgs a, const JMove* m, int count, const unsigned long long* events,
                                std::int32_t* stops) {
-    const int pa = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = lane_id();
+    const int pa = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (pa >= a.k) return;
-    std::int64_t gain = 0, net = 0, best = 0;
-    int stop = -1;
     int low = 0, high = count;
     while (low < high) {
         const int mid = (low + high) / 2;
         if (m[mid].a < pa) low = mid + 1;
         else high = mid;
     }
-    for (int i = low; i < count && m[i].a == pa; ++i) {
-        if (!m[i].size) continue;
-        gain += (std::int64_t)events[i];
-        net += m[i].net;
-        const std::int64_t lo = (std::int64_t)a.loads[pa] - (std::int64_t)a.cap;
-        const std::int64_t hi = (std::int64_t)a.cap - (std::int64_t)a.loads[m[i].b];
-        if (!jgap(net, lo, hi) && gain > best) {
-            best = gain;
-            stop = i;
-        }
+    int end = low;
+    high = count;
+    while (end < high) {
+        const int mid = (end + high) / 2;
+        if (m[mid].a <= pa) end = mid + 1;
+        else high = mid;
     }
-    stops[pa] = stop;
+    const std::int64_t lo = (std::int64_t)a.loads[pa] - (std::int64_t)a.cap;
+    std::int64_t gain = 0, net = 0, best = 0;
+    int stop = -1;
+    for (int base = low; base < end; base += kWave) {
+        const int i = base + lane;
+        std::int64_t g = 0, d = 0;
+        int b = 0;
+        bool live = false;
+        if (i < end) {
+            live = m[i].size != 0;
+            if (live) {
+                g = (std::int64_t)events[i];
+                d = m[i].net;
+                b = m[i].b;
+            }
+        }
+        const std::int64_t pg = gain + wave_inclusive_scan<PlusOp>(g);
+        const std::int64_t pn = net + wave_inclusive_scan<PlusOp>(d);
+        std::int64_t cand = LLONG_MIN;
+        if (live && !jgap(pn, lo, (std::int64_t)a.cap - (std::int64_t)a.loads[b])) cand = pg;
+        const std::int64_t top = wave_max(cand);
+        if (top > best) {
+            best = top;
+            stop = base + __ffsll((unsigned long long)__ballot(cand == top)) - 1;
+        }
+        gain = __shfl(pg, kWave - 1);
+        net = __shfl(pn, kWave - 1);
+    }
+    if (lane == 0) stops[pa] = stop;
 }
 __global__ void k_joint_accept(RefArgs a, const JMove* m, int count, const std::int32_t* stops) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2197,7 +2319,7 @@ public:
         GP_LAUNCH_CHECK();
         int empty = 0;
         for (int pass = 0; pass < passes; ++pass) {
-            hipLaunchKernelGGL(k_joint_match, dim3(1), dim3(kMaxParts), 0, s, k, Q.get(), partner.get(), visits.get());
+            hipLaunchKernelGGL(k_joint_match, dim3(1), dim3(kMatchThreads), 0, s, k, Q.get(), partner.get(), visits.get());
             GP_LAUNCH_CHECK();
             fill<unsigned long long>(c_, seeds.get(), (std::int64_t)k * sw, ~0ULL);
             hipLaunchKernelGGL(k_joint_seeds, dim3(grid_for(n)), dim3(256), 0, s, a_, A.get(), partner.get(),
@@ -2237,7 +2359,7 @@ public:
             fill<unsigned long long>(c_, events.get(), active, 0ULL);
             hipLaunchKernelGGL(k_joint_events, dim3(ceil_div(active, 4)), dim3(256), 0, s, a_, moves.get(), active,
                                owners.get(), events.get());
-            hipLaunchKernelGGL(k_joint_prefix, dim3(ceil_div(k, 128)), dim3(128), 0, s, a_, moves.get(), active,
+            hipLaunchKernelGGL(k_joint_prefix, dim3(ceil_div(k, 4)), dim3(256), 0, s, a_, moves.get(), active,
                                events.get(), stops.get());
             set_lane(c_, kStateFlag, 0);
             hipLaunchKernelGGL(k_joint_accept, dim3(ceil_div(active, 256)), dim3(256), 0, s, a_, moves.get(), active,

@@ -34,7 +34,11 @@ enum Ctr {
     kDonors = 6,
     kActive = 7,
     kRowSum = 8,  // entries of the rows scanned by the last LP proposal sweep
-    kCtrCount = 9
+    // Tier list counters of the second and third sweep of a round, so that
+    // one clear of the whole block at the round start serves all three.
+    kTierB = 9,
+    kTierC = 11,
+    kCtrCount = 13
 };
 
 // Part ids inside refinement: k <= 128 fits a byte, which keeps the label
@@ -612,34 +616,40 @@ __device__ inline RepairPick team_reduce(const BlockTeam& t, RepairPick c) {
 
 // enumerate_repair_destination (rebalance_select.cuh:8-75) on frozen loads,
 // then the gain bucket (incremental.cuh:559-626).
+// load0: the frozen part loads (the block's LDS copy where it has one).
 __device__ inline void repair_finish(const RefArgs& a, std::int64_t v, int s, std::uint64_t w, std::uint64_t current,
-                                     RepairPick best) {
+                                     RepairPick best, const unsigned long long* load0) {
     if (best.p < 0) {
         // Slack-weighted hash routing in part order.
         std::uint64_t tot = 0;
-        for (int p = 0; p < a.k; ++p)
-            if (p != s && a.loads0[p] < a.cap && w <= a.cap - a.loads0[p]) {
-                const std::uint64_t sl = a.cap - a.loads0[p];
+        for (int p = 0; p < a.k; ++p) {
+            const std::uint64_t l = load0[p];
+            if (p != s && l < a.cap && w <= a.cap - l) {
+                const std::uint64_t sl = a.cap - l;
                 tot = ~0ULL - tot < sl ? ~0ULL : tot + sl;
             }
+        }
         best.conn = 0;
         if (tot) {
-            std::uint64_t ticket = (std::uint64_t)mix32((std::uint32_t)v) % tot;
-            for (int p = 0; p < a.k; ++p)
-                if (p != s && a.loads0[p] < a.cap && w <= a.cap - a.loads0[p]) {
-                    const std::uint64_t sl = a.cap - a.loads0[p];
+            const std::uint32_t h = mix32((std::uint32_t)v);
+            std::uint64_t ticket = tot <= 0xffffffffULL ? (std::uint64_t)(h % (std::uint32_t)tot) : (std::uint64_t)h % tot;
+            for (int p = 0; p < a.k; ++p) {
+                const std::uint64_t l = load0[p];
+                if (p != s && l < a.cap && w <= a.cap - l) {
+                    const std::uint64_t sl = a.cap - l;
                     if (ticket < sl) {
                         best.p = p;
                         break;
                     }
                     ticket -= sl;
                 }
+            }
         }
     }
     a.targets[v] = best.p;
     if (best.p < 0) return;
     const double gain = best.conn >= current ? (double)(best.conn - current) : -(double)(current - best.conn);
-    const std::uint64_t excess = a.loads0[s] - a.cap;
+    const std::uint64_t excess = load0[s] - a.cap;
     const std::uint64_t relief = w < excess ? w : excess;
     const double gpw = gain / (double)relief;
     const int b = s * kBalanceWidth + gain_bucket(a, gpw) * kBalanceSections +
@@ -688,7 +698,7 @@ struct RepairProposeBody : RowBody {
                 for (int p = t.rank(); p < a.k; p += Team::size) consider(p, hist[p]);
             best = team_reduce(t, best);
         }
-        if (t.leader()) repair_finish(a, v, s, w, current, best);
+        if (t.leader()) repair_finish(a, v, s, w, current, best, load0);
         return RowOut{false, 0};
     }
 };
@@ -2064,11 +2074,14 @@ public:
     // than 2^32 (the level's incident-weight bound).
     bool narrow_bins() const { return g_.incident_bound <= 0xffffffffULL; }
 
+    // slot: the two tier list counters; zeroed: they are known to be zero
+    // (cleared with the counter block since their last use).
     template <class Body>
     void tiers(const Body& body, const std::int32_t* list, std::int64_t n, const unsigned long long* count,
-               std::int32_t* out, unsigned long long* out_count, unsigned long long* acc, int scratch) {
-        run_tiers(c_, body, list, n, count, medium_.get(), heavy_.get(), ctr_.get() + kTier, out, out_count, acc,
-                  scratch, a_.n, g_.max_degree > static_cast<std::uint64_t>(kTierMedium));
+               std::int32_t* out, unsigned long long* out_count, unsigned long long* acc, int scratch,
+               int slot = kTier, bool zeroed = false) {
+        run_tiers(c_, body, list, n, count, medium_.get(), heavy_.get(), ctr_.get() + slot, out, out_count, acc,
+                  scratch, a_.n, g_.max_degree > static_cast<std::uint64_t>(kTierMedium), zeroed);
     }
 
     // One LP round over the frontier; returns the number of movers and the cut delta.
@@ -2086,21 +2099,24 @@ public:
         c_.timed("lp_propose", 0.0, [&] {
             if (narrow_bins())
                 tiers(LpProposeBody<unsigned>{{a_}}, frontier_.get(), 0, ctr_.get() + kFrontier, cand_.get(),
-                      ctr_.get() + kCand, ctr_.get() + kRowSum, a_.k);
+                      ctr_.get() + kCand, ctr_.get() + kRowSum, a_.k, kTier, true);
             else
                 tiers(LpProposeBody<unsigned long long>{{a_}}, frontier_.get(), 0, ctr_.get() + kFrontier, cand_.get(),
-                      ctr_.get() + kCand, ctr_.get() + kRowSum, a_.k);
+                      ctr_.get() + kCand, ctr_.get() + kRowSum, a_.k, kTier, true);
         });
-        tiers(AfterBody{{a_}}, cand_.get(), 0, ctr_.get() + kCand, movers_.get(), ctr_.get() + kMovers, nullptr, 0);
-        return apply(true, false, delta);
+        tiers(AfterBody{{a_}}, cand_.get(), 0, ctr_.get() + kCand, movers_.get(), ctr_.get() + kMovers, nullptr, 0,
+              kTierB, true);
+        return apply(true, false, delta, true);
     }
 
-    // Update + commit of the current mover list.
-    std::int64_t apply(bool lp, bool samples, std::int64_t& delta) {
+    // Update + commit of the current mover list.  zeroed: the delta and
+    // third-sweep tier counters are still zero from the counter block's
+    // clear at the start of the round or repair iteration.
+    std::int64_t apply(bool lp, bool samples, std::int64_t& delta, bool zeroed = false) {
         const hipStream_t s = c_.stream();
-        fill<unsigned long long>(c_, ctr_.get() + kDelta, 1, 0ULL);
+        if (!zeroed) fill<unsigned long long>(c_, ctr_.get() + kDelta, 1, 0ULL);
         tiers(UpdateBody{{a_}, samples}, movers_.get(), 0, ctr_.get() + kMovers, nullptr, nullptr,
-              ctr_.get() + kDelta, 0);
+              ctr_.get() + kDelta, 0, kTierC, zeroed);
         hipLaunchKernelGGL(k_commit, dim3(grid_for(a_.n, 256, 1024)), dim3(256), 0, s, a_, movers_.get(),
                            ctr_.get() + kMovers, lp);
         GP_LAUNCH_CHECK();
@@ -2263,10 +2279,10 @@ public:
             GP_LAUNCH_CHECK();
             if (narrow_bins())
                 tiers(RepairProposeBody<unsigned>{{a_}}, cand_.get(), 0, ctr_.get() + kDonors, nullptr, nullptr, nullptr,
-                      a_.k);
+                      a_.k, kTier, true);
             else
                 tiers(RepairProposeBody<unsigned long long>{{a_}}, cand_.get(), 0, ctr_.get() + kDonors, nullptr,
-                      nullptr, nullptr, a_.k);
+                      nullptr, nullptr, a_.k, kTier, true);
             hipLaunchKernelGGL(k_bucket_scan, dim3(k), dim3(kScanThreads), 0, s, a_, prefix.get(), bstar.get(),
                                excess.get());
             GP_LAUNCH_CHECK();
@@ -2279,7 +2295,7 @@ public:
                                ctr_.get() + kDonors);
             GP_LAUNCH_CHECK();
             std::int64_t delta = 0;
-            count = apply(false, true, delta);
+            count = apply(false, true, delta, true);
             ++iterations;
             if (count) {
                 update_cut(cut, delta);

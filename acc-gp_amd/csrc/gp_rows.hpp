@@ -309,7 +309,8 @@ __global__ void __launch_bounds__(kHeavyThreads) k_tier_heavy(Body body, TierLis
 }
 
 // Host launcher: three launches, counters in dev[0] (medium) and dev[1]
-// (heavy) are reset here.  scratch is per-team LDS in u64 units.
+// (heavy) are reset here unless the caller knows them to be zero
+// (counts_zeroed).  scratch is per-team LDS in u64 units.
 // heavy_rows: the rows may include some longer than kTierMedium (a bound
 // the caller knows); when false every longer row runs in the medium launch
 // (correct for any length) and no heavy launch is made.
@@ -317,9 +318,9 @@ template <class Body>
 void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int64_t n, const unsigned long long* count_ptr,
                std::int32_t* medium, std::int32_t* heavy, unsigned long long* tier_counts, std::int32_t* out,
                unsigned long long* out_count, unsigned long long* acc, int scratch, std::int64_t max_rows,
-               bool heavy_rows = true) {
+               bool heavy_rows = true, bool counts_zeroed = false) {
     const hipStream_t s = c.stream();
-    fill<unsigned long long>(c, tier_counts, 2, 0ULL);
+    if (!counts_zeroed) fill<unsigned long long>(c, tier_counts, 2, 0ULL);
     TierLists t{medium, tier_counts, heavy, tier_counts + 1, out, out_count, acc};
     const int lscratch = light_scratch_words<Body>(scratch);
     int lanes = c.light_lanes();

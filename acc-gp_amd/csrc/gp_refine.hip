@@ -1216,10 +1216,42 @@ __global__ void k_gather_info(const DonorInfo* src, const std::int32_t* idx, std
         dst[j] = src[idx[j]];
 }
 
-__global__ void k_slack(RefArgs a, unsigned long long* slack, unsigned long long* requests) {
-    for (int p = threadIdx.x; p < a.k; p += blockDim.x) {
-        slack[p] = a.loads0[p] < a.cap ? a.cap - a.loads0[p] : 0;
-        requests[p] = 0;
+// Reservation set-up in one launch: clears the per-donor done flags and
+// tried masks, and sets each part's slack and request total.
+__global__ void k_reserve_begin(RefArgs a, std::int64_t ne, std::uint8_t* done, unsigned long long* tried,
+                                unsigned long long* slack, unsigned long long* requests) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < 2 * ne;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        tried[i] = 0;
+        if (i < ne) done[i] = 0;
+    }
+    if (blockIdx.x == 0)
+        for (int p = threadIdx.x; p < a.k; p += blockDim.x) {
+            slack[p] = a.loads0[p] < a.cap ? a.cap - a.loads0[p] : 0;
+            requests[p] = 0;
+        }
+}
+
+// Start of a repair iteration in one launch: freezes the part loads
+// (loads0 = loads), clears the bucket weights and the counter block.
+__global__ void k_repair_begin(RefArgs a, unsigned long long* loads0, unsigned long long* counts, std::int64_t ncounts,
+                               unsigned long long* ctr, int nctr) {
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < ncounts;
+         i += (std::int64_t)gridDim.x * blockDim.x) {
+        counts[i] = 0;
+        if (i < a.k) loads0[i] = a.loads[i];
+        if (i < nctr) ctr[i] = 0;
+    }
+}
+
+// New repair prices from the samples of the last repair (parts without a
+// sample keep their price): loss / weight for a positive loss, else 0.
+__global__ void k_publish_prices(const RepairSample* samples, double* prices, int k) {
+    for (int p = threadIdx.x; p < k; p += blockDim.x) {
+        const RepairSample x = samples[p];
+        if (!x.weight) continue;
+        const long long loss = (long long)x.loss;
+        prices[p] = loss > 0 ? (double)loss / (double)x.weight : 0.0;
     }
 }
 
@@ -2172,9 +2204,8 @@ public:
         DBuf<unsigned long long> slack = c_.buf<unsigned long long>(static_cast<std::size_t>(k));
         DBuf<std::uint8_t> done = c_.buf<std::uint8_t>(static_cast<std::size_t>(ne));
         DBuf<unsigned long long> tried = c_.buf<unsigned long long>(static_cast<std::size_t>(ne) * 2);
-        fill<std::uint8_t>(c_, done.get(), ne, 0);
-        fill<unsigned long long>(c_, tried.get(), 2 * ne, 0ULL);
-        hipLaunchKernelGGL(k_slack, dim3(1), dim3(kMaxParts), 0, s, a_, slack.get(), requests.get());
+        hipLaunchKernelGGL(k_reserve_begin, dim3(grid_for(2 * ne)), dim3(256), 0, s, a_, ne, done.get(), tried.get(),
+                           slack.get(), requests.get());
         GP_LAUNCH_CHECK();
         DBuf<DonorInfo> live = c_.buf<DonorInfo>(static_cast<std::size_t>(ne));
         hipLaunchKernelGGL(k_donor_info, dim3(grid_for(ne)), dim3(256), 0, s, a_, E.get(), ne, live.get());
@@ -2271,9 +2302,10 @@ public:
         DBuf<std::int32_t> list = c_.buf<std::int32_t>(n);
         DBuf<std::int64_t> lcount = c_.buf<std::int64_t>(1);
         do {
-            c_.copy_d2d(loads0_.get(), a_.loads, static_cast<std::size_t>(k) * sizeof(unsigned long long));
-            fill<unsigned long long>(c_, counts_.get(), (std::int64_t)k * kBalanceWidth, 0ULL);
-            fill<unsigned long long>(c_, ctr_.get(), kCtrCount, 0ULL);
+            static_assert(kCtrCount <= kBalanceWidth, "the counter block clears inside the bucket clear");
+            hipLaunchKernelGGL(k_repair_begin, dim3(grid_for((std::int64_t)k * kBalanceWidth)), dim3(256), 0, s, a_,
+                               loads0_.get(), counts_.get(), (std::int64_t)k * kBalanceWidth, ctr_.get(), (int)kCtrCount);
+            GP_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_repair_donors, dim3(grid_for(n, 256 * kScanRounds, 2048)), dim3(256), 0, s, a_, cand_.get(),
                                ctr_.get() + kDonors);
             GP_LAUNCH_CHECK();
@@ -2461,17 +2493,9 @@ public:
 
     void publish_prices() {
         if (!prices_) return;
-        std::vector<RepairSample> sm(static_cast<std::size_t>(a_.k));
-        c_.read(sm.data(), samples_.get(), sm.size() * sizeof(RepairSample));
-        std::vector<double> pr(static_cast<std::size_t>(a_.k));
-        c_.read(pr.data(), prices_.get(), pr.size() * sizeof(double));
-        for (int p = 0; p < a_.k; ++p) {
-            const RepairSample& x = sm[static_cast<std::size_t>(p)];
-            if (!x.weight) continue;
-            const long long loss = (long long)x.loss;
-            pr[static_cast<std::size_t>(p)] = loss > 0 ? (double)loss / (double)x.weight : 0.0;
-        }
-        c_.upload(prices_.get(), pr.data(), pr.size() * sizeof(double));
+        hipLaunchKernelGGL(k_publish_prices, dim3(1), dim3(kMaxParts), 0, c_.stream(), samples_.get(), prices_.get(),
+                           a_.k);
+        GP_LAUNCH_CHECK();
     }
 
     void reset_samples() {

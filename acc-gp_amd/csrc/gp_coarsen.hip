@@ -1458,6 +1458,7 @@ void compute_level0_metadata(Context& c, DGraph& g, CoarsenState& st) {
     st.positive_ew = (h.flags & 1ULL) == 0;
     st.unit_ew = (h.flags & 2ULL) == 0;
     g.max_degree = h.max_deg;
+    g.row_len_bound = h.max_deg;
     g.has_heavy = h.max_deg >= 4096;
     st.total_weight = h.total_vw;
     st.max_vertex_weight = h.max_vw;
@@ -1880,7 +1881,7 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
         const ScatterBody body{fine.off.get(), fine.nbr.get(), fine.ew_ptr(), map, cursor.get(), rkey.get(), rval.get()};
         c.timed("contract_scatter", bytes,
                 [&] { run_tiers(c, body, nullptr, n, nullptr, tier_a.get(), tier_b.get(), tier_n.get(), nullptr, nullptr,
-                                nullptr, 0, n, fine.max_degree > static_cast<std::uint64_t>(kTierMedium)); });
+                                nullptr, 0, n, fine.row_len_bound); });
     }
     cursor.reset();
     // Row classes by raw length.
@@ -1973,7 +1974,7 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
                         out->ew.get(), nc};
     // Unique coarse rows are no longer than the raw ones (max rb[1]).
     run_tiers(c, copy, nullptr, nc, nullptr, tier_a.get(), tier_b.get(), tier_n.get(), nullptr, nullptr, nullptr, 0, nc,
-              rb[1] > static_cast<unsigned long long>(kTierMedium));
+              rb[1]);
     out->vw = std::move(coarse_vw);
     out->incident_bound = crb;
     if (st.positive_ew) {
@@ -1983,6 +1984,9 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
         out->max_degree = ~0ULL;
         out->has_heavy = true;
     }
+    // Both bound the row length: the longest raw row, and (positive edge
+    // weights, each unique entry weighing at least 1) the degree bound.
+    out->row_len_bound = std::min<std::uint64_t>(rb[1], out->max_degree);
     st.max_row_weight = crb;
     st.row_bound = c.buf<std::uint64_t>(static_cast<std::size_t>(nc));
     c.copy_d2d(st.row_bound.get(), next_rb.get(), static_cast<std::size_t>(nc) * sizeof(std::uint64_t));

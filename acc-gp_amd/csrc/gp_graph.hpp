@@ -19,6 +19,9 @@ struct DGraph {
     std::uint64_t max_degree = std::numeric_limits<std::uint64_t>::max();
     bool has_heavy = true;
     std::uint64_t incident_bound = std::numeric_limits<std::uint64_t>::max();
+    // Upper bound on the entries of any row (exact at the finest level,
+    // the longest raw row for coarse levels); picks the row tiers to launch.
+    std::uint64_t row_len_bound = std::numeric_limits<std::uint64_t>::max();
 
     const std::uint32_t* ew_ptr() const { return ew ? ew.get() : nullptr; }
     bool unit_edges() const { return !ew; }

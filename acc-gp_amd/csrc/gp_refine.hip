@@ -2113,7 +2113,7 @@ public:
                std::int32_t* out, unsigned long long* out_count, unsigned long long* acc, int scratch,
                int slot = kTier, bool zeroed = false) {
         run_tiers(c_, body, list, n, count, medium_.get(), heavy_.get(), ctr_.get() + slot, out, out_count, acc,
-                  scratch, a_.n, g_.max_degree > static_cast<std::uint64_t>(kTierMedium), zeroed);
+                  scratch, a_.n, g_.row_len_bound, zeroed);
     }
 
     // One LP round over the frontier; returns the number of movers and the cut delta.

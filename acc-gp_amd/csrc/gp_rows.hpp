@@ -311,15 +311,18 @@ __global__ void __launch_bounds__(kHeavyThreads) k_tier_heavy(Body body, TierLis
 // Host launcher: three launches, counters in dev[0] (medium) and dev[1]
 // (heavy) are reset here unless the caller knows them to be zero
 // (counts_zeroed).  scratch is per-team LDS in u64 units.
-// heavy_rows: the rows may include some longer than kTierMedium (a bound
-// the caller knows); when false every longer row runs in the medium launch
-// (correct for any length) and no heavy launch is made.
+// row_bound: an upper bound on the entries of the listed rows.  With no
+// row above kTierLight no medium launch is made; with none above
+// kTierMedium no heavy launch is made (longer rows would run in the medium
+// launch, which is correct for any length).
 template <class Body>
 void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int64_t n, const unsigned long long* count_ptr,
                std::int32_t* medium, std::int32_t* heavy, unsigned long long* tier_counts, std::int32_t* out,
                unsigned long long* out_count, unsigned long long* acc, int scratch, std::int64_t max_rows,
-               bool heavy_rows = true, bool counts_zeroed = false) {
+               std::uint64_t row_bound = ~0ULL, bool counts_zeroed = false) {
     const hipStream_t s = c.stream();
+    const bool heavy_rows = row_bound > static_cast<std::uint64_t>(kTierMedium);
+    const bool medium_rows = row_bound > static_cast<std::uint64_t>(kTierLight);
     if (!counts_zeroed) fill<unsigned long long>(c, tier_counts, 2, 0ULL);
     TierLists t{medium, tier_counts, heavy, tier_counts + 1, out, out_count, acc};
     const int lscratch = light_scratch_words<Body>(scratch);
@@ -351,17 +354,19 @@ void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int6
         hipLaunchKernelGGL((k_tier_heavy<Body>), dim3(kHeavyGrid), dim3(kHeavyThreads), heavy_lds, c.aux(), body, t);
         GP_LAUNCH_CHECK();
     }
-    const int mlanes = c.medium_lanes();
-    const int medium_blocks = static_cast<int>(
-        std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(max_rows, kTierThreads / mlanes), 2048)));
-    const std::size_t medium_lds = part + (kTierThreads / mlanes) * scratch * sizeof(unsigned long long);
-    if (mlanes == 32)
-        hipLaunchKernelGGL((k_tier_medium<Body, 32>), dim3(medium_blocks), dim3(kTierThreads), medium_lds, s, body, t,
-                           scratch);
-    else
-        hipLaunchKernelGGL((k_tier_medium<Body, 64>), dim3(medium_blocks), dim3(kTierThreads), medium_lds, s, body, t,
-                           scratch);
-    GP_LAUNCH_CHECK();
+    if (medium_rows) {
+        const int mlanes = c.medium_lanes();
+        const int medium_blocks = static_cast<int>(
+            std::max<std::int64_t>(1, std::min<std::int64_t>(ceil_div(max_rows, kTierThreads / mlanes), 2048)));
+        const std::size_t medium_lds = part + (kTierThreads / mlanes) * scratch * sizeof(unsigned long long);
+        if (mlanes == 32)
+            hipLaunchKernelGGL((k_tier_medium<Body, 32>), dim3(medium_blocks), dim3(kTierThreads), medium_lds, s, body,
+                               t, scratch);
+        else
+            hipLaunchKernelGGL((k_tier_medium<Body, 64>), dim3(medium_blocks), dim3(kTierThreads), medium_lds, s, body,
+                               t, scratch);
+        GP_LAUNCH_CHECK();
+    }
     // Heavy rows are few but long; blocks beyond the row count exit at once.
     if (side) {
         c.join();

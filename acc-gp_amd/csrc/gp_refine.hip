@@ -1245,10 +1245,12 @@ __global__ void k_repair_begin(RefArgs a, unsigned long long* loads0, unsigned l
 }
 
 // New repair prices from the samples of the last repair (parts without a
-// sample keep their price): loss / weight for a positive loss, else 0.
-__global__ void k_publish_prices(const RepairSample* samples, double* prices, int k) {
+// sample keep their price): loss / weight for a positive loss, else 0.  The
+// samples are cleared for the next repair.
+__global__ void k_publish_prices(RepairSample* samples, double* prices, int k) {
     for (int p = threadIdx.x; p < k; p += blockDim.x) {
         const RepairSample x = samples[p];
+        samples[p] = RepairSample{0, 0};
         if (!x.weight) continue;
         const long long loss = (long long)x.loss;
         prices[p] = loss > 0 ? (double)loss / (double)x.weight : 0.0;
@@ -2120,7 +2122,8 @@ public:
     std::int64_t lp_round(std::int64_t& delta) {
         const hipStream_t s = c_.stream();
         const std::int64_t n = a_.n;
-        fill<unsigned long long>(c_, ctr_.get(), kCtrCount, 0ULL);
+        if (!ctr_clear_) fill<unsigned long long>(c_, ctr_.get(), kCtrCount, 0ULL);
+        ctr_clear_ = false;
         hipLaunchKernelGGL(k_compact_flags<StateLane<std::uint8_t>>, dim3(grid_for(n, 256 * kScanRounds, 2048)), dim3(256), 0, s, n,
                            a_.member,
                            frontier_.get(), ctr_.get() + kFrontier);
@@ -2152,8 +2155,11 @@ public:
         hipLaunchKernelGGL(k_commit, dim3(grid_for(a_.n, 256, 1024)), dim3(256), 0, s, a_, movers_.get(),
                            ctr_.get() + kMovers, lp);
         GP_LAUNCH_CHECK();
+        if (samples) samples_clear_ = false;
         std::vector<unsigned long long> h(static_cast<std::size_t>(kCtrCount + a_.k));
-        c_.read(h.data(), ctr_.get(), h.size() * sizeof(unsigned long long));
+        // The read also clears the counters (not the part loads behind them).
+        c_.read_clear(h.data(), ctr_.get(), h.size(), kCtrCount);
+        ctr_clear_ = true;
         for (int p = 0; p < a_.k; ++p) loads_host_[static_cast<std::size_t>(p)] = h[static_cast<std::size_t>(kCtrCount + p)];
         delta = (std::int64_t)h[kDelta];
         if (lp && c_.profiling())
@@ -2290,6 +2296,7 @@ public:
     // Capacity repair loop (rebalance_graph.cuh:160-280); returns feasibility.
     bool rebalance(std::uint64_t& cut) {
         const hipStream_t s = c_.stream();
+        ctr_clear_ = false;
         const std::int64_t n = a_.n;
         const int k = a_.k;
         int iterations = 0;
@@ -2342,6 +2349,7 @@ public:
     // Capacity-complementary exchange (incremental.cuh:974-1041).
     void exchange(int passes, std::uint64_t& cut, std::uint64_t& best_cut) {
         const hipStream_t s = c_.stream();
+        ctr_clear_ = false;
         const std::int64_t n = a_.n;
         const int k = a_.k;
         int sw = std::min(kJointSeeds, std::max(32, 4096 / k));
@@ -2432,6 +2440,8 @@ public:
             best_cut = cut;
             dense();
         }
+        // Later passes wrote the mover counter after the last read.
+        ctr_clear_ = false;
     }
 
     // Stable compaction of non-empty moves, sort by cmp, optional unique.
@@ -2496,10 +2506,13 @@ public:
         hipLaunchKernelGGL(k_publish_prices, dim3(1), dim3(kMaxParts), 0, c_.stream(), samples_.get(), prices_.get(),
                            a_.k);
         GP_LAUNCH_CHECK();
+        samples_clear_ = true;
     }
 
     void reset_samples() {
-        if (samples_) fill<unsigned long long>(c_, reinterpret_cast<unsigned long long*>(samples_.get()), 2 * a_.k, 0ULL);
+        if (samples_ && !samples_clear_)
+            fill<unsigned long long>(c_, reinterpret_cast<unsigned long long*>(samples_.get()), 2 * a_.k, 0ULL);
+        samples_clear_ = true;
     }
 
     Context& c_;
@@ -2514,6 +2527,10 @@ public:
     DBuf<unsigned long long> ctr_, loads0_, counts_;
     DBuf<double> prices_;
     DBuf<RepairSample> samples_;
+    // Host knowledge that the counter block (first kCtrCount words) or the
+    // repair samples are zero on the device, so their clear can be skipped.
+    bool ctr_clear_ = false;
+    bool samples_clear_ = false;
 };
 
 }  // namespace

@@ -154,6 +154,16 @@ __global__ void __launch_bounds__(256) k_copy_out(const T* src, T* dst, std::siz
     for (std::size_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
+// The same for counter words, zeroing the first nclear of them on the
+// device after they are copied (each word is read and cleared by one thread).
+template <typename T>
+__global__ void __launch_bounds__(256) k_copy_clear(T* src, T* dst, std::size_t n, std::size_t nclear) {
+    for (std::size_t i = threadIdx.x; i < n; i += blockDim.x) {
+        dst[i] = src[i];
+        if (i < nclear) src[i] = 0;
+    }
+}
+
 // Accumulated device time and algorithmic bytes per kernel class, measured
 // with HIP events recorded on the context stream around each launch.
 struct ProfileEntry {
@@ -272,6 +282,18 @@ public:
         GP_HIP(hipGetLastError());
         sync();
         std::memcpy(host, pinned_, bytes);
+    }
+    // read() of counter words that also zeroes the first nclear words on the
+    // device, so the next user of the counters needs no clearing launch.
+    void read_clear(unsigned long long* host, unsigned long long* dev, std::size_t words, std::size_t nclear) {
+        if (!words) return;
+        if (words * sizeof(unsigned long long) > kPinnedBytes || nclear > words)
+            throw std::invalid_argument("read_clear: region exceeds the read-back buffer");
+        hipLaunchKernelGGL(k_copy_clear<unsigned long long>, dim3(1), dim3(256), 0, stream_, dev,
+                           static_cast<unsigned long long*>(pinned_), words, nclear);
+        GP_HIP(hipGetLastError());
+        sync();
+        std::memcpy(host, pinned_, words * sizeof(unsigned long long));
     }
     template <typename T>
     T read_scalar(const T* dev) {

@@ -510,6 +510,7 @@ __global__ void __launch_bounds__(Threads) k_affinity_block_hash(AffinityArgs a,
 // larger table (one block fills a CU's LDS, so it brings 16 waves);
 // longer rows go through records and the global sort.
 constexpr int kBigRowDegree = 12288;
+constexpr int kBlockHashThreads = 512;
 using BigHashSmem = BlockHashT<16384>;
 
 // Splits the hub list by row length (order inside each part is arbitrary:
@@ -1593,9 +1594,11 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
         };
         auto block = [&](const std::int32_t* list, std::int64_t count) {
             if (!count) return;
+            // The 64 KiB table allows two blocks per CU, so each block
+            // brings eight waves.
             if (st.max_row_weight <= 0xffffffffULL)
-                hipLaunchKernelGGL((k_affinity_block_hash<BlockHashSmem, kBlockDegree, kBlockSortThreads>),
-                                   dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, s, a, list, count);
+                hipLaunchKernelGGL((k_affinity_block_hash<BlockHashSmem, kBlockDegree, kBlockHashThreads>),
+                                   dim3(grid_for(count, 1, 4096)), dim3(kBlockHashThreads), 0, s, a, list, count);
             else
                 hipLaunchKernelGGL(k_affinity_block, dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, s, a,
                                    list, count);

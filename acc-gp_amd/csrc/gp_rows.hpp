@@ -285,7 +285,11 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLis
     }
 }
 
-constexpr int kHeavyGrid = 512;  // blocks beyond the heavy row count exit at once
+// More blocks than stay resident: with about one row per block the
+// dispatcher hands the next row to whichever block slot frees first, which
+// balances rows of very different lengths.  Blocks beyond the heavy row
+// count exit at once.
+constexpr int kHeavyGrid = 2048;
 
 template <class Body>
 __global__ void __launch_bounds__(kHeavyThreads) k_tier_heavy(Body body, TierLists t) {

@@ -383,7 +383,7 @@ def _stats_dict(req):
     return dict((f, getattr(req.stats, f)) for f, _ in PartitionStats._fields_)
 
 
-def partition_batch(items, workers=4, hbm_budget=0, device=0):
+def partition_batch(items, workers=6, hbm_budget=0, device=0):
     """Partitions BatchItems concurrently; returns per-item stats dicts."""
     reqs = (Request * len(items))()
     for it, r in zip(items, reqs):
@@ -399,7 +399,7 @@ def partition_batch(items, workers=4, hbm_budget=0, device=0):
 class Server:
     """Continuous-batching request loop (gp_server_*)."""
 
-    def __init__(self, workers=4, hbm_budget=0, device=0):
+    def __init__(self, workers=6, hbm_budget=0, device=0):
         self.h = ctypes.c_void_p()
         check(lib().gp_server_create(device, workers, hbm_budget, ctypes.byref(self.h)))
         self._live = {}

@@ -32,7 +32,7 @@ def main():
     ap.add_argument("--cpu-scale", type=int, default=19, help="R-MAT scale of the bounded CPU-oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-graphs", type=int, default=256, help="graphs in the batched mix (0: skip)")
-    ap.add_argument("--batch-workers", type=int, default=8)
+    ap.add_argument("--batch-workers", type=int, default=6)
     ap.add_argument("--serve-requests", type=int, default=128, help="requests of the serving leg (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 1 GiB device-to-device copies (counter calibration) and exit")

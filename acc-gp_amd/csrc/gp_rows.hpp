@@ -17,7 +17,7 @@ namespace gp {
 constexpr int kTierLight = 64;
 constexpr int kTierMedium = 4096;
 constexpr int kTierThreads = 256;
-constexpr int kHeavyThreads = 1024;  // one block per heavy row: many entries in flight
+constexpr int kHeavyThreads = 512;  // one block per heavy row: many entries in flight
 constexpr int kBlockRed = 3 * (kHeavyThreads / kWave);  // u64 words of block-team reduction scratch
 
 // Appends v for every lane with pred set; all lanes of the wave must call it.
@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLis
 // dispatcher hands the next row to whichever block slot frees first, which
 // balances rows of very different lengths.  Blocks beyond the heavy row
 // count exit at once.
-constexpr int kHeavyGrid = 2048;
+constexpr int kHeavyGrid = 4096;
 
 template <class Body>
 __global__ void __launch_bounds__(kHeavyThreads) k_tier_heavy(Body body, TierLists t) {

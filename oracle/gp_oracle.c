@@ -280,6 +280,138 @@ static int64_t og_count_clusters(const uint64_t* cw, int64_t n) {
     return c;
 }
 
+/* seed_matching.cuh:12-91, call coarsen.cu:505-539.  Every vertex of degree at
+ * most 1024 picks its admissible neighbour of largest (edge weight, pair hash
+ * mix32(lo ^ mix32(hi ^ salt)), id); a neighbour is admissible when the two
+ * vertex weights fit the cluster capacity together.  Mutual picks merge into
+ * a cluster rooted at the smaller id.  Writes clusters and cluster weights
+ * (cw must start zero); returns the number of clusters. */
+static int64_t og_seed_match(og_agg* A, uint32_t salt) {
+    const og_graph* g = A->g;
+    const int64_t n = A->n;
+    const uint64_t cap = A->cap;
+    int32_t* choices = og_alloc((size_t)n, sizeof(int32_t));
+    for (int64_t v = 0; v < n; ++v) {
+        choices[v] = -1;
+        const int64_t deg = g->off[v + 1] - g->off[v];
+        if (deg > 1024) continue;
+        const uint64_t wv = g->vw[v];
+        int32_t best = -1; uint64_t bw = 0; uint32_t bh = 0;
+        for (int64_t e = g->off[v]; e < g->off[v + 1]; ++e) {
+            const int32_t u = g->nbr[e];
+            if (u == v || wv > cap || (uint64_t)g->vw[u] > cap - wv) continue;
+            const uint32_t lo = (uint32_t)(u < v ? u : v), hi = (uint32_t)(u < v ? v : u);
+            const uint32_t h = mix32(lo ^ mix32(hi ^ salt));
+            const uint64_t w = ew_at(g->ew, e);
+            if (best < 0 || w > bw || (w == bw && (h > bh || (h == bh && u > best)))) {
+                best = u; bw = w; bh = h;
+            }
+        }
+        choices[v] = best;
+    }
+    int32_t* matched = og_alloc((size_t)n, sizeof(int32_t));
+    for (int64_t v = 0; v < n; ++v) matched[v] = -1;
+    for (int64_t v = 0; v < n; ++v) {
+        const int32_t u = choices[v];
+        if (u > v && choices[u] == v) { matched[v] = u; matched[u] = (int32_t)v; }
+    }
+    int64_t count = 0;
+    for (int64_t v = 0; v < n; ++v) {
+        const int32_t p = matched[v];
+        const int32_t root = p < 0 ? (int32_t)v : (p < v ? p : (int32_t)v);
+        A->clusters[v] = root;
+        count += root == v;
+    }
+    for (int64_t v = 0; v < n; ++v) A->cw[A->clusters[v]] += g->vw[v];
+    free(choices); free(matched);
+    return count;
+}
+
+/* Mover threshold of one growth round (coarsen.cu:984-990): a singleton root
+ * moves when mix32(v ^ role_salt) is below floor(2^32 * min(1/2,
+ * max(0, clusters - ceil(n/2)) / clusters)), taken in double. */
+static uint32_t og_mover_threshold(int64_t cluster_count, int64_t n) {
+    const int64_t desired = (n + 1) / 2;
+    const int64_t removable = cluster_count - desired > 0 ? cluster_count - desired : 0;
+    double frac = (double)removable / (double)cluster_count;
+    if (frac > 0.50) frac = 0.50;
+    return (uint32_t)(frac * 4294967296.0);
+}
+
+/* Isolated packing: coarsen.cu:1130-1161, aggregation.cuh:705-745.  Zero-degree
+ * vertices in id order are laid on the weight line; bin b covers
+ * [b*cap, (b+1)*cap) of the running weight prefix.  A vertex whose interval
+ * [P-w, P) lies inside one bin joins the smallest such vertex of that bin;
+ * a vertex straddling a bin edge stays alone.  Returns 1 when it ran. */
+static int og_pack_isolated(og_agg* A) {
+    const og_graph* g = A->g;
+    const int64_t n = A->n;
+    const uint64_t cap = A->cap;
+    int64_t count = 0;
+    for (int64_t v = 0; v < n; ++v) count += g->off[v] == g->off[v + 1];
+    if (count <= 1) return 0;
+    int32_t* iso = og_alloc((size_t)count, sizeof(int32_t));
+    uint64_t* prefix = og_alloc((size_t)count, sizeof(uint64_t));
+    int64_t c = 0; uint64_t run = 0;
+    for (int64_t v = 0; v < n; ++v)
+        if (g->off[v] == g->off[v + 1]) { iso[c] = (int32_t)v; run += g->vw[v]; prefix[c] = run; ++c; }
+    const uint64_t tot = run;
+    const uint64_t bins = tot / cap + (tot % cap != 0);
+    int32_t* rep = og_alloc((size_t)bins, sizeof(int32_t));
+    for (uint64_t b = 0; b < bins; ++b) rep[b] = INT32_MAX;
+    for (int64_t i = 0; i < count; ++i) {
+        const uint64_t w = g->vw[iso[i]];
+        const uint64_t fb = (prefix[i] - w) / cap, lb = (prefix[i] - 1) / cap;
+        if (fb == lb && iso[i] < rep[fb]) rep[fb] = iso[i];
+    }
+    for (int64_t i = 0; i < count; ++i) {
+        const int32_t v = iso[i];
+        const uint64_t w = g->vw[v];
+        const uint64_t fb = (prefix[i] - w) / cap, lb = (prefix[i] - 1) / cap;
+        if (fb != lb) continue;
+        const int32_t t = rep[fb];
+        if (t == v) continue;
+        A->clusters[v] = t; A->cw[v] = 0; A->cw[t] += w;
+    }
+    free(iso); free(prefix); free(rep);
+    return 1;
+}
+
+/* Two-hop pairing: coarsen.cu:1166-1259, aggregation.cuh:654-696.  Every
+ * vertex still alone in its cluster takes its unrestricted favourite cluster
+ * (the exact top-1 of og_form_proposals without roles or capacity); vertices
+ * sharing a favourite are ordered by (gain desc, tie desc, id asc) and the
+ * 2nd, 4th, ... of each group joins the cluster of the one before it when
+ * their weights fit the capacity.  Only the first `budget` such pairs merge;
+ * adm is scratch of n records. */
+static void og_two_hop(og_agg* A, uint32_t seed, int level, int64_t budget, og_adm* adm) {
+    const og_graph* g = A->g;
+    const int64_t n = A->n;
+    const uint64_t cap = A->cap;
+    og_form_proposals(A, seed, level, 4, 0, 0, 0);
+    int64_t cnt = 0;
+    for (int64_t v = 0; v < n; ++v) {
+        const int32_t fav = A->cw[A->clusters[v]] == (uint64_t)g->vw[v] ? A->proposals[v] : OG_INVALID;
+        if (fav == OG_INVALID) continue;
+        adm[cnt].target = fav; adm[cnt].key = A->gains[v]; adm[cnt].tie = A->ties[v];
+        adm[cnt].v = (int32_t)v; ++cnt;
+    }
+    qsort(adm, (size_t)cnt, sizeof(og_adm), cmp_adm);
+    int64_t position = 0, rank = 0;
+    for (int64_t i = 0; i < cnt; ++i) {
+        rank = (i == 0 || adm[i].target != adm[i - 1].target) ? 1 : rank + 1;
+        const uint64_t wi = g->vw[adm[i].v];
+        const int flag = (rank % 2 == 0) && i > 0 && (uint64_t)g->vw[adm[i - 1].v] <= cap - wi;
+        if (!flag) continue;
+        if (position < budget) {
+            const int32_t v = adm[i].v;
+            const int32_t s = A->clusters[v], t = A->clusters[adm[i - 1].v];
+            if (s != t) { A->cw[s] -= wi; A->cw[t] += wi; A->clusters[v] = t; }
+        }
+        ++position;
+    }
+}
+
 typedef struct {
     int64_t coarse_n;
     uint64_t capacity;
@@ -315,46 +447,9 @@ static int og_aggregate(const og_graph* g, og_cstate* st, int parts, uint32_t se
     A.scratch = og_alloc((size_t)maxdeg_row + 1, sizeof(og_cw_pair));
 
     int64_t initial_count = n;
-    if (seed_matching) {
-        /* seed_matching.cuh:12-91, call coarsen.cu:505-539 */
-        int32_t* choices = og_alloc((size_t)n, sizeof(int32_t));
-        const uint32_t salt = seed ^ ((uint32_t)level * 0x9e3779b9U);
-        for (int64_t v = 0; v < n; ++v) {
-            choices[v] = -1;
-            const int64_t deg = g->off[v + 1] - g->off[v];
-            if (deg > 1024) continue;
-            const uint64_t wv = g->vw[v];
-            int32_t best = -1; uint64_t bw = 0; uint32_t bh = 0;
-            for (int64_t e = g->off[v]; e < g->off[v + 1]; ++e) {
-                const int32_t u = g->nbr[e];
-                if (u == v || wv > cap || (uint64_t)g->vw[u] > cap - wv) continue;
-                const uint32_t lo = (uint32_t)(u < v ? u : v), hi = (uint32_t)(u < v ? v : u);
-                const uint32_t h = mix32(lo ^ mix32(hi ^ salt));
-                const uint64_t w = ew_at(g->ew, e);
-                if (best < 0 || w > bw || (w == bw && (h > bh || (h == bh && u > best)))) {
-                    best = u; bw = w; bh = h;
-                }
-            }
-            choices[v] = best;
-        }
-        int32_t* matched = og_alloc((size_t)n, sizeof(int32_t));
-        for (int64_t v = 0; v < n; ++v) matched[v] = -1;
-        for (int64_t v = 0; v < n; ++v) {
-            const int32_t u = choices[v];
-            if (u > v && choices[u] == v) { matched[v] = u; matched[u] = (int32_t)v; }
-        }
-        initial_count = 0;
-        for (int64_t v = 0; v < n; ++v) {
-            const int32_t p = matched[v];
-            const int32_t root = p < 0 ? (int32_t)v : (p < v ? p : (int32_t)v);
-            A.clusters[v] = root;
-            initial_count += root == v;
-        }
-        for (int64_t v = 0; v < n; ++v) A.cw[A.clusters[v]] += g->vw[v];
-        free(choices); free(matched);
-    } else {
+    if (seed_matching) initial_count = og_seed_match(&A, seed ^ ((uint32_t)level * 0x9e3779b9U));
+    else
         for (int64_t v = 0; v < n; ++v) { A.clusters[v] = (int32_t)v; A.cw[v] = g->vw[v]; }
-    }
 
     /* Degree statistics: coarsen.cu:547-579.  A bound <= 8 is trusted. */
     uint64_t stats_maxdeg; int stats_heavy;
@@ -370,10 +465,7 @@ static int og_aggregate(const og_graph* g, og_cstate* st, int parts, uint32_t se
     og_adm* adm = og_alloc((size_t)n, sizeof(og_adm));
     for (int round = 0; round < 4; ++round) {                                /* coarsen.cu:516-1128 */
         if ((double)cluster_count <= 0.60 * (double)n) break;
-        const int64_t removable = cluster_count - desired > 0 ? cluster_count - desired : 0;
-        double frac = (double)removable / (double)cluster_count;
-        if (frac > 0.50) frac = 0.50;
-        const uint32_t threshold = (uint32_t)(frac * 4294967296.0);
+        const uint32_t threshold = og_mover_threshold(cluster_count, n);
         const uint64_t room = cap - bound;
         const uint64_t degree = stats_maxdeg;
         const int fit = fast_path && (degree == 0 || bound <= room / maxv / degree);
@@ -412,64 +504,10 @@ static int og_aggregate(const og_graph* g, og_cstate* st, int parts, uint32_t se
         if ((double)cluster_count <= 0.60 * (double)n) break;
     }
 
-    if (pack_isolated) {
-        /* coarsen.cu:1130-1161, aggregation.cuh:705-745 */
-        int64_t count = 0;
-        for (int64_t v = 0; v < n; ++v) count += g->off[v] == g->off[v + 1];
-        if (count > 1) {
-            int32_t* iso = og_alloc((size_t)count, sizeof(int32_t));
-            uint64_t* prefix = og_alloc((size_t)count, sizeof(uint64_t));
-            int64_t c = 0; uint64_t run = 0;
-            for (int64_t v = 0; v < n; ++v)
-                if (g->off[v] == g->off[v + 1]) { iso[c] = (int32_t)v; run += g->vw[v]; prefix[c] = run; ++c; }
-            const uint64_t tot = run;
-            const uint64_t bins = tot / cap + (tot % cap != 0);
-            int32_t* rep = og_alloc((size_t)bins, sizeof(int32_t));
-            for (uint64_t b = 0; b < bins; ++b) rep[b] = INT32_MAX;
-            for (int64_t i = 0; i < count; ++i) {
-                const uint64_t w = g->vw[iso[i]];
-                const uint64_t fb = (prefix[i] - w) / cap, lb = (prefix[i] - 1) / cap;
-                if (fb == lb && iso[i] < rep[fb]) rep[fb] = iso[i];
-            }
-            for (int64_t i = 0; i < count; ++i) {
-                const int32_t v = iso[i];
-                const uint64_t w = g->vw[v];
-                const uint64_t fb = (prefix[i] - w) / cap, lb = (prefix[i] - 1) / cap;
-                if (fb != lb) continue;
-                const int32_t t = rep[fb];
-                if (t == v) continue;
-                A.clusters[v] = t; A.cw[v] = 0; A.cw[t] += w;
-            }
-            cluster_count = og_count_clusters(A.cw, n);
-            free(iso); free(prefix); free(rep);
-        }
-    }
+    if (pack_isolated && og_pack_isolated(&A)) cluster_count = og_count_clusters(A.cw, n);
 
     if ((double)cluster_count > 0.60 * (double)n) {
-        /* Two-hop pairing: coarsen.cu:1166-1259, aggregation.cuh:654-696 */
-        og_form_proposals(&A, seed, level, 4, 0, 0, 0);
-        int64_t cnt = 0;
-        for (int64_t v = 0; v < n; ++v) {
-            const int32_t fav = A.cw[A.clusters[v]] == (uint64_t)g->vw[v] ? A.proposals[v] : OG_INVALID;
-            if (fav == OG_INVALID) continue;
-            adm[cnt].target = fav; adm[cnt].key = A.gains[v]; adm[cnt].tie = A.ties[v];
-            adm[cnt].v = (int32_t)v; ++cnt;
-        }
-        qsort(adm, (size_t)cnt, sizeof(og_adm), cmp_adm);
-        const int64_t budget = cluster_count - desired > 0 ? cluster_count - desired : 0;
-        int64_t position = 0, rank = 0;
-        for (int64_t i = 0; i < cnt; ++i) {
-            rank = (i == 0 || adm[i].target != adm[i - 1].target) ? 1 : rank + 1;
-            const uint64_t wi = g->vw[adm[i].v];
-            const int flag = (rank % 2 == 0) && i > 0 && (uint64_t)g->vw[adm[i - 1].v] <= cap - wi;
-            if (!flag) continue;
-            if (position < budget) {
-                const int32_t v = adm[i].v;
-                const int32_t s = A.clusters[v], t = A.clusters[adm[i - 1].v];
-                if (s != t) { A.cw[s] -= wi; A.cw[t] += wi; A.clusters[v] = t; }
-            }
-            ++position;
-        }
+        og_two_hop(&A, seed, level, cluster_count - desired > 0 ? cluster_count - desired : 0, adm);
         cluster_count = og_count_clusters(A.cw, n);
     }
 
@@ -791,15 +829,97 @@ static int og_update_cut(og_ref* R, int64_t delta) {
     return 0;
 }
 
-/* balance_gain_bucket (incremental.cuh:553-557) on an exact power table. */
-static int og_gain_bucket(const og_ref* R, double gain) {
+/* LP proposals: incremental.cuh:166-198,229-296.  A vertex locked by the
+ * previous round is skipped (and unlocked); an interior vertex (all incident
+ * weight in its own part) is skipped.  The target is the part p != s of
+ * largest A(v,p) > 0, the lowest id on ties; with prices on, the largest
+ * score A(v,p) - pi_p * max(0, w - slack_p), the first on ties.  The vertex
+ * proposes when best >= A(v,s) or A(v,s) - best < floor(filter * A(v,s)).
+ * Candidates are listed in vertex order in movers; returns their count. */
+static int64_t og_lp_propose(og_ref* R, double filter, int32_t* movers) {
+    const og_graph* g = R->g;
+    const int64_t n = R->n;
+    const int k = R->k;
+    int64_t ncand = 0;
+    for (int64_t v = 0; v < n; ++v) {
+        R->targets[v] = -1; R->gains[v] = INT64_MIN; R->flags[v] = 0;
+        if (R->locks[v]) { R->locks[v] = 0; continue; }
+        const int s = R->labels[v];
+        const uint64_t total = og_affinity(R, v, R->conn);
+        const uint64_t current = R->conn[s];
+        if (current == total) continue;
+        const uint64_t w = g->vw[v];
+        uint64_t best = 0; int target = -1; double best_score = -1.7976931348623157e308;
+        for (int p = 0; p < k; ++p) {
+            const uint64_t c = R->conn[p];
+            if (p == s || !c) continue;
+            if (R->prices) {
+                const double score = (double)c - og_repair_cost(R, p, w);
+                if (target < 0 || score > best_score) { best_score = score; best = c; target = p; }
+            } else if (target < 0 || c > best) { best = c; target = p; }
+        }
+        if (target >= 0 && (best >= current || current - best < (uint64_t)(filter * (double)current))) {
+            R->targets[v] = target;
+            R->gains[v] = best >= current ? (int64_t)(best - current) : -(int64_t)(current - best);
+            movers[ncand++] = (int32_t)v;
+        }
+    }
+    return ncand;
+}
+
+/* Afterburner: incremental.cuh:361-438.  Candidate v keeps its move when
+ * g_v + sum over candidate neighbours u ranked before v (g_u > g_v, or equal
+ * and u < v) of w_vu([t_u=t_v] - [t_u=s_v] - [p(u)=t_v] + [p(u)=s_v]) is >= 0,
+ * plus the price difference of leaving s_v for t_v when prices are on: the
+ * exact cut change of moving v after every candidate ranked before it.  Sets
+ * flags, compacts movers to the kept candidates and returns their count. */
+static int64_t og_afterburner(og_ref* R, int32_t* movers, int64_t ncand) {
+    const og_graph* g = R->g;
+    int64_t count = 0;
+    for (int64_t i = 0; i < ncand; ++i) {
+        const int32_t v = movers[i];
+        const int s = R->labels[v], t = R->targets[v];
+        const int64_t gv = R->gains[v];
+        int64_t corr = 0;
+        for (int64_t e = g->off[v]; e < g->off[v + 1]; ++e) {
+            const int32_t u = g->nbr[e];
+            if (u == v || R->targets[u] < 0 || !(R->gains[u] > gv || (R->gains[u] == gv && u < v))) continue;
+            corr += (int64_t)ew_at(g->ew, e) *
+                    ((R->targets[u] == t) - (R->targets[u] == s) - (R->labels[u] == t) + (R->labels[u] == s));
+        }
+        int keep;
+        if (R->prices) {
+            const uint64_t w = g->vw[v];
+            const double adj = og_repair_cost(R, s, w) - og_repair_cost(R, t, w);
+            keep = (double)(gv + corr) + adj >= 0;
+        } else keep = gv + corr >= 0;
+        R->flags[v] = keep;
+    }
+    for (int64_t i = 0; i < ncand; ++i) if (R->flags[movers[i]]) movers[count++] = movers[i];
+    return count;
+}
+
+/* 1.5^j for j = -23..24 at index j + 23: exact products upward from 1, and
+ * quotients by 1.5 downward (both directions in IEEE double). */
+static void og_pow15_table(double* t) {
+    t[23] = 1.0;
+    for (int j = 1; j <= 24; ++j) t[23 + j] = t[22 + j] * 1.5;
+    for (int j = 1; j <= 23; ++j) t[23 - j] = t[24 - j] / 1.5;
+}
+
+/* balance_gain_bucket (incremental.cuh:553-557): 0 for a positive gain per
+ * unit of relief, 1 for zero, else max(2, min(49, 25 + ceil(log(-g)/log 1.5))),
+ * taken as the smallest j with -g <= 1.5^j on the table above instead of the
+ * libm logarithms (same bucket wherever log is correctly rounded). */
+static int og_gain_bucket_table(const double* pow15, double gain) {
     if (gain > 0) return 0;
     if (gain == 0) return 1;
     const double x = -gain;
     for (int j = -23; j <= 23; ++j)
-        if (x <= R->pow15[j + 23]) return 25 + j < 2 ? 2 : 25 + j;
+        if (x <= pow15[j + 23]) return 25 + j < 2 ? 2 : 25 + j;
     return 49;
 }
+static int og_gain_bucket(const og_ref* R, double gain) { return og_gain_bucket_table(R->pow15, gain); }
 
 /* enumerate_repair_destination: rebalance_select.cuh:8-75 (with the dense
  * ascending scan and the slack-weighted hash fallback). */
@@ -836,111 +956,172 @@ static int og_repair_destination(const og_ref* R, int64_t v, int source, uint64_
     return -1;
 }
 
-/* One capacity-repair loop: rebalance_graph.cuh:160-280 (body 215-262,
- * loop control 48-63), proposals incremental.cuh:559-626, bucket scan
- * 628-652, reservation rebalance_graph.cuh:86-156. Returns feasibility. */
-static int og_rebalance(og_ref* R, int* ok) {
+/* Scratch of one capacity-repair iteration. */
+typedef struct {
+    int32_t* bucket;     /* n: donor bucket index (part*800 + gain bucket*16 + section), -1 none */
+    uint64_t* counts;    /* k*800 bucket weights */
+    uint64_t* prefix;    /* k*800 exclusive prefix of counts within each part */
+    uint64_t* run;       /* k*800 running same-bucket weight (canonical rank) */
+    uint64_t* loads0;    /* k: loads frozen at the start of the iteration */
+    uint64_t* slack;     /* k: cap - loads0 (0 when overloaded) */
+    uint64_t* requests;  /* k: admitted request weight per receiver */
+    uint64_t* snapshot;  /* k */
+    uint64_t* batch;     /* k */
+    uint8_t* eligible;   /* n */
+    uint8_t* done;       /* n */
+    uint8_t* tried;      /* n*k */
+    int32_t* choice;     /* n */
+    int32_t* elist;      /* n: eligible donors in vertex order */
+} og_repair_ws;
+
+static void og_repair_ws_alloc(og_repair_ws* W, int64_t n, int k) {
+    W->bucket = og_alloc((size_t)n, sizeof(int32_t));
+    W->counts = og_alloc((size_t)k * OG_BALANCE_WIDTH, sizeof(uint64_t));
+    W->prefix = og_alloc((size_t)k * OG_BALANCE_WIDTH, sizeof(uint64_t));
+    W->run = og_alloc((size_t)k * OG_BALANCE_WIDTH, sizeof(uint64_t));
+    W->loads0 = og_alloc((size_t)k, sizeof(uint64_t));
+    W->slack = og_alloc((size_t)k, sizeof(uint64_t));
+    W->requests = og_alloc((size_t)k, sizeof(uint64_t));
+    W->snapshot = og_alloc((size_t)k, sizeof(uint64_t));
+    W->batch = og_alloc((size_t)k, sizeof(uint64_t));
+    W->eligible = og_alloc((size_t)n, 1);
+    W->done = og_alloc((size_t)n, 1);
+    W->tried = og_alloc((size_t)n * (size_t)k, 1);
+    W->choice = og_alloc((size_t)n, sizeof(int32_t));
+    W->elist = og_alloc((size_t)n, sizeof(int32_t));
+}
+
+static void og_repair_ws_free(og_repair_ws* W) {
+    free(W->bucket); free(W->counts); free(W->prefix); free(W->run); free(W->loads0); free(W->slack);
+    free(W->requests); free(W->snapshot); free(W->batch); free(W->eligible); free(W->done); free(W->tried);
+    free(W->choice); free(W->elist);
+}
+
+/* Donor proposals and eligibility of one repair iteration on the loads frozen
+ * at its start: incremental.cuh:559-652.  Every vertex of an overloaded part
+ * takes its repair destination (og_repair_destination); its gain per unit of
+ * relief min(w, excess) picks one of 50 gain buckets and mix32(v) % 16 one of
+ * 16 sections.  Buckets are ordered within the part; a donor is eligible when
+ * the weight of the part's donors before it -- earlier buckets, then smaller
+ * ids in its own bucket (the reference ranks by atomic arrival instead,
+ * incremental.cuh:618) -- is below the part's excess.  Sets targets to the
+ * first-choice destination and returns the eligible donors in vertex order
+ * in W->elist. */
+static int64_t og_repair_select(og_ref* R, og_repair_ws* W) {
     const og_graph* g = R->g;
     const int64_t n = R->n;
     const int k = R->k;
     const uint64_t cap = R->cap;
+    memcpy(W->loads0, R->loads, (size_t)k * sizeof(uint64_t));
+    memset(W->counts, 0, (size_t)k * OG_BALANCE_WIDTH * sizeof(uint64_t));
+    for (int p = 0; p < k; ++p) { W->slack[p] = W->loads0[p] < cap ? cap - W->loads0[p] : 0; W->requests[p] = 0; }
+    for (int64_t v = 0; v < n; ++v) {
+        W->bucket[v] = -1; R->targets[v] = -1; R->flags[v] = 0;
+        const int s = R->labels[v];
+        if (!(W->loads0[s] > cap)) continue;
+        const uint64_t w = g->vw[v];
+        const uint64_t total = og_affinity(R, v, R->conn);
+        uint64_t best_conn, current;
+        const int best = og_repair_destination(R, v, s, w, R->conn, total, &best_conn, &current);
+        if (best < 0) continue;
+        const double gain = best_conn >= current ? (double)(best_conn - current) : -(double)(current - best_conn);
+        const uint64_t excess = W->loads0[s] - cap;
+        const uint64_t relief = w < excess ? w : excess;
+        const double gpw = gain / (double)relief;
+        R->targets[v] = best;
+        W->bucket[v] = s * OG_BALANCE_WIDTH + og_gain_bucket(R, gpw) * OG_BALANCE_SECTIONS +
+                       (int)(mix32((uint32_t)v) % OG_BALANCE_SECTIONS);
+        W->counts[W->bucket[v]] += w;
+    }
+    for (int p = 0; p < k; ++p) {
+        uint64_t sum = 0;
+        for (int b = 0; b < OG_BALANCE_WIDTH; ++b) {
+            W->prefix[p * OG_BALANCE_WIDTH + b] = sum;
+            sum += W->counts[p * OG_BALANCE_WIDTH + b];
+        }
+    }
+    memset(W->run, 0, (size_t)k * OG_BALANCE_WIDTH * sizeof(uint64_t));
+    int64_t ecount = 0;
+    for (int64_t v = 0; v < n; ++v) {
+        W->eligible[v] = 0; W->done[v] = 0;
+        const int b = W->bucket[v];
+        if (b < 0) continue;
+        const int s = b / OG_BALANCE_WIDTH;
+        const uint64_t rank = W->run[b];
+        W->run[b] += g->vw[v];
+        if (W->prefix[b] + rank < W->loads0[s] - cap) { W->eligible[v] = 1; W->elist[ecount++] = (int32_t)v; }
+    }
+    return ecount;
+}
+
+/* Receiver reservation: rebalance_graph.cuh:86-156, at most k attempts.  In
+ * attempt 0 every eligible donor requests its first-choice destination; in
+ * later attempts a donor not yet admitted requests the untried part p != s of
+ * largest (A(v,p), room) whose room -- slack minus the requests admitted
+ * before the attempt -- holds it.  Per attempt, requests to one receiver are
+ * ranked in vertex order on top of its admitted total; a request is admitted
+ * when it ends within the receiver's slack.  Admitted donors get flags = 1
+ * and their receiver in targets; an attempt without any request ends it. */
+static void og_repair_reserve(og_ref* R, og_repair_ws* W, int64_t ecount) {
+    const og_graph* g = R->g;
+    const int k = R->k;
+    for (int64_t i = 0; i < ecount; ++i) memset(W->tried + (size_t)W->elist[i] * k, 0, (size_t)k);
+    for (int attempt = 0; attempt < k; ++attempt) {
+        memcpy(W->snapshot, W->requests, (size_t)k * sizeof(uint64_t));
+        int any = 0;
+        for (int64_t i = 0; i < ecount; ++i) {
+            const int32_t v = W->elist[i];
+            W->choice[v] = -1;
+            if (W->done[v]) continue;
+            if (attempt == 0) W->choice[v] = R->targets[v];
+            else {
+                const uint64_t w = g->vw[v];
+                og_affinity(R, v, R->conn);
+                int t = -1; uint64_t best = 0, room = 0;
+                for (int p = 0; p < k; ++p) {
+                    if (p == R->labels[v] || W->tried[(size_t)v * k + p]) continue;
+                    const uint64_t avail = W->snapshot[p] < W->slack[p] ? W->slack[p] - W->snapshot[p] : 0;
+                    if (avail < w) continue;
+                    if (t < 0 || R->conn[p] > best || (R->conn[p] == best && avail > room)) {
+                        t = p; best = R->conn[p]; room = avail;
+                    }
+                }
+                W->choice[v] = t;
+            }
+            if (W->choice[v] >= 0) any = 1;
+        }
+        if (!any) break;
+        memset(W->batch, 0, (size_t)k * sizeof(uint64_t));
+        for (int64_t i = 0; i < ecount; ++i) {
+            const int32_t v = W->elist[i];
+            const int t = W->choice[v];
+            if (t < 0) continue;
+            const uint64_t w = g->vw[v];
+            W->tried[(size_t)v * k + t] = 1;
+            const uint64_t begin = W->requests[t] + W->batch[t];
+            W->batch[t] += w;
+            if (begin <= W->slack[t] && w <= W->slack[t] - begin) { R->targets[v] = t; R->flags[v] = 1; W->done[v] = 1; }
+        }
+        for (int p = 0; p < k; ++p) W->requests[p] += W->batch[p];
+    }
+}
+
+/* One capacity-repair loop: rebalance_graph.cuh:160-280 (body 215-262, loop
+ * control 48-63): select, reserve, apply; repeated while infeasible, moves
+ * were made and fewer than 2k+4 iterations ran.  Returns feasibility. */
+static int og_rebalance(og_ref* R, int* ok) {
+    const int64_t n = R->n;
+    const int k = R->k;
+    const uint64_t cap = R->cap;
     *ok = 0;
-    int32_t* bucket = og_alloc((size_t)n, sizeof(int32_t));
+    og_repair_ws W;
+    og_repair_ws_alloc(&W, n, k);
     int32_t* movers = og_alloc((size_t)n, sizeof(int32_t));
-    uint64_t* counts = og_alloc((size_t)k * OG_BALANCE_WIDTH, sizeof(uint64_t));
-    uint64_t* prefix = og_alloc((size_t)k * OG_BALANCE_WIDTH, sizeof(uint64_t));
-    uint64_t* run = og_alloc((size_t)k * OG_BALANCE_WIDTH, sizeof(uint64_t));
-    uint64_t* loads0 = og_alloc((size_t)k, sizeof(uint64_t));
-    uint64_t* slack = og_alloc((size_t)k, sizeof(uint64_t));
-    uint64_t* requests = og_alloc((size_t)k, sizeof(uint64_t));
-    uint64_t* snapshot = og_alloc((size_t)k, sizeof(uint64_t));
-    uint64_t* batch = og_alloc((size_t)k, sizeof(uint64_t));
-    uint8_t* eligible = og_alloc((size_t)n, 1);
-    uint8_t* done = og_alloc((size_t)n, 1);
-    uint8_t* tried = og_alloc((size_t)n * (size_t)k, 1);
-    int32_t* choice = og_alloc((size_t)n, sizeof(int32_t));
-    int32_t* elist = og_alloc((size_t)n, sizeof(int32_t));
     int iterations = 0, feasible = 0;
     int64_t count = 0;
     do {
-        memcpy(loads0, R->loads, (size_t)k * sizeof(uint64_t));
-        memset(counts, 0, (size_t)k * OG_BALANCE_WIDTH * sizeof(uint64_t));
-        for (int p = 0; p < k; ++p) { slack[p] = loads0[p] < cap ? cap - loads0[p] : 0; requests[p] = 0; }
-        /* Donor proposals on frozen loads. */
-        for (int64_t v = 0; v < n; ++v) {
-            bucket[v] = -1; R->targets[v] = -1; R->flags[v] = 0;
-            const int s = R->labels[v];
-            if (!(loads0[s] > cap)) continue;
-            const uint64_t w = g->vw[v];
-            const uint64_t total = og_affinity(R, v, R->conn);
-            uint64_t best_conn, current;
-            const int best = og_repair_destination(R, v, s, w, R->conn, total, &best_conn, &current);
-            if (best < 0) continue;
-            const double gain = best_conn >= current ? (double)(best_conn - current) : -(double)(current - best_conn);
-            const uint64_t excess = loads0[s] - cap;
-            const uint64_t relief = w < excess ? w : excess;
-            const double gpw = gain / (double)relief;
-            R->targets[v] = best;
-            bucket[v] = s * OG_BALANCE_WIDTH + og_gain_bucket(R, gpw) * OG_BALANCE_SECTIONS +
-                        (int)(mix32((uint32_t)v) % OG_BALANCE_SECTIONS);
-            counts[bucket[v]] += w;
-        }
-        for (int p = 0; p < k; ++p) {
-            uint64_t sum = 0;
-            for (int b = 0; b < OG_BALANCE_WIDTH; ++b) { prefix[p * OG_BALANCE_WIDTH + b] = sum; sum += counts[p * OG_BALANCE_WIDTH + b]; }
-        }
-        /* Canonical rank: same-bucket donors in vertex order. */
-        memset(run, 0, (size_t)k * OG_BALANCE_WIDTH * sizeof(uint64_t));
-        int64_t ecount = 0;
-        for (int64_t v = 0; v < n; ++v) {
-            eligible[v] = 0; done[v] = 0;
-            const int b = bucket[v];
-            if (b < 0) continue;
-            const int s = b / OG_BALANCE_WIDTH;
-            const uint64_t rank = run[b];
-            run[b] += g->vw[v];
-            if (prefix[b] + rank < loads0[s] - cap) { eligible[v] = 1; elist[ecount++] = (int32_t)v; }
-        }
-        /* Receiver reservation in attempts. */
-        for (int64_t i = 0; i < ecount; ++i) memset(tried + (size_t)elist[i] * k, 0, (size_t)k);
-        for (int attempt = 0; attempt < k; ++attempt) {
-            memcpy(snapshot, requests, (size_t)k * sizeof(uint64_t));
-            int any = 0;
-            for (int64_t i = 0; i < ecount; ++i) {
-                const int32_t v = elist[i];
-                choice[v] = -1;
-                if (done[v]) continue;
-                if (attempt == 0) choice[v] = R->targets[v];
-                else {
-                    const uint64_t w = g->vw[v];
-                    og_affinity(R, v, R->conn);
-                    int t = -1; uint64_t best = 0, room = 0;
-                    for (int p = 0; p < k; ++p) {
-                        if (p == R->labels[v] || tried[(size_t)v * k + p]) continue;
-                        const uint64_t avail = snapshot[p] < slack[p] ? slack[p] - snapshot[p] : 0;
-                        if (avail < w) continue;
-                        if (t < 0 || R->conn[p] > best || (R->conn[p] == best && avail > room)) {
-                            t = p; best = R->conn[p]; room = avail;
-                        }
-                    }
-                    choice[v] = t;
-                }
-                if (choice[v] >= 0) any = 1;
-            }
-            if (!any) break;
-            memset(batch, 0, (size_t)k * sizeof(uint64_t));
-            for (int64_t i = 0; i < ecount; ++i) {
-                const int32_t v = elist[i];
-                const int t = choice[v];
-                if (t < 0) continue;
-                const uint64_t w = g->vw[v];
-                tried[(size_t)v * k + t] = 1;
-                const uint64_t begin = requests[t] + batch[t];
-                batch[t] += w;
-                if (begin <= slack[t] && w <= slack[t] - begin) { R->targets[v] = t; R->flags[v] = 1; done[v] = 1; }
-            }
-            for (int p = 0; p < k; ++p) requests[p] += batch[p];
-        }
+        const int64_t ecount = og_repair_select(R, &W);
+        og_repair_reserve(R, &W, ecount);
         count = 0;
         for (int64_t v = 0; v < n; ++v) {
             if (R->flags[v]) movers[count++] = (int32_t)v;
@@ -957,8 +1138,8 @@ static int og_rebalance(og_ref* R, int* ok) {
     R->st->repair_iterations += iterations;
     *ok = 1;
 fail:
-    free(bucket); free(movers); free(counts); free(prefix); free(run); free(loads0); free(slack);
-    free(requests); free(snapshot); free(batch); free(eligible); free(done); free(tried); free(choice); free(elist);
+    og_repair_ws_free(&W);
+    free(movers);
     return feasible;
 }
 
@@ -1328,9 +1509,7 @@ static int og_refine_level(const og_graph* g, int32_t* labels, uint64_t* loads, 
     R.locks = og_alloc((size_t)n, sizeof(int32_t));
     R.flags = og_alloc((size_t)n, sizeof(int32_t));
     R.conn = og_alloc((size_t)k, sizeof(uint64_t));
-    R.pow15[23] = 1.0;
-    for (int j = 1; j <= 24; ++j) R.pow15[23 + j] = R.pow15[22 + j] * 1.5;
-    for (int j = 1; j <= 23; ++j) R.pow15[23 - j] = R.pow15[24 - j] / 1.5;
+    og_pow15_table(R.pow15);
     uint64_t total_mass = 0;
     for (int p = 0; p < k; ++p) total_mass += loads[p];
     if (total_mass > (uint64_t)n) {
@@ -1365,53 +1544,8 @@ static int og_refine_level(const og_graph* g, int32_t* labels, uint64_t* loads, 
         for (int phase = 0; phase < lp_rounds && stale < o->stale_rounds; ++phase) {
             st->rounds = phase + 1;
             if (R.samples) memset(R.samples, 0, (size_t)k * sizeof(og_sample));
-            /* LP proposals: incremental.cuh:166-198,229-296 */
-            int64_t ncand = 0;
-            for (int64_t v = 0; v < n; ++v) {
-                R.targets[v] = -1; R.gains[v] = INT64_MIN; R.flags[v] = 0;
-                if (R.locks[v]) { R.locks[v] = 0; continue; }
-                const int s = labels[v];
-                const uint64_t total = og_affinity(&R, v, R.conn);
-                const uint64_t current = R.conn[s];
-                if (current == total) continue;
-                const uint64_t w = g->vw[v];
-                uint64_t best = 0; int target = -1; double best_score = -1.7976931348623157e308;
-                for (int p = 0; p < k; ++p) {
-                    const uint64_t c = R.conn[p];
-                    if (p == s || !c) continue;
-                    if (R.prices) {
-                        const double score = (double)c - og_repair_cost(&R, p, w);
-                        if (target < 0 || score > best_score) { best_score = score; best = c; target = p; }
-                    } else if (target < 0 || c > best) { best = c; target = p; }
-                }
-                if (target >= 0 && (best >= current || current - best < (uint64_t)(filter * (double)current))) {
-                    R.targets[v] = target;
-                    R.gains[v] = best >= current ? (int64_t)(best - current) : -(int64_t)(current - best);
-                    movers[ncand++] = (int32_t)v;
-                }
-            }
-            /* Afterburner: incremental.cuh:361-438 */
-            int64_t count = 0;
-            for (int64_t i = 0; i < ncand; ++i) {
-                const int32_t v = movers[i];
-                const int s = labels[v], t = R.targets[v];
-                const int64_t gv = R.gains[v];
-                int64_t corr = 0;
-                for (int64_t e = g->off[v]; e < g->off[v + 1]; ++e) {
-                    const int32_t u = g->nbr[e];
-                    if (u == v || R.targets[u] < 0 || !(R.gains[u] > gv || (R.gains[u] == gv && u < v))) continue;
-                    corr += (int64_t)ew_at(g->ew, e) *
-                            ((R.targets[u] == t) - (R.targets[u] == s) - (labels[u] == t) + (labels[u] == s));
-                }
-                int keep;
-                if (R.prices) {
-                    const uint64_t w = g->vw[v];
-                    const double adj = og_repair_cost(&R, s, w) - og_repair_cost(&R, t, w);
-                    keep = (double)(gv + corr) + adj >= 0;
-                } else keep = gv + corr >= 0;
-                R.flags[v] = keep;
-            }
-            for (int64_t i = 0; i < ncand; ++i) if (R.flags[movers[i]]) movers[count++] = movers[i];
+            const int64_t ncand = og_lp_propose(&R, filter, movers);
+            const int64_t count = og_afterburner(&R, movers, ncand);
             if (!count) { if (++empty >= 2) break; ++stale; continue; }
             empty = 0;
             const int64_t delta = og_apply(&R, movers, count, 1, 0);
@@ -1542,4 +1676,188 @@ int og_refine_graph(int64_t n, int64_t m, const int32_t* off, const int32_t* nbr
                        : og_fail("invalid capacity");
     free(vv); free(loads);
     return rc;
+}
+
+/* ====================================================== single stages
+ * Entry points for the oracle's own checks: each runs one stage of the
+ * restated schedule on caller arrays through the same functions the level
+ * loops call. */
+static og_graph og_view(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew, const uint32_t* vw) {
+    og_graph g;
+    memset(&g, 0, sizeof g);
+    g.n = n; g.m = off[n];
+    g.off = (int32_t*)off; g.nbr = (int32_t*)nbr; g.ew = (uint32_t*)ew; g.vw = (uint32_t*)vw;
+    return g;
+}
+
+static void og_agg_view(og_agg* A, const og_graph* g, uint64_t cap, int32_t* clusters, uint64_t* cw) {
+    memset(A, 0, sizeof *A);
+    A->g = g; A->n = g->n; A->cap = cap; A->clusters = clusters; A->cw = cw;
+    A->roles = og_alloc((size_t)g->n, 1);
+    A->proposals = og_alloc((size_t)g->n, sizeof(int32_t));
+    A->gains = og_alloc((size_t)g->n, sizeof(uint64_t));
+    A->regret = og_alloc((size_t)g->n, sizeof(uint64_t));
+    A->ties = og_alloc((size_t)g->n, sizeof(uint32_t));
+    int64_t maxdeg = 0;
+    for (int64_t v = 0; v < g->n; ++v) if (g->off[v + 1] - g->off[v] > maxdeg) maxdeg = g->off[v + 1] - g->off[v];
+    A->scratch = og_alloc((size_t)maxdeg + 1, sizeof(og_cw_pair));
+}
+
+static void og_agg_view_free(og_agg* A) {
+    free(A->roles); free(A->proposals); free(A->gains); free(A->regret); free(A->ties); free(A->scratch);
+}
+
+int64_t og_t_seed_match(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew, const uint32_t* vw,
+                        uint64_t cap, uint32_t salt, int32_t* roots) {
+    og_graph g = og_view(n, off, nbr, ew, vw);
+    uint64_t* cw = og_alloc((size_t)n, sizeof(uint64_t));
+    og_agg A;
+    og_agg_view(&A, &g, cap, roots, cw);
+    const int64_t count = og_seed_match(&A, salt);
+    og_agg_view_free(&A);
+    free(cw);
+    return count;
+}
+
+uint32_t og_t_mover_threshold(int64_t clusters, int64_t n) { return og_mover_threshold(clusters, n); }
+
+void og_t_proposals(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew, const uint32_t* vw,
+                    const int32_t* clusters, const uint64_t* cw, uint64_t cap, uint32_t seed, int level, int round,
+                    int filter_roles, uint32_t threshold, int need_regret, uint8_t* roles, int32_t* proposals,
+                    uint64_t* gains, uint64_t* regret, uint32_t* ties) {
+    og_graph g = og_view(n, off, nbr, ew, vw);
+    og_agg A;
+    og_agg_view(&A, &g, cap, (int32_t*)clusters, (uint64_t*)cw);
+    og_form_proposals(&A, seed, level, round, filter_roles, threshold, need_regret);
+    memcpy(roles, A.roles, (size_t)n);
+    memcpy(proposals, A.proposals, (size_t)n * sizeof(int32_t));
+    memcpy(gains, A.gains, (size_t)n * sizeof(uint64_t));
+    memcpy(regret, A.regret, (size_t)n * sizeof(uint64_t));
+    memcpy(ties, A.ties, (size_t)n * sizeof(uint32_t));
+    og_agg_view_free(&A);
+}
+
+int og_t_pack_isolated(int64_t n, const int32_t* off, const uint32_t* vw, uint64_t cap, int32_t* clusters,
+                       uint64_t* cw) {
+    og_graph g = og_view(n, off, NULL, NULL, vw);
+    og_agg A;
+    og_agg_view(&A, &g, cap, clusters, cw);
+    const int ran = og_pack_isolated(&A);
+    og_agg_view_free(&A);
+    return ran;
+}
+
+void og_t_two_hop(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew, const uint32_t* vw,
+                  int32_t* clusters, uint64_t* cw, uint64_t cap, uint32_t seed, int level, int64_t budget) {
+    og_graph g = og_view(n, off, nbr, ew, vw);
+    og_agg A;
+    og_agg_view(&A, &g, cap, clusters, cw);
+    og_adm* adm = og_alloc((size_t)n, sizeof(og_adm));
+    og_two_hop(&A, seed, level, budget, adm);
+    free(adm);
+    og_agg_view_free(&A);
+}
+
+static void og_ref_view(og_ref* R, const og_graph* g, int k, uint64_t cap, int32_t* labels, uint64_t* loads,
+                        const double* prices, og_level_stats* st) {
+    memset(R, 0, sizeof *R);
+    R->g = g; R->n = g->n; R->k = k; R->cap = cap; R->labels = labels; R->loads = loads; R->st = st;
+    R->targets = og_alloc((size_t)g->n, sizeof(int32_t));
+    R->gains = og_alloc((size_t)g->n, sizeof(int64_t));
+    R->locks = og_alloc((size_t)g->n, sizeof(int32_t));
+    R->flags = og_alloc((size_t)g->n, sizeof(int32_t));
+    R->conn = og_alloc((size_t)k, sizeof(uint64_t));
+    og_pow15_table(R->pow15);
+    if (prices) {
+        R->prices = og_alloc((size_t)k, sizeof(double));
+        memcpy(R->prices, prices, (size_t)k * sizeof(double));
+    }
+}
+
+static void og_ref_view_free(og_ref* R) {
+    free(R->targets); free(R->gains); free(R->locks); free(R->flags); free(R->conn); free(R->prices);
+}
+
+int64_t og_t_lp_propose(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew, const uint32_t* vw,
+                        int k, const int32_t* labels, int32_t* locks, const uint64_t* loads, uint64_t cap,
+                        const double* prices, double filter, int32_t* targets, int64_t* gains) {
+    og_graph g = og_view(n, off, nbr, ew, vw);
+    og_ref R;
+    og_level_stats st;
+    memset(&st, 0, sizeof st);
+    og_ref_view(&R, &g, k, cap, (int32_t*)labels, (uint64_t*)loads, prices, &st);
+    memcpy(R.locks, locks, (size_t)n * sizeof(int32_t));
+    int32_t* movers = og_alloc((size_t)n, sizeof(int32_t));
+    const int64_t ncand = og_lp_propose(&R, filter, movers);
+    memcpy(locks, R.locks, (size_t)n * sizeof(int32_t));
+    memcpy(targets, R.targets, (size_t)n * sizeof(int32_t));
+    memcpy(gains, R.gains, (size_t)n * sizeof(int64_t));
+    free(movers);
+    og_ref_view_free(&R);
+    return ncand;
+}
+
+int64_t og_t_afterburner(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew, const uint32_t* vw,
+                         int k, const int32_t* labels, const uint64_t* loads, uint64_t cap, const double* prices,
+                         const int32_t* targets, const int64_t* gains, uint8_t* keep) {
+    og_graph g = og_view(n, off, nbr, ew, vw);
+    og_ref R;
+    og_level_stats st;
+    memset(&st, 0, sizeof st);
+    og_ref_view(&R, &g, k, cap, (int32_t*)labels, (uint64_t*)loads, prices, &st);
+    memcpy(R.targets, targets, (size_t)n * sizeof(int32_t));
+    memcpy(R.gains, gains, (size_t)n * sizeof(int64_t));
+    int32_t* movers = og_alloc((size_t)n, sizeof(int32_t));
+    int64_t ncand = 0;
+    for (int64_t v = 0; v < n; ++v) if (targets[v] >= 0) movers[ncand++] = (int32_t)v;
+    const int64_t count = og_afterburner(&R, movers, ncand);
+    for (int64_t v = 0; v < n; ++v) keep[v] = (uint8_t)(targets[v] >= 0 && R.flags[v]);
+    free(movers);
+    og_ref_view_free(&R);
+    return count;
+}
+
+int og_t_gain_bucket(double gain) {
+    double t[48];
+    og_pow15_table(t);
+    return og_gain_bucket_table(t, gain);
+}
+
+int og_t_repair_destination(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew,
+                            const uint32_t* vw, int k, const int32_t* labels, const uint64_t* loads, uint64_t cap,
+                            int32_t v, uint64_t* best_conn, uint64_t* current) {
+    og_graph g = og_view(n, off, nbr, ew, vw);
+    og_ref R;
+    og_level_stats st;
+    memset(&st, 0, sizeof st);
+    og_ref_view(&R, &g, k, cap, (int32_t*)labels, (uint64_t*)loads, NULL, &st);
+    const uint64_t total = og_affinity(&R, v, R.conn);
+    const int dest = og_repair_destination(&R, v, labels[v], vw[v], R.conn, total, best_conn, current);
+    og_ref_view_free(&R);
+    return dest;
+}
+
+int64_t og_t_repair_iteration(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew,
+                              const uint32_t* vw, int k, const int32_t* labels, const uint64_t* loads, uint64_t cap,
+                              int32_t* first_choice, int32_t* bucket, uint8_t* eligible, int32_t* receiver) {
+    og_graph g = og_view(n, off, nbr, ew, vw);
+    og_ref R;
+    og_level_stats st;
+    memset(&st, 0, sizeof st);
+    og_ref_view(&R, &g, k, cap, (int32_t*)labels, (uint64_t*)loads, NULL, &st);
+    og_repair_ws W;
+    og_repair_ws_alloc(&W, n, k);
+    const int64_t ecount = og_repair_select(&R, &W);
+    memcpy(first_choice, R.targets, (size_t)n * sizeof(int32_t));
+    memcpy(bucket, W.bucket, (size_t)n * sizeof(int32_t));
+    memcpy(eligible, W.eligible, (size_t)n);
+    og_repair_reserve(&R, &W, ecount);
+    int64_t admitted = 0;
+    for (int64_t v = 0; v < n; ++v) {
+        receiver[v] = R.flags[v] ? R.targets[v] : -1;
+        admitted += R.flags[v] != 0;
+    }
+    og_repair_ws_free(&W);
+    og_ref_view_free(&R);
+    return admitted;
 }

@@ -106,6 +106,47 @@ int og_admission_batch(int64_t cnt, const int32_t* targets, const uint64_t* keys
                        const int32_t* vertices, const uint64_t* weights, const uint64_t* bases, uint64_t cap,
                        uint8_t* accepted);
 
+/* ---- single stages of the schedule (the oracle's own checks) ----------
+ * Each runs one stage on caller arrays through the functions the level loops
+ * use.  vw is required (no implicit unit vertex weights); ew may be NULL. */
+/* Seed matching with the given salt: roots[v] = cluster root; returns the cluster count. */
+int64_t og_t_seed_match(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew, const uint32_t* vw,
+                        uint64_t cap, uint32_t salt, int32_t* roots);
+/* Mover threshold of a growth round with `clusters` clusters over n vertices. */
+uint32_t og_t_mover_threshold(int64_t clusters, int64_t n);
+/* Roles and exact top-two proposals of one growth round (filter_roles = 1) or
+ * the unrestricted favourites of two-hop pairing (filter_roles = 0). */
+void og_t_proposals(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew, const uint32_t* vw,
+                    const int32_t* clusters, const uint64_t* cw, uint64_t cap, uint32_t seed, int level, int round,
+                    int filter_roles, uint32_t threshold, int need_regret, uint8_t* roles, int32_t* proposals,
+                    uint64_t* gains, uint64_t* regret, uint32_t* ties);
+/* Isolated packing on clusters/cw in place; returns 1 when it ran (>1 isolated vertex). */
+int og_t_pack_isolated(int64_t n, const int32_t* off, const uint32_t* vw, uint64_t cap, int32_t* clusters,
+                       uint64_t* cw);
+/* Two-hop pairing on clusters/cw in place with a pair budget. */
+void og_t_two_hop(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew, const uint32_t* vw,
+                  int32_t* clusters, uint64_t* cw, uint64_t cap, uint32_t seed, int level, int64_t budget);
+/* One LP proposal sweep; locks in/out; prices NULL = off; returns the candidate count. */
+int64_t og_t_lp_propose(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew, const uint32_t* vw,
+                        int k, const int32_t* labels, int32_t* locks, const uint64_t* loads, uint64_t cap,
+                        const double* prices, double filter, int32_t* targets, int64_t* gains);
+/* Afterburner over the candidates (targets >= 0); keep[v] = 1 for kept moves; returns their count. */
+int64_t og_t_afterburner(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew, const uint32_t* vw,
+                         int k, const int32_t* labels, const uint64_t* loads, uint64_t cap, const double* prices,
+                         const int32_t* targets, const int64_t* gains, uint8_t* keep);
+/* Repair gain bucket of a gain per unit of relief. */
+int og_t_gain_bucket(double gain);
+/* Repair destination of vertex v (-1 none), with A(v, dest) and A(v, own). */
+int og_t_repair_destination(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew,
+                            const uint32_t* vw, int k, const int32_t* labels, const uint64_t* loads, uint64_t cap,
+                            int32_t v, uint64_t* best_conn, uint64_t* current);
+/* One repair iteration without applying it: first-choice destinations,
+ * donor buckets (-1 none), eligibility, and the admitted receiver (-1 not
+ * admitted); returns the number admitted. */
+int64_t og_t_repair_iteration(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew,
+                              const uint32_t* vw, int k, const int32_t* labels, const uint64_t* loads, uint64_t cap,
+                              int32_t* first_choice, int32_t* bucket, uint8_t* eligible, int32_t* receiver);
+
 const char* og_last_error(void);
 
 #ifdef __cplusplus

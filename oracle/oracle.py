@@ -204,3 +204,150 @@ def admission_batch(targets, keys, ties, vertices, weights, bases, cap):
     out = np.zeros(max(n, 1), np.uint8)
     lib().og_admission_batch(n, _p(t), _p(k), _p(ti), _p(v), _p(w), _p(b), cap, _p(out))
     return out[:n].astype(bool)
+
+
+# ------------------------------------------------------------ single stages
+_STAGES_BOUND = False
+
+
+def _bind_stages():
+    global _STAGES_BOUND
+    if _STAGES_BOUND:
+        return
+    L = lib()
+    vp, i64, u64, u32, ci, dbl = (ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32,
+                                  ctypes.c_int, ctypes.c_double)
+    L.og_t_seed_match.restype = i64
+    L.og_t_seed_match.argtypes = [i64, vp, vp, vp, vp, u64, u32, vp]
+    L.og_t_mover_threshold.restype = u32
+    L.og_t_mover_threshold.argtypes = [i64, i64]
+    L.og_t_proposals.argtypes = [i64, vp, vp, vp, vp, vp, vp, u64, u32, ci, ci, ci, u32, ci, vp, vp, vp, vp, vp]
+    L.og_t_pack_isolated.argtypes = [i64, vp, vp, u64, vp, vp]
+    L.og_t_two_hop.argtypes = [i64, vp, vp, vp, vp, vp, vp, u64, u32, ci, i64]
+    L.og_t_lp_propose.restype = i64
+    L.og_t_lp_propose.argtypes = [i64, vp, vp, vp, vp, ci, vp, vp, vp, u64, vp, dbl, vp, vp]
+    L.og_t_afterburner.restype = i64
+    L.og_t_afterburner.argtypes = [i64, vp, vp, vp, vp, ci, vp, vp, u64, vp, vp, vp, vp]
+    L.og_t_gain_bucket.argtypes = [dbl]
+    L.og_t_repair_destination.argtypes = [i64, vp, vp, vp, vp, ci, vp, vp, u64, ctypes.c_int32, vp, vp]
+    L.og_t_repair_iteration.restype = i64
+    L.og_t_repair_iteration.argtypes = [i64, vp, vp, vp, vp, ci, vp, vp, u64, vp, vp, vp, vp]
+    _STAGES_BOUND = True
+
+
+def _graph_arrays(off, nbr, ew, vw):
+    off = np.ascontiguousarray(off, np.int32)
+    nbr = np.ascontiguousarray(nbr if nbr is not None else np.zeros(0, np.int32), np.int32)
+    ew = None if ew is None else np.ascontiguousarray(ew, np.uint32)
+    n = len(off) - 1
+    vw = np.ones(n, np.uint32) if vw is None else np.ascontiguousarray(vw, np.uint32)
+    return n, off, nbr, ew, vw
+
+
+def seed_match(off, nbr, ew, vw, cap, salt):
+    """Seed matching: (cluster root per vertex, cluster count)."""
+    _bind_stages()
+    n, off, nbr, ew, vw = _graph_arrays(off, nbr, ew, vw)
+    roots = np.zeros(n, np.int32)
+    count = lib().og_t_seed_match(n, _p(off), _p(nbr), _p(ew), _p(vw), cap, salt, _p(roots))
+    return roots, int(count)
+
+
+def mover_threshold(clusters, n):
+    _bind_stages()
+    return int(lib().og_t_mover_threshold(clusters, n))
+
+
+def proposals(off, nbr, ew, vw, clusters, cw, cap, seed, level, round_, filter_roles, threshold, need_regret):
+    """Roles and top-two proposals: dict of roles, proposals, gains, regret, ties."""
+    _bind_stages()
+    n, off, nbr, ew, vw = _graph_arrays(off, nbr, ew, vw)
+    clusters = np.ascontiguousarray(clusters, np.int32)
+    cw = np.ascontiguousarray(cw, np.uint64)
+    out = dict(roles=np.zeros(n, np.uint8), proposals=np.zeros(n, np.int32), gains=np.zeros(n, np.uint64),
+               regret=np.zeros(n, np.uint64), ties=np.zeros(n, np.uint32))
+    lib().og_t_proposals(n, _p(off), _p(nbr), _p(ew), _p(vw), _p(clusters), _p(cw), cap, seed, level, round_,
+                         filter_roles, threshold, need_regret, _p(out["roles"]), _p(out["proposals"]),
+                         _p(out["gains"]), _p(out["regret"]), _p(out["ties"]))
+    return out
+
+
+def pack_isolated(off, vw, cap, clusters, cw):
+    """Isolated packing in place on copies: (clusters, cw, ran)."""
+    _bind_stages()
+    n, off, _, _, vw = _graph_arrays(off, None, None, vw)
+    clusters = np.array(clusters, np.int32, copy=True)
+    cw = np.array(cw, np.uint64, copy=True)
+    ran = lib().og_t_pack_isolated(n, _p(off), _p(vw), cap, _p(clusters), _p(cw))
+    return clusters, cw, bool(ran)
+
+
+def two_hop(off, nbr, ew, vw, clusters, cw, cap, seed, level, budget):
+    _bind_stages()
+    n, off, nbr, ew, vw = _graph_arrays(off, nbr, ew, vw)
+    clusters = np.array(clusters, np.int32, copy=True)
+    cw = np.array(cw, np.uint64, copy=True)
+    lib().og_t_two_hop(n, _p(off), _p(nbr), _p(ew), _p(vw), _p(clusters), _p(cw), cap, seed, level, budget)
+    return clusters, cw
+
+
+def lp_propose(off, nbr, ew, vw, k, labels, locks, loads, cap, prices, filter_):
+    """One LP proposal sweep: (targets, gains, locks after, candidate count)."""
+    _bind_stages()
+    n, off, nbr, ew, vw = _graph_arrays(off, nbr, ew, vw)
+    labels = np.ascontiguousarray(labels, np.int32)
+    locks = np.array(locks, np.int32, copy=True)
+    loads = np.ascontiguousarray(loads, np.uint64)
+    prices = None if prices is None else np.ascontiguousarray(prices, np.float64)
+    targets = np.zeros(n, np.int32)
+    gains = np.zeros(n, np.int64)
+    c = lib().og_t_lp_propose(n, _p(off), _p(nbr), _p(ew), _p(vw), k, _p(labels), _p(locks), _p(loads), cap,
+                              _p(prices), filter_, _p(targets), _p(gains))
+    return targets, gains, locks, int(c)
+
+
+def afterburner(off, nbr, ew, vw, k, labels, loads, cap, prices, targets, gains):
+    """Afterburner over candidates (targets >= 0): keep flags."""
+    _bind_stages()
+    n, off, nbr, ew, vw = _graph_arrays(off, nbr, ew, vw)
+    labels = np.ascontiguousarray(labels, np.int32)
+    loads = np.ascontiguousarray(loads, np.uint64)
+    prices = None if prices is None else np.ascontiguousarray(prices, np.float64)
+    targets = np.ascontiguousarray(targets, np.int32)
+    gains = np.ascontiguousarray(gains, np.int64)
+    keep = np.zeros(n, np.uint8)
+    lib().og_t_afterburner(n, _p(off), _p(nbr), _p(ew), _p(vw), k, _p(labels), _p(loads), cap, _p(prices),
+                           _p(targets), _p(gains), _p(keep))
+    return keep.astype(bool)
+
+
+def gain_bucket(gain):
+    _bind_stages()
+    return int(lib().og_t_gain_bucket(float(gain)))
+
+
+def repair_destination(off, nbr, ew, vw, k, labels, loads, cap, v):
+    """(destination or -1, A(v, destination), A(v, own part))."""
+    _bind_stages()
+    n, off, nbr, ew, vw = _graph_arrays(off, nbr, ew, vw)
+    labels = np.ascontiguousarray(labels, np.int32)
+    loads = np.ascontiguousarray(loads, np.uint64)
+    b = ctypes.c_uint64(); c = ctypes.c_uint64()
+    d = lib().og_t_repair_destination(n, _p(off), _p(nbr), _p(ew), _p(vw), k, _p(labels), _p(loads), cap, v,
+                                      ctypes.byref(b), ctypes.byref(c))
+    return int(d), b.value, c.value
+
+
+def repair_iteration(off, nbr, ew, vw, k, labels, loads, cap):
+    """One repair iteration, not applied: dict of first_choice, bucket, eligible, receiver."""
+    _bind_stages()
+    n, off, nbr, ew, vw = _graph_arrays(off, nbr, ew, vw)
+    labels = np.ascontiguousarray(labels, np.int32)
+    loads = np.ascontiguousarray(loads, np.uint64)
+    out = dict(first_choice=np.zeros(n, np.int32), bucket=np.zeros(n, np.int32), eligible=np.zeros(n, np.uint8),
+               receiver=np.zeros(n, np.int32))
+    lib().og_t_repair_iteration(n, _p(off), _p(nbr), _p(ew), _p(vw), k, _p(labels), _p(loads), cap,
+                                _p(out["first_choice"]), _p(out["bucket"]), _p(out["eligible"]),
+                                _p(out["receiver"]))
+    out["eligible"] = out["eligible"].astype(bool)
+    return out

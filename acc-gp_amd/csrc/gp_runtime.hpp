@@ -74,7 +74,10 @@ public:
         std::lock_guard<std::mutex> lock(mutex_);
         trim_locked();
     }
-    std::size_t reserved_bytes() const { return total_; }
+    std::size_t reserved_bytes() const {
+        std::lock_guard<std::mutex> lock(mutex_);
+        return total_;
+    }
 
 private:
     void trim_locked() {
@@ -96,7 +99,7 @@ private:
         total_ = 0;
     }
     hipStream_t stream_;
-    std::mutex mutex_;
+    mutable std::mutex mutex_;
     std::map<std::size_t, std::vector<void*>> free_;
     std::unordered_map<void*, std::size_t> live_;
     std::size_t total_ = 0;

@@ -18,6 +18,7 @@
 #include <numeric>
 #include <string>
 #include <thread>
+#include <unordered_set>
 #include <vector>
 
 namespace {
@@ -47,6 +48,8 @@ struct gp_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<Job>> queue;
+    // Tickets handed out and not yet waited for (released by destroy).
+    std::unordered_set<std::shared_ptr<Job>*> tickets;
     std::uint64_t in_flight = 0;
     int running = 0;
     bool stopping = false;
@@ -145,22 +148,28 @@ int gp_server_submit(gp_server* s, gp_request* r, gp_ticket** ticket) {
     job->bytes = footprint_of(r);
     job->submitted = std::chrono::steady_clock::now();
     r->status = -1;
+    auto* handle = new std::shared_ptr<Job>(job);
     {
         std::lock_guard<std::mutex> g(s->mu);
-        if (s->init_failures.load() == static_cast<int>(s->workers.size())) return GP_ERR_HIP;
+        if (s->init_failures.load() == static_cast<int>(s->workers.size())) {
+            delete handle;
+            return GP_ERR_HIP;
+        }
         s->queue.push_back(job);
+        s->tickets.insert(handle);
     }
     s->cv_work.notify_one();
-    *ticket = reinterpret_cast<gp_ticket*>(new std::shared_ptr<Job>(job));
+    *ticket = reinterpret_cast<gp_ticket*>(handle);
     return GP_OK;
 }
 
 int gp_server_wait(gp_server* s, gp_ticket* ticket) {
     if (!s || !ticket) return GP_ERR_INVALID;
     auto* jp = reinterpret_cast<std::shared_ptr<Job>*>(ticket);
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (!s->tickets.erase(jp)) return GP_ERR_INVALID;
     std::shared_ptr<Job> job = *jp;
     delete jp;
-    std::unique_lock<std::mutex> lk(s->mu);
     s->cv_done.wait(lk, [&] {
         return job->done || s->init_failures.load() == static_cast<int>(s->workers.size());
     });
@@ -176,6 +185,9 @@ void gp_server_destroy(gp_server* s) {
     }
     s->cv_work.notify_all();
     for (auto& t : s->workers) t.join();
+    // Queued requests have run; tickets never waited for are released here.
+    for (auto* jp : s->tickets) delete jp;
+    s->tickets.clear();
     delete s;
 }
 

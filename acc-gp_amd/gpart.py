@@ -420,6 +420,20 @@ class Server:
         return dict(_stats_dict(req), queue_seconds=req.queue_seconds, device_seconds=req.device_seconds)
 
     def close(self):
+        """Finishes every queued request, stops the workers and releases
+        tickets not waited for.  The request buffers stay referenced until
+        the workers have stopped."""
         if self.h:
             lib().gp_server_destroy(self.h)
             self.h = None
+        self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.close()

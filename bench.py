@@ -146,56 +146,22 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def batch_mix(gp, count, seed=5):
-    """Mixed-size workload: 1/3 grids (side 32-1024), 1/3 R-MAT (scale 10-18,
-    edge factor 8-16), 1/3 random geometric 2-D graphs (2^10-2^20 points,
-    mean degree 8); k drawn from {4, 8, 16, 32, 64}; eps 1.03."""
-    rng = np.random.default_rng(seed)
-    mix = []
-    for i in range(count):
-        k = int(rng.choice([4, 8, 16, 32, 64]))
-        kind = i % 3
-        if kind == 0:
-            side = int(rng.integers(32, 1025))
-            off, nbr = gp.generate_grid(side, side)
-        elif kind == 1:
-            off, nbr = gp.generate_rmat(int(rng.integers(10, 19)), int(rng.integers(8, 17)), 0.57, 0.19, 0.19,
-                                        seed=int(rng.integers(1, 1 << 30)))
-        else:
-            n = int(round(2.0 ** rng.uniform(10, 20)))
-            off, nbr = gp.generate_rgg2d(n, (8.0 / (np.pi * n)) ** 0.5, seed=int(rng.integers(1, 1 << 30)))
-        if feasible(gp, len(off) - 1, k):
-            mix.append((off, nbr, k))
-    return mix
-
-
-def feasible(gp, n, k):
-    """The integer capacity floor(1.03 n / k) must hold ceil(n / k) (the
-    reference rejects the request otherwise)."""
-    if n < 2 * k:
-        return False
-    try:
-        gp.capacity(n, k, 1.03)
-        return True
-    except RuntimeError:
-        return False
-
-
 def batched_leg(gp, ctx, count, workers):
     """The same mix partitioned by the plain per-graph loop (one context,
     upload + partition + download per graph) and by the batched form
     (concurrent requests); results must be identical."""
+    from tests import workloads
     t = time.time()
-    mix = batch_mix(gp, count)
-    edges = sum(len(nbr) // 2 for _, nbr, _ in mix)
+    mix = workloads.batch_mix(gp, count)
+    edges = sum(len(nbr) // 2 for _, _, nbr, _ in mix)
     print("bench: batched mix of %d graphs, %d undirected edges, generated in %.1fs" % (len(mix), edges, time.time() - t),
           file=sys.stderr, flush=True)
     # Warm both paths once on a small graph.
-    small = [gp.BatchItem(mix[0][0], mix[0][1], mix[0][2])]
+    small = [gp.BatchItem(mix[0][1], mix[0][2], mix[0][3])]
     gp.partition_batch(small, workers=workers)
     plain_labels, plain_cuts, plain_lat = [], [], []
     t0 = time.perf_counter()
-    for off, nbr, k in mix:
+    for _, off, nbr, k in mix:
         s0 = time.perf_counter()
         g = gp.Graph(ctx, off, nbr)
         lab, st = g.partition(k)
@@ -204,7 +170,7 @@ def batched_leg(gp, ctx, count, workers):
         plain_labels.append(lab)
         plain_cuts.append(st["final_cut"])
     plain_s = time.perf_counter() - t0
-    items = [gp.BatchItem(off, nbr, k) for off, nbr, k in mix]
+    items = [gp.BatchItem(off, nbr, k) for _, off, nbr, k in mix]
     t0 = time.perf_counter()
     stats = gp.partition_batch(items, workers=workers)
     batch_s = time.perf_counter() - t0
@@ -224,51 +190,17 @@ def batched_leg(gp, ctx, count, workers):
             "plain_p99_request_ms": 1000.0 * float(np.percentile(plain_lat, 99))}, mix, items
 
 
-def perturb(off, nbr, frac, rng):
-    """Applies frac*m/2 random edge deletions and as many insertions."""
-    n = len(off) - 1
-    src = np.repeat(np.arange(n, dtype=np.int64), np.diff(off))
-    keep = src < nbr
-    a, b = src[keep], nbr[keep].astype(np.int64)
-    k = max(1, int(frac * len(a)))
-    drop = rng.choice(len(a), size=k, replace=False)
-    mask = np.ones(len(a), bool)
-    mask[drop] = False
-    a, b = a[mask], b[mask]
-    u, v = rng.integers(0, n, k), rng.integers(0, n, k)
-    ok = u != v
-    a = np.concatenate([a, np.minimum(u, v)[ok]])
-    b = np.concatenate([b, np.maximum(u, v)[ok]])
-    key = np.unique(a * n + b)
-    a, b = key // n, key % n
-    s2 = np.concatenate([a, b])
-    d2 = np.concatenate([b, a])
-    order = np.lexsort((d2, s2))
-    s2, d2 = s2[order], d2[order]
-    off2 = np.zeros(n + 1, np.int64)
-    np.add.at(off2, s2 + 1, 1)
-    return np.cumsum(off2).astype(np.int32), d2.astype(np.int32)
-
-
 def serving_leg(gp, ctx, mix, done_items, workers, count, rate, seed=6):
     """Poisson arrivals at `rate` requests/s (70% of the batched capacity):
     cold requests from the mix with k in [2, 128], and 10% repartitions of
     resident graphs with 1% edge insertions and deletions, warm-started from
     their labels.  Latency = queueing + upload + partition + download."""
-    rng = np.random.default_rng(seed)
+    from tests import workloads
     reqs = []
-    for i in range(count):
-        j = int(rng.integers(0, len(mix)))
-        off, nbr, k0 = mix[j]
-        if i % 10 == 9:
-            off2, nbr2 = perturb(off, nbr, 0.01, rng)
-            reqs.append(("warm", gp.BatchItem(off2, nbr2, k0, warm=done_items[j].labels), (off2, nbr2, k0)))
-        else:
-            k = int(rng.integers(2, 129))
-            while k > 2 and not feasible(gp, len(off) - 1, k):
-                k -= 1
-            reqs.append(("cold", gp.BatchItem(off, nbr, k), None))
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    for kind, j, off, nbr, k in workloads.serving_requests(gp, mix, count, seed):
+        warm = done_items[j].labels if kind == "warm" else None
+        reqs.append((kind, gp.BatchItem(off, nbr, k, warm=warm), (off, nbr, k)))
+    arrivals = workloads.poisson_arrivals(count, rate, seed)
     srv = gp.Server(workers=workers)
     try:
         tickets = []

@@ -203,9 +203,12 @@ typedef struct gp_ticket gp_ticket;
 /* workers: concurrent requests (contexts); hbm_budget_bytes 0 = 64 GiB. */
 int gp_server_create(int device, int workers, uint64_t hbm_budget_bytes, gp_server** out);
 int gp_server_submit(gp_server* s, gp_request* r, gp_ticket** ticket);
-/* Blocks until the request is done; releases the ticket; returns its status. */
+/* Blocks until the request is done; releases the ticket; returns its status
+ * (GP_ERR_INVALID for a ticket this server does not hold). */
 int gp_server_wait(gp_server* s, gp_ticket* ticket);
-/* Finishes queued requests, then stops the workers. */
+/* Finishes queued requests, then stops the workers and releases the tickets
+ * not waited for (their requests have completed; the caller's request
+ * buffers must stay valid until this returns). */
 void gp_server_destroy(gp_server* s);
 /* Whole batch through a temporary server, largest request first. */
 int gp_partition_batch(int device, gp_request* items, int64_t count, int workers, uint64_t hbm_budget_bytes);

@@ -65,3 +65,17 @@ def hub_ring(n, extra, seed):
     off, nbr, ew = csr_from_edges(n, sorted(edges))
     vw = np.exp(rng.uniform(0.0, np.log(1000.0), n)).astype(np.uint32).clip(1, 1000)
     return off, nbr, ew, vw
+
+
+def heavy_weights(n, m, seed):
+    """Random graph whose edge weights reach 2^30: about 30% of edges weigh
+    2^30 + 1000 h (h < 300), the rest 1 + h, with h a hash of the edge.  Rows
+    with several heavy edges have incident weights above 2^32, and merged
+    rows grow further after contraction.  Vertex weights 1..4."""
+    off, nbr, _ = random_graph(n, m, seed=seed)
+    src = np.repeat(np.arange(n), np.diff(off))
+    lo, hi = np.minimum(src, nbr).astype(np.uint64), np.maximum(src, nbr).astype(np.uint64)
+    h = (lo * np.uint64(2654435761) + hi * np.uint64(40503)) % np.uint64(1000)
+    ew = np.where(h < 300, np.uint64(1 << 30) + h * np.uint64(1000), np.uint64(1) + h).astype(np.uint32)
+    vw = np.random.default_rng(seed + 1).integers(1, 5, n).astype(np.uint32)
+    return off, nbr, ew, vw

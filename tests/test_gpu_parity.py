@@ -41,6 +41,14 @@ def _cases(gp):
     cases.append(("rmat16_k64_heavy_rows", off, nbr, None, None, 64))
     off, nbr, ew, vw = graphs.hub_ring(12000, 6000, seed=4)
     cases.append(("hub12000_k8_skewed_weights", off, nbr, ew, vw, 8))
+    # A hub row of 19,999 entries: above the 12,288-entry limit of the
+    # one-block LDS hash, so its affinity goes through the global records.
+    off, nbr, ew, vw = graphs.hub_ring(20000, 10000, seed=6)
+    cases.append(("hub20000_k8_row_beyond_block_hash", off, nbr, ew, vw, 8))
+    # Row weights above 2^32 at level 0 and after contraction: 64-bit part
+    # tables, the sort-based affinity and the plain record path.
+    off, nbr, ew, vw = graphs.heavy_weights(3000, 9000, seed=12)
+    cases.append(("heavy3000_k4_row_weight_beyond_u32", off, nbr, ew, vw, 4))
     return cases
 
 
@@ -120,3 +128,18 @@ def test_refine_partition_single_level(gp, og, ctx):
     assert np.array_equal(lab, olab)
     assert st["final_cut"] == ost["final_cut"]
     g.close()
+
+
+def test_coarse_edge_weight_overflow_is_an_error_on_both(gp, og, ctx):
+    # At k=16 two heavy edges merge into one coarse edge above 2^32 - 1: the
+    # oracle and the device both refuse (coarsen.cu throws likewise).
+    off, nbr, ew, vw = graphs.heavy_weights(3000, 9000, seed=12)
+    _say("case heavy3000_k16: overflow")
+    with pytest.raises(RuntimeError, match="coarse edge weight exceeds uint32"):
+        og.Hierarchy(off, nbr, ew, vw, og.coarsen_opts(16))
+    g = gp.Graph(ctx, off, nbr, ew, vw)
+    try:
+        with pytest.raises(RuntimeError, match="coarse edge weight exceeds uint32"):
+            gp.Hierarchy(ctx, g, gp.coarsen_options(16))
+    finally:
+        g.close()

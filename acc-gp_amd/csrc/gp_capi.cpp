@@ -3,6 +3,7 @@
 
 #include "gp_coarsen.hpp"
 #include "gp_host.hpp"
+#include "gp_metrics.hpp"
 #include "gp_refine.hpp"
 
 #include <chrono>
@@ -364,6 +365,16 @@ int gp_result_level_stats(const gp_result* r, int index, gp_level_stats* out) {
     return GP_OK;
 }
 void gp_result_destroy(gp_result* r) { delete r; }
+const int32_t* gp_result_device_labels(const gp_result* r) { return r ? r->r.labels.get() : nullptr; }
+
+int gp_edge_cut(gp_context* ctx, const gp_graph* g, const int32_t* labels, int parts, uint64_t* cut,
+                uint64_t* loads) {
+    return guarded([&] {
+        if (!ctx || !g || !g->view || !cut || !loads) throw std::invalid_argument("null argument");
+        ctx->ctx.activate();
+        gp::edge_cut(ctx->ctx, *g->view, labels, parts, cut, loads);
+    });
+}
 
 int gp_refine_partition(gp_context* ctx, const gp_graph* g, int32_t* labels, const gp_refine_options* opts,
                         gp_level_stats* stats) {

@@ -70,7 +70,7 @@ EXPORTS = [
     "gp_hierarchy_map_download", "gp_hierarchy_level_info", "gp_hierarchy_destroy",
     "gp_initial_partition_host", "gp_initial_partition", "gp_default_refine_options", "gp_uncoarsen",
     "gp_result_labels", "gp_result_cut", "gp_result_loads", "gp_result_levels", "gp_result_level_stats",
-    "gp_result_destroy", "gp_refine_partition", "gp_partition", "gp_generate_grid", "gp_generate_rmat",
+    "gp_result_destroy", "gp_result_device_labels", "gp_edge_cut", "gp_refine_partition", "gp_partition", "gp_generate_grid", "gp_generate_rmat",
     "gp_generate_rgg2d", "gp_generate_knn3d", "gp_free_host", "gp_load_csr", "gp_save_csr",
     "gp_profile_enable", "gp_profile_query", "gp_profile_reset", "gp_server_create", "gp_server_submit",
     "gp_server_wait", "gp_server_destroy", "gp_partition_batch",
@@ -116,6 +116,9 @@ def lib():
         L.gp_result_levels.argtypes = [vp]
         L.gp_result_level_stats.argtypes = [vp, i32, vp]
         L.gp_result_destroy.argtypes = [vp]
+        L.gp_result_device_labels.argtypes = [vp]
+        L.gp_result_device_labels.restype = vp
+        L.gp_edge_cut.argtypes = [vp, vp, vp, i32, vp, vp]
         L.gp_refine_partition.argtypes = [vp, vp, vp, vp, vp]
         L.gp_partition.argtypes = [vp, vp, i32, dbl, u32, vp, vp]
         L.gp_generate_grid.argtypes = [i64, i64, vp, vp, vp, vp]
@@ -291,11 +294,22 @@ class Graph:
         check(lib().gp_partition(self.ctx.h, self.h, parts, imbalance, seed, _p(labels), ctypes.byref(st)))
         return labels, dict((f, getattr(st, f)) for f, _ in PartitionStats._fields_)
 
+    def edge_cut(self, device_labels, parts):
+        """(cut, loads) of labels already in device memory (gp_edge_cut)."""
+        return _edge_cut(self.ctx, self.h, device_labels, parts)
+
     def refine(self, labels, ropts):
         out = np.array(labels, np.int32, copy=True)
         st = LevelStats()
         check(lib().gp_refine_partition(self.ctx.h, self.h, _p(out), ctypes.byref(ropts), ctypes.byref(st)))
         return out, dict((f, getattr(st, f)) for f, _ in LevelStats._fields_)
+
+
+def _edge_cut(ctx, g_handle, device_labels, parts):
+    cut = ctypes.c_uint64()
+    loads = np.zeros(parts, np.uint64)
+    check(lib().gp_edge_cut(ctx.h, g_handle, ctypes.c_void_p(device_labels), parts, ctypes.byref(cut), _p(loads)))
+    return cut.value, loads
 
 
 def download_graph(ctx, g_handle):
@@ -340,11 +354,17 @@ class Hierarchy:
         check(lib().gp_hierarchy_level_info(self.h, level, ctypes.byref(md), ctypes.byref(hh), ctypes.byref(ib)))
         return dict(max_degree=md.value, has_heavy=hh.value, incident_bound=ib.value)
 
-    def uncoarsen(self, coarsest_labels, ropts):
+    def uncoarsen(self, coarsest_labels, ropts, evaluate_parts=None):
+        """Refined labels, cut, loads and per-level stats.  With
+        evaluate_parts, also the (cut, loads) that gp_edge_cut counts on the
+        result's device labels with that many parts (appended)."""
         labels = np.ascontiguousarray(coarsest_labels, np.int32)
         r = ctypes.c_void_p()
         check(lib().gp_uncoarsen(self.ctx.h, self.h, _p(labels), ctypes.byref(ropts), ctypes.byref(r)))
         try:
+            evaluated = None
+            if evaluate_parts is not None:
+                evaluated = _edge_cut(self.ctx, self.graph.h, lib().gp_result_device_labels(r), evaluate_parts)
             out = np.zeros(self.graph.n, np.int32)
             check(lib().gp_result_labels(self.ctx.h, r, _p(out)))
             loads = np.zeros(ropts.parts, np.uint64)
@@ -354,6 +374,8 @@ class Hierarchy:
                 st = LevelStats()
                 check(lib().gp_result_level_stats(r, i, ctypes.byref(st)))
                 stats.append(dict((f, getattr(st, f)) for f, _ in LevelStats._fields_))
+            if evaluate_parts is not None:
+                return out, lib().gp_result_cut(r), loads, stats, evaluated
             return out, lib().gp_result_cut(r), loads, stats
         finally:
             lib().gp_result_destroy(r)

@@ -145,6 +145,16 @@ int gp_result_loads(const gp_result* r, uint64_t* loads);
 int gp_result_levels(const gp_result* r);
 int gp_result_level_stats(const gp_result* r, int index, gp_level_stats* out);
 void gp_result_destroy(gp_result* r);
+/* device_partition of DeviceUncoarsenResult (include/refine.hpp:74-82): the
+ * result's labels in device memory, valid until gp_result_destroy. */
+const int32_t* gp_result_device_labels(const gp_result* r);
+/* Edge cut and part loads of device-resident labels (host_cut,
+ * include/graph.hpp:67-69; the device recounts of src/refine.cu:63-98):
+ * labels is a device pointer to n entries in [0, parts); cut and loads
+ * (parts words) are host outputs.  GP_ERR_INVALID for labels out of range.
+ * Partials come from the context arena; synchronizes the context stream. */
+int gp_edge_cut(gp_context* ctx, const gp_graph* g, const int32_t* labels, int parts, uint64_t* cut,
+                uint64_t* loads);
 /* refine_partition(graph, labels, options): one level, labels in/out (host). */
 int gp_refine_partition(gp_context* ctx, const gp_graph* g, int32_t* labels, const gp_refine_options* opts,
                         gp_level_stats* stats);

@@ -143,3 +143,29 @@ def test_coarse_edge_weight_overflow_is_an_error_on_both(gp, og, ctx):
             gp.Hierarchy(ctx, g, gp.coarsen_options(16))
     finally:
         g.close()
+
+
+def test_edge_cut_of_device_labels(gp, og, ctx):
+    # gp_edge_cut on the result's device labels counts the same cut and
+    # loads as the refinement and as a host recount; labels outside the
+    # part range are refused.
+    cases = [("grid64_k4", graphs.grid(64, 64) + (None, None), 4)]
+    off, nbr, ew, vw = gp.generate_knn3d(5000, 8, seed=3)
+    cases.append(("knn5000_k16_weighted", (off, nbr, ew, vw), 16))
+    off, nbr = gp.generate_rmat(14, 16, seed=2)
+    cases.append(("rmat14_k64", (off, nbr, None, None), 64))
+    for name, (off, nbr, ew, vw), k in cases:
+        _say("case %s: edge cut of device labels" % name)
+        g = gp.Graph(ctx, off, nbr, ew, vw)
+        h = gp.Hierarchy(ctx, g, gp.coarsen_options(k))
+        try:
+            coff, cnbr, cew, cvw = h.graph_at(h.levels - 1)
+            init = gp.initial_partition_host(coff, cnbr, cew, cvw, k)
+            lab, cut, loads, _, (ecut, eloads) = h.uncoarsen(init, gp.refine_options(k), evaluate_parts=k)
+            assert ecut == cut == og.cut(off, nbr, ew, lab), name
+            assert np.array_equal(eloads, loads), name
+            with pytest.raises(RuntimeError, match="labels outside"):
+                h.uncoarsen(init, gp.refine_options(k), evaluate_parts=2)
+        finally:
+            h.close()
+            g.close()

@@ -296,24 +296,42 @@ void gp_hierarchy_destroy(gp_hierarchy* h) { delete h; }
 
 int gp_initial_partition_host(int64_t n, const int32_t* offsets, const int32_t* neighbors,
                               const uint32_t* edge_weights, const uint32_t* vertex_weights, int parts,
-                              double imbalance, uint32_t seed, int32_t* labels) {
+                              double imbalance, uint32_t seed, int require_feasible, int32_t* labels) {
     if (!offsets || !labels) return fail(GP_ERR_INVALID, "null argument");
     return guarded([&] {
-        if (parts < 2 || parts > gp::kMaxParts || parts > n) throw std::invalid_argument("invalid initial partition options");
-        gp::initial_partition_host(n, offsets, neighbors, edge_weights, vertex_weights, parts, imbalance, seed, labels);
+        gp::initial_partition(n, offsets, neighbors, edge_weights, vertex_weights, parts, imbalance, seed,
+                              require_feasible != 0, labels);
     });
 }
 
 int gp_initial_partition(gp_context* ctx, const gp_graph* g, int parts, double imbalance, uint32_t seed,
-                         int32_t* labels) {
+                         int require_feasible, int32_t* labels) {
     if (!ctx || !g || !labels) return fail(GP_ERR_INVALID, "null argument");
     return guarded([&] {
         gp::HostGraph h = download_graph(ctx->ctx, *g->view);
-        if (parts < 2 || parts > gp::kMaxParts || parts > h.n) throw std::invalid_argument("invalid initial partition options");
-        gp::initial_partition_host(h.n, h.off.data(), h.nbr.data(), h.ew.empty() ? nullptr : h.ew.data(), h.vw.data(),
-                                   parts, imbalance, seed, labels);
+        gp::initial_partition(h.n, h.off.data(), h.nbr.data(), h.ew.empty() ? nullptr : h.ew.data(), h.vw.data(),
+                              parts, imbalance, seed, require_feasible != 0, labels);
     });
 }
+
+int gp_repair_capacity_host(int64_t n, const int32_t* offsets, const int32_t* neighbors,
+                            const uint32_t* edge_weights, const uint32_t* vertex_weights, int parts, uint64_t capacity,
+                            int32_t* labels, uint64_t* moved) {
+    if (!offsets || !labels || (offsets[n] > 0 && !neighbors)) return fail(GP_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (n <= 0 || parts < 2 || parts > gp::kMaxParts) throw std::invalid_argument("invalid repair options");
+        std::vector<std::uint64_t> loads(static_cast<std::size_t>(parts), 0);
+        for (int64_t v = 0; v < n; ++v) {
+            if (labels[v] < 0 || labels[v] >= parts) throw std::invalid_argument("labels outside [0, parts)");
+            loads[static_cast<std::size_t>(labels[v])] += vertex_weights ? vertex_weights[v] : 1u;
+        }
+        const std::size_t k = gp::repair_capacity(n, offsets, neighbors, edge_weights, vertex_weights, parts, capacity,
+                                                  labels, loads);
+        if (moved) *moved = k;
+    });
+}
+
+const char* gp_initial_partition_backend(void) { return gp::initial_partition_backend(); }
 
 void gp_default_refine_options(int parts, double imbalance, gp_refine_options* out) {
     if (!out) return;
@@ -415,8 +433,10 @@ int gp_partition(gp_context* ctx, gp_graph* g, int parts, double imbalance, uint
         gp::HostGraph hc = download_graph(c, cg);
         std::vector<std::int32_t> coarse_labels(static_cast<std::size_t>(cg.n));
         if (parts > cg.n) throw std::invalid_argument("more parts than coarsest vertices");
-        gp::initial_partition_host(hc.n, hc.off.data(), hc.nbr.data(), hc.ew.empty() ? nullptr : hc.ew.data(),
-                                   hc.vw.data(), parts, imbalance, seed, coarse_labels.data());
+        // Production defers coarse imbalance to the repair of the refinement
+        // (app/partition_main.cu:111-112: require_feasible = false).
+        gp::initial_partition(hc.n, hc.off.data(), hc.nbr.data(), hc.ew.empty() ? nullptr : hc.ew.data(),
+                              hc.vw.data(), parts, imbalance, seed, false, coarse_labels.data());
         gp::DBuf<std::int32_t> lab = c.buf<std::int32_t>(static_cast<std::size_t>(cg.n));
         c.upload(lab.get(), coarse_labels.data(), coarse_labels.size() * 4);
         const double t_initial = seconds_since(t1);

@@ -21,6 +21,31 @@ void initial_partition_host(std::int64_t n, const std::int32_t* off, const std::
                             const std::uint32_t* vw, int parts, double imbalance, std::uint32_t seed,
                             std::int32_t* labels);
 
+struct InitialReport {
+    const char* backend = "";
+    std::uint64_t vertex_gcd = 0, edge_gcd = 0;
+    std::uint64_t max_load = 0;   // largest part load before any repair
+    std::uint64_t repaired = 0;   // vertices moved by the strict repair
+    std::uint64_t edgecut = 0;    // exact cut of the returned labels
+    double seconds = 0.0;
+};
+
+// The initial-partition boundary of initial_partition_metis
+// (src/initial_partition_metis.cu:86-183): checks, exact GCD normalisation,
+// METIS when built with GP_WITH_METIS and the recursive bisection otherwise,
+// and with require_feasible the strict single-move repair.
+void initial_partition(std::int64_t n, const std::int32_t* off, const std::int32_t* nbr, const std::uint32_t* ew,
+                       const std::uint32_t* vw, int parts, double imbalance, std::uint32_t seed, bool require_feasible,
+                       std::int32_t* labels, InitialReport* report = nullptr);
+// repair_capacity (src/initial_partition_metis.cu:35-83): single moves until
+// no part exceeds capacity; loads in/out; returns the vertices moved; throws
+// when no feasible move is left.
+std::size_t repair_capacity(std::int64_t n, const std::int32_t* off, const std::int32_t* nbr, const std::uint32_t* ew,
+                            const std::uint32_t* vw, int parts, std::uint64_t capacity, std::int32_t* labels,
+                            std::vector<std::uint64_t>& loads);
+// "metis" or "bisection" (the build's initial partitioner).
+const char* initial_partition_backend();
+
 // Generators (rows sorted, symmetric, no self loops, no duplicates).
 HostGraph generate_grid(std::int64_t rows, std::int64_t cols);
 HostGraph generate_rmat(int scale, int edge_factor, double a, double b, double c, std::uint64_t seed);

@@ -49,7 +49,7 @@ int main(int argc, char** argv) {
         const auto load_end = std::chrono::steady_clock::now();
         std::cout << "vertices=" << n << " edge_entries=" << m << " device_count=1\n";
         std::cout << "verify=" << (verify ? 1 : 0) << '\n';
-        std::cout << "coarsen_mode=regret initial_partition_backend=host-recursive-bisection (not METIS)\n";
+        std::cout << "coarsen_mode=regret initial_partition_backend=" << gp_initial_partition_backend() << '\n';
         gp_context* ctx = nullptr;
         check(gp_context_create(0, &ctx));
         const auto prepare_start = std::chrono::steady_clock::now();

@@ -68,7 +68,8 @@ EXPORTS = [
     "gp_graph_create", "gp_graph_info", "gp_graph_download", "gp_graph_destroy", "gp_capacity",
     "gp_default_coarsen_options", "gp_coarsen", "gp_hierarchy_levels", "gp_hierarchy_level",
     "gp_hierarchy_map_download", "gp_hierarchy_level_info", "gp_hierarchy_destroy",
-    "gp_initial_partition_host", "gp_initial_partition", "gp_default_refine_options", "gp_uncoarsen",
+    "gp_initial_partition_host", "gp_initial_partition", "gp_repair_capacity_host", "gp_initial_partition_backend",
+    "gp_default_refine_options", "gp_uncoarsen",
     "gp_result_labels", "gp_result_cut", "gp_result_loads", "gp_result_levels", "gp_result_level_stats",
     "gp_result_destroy", "gp_result_device_labels", "gp_edge_cut", "gp_refine_partition", "gp_partition", "gp_generate_grid", "gp_generate_rmat",
     "gp_generate_rgg2d", "gp_generate_knn3d", "gp_free_host", "gp_load_csr", "gp_save_csr",
@@ -105,8 +106,10 @@ def lib():
         L.gp_hierarchy_map_download.argtypes = [vp, vp, i32, vp]
         L.gp_hierarchy_level_info.argtypes = [vp, i32, vp, vp, vp]
         L.gp_hierarchy_destroy.argtypes = [vp]
-        L.gp_initial_partition_host.argtypes = [i64, vp, vp, vp, vp, i32, dbl, u32, vp]
-        L.gp_initial_partition.argtypes = [vp, vp, i32, dbl, u32, vp]
+        L.gp_initial_partition_host.argtypes = [i64, vp, vp, vp, vp, i32, dbl, u32, i32, vp]
+        L.gp_initial_partition.argtypes = [vp, vp, i32, dbl, u32, i32, vp]
+        L.gp_repair_capacity_host.argtypes = [i64, vp, vp, vp, vp, i32, u64, vp, vp]
+        L.gp_initial_partition_backend.restype = ctypes.c_char_p
         L.gp_default_refine_options.argtypes = [i32, dbl, vp]
         L.gp_uncoarsen.argtypes = [vp, vp, vp, vp, pp]
         L.gp_result_labels.argtypes = [vp, vp, vp]
@@ -212,15 +215,34 @@ def save_csr(indptr, indices, off, nbr):
     check(lib().gp_save_csr(indptr.encode(), indices.encode(), len(off) - 1, len(nbr), _p(off), _p(nbr)))
 
 
-def initial_partition_host(off, nbr, ew, vw, parts, imbalance=1.03, seed=0):
+def initial_partition_host(off, nbr, ew, vw, parts, imbalance=1.03, seed=0, require_feasible=False):
+    """initial_partition_metis on a host CSR; require_feasible=False is the
+    production setting (app/partition_main.cu:111-112)."""
     off = np.ascontiguousarray(off, np.int32)
     nbr = np.ascontiguousarray(nbr, np.int32)
     ew = None if ew is None else np.ascontiguousarray(ew, np.uint32)
     vw = None if vw is None else np.ascontiguousarray(vw, np.uint32)
     out = np.zeros(len(off) - 1, np.int32)
     check(lib().gp_initial_partition_host(len(off) - 1, _p(off), _p(nbr), _p(ew), _p(vw), parts, imbalance, seed,
-                                          _p(out)))
+                                          int(require_feasible), _p(out)))
     return out
+
+
+def repair_capacity_host(off, nbr, ew, vw, parts, capacity, labels):
+    """Strict single-move capacity repair: (labels after, vertices moved)."""
+    off = np.ascontiguousarray(off, np.int32)
+    nbr = np.ascontiguousarray(nbr, np.int32)
+    ew = None if ew is None else np.ascontiguousarray(ew, np.uint32)
+    vw = None if vw is None else np.ascontiguousarray(vw, np.uint32)
+    out = np.array(labels, np.int32, copy=True)
+    moved = ctypes.c_uint64()
+    check(lib().gp_repair_capacity_host(len(off) - 1, _p(off), _p(nbr), _p(ew), _p(vw), parts, capacity, _p(out),
+                                        ctypes.byref(moved)))
+    return out, moved.value
+
+
+def initial_partition_backend():
+    return lib().gp_initial_partition_backend().decode()
 
 
 def coarsen_options(parts, seed=0, **overrides):

@@ -125,14 +125,31 @@ int gp_hierarchy_level_info(const gp_hierarchy* h, int level, uint64_t* max_degr
 void gp_hierarchy_destroy(gp_hierarchy* h);
 
 /* ---- initial partition (include/initial_partition.hpp:6-8) ------------ */
-/* Host partitioner of a host CSR (not METIS: recursive bisection). */
+/* initial_partition_metis(graph, parts, eps, seed, require_feasible) on a host
+ * CSR (src/initial_partition_metis.cu:86-183): positive vertex weights, none
+ * above the integer capacity, exact GCD normalisation of vertex and edge
+ * weights, self loops and zero-weight edges left out; METIS 5 when the
+ * library was built with it (gp_initial_partition_backend() == "metis"),
+ * the in-repo recursive bisection otherwise ("bisection", not METIS).  With
+ * require_feasible, the normalised weights must admit the capacity and the
+ * strict single-move repair of gp_repair_capacity_host follows; production
+ * (gp_partition) passes 0, as the reference CLI does. */
 int gp_initial_partition_host(int64_t n, const int32_t* offsets, const int32_t* neighbors,
                               const uint32_t* edge_weights, const uint32_t* vertex_weights, int parts,
-                              double imbalance, uint32_t seed, int32_t* labels);
-/* initial_partition_metis(graph, parts, eps, seed): downloads the device graph
- * and partitions it on the host; labels are host output. */
+                              double imbalance, uint32_t seed, int require_feasible, int32_t* labels);
+/* The same on a device graph (downloaded to the host first); labels are host output. */
 int gp_initial_partition(gp_context* ctx, const gp_graph* g, int parts, double imbalance, uint32_t seed,
-                         int32_t* labels);
+                         int require_feasible, int32_t* labels);
+/* repair_capacity (src/initial_partition_metis.cu:35-83): while a part is
+ * above capacity, move the vertex of an overloaded part to a part with room
+ * for all of it whose (A(v,p) - A(v,s)) / min(w, load_s - capacity) is
+ * largest (first in vertex, part order on ties).  labels in/out; moved gets
+ * the count; GP_ERR_RUNTIME when no feasible move is left. */
+int gp_repair_capacity_host(int64_t n, const int32_t* offsets, const int32_t* neighbors,
+                            const uint32_t* edge_weights, const uint32_t* vertex_weights, int parts, uint64_t capacity,
+                            int32_t* labels, uint64_t* moved);
+/* "metis" or "bisection". */
+const char* gp_initial_partition_backend(void);
 
 /* ---- uncoarsening / refinement (include/uncoarsen.hpp:17-20, refine.hpp:84-98) */
 void gp_default_refine_options(int parts, double imbalance, gp_refine_options* out);

@@ -189,3 +189,123 @@ def test_server_fails_cleanly_without_a_device(gp):
     off, nbr = gp.generate_grid(8, 8)
     with pytest.raises(RuntimeError):
         gp.partition_batch([gp.BatchItem(off, nbr, 2)], workers=2)
+
+
+# ------------------------------------------------ initial-partition boundary
+def _repair_brute(off, nbr, ew, vw, k, cap, lab):
+    """repair_capacity of src/initial_partition_metis.cu:35-83 in exact
+    rationals: while a part is above capacity, move the (vertex, part) of
+    largest (A(v,p) - A(v,s)) / min(w, load_s - cap), first in (v, p) order."""
+    from fractions import Fraction
+    n = len(off) - 1
+    lab = list(lab)
+    w = [1] * n if vw is None else [int(x) for x in vw]
+    loads = [0] * k
+    for v in range(n):
+        loads[lab[v]] += w[v]
+    moved = 0
+    while max(loads) > cap:
+        best, pick = None, None
+        for v in range(n):
+            s = lab[v]
+            if loads[s] <= cap:
+                continue
+            a = [0] * k
+            for e in range(off[v], off[v + 1]):
+                if nbr[e] != v:
+                    a[lab[nbr[e]]] += 1 if ew is None else int(ew[e])
+            for p in range(k):
+                if p == s or loads[p] > cap or w[v] > cap - loads[p]:
+                    continue
+                score = Fraction(a[p] - a[s], min(w[v], loads[s] - cap))
+                if best is None or score > best:
+                    best, pick = score, (v, p)
+        if pick is None:
+            raise RuntimeError("no feasible move")
+        v, p = pick
+        loads[lab[v]] -= w[v]
+        loads[p] += w[v]
+        lab[v] = p
+        moved += 1
+    return lab, moved
+
+
+def test_repair_capacity_known_answer(gp):
+    # Path 0-1-2-3, part 0 = {0,1,2} over capacity 2: vertex 2 (gain 0) beats
+    # vertex 0 (gain -1) and vertex 1 (gain -2).
+    off, nbr, _ = graphs.csr_from_edges(4, [(0, 1), (1, 2), (2, 3)])
+    lab, moved = gp.repair_capacity_host(off, nbr, None, None, 2, 2, [0, 0, 0, 1])
+    assert lab.tolist() == [0, 0, 1, 1] and moved == 1
+    # Already feasible: nothing moves.
+    lab, moved = gp.repair_capacity_host(off, nbr, None, None, 2, 2, [0, 0, 1, 1])
+    assert lab.tolist() == [0, 0, 1, 1] and moved == 0
+    # No part has room: the strict repair fails.
+    with pytest.raises(RuntimeError, match="strict capacity"):
+        gp.repair_capacity_host(off, nbr, None, np.array([3, 3, 3, 3], np.uint32), 2, 4, [0, 0, 0, 1])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_repair_capacity_brute_force(gp, seed):
+    rng = np.random.default_rng(seed)
+    n, k = 40, 4
+    off, nbr, ew = graphs.random_graph(n, 90, seed=seed + 100, max_weight=5)
+    vw = rng.integers(1, 5, n).astype(np.uint32)
+    lab = rng.integers(0, k, n).astype(np.int32)
+    lab[rng.random(n) < 0.4] = 1
+    cap = int(np.ceil(vw.sum() * 1.1 / k))
+    got, moved = gp.repair_capacity_host(off, nbr, ew, vw, k, cap, lab)
+    want, wmoved = _repair_brute(off, nbr, ew, vw, k, cap, lab)
+    assert got.tolist() == want and moved == wmoved
+    assert np.bincount(got, weights=vw, minlength=k).max() <= cap
+
+
+def test_initial_partition_gcd_normalisation(gp):
+    # Common factors of vertex and edge weights are divided out exactly, so
+    # scaling every vertex weight by 3 and every edge weight by 5 leaves the
+    # labels unchanged; self loops and zero-weight edges are left out.
+    off, nbr, ew = graphs.random_graph(500, 2000, seed=9, max_weight=4)
+    vw = np.random.default_rng(2).integers(1, 5, 500).astype(np.uint32)
+    base = gp.initial_partition_host(off, nbr, ew, vw, 8)
+    scaled = gp.initial_partition_host(off, nbr, ew * 5, vw * 3, 8)
+    assert np.array_equal(base, scaled)
+    # add a self loop on every vertex and a zero-weight edge
+    rows = [[(int(nbr[e]), int(ew[e])) for e in range(off[v], off[v + 1])] + [(v, 7)] for v in range(500)]
+    rows[0].append((499, 0))
+    rows[499].append((0, 0))
+    off2, nbr2, ew2 = [0], [], []
+    for r in rows:
+        for u, w in sorted(r):
+            nbr2.append(u)
+            ew2.append(w)
+        off2.append(len(nbr2))
+    looped = gp.initial_partition_host(np.array(off2, np.int32), np.array(nbr2, np.int32),
+                                       np.array(ew2, np.uint32), vw, 8)
+    assert np.array_equal(base, looped)
+
+
+def test_initial_partition_boundary_checks(gp):
+    # total 130, capacity floor(1.03 * 130 / 2) = 66 < the vertex of weight 70
+    off, nbr, _ = graphs.csr_from_edges(61, [(i, i + 1) for i in range(60)])
+    with pytest.raises(RuntimeError, match="exceeds partition capacity"):
+        gp.initial_partition_host(off, nbr, None, np.array([1] * 30 + [70] + [1] * 30, np.uint32), 2)
+    off, nbr, _ = graphs.csr_from_edges(5, [(0, 1), (1, 2), (2, 3), (3, 4)])
+    with pytest.raises(RuntimeError, match="invalid initial partition options"):
+        gp.initial_partition_host(off, nbr, None, None, 6)
+    # weights all 2: capacity floor(10 / 2) = 5 holds 2 vertices (4), below
+    # ceil(10 / 2) = 5, so a strict request fails; production does not check.
+    even = np.full(5, 2, np.uint32)
+    with pytest.raises(RuntimeError, match="indivisible"):
+        gp.initial_partition_host(off, nbr, None, even, 2, imbalance=1.0, require_feasible=True)
+    lab = gp.initial_partition_host(off, nbr, None, even, 2, imbalance=1.0)
+    assert set(lab.tolist()) <= {0, 1}
+
+
+def test_initial_partition_strict_is_feasible(gp):
+    for seed in range(4):
+        off, nbr, ew = graphs.random_graph(300, 900, seed=seed, max_weight=3)
+        vw = np.random.default_rng(seed).integers(1, 9, 300).astype(np.uint32)
+        for k in (2, 5, 16):
+            lab = gp.initial_partition_host(off, nbr, ew, vw, k, require_feasible=True)
+            cap = gp.capacity(int(vw.sum()), k)
+            assert np.bincount(lab, weights=vw, minlength=k).max() <= cap, (seed, k)
+    assert gp.initial_partition_backend() in ("metis", "bisection")

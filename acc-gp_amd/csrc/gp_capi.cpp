@@ -147,7 +147,13 @@ const char* gp_last_error(void) { return g_error.c_str(); }
 
 int gp_context_create(int device, gp_context** out) {
     if (!out) return fail(GP_ERR_INVALID, "null output");
-    return guarded([&] { *out = new gp_context(device); });
+    return guarded([&] {
+        auto c = std::make_unique<gp_context>(device);
+        // Set-up of gp_edge_cut_noalloc: the device address of its static
+        // scratch on this device, resolved once per device (device current).
+        gp::resolve_noalloc_scratch(device);
+        *out = c.release();
+    });
 }
 void gp_context_destroy(gp_context* ctx) { delete ctx; }
 uint64_t gp_context_reserved_bytes(const gp_context* ctx) {
@@ -391,6 +397,15 @@ int gp_edge_cut(gp_context* ctx, const gp_graph* g, const int32_t* labels, int p
         if (!ctx || !g || !g->view || !cut || !loads) throw std::invalid_argument("null argument");
         ctx->ctx.activate();
         gp::edge_cut(ctx->ctx, *g->view, labels, parts, cut, loads);
+    });
+}
+
+int gp_edge_cut_noalloc(gp_context* ctx, const gp_graph* g, const int32_t* labels, int parts, uint64_t* cut,
+                        uint64_t* loads) {
+    return guarded([&] {
+        if (!ctx || !g || !g->view || !cut || !loads) throw std::invalid_argument("null argument");
+        ctx->ctx.activate();
+        gp::edge_cut_noalloc(ctx->ctx, *g->view, labels, parts, cut, loads);
     });
 }
 

@@ -16,7 +16,17 @@
 // vertex weights + m (4 neighbour + 4 neighbour label [+ 4 edge weight]).
 #include "gp_metrics.hpp"
 
+#include <mutex>
 #include <stdexcept>
+#include <unordered_map>
+
+// Scratch of the allocation-free evaluation (edge_cut_noalloc): room for the
+// partial rows and the totals row of the largest evaluation, 1024 partial
+// blocks + 1 totals row of 130 words (cut, out-of-range count, 128 loads),
+// 1,066,000 bytes.  Every GPU holds its own copy until the process ends.
+// Host code never passes this name to a kernel: it is a host-side stand-in,
+// and the device address comes from hipGetSymbolAddress per device.
+static __device__ __align__(256) unsigned long long scratch[gp::kNoAllocScratchWords];
 
 namespace gp {
 
@@ -135,8 +145,50 @@ void edge_cut_into(Context& c, const DGraph& g, const std::int32_t* labels, int 
 
 void edge_cut(Context& c, const DGraph& g, const std::int32_t* labels, int k, std::uint64_t* cut,
               std::uint64_t* loads) {
-    DBuf<unsigned long long> scratch = c.buf<unsigned long long>(edge_cut_scratch_words(g.n, k));
-    edge_cut_into(c, g, labels, k, scratch.get(), cut, loads);
+    DBuf<unsigned long long> buf = c.buf<unsigned long long>(edge_cut_scratch_words(g.n, k));
+    edge_cut_into(c, g, labels, k, buf.get(), cut, loads);
+}
+
+namespace {
+
+std::mutex g_scratch_mutex;
+std::unordered_map<int, unsigned long long*> g_scratch_address;  // device -> address of `scratch`
+
+}  // namespace
+
+unsigned long long* resolve_noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    auto it = g_scratch_address.find(device);
+    if (it != g_scratch_address.end()) return it->second;
+    int current = -1;
+    GP_HIP(hipGetDevice(&current));
+    if (current != device) throw std::logic_error("resolve_noalloc_scratch: device is not current");
+    void* ptr = nullptr;
+    GP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw HipError("hipGetSymbolAddress returned no address for the evaluation scratch");
+    auto* p = static_cast<unsigned long long*>(ptr);
+    g_scratch_address.emplace(device, p);
+    return p;
+}
+
+void edge_cut_noalloc(Context& c, const DGraph& g, const std::int32_t* labels, int k, std::uint64_t* cut,
+                      std::uint64_t* loads) {
+    if (k < 2 || k > kMaxParts) throw std::invalid_argument("parts must be in [2, 128]");
+    const std::size_t words = edge_cut_scratch_words(g.n, k);
+    if (words > kNoAllocScratchWords)
+        throw std::invalid_argument("evaluation needs " + std::to_string(words) + " scratch words, above the " +
+                                    std::to_string(kNoAllocScratchWords) + " of the static scratch");
+    unsigned long long* p = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        auto it = g_scratch_address.find(c.device());
+        if (it != g_scratch_address.end()) p = it->second;
+    }
+    if (!p) throw std::logic_error("evaluation scratch of this device was not resolved at context creation");
+    // No word of the scratch has to start at zero: the partial pass writes
+    // every row the finishing pass reads, and the finishing pass writes the
+    // totals row that is read back.
+    edge_cut_into(c, g, labels, k, p, cut, loads);
 }
 
 }  // namespace gp

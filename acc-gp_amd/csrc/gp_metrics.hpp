@@ -20,4 +20,17 @@ void edge_cut_into(Context& c, const DGraph& g, const std::int32_t* labels, int 
 void edge_cut(Context& c, const DGraph& g, const std::int32_t* labels, int k, std::uint64_t* cut,
               std::uint64_t* loads_host);
 
+// Words of the static device scratch of edge_cut_noalloc: the largest
+// evaluation, kEdgeCutMaxBlocks partial rows + 1 totals row of k + 2 words.
+constexpr std::size_t kNoAllocScratchWords =
+    static_cast<std::size_t>(kEdgeCutMaxBlocks + 1) * static_cast<std::size_t>(kMaxParts + 2);
+// Resolves (once per device, that device current) and caches the device
+// address of the static scratch; called at context creation.
+unsigned long long* resolve_noalloc_scratch(int device);
+// edge_cut with its partials in the static scratch of the context's device:
+// allocates nothing (no arena, no hipMalloc).  Calls that overlap on one GPU
+// share the scratch and race.
+void edge_cut_noalloc(Context& c, const DGraph& g, const std::int32_t* labels, int k, std::uint64_t* cut,
+                      std::uint64_t* loads_host);
+
 }  // namespace gp

@@ -184,6 +184,9 @@ public:
         // Coherent: the copy kernel's stores reach host memory directly, so
         // the host reads them once the stream has passed the kernel.
         GP_HIP(hipHostMalloc(&pinned_, kPinnedBytes, hipHostMallocCoherent));
+        // Created here, not on first wait, so that a wait (and every read-back
+        // through the pinned buffer) creates no runtime objects.
+        GP_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
         if (const char* e = std::getenv("GP_LIGHT_LANES")) set_light_lanes(std::atoi(e));
         if (const char* e = std::getenv("GP_MEDIUM_LANES")) set_medium_lanes(std::atoi(e));
         if (const char* e = std::getenv("GP_TIER_OVERLAP")) overlap_ = std::atoi(e) != 0;
@@ -252,7 +255,6 @@ public:
             GP_HIP(hipStreamSynchronize(stream_));
             return;
         }
-        if (!done_) GP_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
         GP_HIP(hipEventRecord(done_, stream_));
         for (;;) {
             const hipError_t e = hipEventQuery(done_);

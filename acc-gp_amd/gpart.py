@@ -71,7 +71,7 @@ EXPORTS = [
     "gp_initial_partition_host", "gp_initial_partition", "gp_repair_capacity_host", "gp_initial_partition_backend",
     "gp_default_refine_options", "gp_uncoarsen",
     "gp_result_labels", "gp_result_cut", "gp_result_loads", "gp_result_levels", "gp_result_level_stats",
-    "gp_result_destroy", "gp_result_device_labels", "gp_edge_cut", "gp_refine_partition", "gp_partition", "gp_generate_grid", "gp_generate_rmat",
+    "gp_result_destroy", "gp_result_device_labels", "gp_edge_cut", "gp_edge_cut_noalloc", "gp_refine_partition", "gp_partition", "gp_generate_grid", "gp_generate_rmat",
     "gp_generate_rgg2d", "gp_generate_knn3d", "gp_free_host", "gp_load_csr", "gp_save_csr",
     "gp_profile_enable", "gp_profile_query", "gp_profile_reset", "gp_server_create", "gp_server_submit",
     "gp_server_wait", "gp_server_destroy", "gp_partition_batch",
@@ -122,6 +122,7 @@ def lib():
         L.gp_result_device_labels.argtypes = [vp]
         L.gp_result_device_labels.restype = vp
         L.gp_edge_cut.argtypes = [vp, vp, vp, i32, vp, vp]
+        L.gp_edge_cut_noalloc.argtypes = [vp, vp, vp, i32, vp, vp]
         L.gp_refine_partition.argtypes = [vp, vp, vp, vp, vp]
         L.gp_partition.argtypes = [vp, vp, i32, dbl, u32, vp, vp]
         L.gp_generate_grid.argtypes = [i64, i64, vp, vp, vp, vp]
@@ -327,6 +328,24 @@ class Graph:
         return out, dict((f, getattr(st, f)) for f, _ in LevelStats._fields_)
 
 
+def edge_cut_noalloc(ctx, g_handle, device_labels, parts):
+    """(cut, loads) of device labels through gp_edge_cut_noalloc on ctx."""
+    cut = ctypes.c_uint64()
+    loads = np.zeros(parts, np.uint64)
+    check(lib().gp_edge_cut_noalloc(ctx.h, g_handle, ctypes.c_void_p(device_labels), parts, ctypes.byref(cut),
+                                    _p(loads)))
+    return cut.value, loads
+
+
+def edge_cut(ctx, g_handle, device_labels, parts):
+    """(cut, loads) of device labels through gp_edge_cut on ctx."""
+    return _edge_cut(ctx, g_handle, device_labels, parts)
+
+
+def reserved_bytes(ctx):
+    return lib().gp_context_reserved_bytes(ctx.h)
+
+
 def _edge_cut(ctx, g_handle, device_labels, parts):
     cut = ctypes.c_uint64()
     loads = np.zeros(parts, np.uint64)
@@ -376,16 +395,20 @@ class Hierarchy:
         check(lib().gp_hierarchy_level_info(self.h, level, ctypes.byref(md), ctypes.byref(hh), ctypes.byref(ib)))
         return dict(max_degree=md.value, has_heavy=hh.value, incident_bound=ib.value)
 
-    def uncoarsen(self, coarsest_labels, ropts, evaluate_parts=None):
+    def uncoarsen(self, coarsest_labels, ropts, evaluate_parts=None, evaluate=None):
         """Refined labels, cut, loads and per-level stats.  With
         evaluate_parts, also the (cut, loads) that gp_edge_cut counts on the
-        result's device labels with that many parts (appended)."""
+        result's device labels with that many parts (appended); with
+        evaluate, what evaluate(graph_handle, device_labels) returns while
+        the result is alive (appended)."""
         labels = np.ascontiguousarray(coarsest_labels, np.int32)
         r = ctypes.c_void_p()
         check(lib().gp_uncoarsen(self.ctx.h, self.h, _p(labels), ctypes.byref(ropts), ctypes.byref(r)))
         try:
             evaluated = None
-            if evaluate_parts is not None:
+            if evaluate is not None:
+                evaluated = evaluate(self.graph.h, lib().gp_result_device_labels(r))
+            elif evaluate_parts is not None:
                 evaluated = _edge_cut(self.ctx, self.graph.h, lib().gp_result_device_labels(r), evaluate_parts)
             out = np.zeros(self.graph.n, np.int32)
             check(lib().gp_result_labels(self.ctx.h, r, _p(out)))
@@ -396,7 +419,7 @@ class Hierarchy:
                 st = LevelStats()
                 check(lib().gp_result_level_stats(r, i, ctypes.byref(st)))
                 stats.append(dict((f, getattr(st, f)) for f, _ in LevelStats._fields_))
-            if evaluate_parts is not None:
+            if evaluate is not None or evaluate_parts is not None:
                 return out, lib().gp_result_cut(r), loads, stats, evaluated
             return out, lib().gp_result_cut(r), loads, stats
         finally:

@@ -172,6 +172,18 @@ const int32_t* gp_result_device_labels(const gp_result* r);
  * Partials come from the context arena; synchronizes the context stream. */
 int gp_edge_cut(gp_context* ctx, const gp_graph* g, const int32_t* labels, int parts, uint64_t* cut,
                 uint64_t* loads);
+/* gp_edge_cut without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as gp_edge_cut; the per-block partials live
+ * in a static device array of 1025 x 130 words (1,066,000 bytes, held by
+ * each GPU the library runs on until the process ends) instead of the
+ * context arena, so the call makes no hipMalloc and touches no arena.  It
+ * covers every graph size at parts <= 128 (GP_ERR_INVALID above).  Calls
+ * that overlap on one GPU share that array and race; the array keeps the
+ * last call's partials. */
+int gp_edge_cut_noalloc(gp_context* ctx, const gp_graph* g, const int32_t* labels, int parts, uint64_t* cut,
+                        uint64_t* loads);
 /* refine_partition(graph, labels, options): one level, labels in/out (host). */
 int gp_refine_partition(gp_context* ctx, const gp_graph* g, int32_t* labels, const gp_refine_options* opts,
                         gp_level_stats* stats);

@@ -6,9 +6,11 @@
 #include "gp_metrics.hpp"
 #include "gp_refine.hpp"
 
+#include <atomic>
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <new>
 #include <string>
 
@@ -17,15 +19,45 @@ struct gp_context {
     gp::Context ctx;
 };
 
+namespace {
+std::atomic<std::uint64_t> g_graph_serial{0};
+}
+
 struct gp_graph {
+    gp_graph() : serial(++g_graph_serial) {}
     std::unique_ptr<gp::DGraph> owned;
     gp::DGraph* view = nullptr;
+    const std::uint64_t serial;  // identifies the graph a workspace describes
+};
+
+// SclpWorkspace (include/coarsen.hpp:55-69): the coarsening state carried
+// from one level to the next (row flags, weight bounds, per-vertex row
+// bounds of the current level) and the graph it describes.
+struct gp_workspace {
+    gp::CoarsenState st;
+    std::uint64_t current = 0;  // serial of the graph st describes (0: none)
+};
+
+// DeviceAggregateResult (include/coarsen.hpp:40-46).
+struct gp_aggregate {
+    gp::AggregateResult r;
+    gp::DBuf<std::int32_t> map;
+    gp::DBuf<std::uint32_t> cvw;
+    std::uint64_t graph = 0;  // serial of the aggregated graph
+    std::uint64_t state = 0;  // serial the workspace state must still describe
+    bool consumed = false;
 };
 
 struct gp_hierarchy {
     gp::Hierarchy h;
     std::vector<std::unique_ptr<gp_graph>> views;
     gp_graph* input = nullptr;
+    bool owns_input = false;  // assembled hierarchies own their level 0
+    ~gp_hierarchy() {
+        views.clear();
+        h = gp::Hierarchy();
+        if (owns_input) delete input;
+    }
 };
 
 struct gp_result {
@@ -84,6 +116,7 @@ gp::RefineOptions to_options(const gp_refine_options& o) {
     r.stale_rounds = o.stale_rounds;
     r.growth_ratio = o.growth_ratio;
     r.exchange_rounds = o.exchange_rounds;
+    r.strict_verify = o.strict_verify != 0;
     return r;
 }
 
@@ -107,6 +140,7 @@ gp_level_stats to_c(const gp::LevelStats& s) {
     o.repair_moves = s.repair_moves;
     o.group_moves = s.group_moves;
     o.repair_iterations = s.repair_iterations;
+    o.strict_checks = s.strict_checks;
     return o;
 }
 
@@ -233,6 +267,17 @@ int gp_graph_download(gp_context* ctx, const gp_graph* g, int32_t* offsets, int3
     });
 }
 
+int gp_graph_metadata(const gp_graph* g, uint64_t* max_degree, int* has_heavy, uint64_t* incident_bound,
+                      int* implicit_unit_weights) {
+    if (!g) return fail(GP_ERR_INVALID, "null graph");
+    const gp::DGraph& d = *g->view;
+    if (max_degree) *max_degree = d.max_degree;
+    if (has_heavy) *has_heavy = d.has_heavy;
+    if (incident_bound) *incident_bound = d.incident_bound;
+    if (implicit_unit_weights) *implicit_unit_weights = d.unit_edges();
+    return GP_OK;
+}
+
 void gp_graph_destroy(gp_graph* g) { delete g; }
 
 int gp_capacity(uint64_t total, int parts, double imbalance, uint64_t* out) {
@@ -298,7 +343,145 @@ int gp_hierarchy_level_info(const gp_hierarchy* h, int level, uint64_t* max_degr
     return GP_OK;
 }
 
+int gp_hierarchy_assemble(gp_context* ctx, int levels, gp_graph* const* level_graphs, const int32_t* const* maps,
+                          gp_hierarchy** out) {
+    if (!ctx || !level_graphs || !out || levels < 1 || (levels > 1 && !maps)) return fail(GP_ERR_INVALID, "null argument");
+    return guarded([&] {
+        gp::Context& c = ctx->ctx;
+        for (int i = 0; i < levels; ++i)
+            if (!level_graphs[i] || !level_graphs[i]->owned)
+                throw std::invalid_argument("hierarchy levels must be graphs made by gp_graph_create or gp_contract");
+        for (int i = 0; i + 1 < levels; ++i) {
+            const std::int64_t nf = level_graphs[i]->view->n, nc = level_graphs[i + 1]->view->n;
+            if (!maps[i] || nc > nf) throw std::invalid_argument("resident refinement requires a complete hierarchy");
+            for (std::int64_t v = 0; v < nf; ++v)
+                if (maps[i][v] < 0 || maps[i][v] >= nc) throw std::invalid_argument("hierarchy map contains an invalid coarse id");
+        }
+        auto h = std::make_unique<gp_hierarchy>();
+        for (int i = 0; i < levels; ++i) {
+            gp::DGraph& d = *level_graphs[i]->view;
+            gp::CoarsenState st;
+            gp::compute_level0_metadata(c, d, st);  // exact bounds of a level built elsewhere
+            if (i + 1 < levels) {
+                gp::DBuf<std::int32_t> m = c.buf<std::int32_t>(static_cast<std::size_t>(d.n));
+                c.upload(m.get(), maps[i], static_cast<std::size_t>(d.n) * sizeof(std::int32_t));
+                h->h.maps.push_back(std::move(m));
+                h->h.caps.push_back(std::numeric_limits<std::uint64_t>::max());
+            }
+        }
+        // Ownership moves in only once everything above has succeeded.
+        h->input = level_graphs[0];
+        h->h.input = level_graphs[0]->view;
+        h->h.stop_reason = "assembled";
+        for (int i = 1; i < levels; ++i) {
+            h->h.coarse.push_back(std::move(level_graphs[i]->owned));
+            auto v = std::make_unique<gp_graph>();
+            v->view = h->h.coarse.back().get();
+            h->views.push_back(std::move(v));
+            delete level_graphs[i];
+        }
+        h->owns_input = true;
+        c.sync();
+        *out = h.release();
+    });
+}
+
+int gp_graph_edge_weight(gp_context* ctx, const gp_graph* g, uint64_t* total) {
+    if (!ctx || !g || !total) return fail(GP_ERR_INVALID, "null argument");
+    return guarded([&] {
+        const std::uint64_t directed = gp::device_edge_weight_sum(ctx->ctx, *g->view);
+        if (directed & 1ULL) throw std::runtime_error("symmetric graph has odd directed edge weight");
+        *total = directed / 2;
+    });
+}
+
+int gp_hierarchy_level_capacity(const gp_hierarchy* h, int level, uint64_t* capacity) {
+    if (!h || !capacity || level < 0 || level + 1 >= h->h.levels()) return fail(GP_ERR_INVALID, "invalid level");
+    *capacity = h->h.caps[static_cast<std::size_t>(level)];
+    return GP_OK;
+}
+
+const char* gp_hierarchy_stop_reason(const gp_hierarchy* h) { return h ? h->h.stop_reason.c_str() : ""; }
+
 void gp_hierarchy_destroy(gp_hierarchy* h) { delete h; }
+
+int gp_workspace_create(gp_workspace** out) {
+    if (!out) return fail(GP_ERR_INVALID, "null output");
+    return guarded([&] { *out = new gp_workspace; });
+}
+
+void gp_workspace_destroy(gp_workspace* ws) { delete ws; }
+
+int gp_aggregate_level(gp_context* ctx, gp_workspace* ws, gp_graph* g, int parts, uint32_t seed, int level,
+                       int cluster_size_factor, int pack_isolated, int seed_matching, int fast_path,
+                       gp_aggregate** out) {
+    if (!ctx || !ws || !g || !out) return fail(GP_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (parts < 2 || parts > gp::kMaxParts || cluster_size_factor < 2 || level < 0)
+            throw std::invalid_argument("invalid aggregation options");
+        gp::Context& c = ctx->ctx;
+        gp::DGraph& d = *g->view;
+        if (ws->current != g->serial) {
+            // A graph the workspace has not contracted into: level-0 metadata.
+            ws->st = gp::CoarsenState();
+            gp::compute_level0_metadata(c, d, ws->st);
+            ws->current = g->serial;
+        }
+        auto a = std::make_unique<gp_aggregate>();
+        a->r = gp::aggregate(c, d, ws->st, parts, seed, level, cluster_size_factor, pack_isolated != 0,
+                             seed_matching != 0, fast_path != 0, a->map, a->cvw);
+        // The aggregation pass tightens the level's degree metadata.
+        d.max_degree = a->r.stats_max_degree;
+        d.has_heavy = a->r.stats_has_heavy;
+        a->graph = g->serial;
+        a->state = ws->current;
+        c.sync();
+        *out = a.release();
+    });
+}
+
+int gp_aggregate_info(const gp_aggregate* a, int64_t* coarse_vertices, uint64_t* maximum_weight, uint64_t* capacity) {
+    if (!a) return fail(GP_ERR_INVALID, "null aggregate");
+    if (coarse_vertices) *coarse_vertices = a->r.coarse_n;
+    if (maximum_weight) *maximum_weight = a->r.max_weight;
+    if (capacity) *capacity = a->r.capacity;
+    return GP_OK;
+}
+
+int gp_aggregate_map_download(gp_context* ctx, const gp_aggregate* a, int32_t* map) {
+    if (!ctx || !a || !map) return fail(GP_ERR_INVALID, "null argument");
+    return guarded([&] { ctx->ctx.download(map, a->map.get(), a->map.size() * sizeof(std::int32_t)); });
+}
+
+int gp_aggregate_vertex_weights_download(gp_context* ctx, const gp_aggregate* a, uint32_t* vertex_weights) {
+    if (!ctx || !a || !vertex_weights) return fail(GP_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (a->consumed) throw std::invalid_argument("aggregate was consumed by gp_contract");
+        ctx->ctx.download(vertex_weights, a->cvw.get(), static_cast<std::size_t>(a->r.coarse_n) * sizeof(std::uint32_t));
+    });
+}
+
+void gp_aggregate_destroy(gp_aggregate* a) { delete a; }
+
+int gp_contract(gp_context* ctx, gp_workspace* ws, const gp_graph* fine, gp_aggregate* a, gp_graph** out) {
+    if (!ctx || !ws || !fine || !a || !out) return fail(GP_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (a->consumed) throw std::invalid_argument("aggregate was already contracted");
+        if (a->graph != fine->serial || ws->current != a->state || a->state != fine->serial)
+            throw std::invalid_argument("aggregate, workspace and fine graph do not belong together");
+        gp::Context& c = ctx->ctx;
+        auto coarse = gp::contract(c, *fine->view, ws->st, a->map.get(), a->r.coarse_n, std::move(a->cvw));
+        a->consumed = true;
+        ws->st.max_vertex_weight = a->r.max_weight;
+        ws->st.sorted_unique_rows = true;
+        auto h = std::make_unique<gp_graph>();
+        h->owned = std::move(coarse);
+        h->view = h->owned.get();
+        ws->current = h->serial;
+        c.sync();
+        *out = h.release();
+    });
+}
 
 int gp_initial_partition_host(int64_t n, const int32_t* offsets, const int32_t* neighbors,
                               const uint32_t* edge_weights, const uint32_t* vertex_weights, int parts,
@@ -348,6 +531,7 @@ void gp_default_refine_options(int parts, double imbalance, gp_refine_options* o
     out->stale_rounds = o.stale_rounds;
     out->growth_ratio = o.growth_ratio;
     out->exchange_rounds = o.exchange_rounds;
+    out->strict_verify = 0;
 }
 
 int gp_uncoarsen(gp_context* ctx, const gp_hierarchy* h, const int32_t* coarsest_labels,

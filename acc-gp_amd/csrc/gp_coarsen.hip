@@ -2042,6 +2042,7 @@ Hierarchy coarsen(Context& c, DGraph& input, const CoarsenOptions& o) {
         auto coarse = contract(c, *cur, st, map.get(), r.coarse_n, std::move(cvw));
         st.max_vertex_weight = r.max_weight;
         h.maps.push_back(std::move(map));
+        h.caps.push_back(r.capacity);
         h.coarse.push_back(std::move(coarse));
         cur = h.coarse.back().get();
         cur_max_degree = cur->max_degree;

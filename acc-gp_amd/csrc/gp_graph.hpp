@@ -63,6 +63,9 @@ struct RefineOptions {
     int exchange_rounds = 2;
     std::int64_t graph_loop_threshold = 65536;
     std::int64_t exchange_max_n = 65536;
+    // Recount the cut and the loads after every committed batch and every
+    // projection (include/refine.hpp:17, the CLI's --verify).
+    bool strict_verify = false;
 };
 
 // Coarsening state carried across levels (SclpWorkspace::Impl, src/coarsen.cu:299-396).
@@ -80,6 +83,7 @@ struct LevelStats {
     std::uint64_t initial_cut = 0, final_cut = 0;
     std::uint64_t lp_moves = 0, repair_moves = 0, group_moves = 0;
     int repair_iterations = 0;
+    std::uint64_t strict_checks = 0;  // recounts that matched (strict_verify)
 };
 
 class Hierarchy {
@@ -87,6 +91,7 @@ public:
     const DGraph* input = nullptr;          // level 0, borrowed
     std::vector<std::unique_ptr<DGraph>> coarse;  // levels 1..
     std::vector<DBuf<std::int32_t>> maps;         // maps[i]: level i -> level i+1
+    std::vector<std::uint64_t> caps;              // caps[i]: cluster weight cap of level i's aggregation
     std::string stop_reason = "capacity_floor";
 
     int levels() const { return 1 + static_cast<int>(coarse.size()); }

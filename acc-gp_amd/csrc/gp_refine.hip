@@ -38,7 +38,12 @@ enum Ctr {
     // one clear of the whole block at the round start serves all three.
     kTierB = 9,
     kTierC = 11,
-    kCtrCount = 13
+    // (entries, rows) handled by the afterburner, update and repair-proposal
+    // sweeps, counted only when the context profiles kernel classes.
+    kEntAfter = 13,
+    kEntUpdate = 15,
+    kEntRepair = 17,
+    kCtrCount = 19
 };
 
 // Part ids inside refinement: k <= 128 fits a byte, which keeps the label
@@ -1390,19 +1395,131 @@ __global__ void k_joint_seeds(RefArgs a, const unsigned long long* A, const std:
     }
 }
 
-// joint_grow (exchange.cuh:96-204): one wave per (seed, policy) slot.
-__global__ void k_joint_grow(RefArgs a, const unsigned long long* A, int slots, int sw,
-                             const unsigned long long* seeds, const std::int32_t* partner, JMove* moves,
-                             JMove* archive) {
-    __shared__ std::int32_t mem[4][8];
+// ------------------------------------------------ exchange: transaction growth
+// Growth of capacity-complementary transactions (the semantics of
+// exchange.cuh:96-204).  A slot is a (partner seed, policy) pair; its
+// transaction starts at the seed vertex and adds, up to eight members and at
+// most four per side, the admissible vertex that ranks first by (marginal
+// gain, balance gap, id), or by (gap, gain, id) under policy 0 while the net
+// weight lies outside the feasible window.  Candidates are the vertices of a
+// member's first 128 row entries that share the member's part, and, while no
+// member sits on the partner side, the partner part's seeds; vertices of
+// degree above 1024 never join.  Each step records the transaction as an
+// archive fragment (positive gain, at most four members) and as the slot's
+// best feasible transaction.
+//
+// CDNA design: one wave per slot with a candidate pool in LDS.  When a
+// member joins, its row is read once and the entries that pass the static
+// tests are appended to the pool by ballot compaction, together with their
+// static marginal gain A(u, other) - A(u, own) (+2w for a row entry) and
+// weight.  A step then scans at most 1024 pool records in LDS instead of
+// re-reading every member row and the affinity rows behind it; records of a
+// vertex that joins are retired.  The step's winner comes from a wave
+// butterfly whose in-row exchanges are DPP moves, and the joining vertex's
+// correction term (its edges to the members) is a DPP sum over its row.
+constexpr int kGrowNominate = 128;                          // row entries that nominate
+constexpr int kGrowPool = 7 * kGrowNominate + kJointSeeds;  // members 1..7 nominate; seeds once
+constexpr int kGrowWaves = 2;                               // slots per block
+constexpr std::uint8_t kPoolPartner = 1, kPoolSeed = 2, kPoolRetired = 4;
+
+struct GrowPool {
+    std::int64_t marginal[kGrowPool];
+    std::int32_t u[kGrowPool];
+    std::uint32_t w[kGrowPool];
+    std::uint8_t kind[kGrowPool];
+    std::int32_t members[8];
+    std::int32_t member_label[8];
+    int count;
+};
+
+struct GrowPick {
+    std::int64_t marginal, gap;
+    std::int32_t u, pad;
+};
+
+// x ranks before y (the step order of exchange.cuh:166-176).
+__device__ inline bool grow_before(const GrowPick& x, const GrowPick& y, bool gap_first) {
+    if (gap_first) {
+        if (x.gap != y.gap) return x.gap < y.gap;
+        if (x.marginal != y.marginal) return x.marginal > y.marginal;
+    } else {
+        if (x.marginal != y.marginal) return x.marginal > y.marginal;
+        if (x.gap != y.gap) return x.gap < y.gap;
+    }
+    return x.u < y.u;
+}
+
+struct GrowBest {
+    bool gap_first;
+    __device__ GrowPick operator()(const GrowPick& x, const GrowPick& y) const {
+        return grow_before(y, x, gap_first) ? y : x;
+    }
+};
+
+// Appends what one new member nominates (member >= 0: its first 128 row
+// entries in its own part) or, with member < 0, the partner part's seeds.
+// Wave-uniform; every lane takes part.
+__device__ inline void grow_nominate(const RefArgs& a, const unsigned long long* A, const unsigned long long* seeds,
+                                     int sw, int source, int target, std::int32_t member, int size, GrowPool& P,
+                                     int lane) {
+    const int k = a.k;
+    std::int64_t begin = 0, end = sw;
+    int own = -1;
+    if (member >= 0) {
+        begin = a.off[member];
+        end = a.off[member + 1];
+        if (end > begin + kGrowNominate) end = begin + kGrowNominate;
+        own = a.labels[member];
+    }
+    for (std::int64_t base = begin; base < end; base += kWave) {
+        const std::int64_t e = base + lane;
+        bool keep = false;
+        std::int32_t u = -1;
+        int p = -1;
+        if (e < end) {
+            if (member < 0) {
+                const unsigned long long val = seeds[(std::int64_t)target * sw + e];
+                if (val != ~0ULL) u = (std::int32_t)(std::uint32_t)val;
+            } else {
+                u = a.nbr[e];
+            }
+            if (u >= 0) {
+                p = a.labels[u];
+                keep = (p == source || p == target) && (member < 0 || p == own) && a.off[u + 1] - a.off[u] <= 1024;
+                for (int j = 0; j < size; ++j) keep &= u != P.members[j];
+            }
+        }
+        const unsigned long long ballot = __ballot(keep);
+        if (keep) {
+            const int r = P.count + __popcll(ballot & ((1ULL << lane) - 1ULL));
+            const int other = p == source ? target : source;
+            std::int64_t marginal = (std::int64_t)A[(std::int64_t)u * k + other] - (std::int64_t)A[(std::int64_t)u * k + p];
+            if (member >= 0) marginal += 2 * (std::int64_t)ew_at(a.ew, e);
+            P.marginal[r] = marginal;
+            P.u[r] = u;
+            P.w[r] = a.vw[u];
+            P.kind[r] = (std::uint8_t)((p == source ? 0 : kPoolPartner) | (member < 0 ? kPoolSeed : 0));
+        }
+        wave_sync();
+        if (lane == 0) P.count += __popcll(ballot);
+        wave_sync();
+    }
+}
+
+__global__ void __launch_bounds__(kGrowWaves* kWave) k_joint_grow(RefArgs a, const unsigned long long* A, int slots,
+                                                                   int sw, const unsigned long long* seeds,
+                                                                   const std::int32_t* partner, JMove* moves,
+                                                                   JMove* archive) {
+    __shared__ GrowPool pools[kGrowWaves];
     const int lane = lane_id(), wave = threadIdx.x / kWave;
-    const int slot = blockIdx.x * (blockDim.x / kWave) + wave;
+    const int slot = blockIdx.x * kGrowWaves + wave;
     if (slot >= slots) return;
+    GrowPool& P = pools[wave];
     const int policy = slot % 2;
     const int k = a.k;
     JMove best = jmove_empty();
     best.id = slot;
-    if (lane == 0) {
+    if (lane == 0) {  // the same lane writes these slots again below
         moves[slot] = best;
         for (int j = 0; j < kJointArchive; ++j) archive[slot * kJointArchive + j] = jmove_empty();
     }
@@ -1411,8 +1528,11 @@ __global__ void k_joint_grow(RefArgs a, const unsigned long long* A, int slots, 
     const std::int32_t root = (std::int32_t)(std::uint32_t)enc;
     const int source = a.labels[root], target = partner[source];
     if (target < 0) return;
-    std::int32_t* members = mem[wave];
-    if (lane < 8) members[lane] = lane == 0 ? root : -1;
+    if (lane == 0) {
+        P.count = 0;
+        P.members[0] = root;
+        P.member_label[0] = source;
+    }
     wave_sync();
     int size = 1, c0 = 1, c1 = 0;
     std::int64_t net = a.vw[root];
@@ -1420,7 +1540,9 @@ __global__ void k_joint_grow(RefArgs a, const unsigned long long* A, int slots, 
     const std::int64_t hi = (std::int64_t)a.cap - (std::int64_t)a.loads[target];
     std::int64_t gain = (std::int64_t)A[(std::int64_t)root * k + target] - (std::int64_t)A[(std::int64_t)root * k + source];
     const int pa = source < target ? source : target, pb = source < target ? target : source;
-    for (int step = 0; step < 8; ++step) {
+    grow_nominate(a, A, seeds, sw, source, target, -1, size, P, lane);
+    grow_nominate(a, A, seeds, sw, source, target, root, size, P, lane);
+    for (int step = 0;; ++step) {
         if (lane == 0 && gain > 0 && size <= kJointArchive) {
             JMove st = jmove_empty();
             st.size = size;
@@ -1429,13 +1551,13 @@ __global__ void k_joint_grow(RefArgs a, const unsigned long long* A, int slots, 
             st.gain = gain;
             st.net = source == pa ? net : -net;
             st.id = slot * kJointArchive + step;
-            for (int j = 0; j < size; ++j) {
+            for (int j = 0; j < size; ++j) {  // members in ascending id order
                 int pos = j;
-                while (pos && st.v[pos - 1] > members[j]) {
+                while (pos && st.v[pos - 1] > P.members[j]) {
                     st.v[pos] = st.v[pos - 1];
                     --pos;
                 }
-                st.v[pos] = members[j];
+                st.v[pos] = P.members[j];
             }
             archive[slot * kJointArchive + step] = st;
         }
@@ -1445,84 +1567,47 @@ __global__ void k_joint_grow(RefArgs a, const unsigned long long* A, int slots, 
             best.b = pb;
             best.gain = gain;
             best.net = source == pa ? net : -net;
-            for (int j = 0; j < 8; ++j) best.v[j] = j < size ? members[j] : -1;
+            for (int j = 0; j < 8; ++j) best.v[j] = j < size ? P.members[j] : -1;
         }
         if (size == 8) break;
-        std::int64_t cg = LLONG_MIN, cgap = LLONG_MAX;
-        std::int32_t chosen = INT_MAX;
         const bool gap_first = policy == 0 && jgap(net, lo, hi) != 0;
-        for (int origin = -1; origin < size; ++origin) {
-            const bool rev = origin < 0;
-            if (rev && c1) continue;
-            const std::int32_t mm = rev ? -1 : members[origin];
-            const std::int64_t begin = rev ? 0 : a.off[mm];
-            std::int64_t end = rev ? sw : a.off[mm + 1];
-            if (!rev && end > begin + 128) end = begin + 128;
-            const int lm = rev ? -1 : a.labels[mm];
-            for (std::int64_t e = begin + lane; e < end; e += kWave) {
-                std::int32_t u;
-                if (rev) {
-                    const unsigned long long val = seeds[(std::int64_t)target * sw + e];
-                    if (val == ~0ULL) continue;
-                    u = (std::int32_t)(std::uint32_t)val;
-                } else
-                    u = a.nbr[e];
-                const int p = a.labels[u];
-                if (p != source && p != target) continue;
-                const int side = p == source ? 0 : 1;
-                if ((side ? c1 : c0) >= kJointSide) continue;
-                if (!rev && p != lm) continue;
-                bool dup = false;
-                for (int j = 0; j < size; ++j) dup |= u == members[j];
-                if (dup || a.off[u + 1] - a.off[u] > 1024) continue;
-                const std::int64_t cnet = net + (side ? -(std::int64_t)a.vw[u] : (std::int64_t)a.vw[u]);
-                const std::int64_t gp = jgap(cnet, lo, hi);
-                if (net > hi && !side) continue;
-                if (net < lo && side) continue;
-                std::int64_t marginal = (std::int64_t)A[(std::int64_t)u * k + (side ? source : target)] -
-                                        (std::int64_t)A[(std::int64_t)u * k + p];
-                if (!rev) marginal += 2 * (std::int64_t)ew_at(a.ew, e);
-                const bool better = gap_first
-                    ? (gp < cgap || (gp == cgap && (marginal > cg || (marginal == cg && u < chosen))))
-                    : (marginal > cg || (marginal == cg && (gp < cgap || (gp == cgap && u < chosen))));
-                if (better) {
-                    chosen = u;
-                    cg = marginal;
-                    cgap = gp;
-                }
-            }
+        GrowPick pick{LLONG_MIN, LLONG_MAX, INT_MAX, 0};
+        for (int r = lane; r < P.count; r += kWave) {
+            const int kind = P.kind[r];
+            const int side = kind & kPoolPartner;
+            if ((kind & kPoolRetired) || ((kind & kPoolSeed) && c1) || (side ? c1 : c0) >= kJointSide) continue;
+            if ((net > hi && !side) || (net < lo && side)) continue;
+            const std::int64_t cnet = net + (side ? -(std::int64_t)P.w[r] : (std::int64_t)P.w[r]);
+            const GrowPick c{P.marginal[r], jgap(cnet, lo, hi), P.u[r], 0};
+            if (grow_before(c, pick, gap_first)) pick = c;
         }
-        for (int d = kWave / 2; d; d >>= 1) {
-            const std::int32_t ou = __shfl_xor(chosen, d);
-            const std::int64_t og = __shfl_xor(cg, d);
-            const std::int64_t ogp = __shfl_xor(cgap, d);
-            const bool better = gap_first
-                ? (ogp < cgap || (ogp == cgap && (og > cg || (og == cg && ou < chosen))))
-                : (og > cg || (og == cg && (ogp < cgap || (ogp == cgap && ou < chosen))));
-            if (better) {
-                chosen = ou;
-                cg = og;
-                cgap = ogp;
-            }
-        }
-        if (chosen == INT_MAX) break;
+        pick = wave_allreduce(pick, GrowBest{gap_first});
+        if (pick.u == INT_MAX) break;
+        const std::int32_t chosen = pick.u;
         const int lc = a.labels[chosen];
         const int side = lc == source ? 0 : 1;
         std::int64_t corr = 0;
-        for (std::int64_t e = a.off[chosen] + lane; e < a.off[chosen + 1]; e += kWave)
+        for (std::int64_t e = a.off[chosen] + lane; e < a.off[chosen + 1]; e += kWave) {
+            const std::int32_t nb = a.nbr[e];
             for (int j = 0; j < size; ++j)
-                if (a.nbr[e] == members[j])
-                    corr += (a.labels[members[j]] == lc ? 2 : -2) * (std::int64_t)ew_at(a.ew, e);
-        corr = wave_sum(corr);
-        const std::int64_t cgain = (std::int64_t)A[(std::int64_t)chosen * k + (side ? source : target)] -
-                                   (std::int64_t)A[(std::int64_t)chosen * k + lc] + corr;
-        wave_sync();
-        if (lane == 0) members[size] = chosen;
+                if (nb == P.members[j]) corr += (P.member_label[j] == lc ? 2 : -2) * (std::int64_t)ew_at(a.ew, e);
+        }
+        corr = wave_allreduce(corr, PlusI64{});
+        gain += (std::int64_t)A[(std::int64_t)chosen * k + (side ? source : target)] -
+                (std::int64_t)A[(std::int64_t)chosen * k + lc] + corr;
+        net += side ? -(std::int64_t)a.vw[chosen] : (std::int64_t)a.vw[chosen];
+        if (side) ++c1;
+        else ++c0;
+        // Retire the newcomer's records, then let it nominate.
+        for (int r = lane; r < P.count; r += kWave)
+            if (P.u[r] == chosen) P.kind[r] |= kPoolRetired;
+        if (lane == 0) {
+            P.members[size] = chosen;
+            P.member_label[size] = lc;
+        }
         wave_sync();
         ++size;
-        if (side) ++c1; else ++c0;
-        gain += cgain;
-        net += side ? -(std::int64_t)a.vw[chosen] : (std::int64_t)a.vw[chosen];
+        if (size < 8) grow_nominate(a, A, seeds, sw, source, target, chosen, size, P, lane);
     }
     if (lane == 0) moves[slot] = best;
 }
@@ -1812,38 +1897,115 @@ __global__ void k_joint_mark(RefArgs a, const JMove* m, int count, std::int32_t*
         a.targets[v] = a.labels[v] == m[i].a ? m[i].b : m[i].a;
     }
 }
-// joint_events (exchange.cuh:227-262): one wave per transaction.
-__global__ void k_joint_events(RefArgs a, const JMove* m, int count, const std::int32_t* owners,
-                               unsigned long long* events) {
-    const int lane = lane_id();
-    const int i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (i >= count || !m[i].size) return;
-    unsigned long long own = 0;
-    const JMove mv = m[i];
-    for (int j = 0; j < mv.size; ++j) {
-        const std::int32_t v = mv.v[j];
-        for (std::int64_t e = a.off[v] + lane; e < a.off[v + 1]; e += kWave) {
+// Cut events of the claimed transactions (the semantics of
+// exchange.cuh:205-262).  Every undirected edge with an end in a transaction
+// contributes the change of its cut state once, counted from its lower end
+// when both ends move: an edge inside one transaction, or to a vertex that
+// stays, is charged whole to that transaction; an edge between transactions
+// i < j is split into the change when i applies (charged to i) and the rest
+// when j applies (charged to j), so prefix sums in list order are exact.
+// Edges to a vertex of another transaction on other parts do not change.
+//
+// CDNA design: a block takes 32 consecutive transactions, stages their
+// member rows (start, length, transaction) in LDS and walks them as one
+// flattened edge range, so lanes stay busy whatever the row lengths.  Each
+// wave reduces its lanes' charges to their own transactions by masked DPP
+// butterflies, one per distinct transaction among its lanes (found by
+// ballot), into 64-bit LDS totals; a charge to another transaction goes to
+// global memory at once.  The totals are flushed once per block.  All sums
+// are modulo 2^64, as the signed contributions of the reference.
+constexpr int kEventTx = 32;
+constexpr int kEventRows = kEventTx * 8;
+constexpr int kEventThreads = 256;
+
+__global__ void __launch_bounds__(kEventThreads) k_joint_events(RefArgs a, const JMove* m, int count,
+                                                                const std::int32_t* owners,
+                                                                unsigned long long* events) {
+    __shared__ std::int32_t row_start[kEventRows];
+    __shared__ std::int32_t row_first[kEventRows + 1];  // exclusive prefix of row lengths
+    __shared__ std::int32_t row_vertex[kEventRows];
+    __shared__ std::int32_t tx_a[kEventTx], tx_b[kEventTx];
+    __shared__ unsigned long long totals[kEventTx];
+    __shared__ std::int32_t wave_sums[kEventThreads / kWave];
+    const int t = threadIdx.x, lane = lane_id(), w = t / kWave;
+    const int tx0 = blockIdx.x * kEventTx;
+    // Row t is member t % 8 of transaction t / 8 (length 0 if absent).
+    {
+        const int tx = t / 8, j = t % 8, i = tx0 + tx;
+        std::int32_t len = 0, v = -1, start = 0;
+        if (i < count && j < m[i].size) {
+            v = m[i].v[j];
+            start = a.off[v];
+            len = a.off[v + 1] - start;
+        }
+        row_start[t] = start;
+        row_vertex[t] = v;
+        if (j == 0) {
+            tx_a[tx] = i < count ? m[i].a : -1;
+            tx_b[tx] = i < count ? m[i].b : -1;
+            totals[tx] = 0;
+        }
+        const std::int32_t inc = wave_inclusive_scan<PlusOp>(len);
+        if (lane == kWave - 1) wave_sums[w] = inc;
+        __syncthreads();
+        std::int32_t before = 0;
+        for (int x = 0; x < w; ++x) before += wave_sums[x];
+        row_first[t] = before + inc - len;
+        if (t == kEventThreads - 1) row_first[kEventRows] = before + inc;
+    }
+    __syncthreads();
+    const std::int32_t total = row_first[kEventRows];
+    for (std::int32_t base = 0; base < total; base += kEventThreads) {
+        const std::int32_t f = base + t;
+        int key = -1;
+        unsigned long long own = 0;
+        if (f < total) {
+            int lo = 0, hi = kEventRows - 1;  // last row whose first entry is <= f
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (row_first[mid] <= f) lo = mid;
+                else hi = mid - 1;
+            }
+            const int tx = lo / 8, i = tx0 + tx;
+            const std::int32_t v = row_vertex[lo];
+            const std::int64_t e = (std::int64_t)row_start[lo] + (f - row_first[lo]);
             const std::int32_t u = a.nbr[e];
             const std::int32_t other = owners[u];
-            if (u == v || (other != INT_MAX && v > u)) continue;
-            if (other != INT_MAX && a.labels[u] != mv.a && a.labels[u] != mv.b) continue;
-            const std::int64_t w = (std::int64_t)ew_at(a.ew, e);
-            const int before = a.labels[v] != a.labels[u];
-            if (other == i || other == INT_MAX) {
-                const int after = a.targets[v] != (other == i ? a.targets[u] : a.labels[u]);
-                own += (unsigned long long)((before - after) * w);
-            } else {
-                const int middle = i < other ? a.targets[v] != a.labels[u] : a.labels[v] != a.targets[u];
-                const int after = a.targets[v] != a.targets[u];
-                const unsigned long long first = (unsigned long long)((before - middle) * w);
-                const unsigned long long last = (unsigned long long)((middle - after) * w);
-                own += i < other ? first : last;
-                atomicAdd(&events[other], i < other ? last : first);
+            const int lu = a.labels[u], lv = a.labels[v];
+            const bool counted = u != v && (other == INT_MAX || (v < u && (lu == tx_a[tx] || lu == tx_b[tx])));
+            if (counted) {
+                const std::int64_t we = (std::int64_t)ew_at(a.ew, e);
+                const int tv = a.targets[v];
+                const int before = lv != lu;
+                if (other == i || other == INT_MAX) {
+                    const int after = tv != (other == i ? (int)a.targets[u] : lu);
+                    own = (unsigned long long)((before - after) * we);
+                } else {
+                    const int tu = a.targets[u];
+                    const int middle = i < other ? tv != lu : lv != tu;
+                    const int after = tv != tu;
+                    const unsigned long long first = (unsigned long long)((before - middle) * we);
+                    const unsigned long long last = (unsigned long long)((middle - after) * we);
+                    own = i < other ? first : last;
+                    const unsigned long long theirs = i < other ? last : first;
+                    if (theirs) atomicAdd(&events[other], theirs);
+                }
+                if (own) key = tx;
             }
         }
+        // Per distinct transaction among the lanes: one masked butterfly sum.
+        unsigned long long pending = __ballot(key >= 0);
+        while (pending) {
+            const int leader = __ffsll(pending) - 1;
+            const int lk = __shfl(key, leader);
+            const bool mine = key == lk;
+            const unsigned long long s = wave_allreduce(mine ? own : 0ULL, PlusU64{});
+            if (lane == leader) atomicAdd(&totals[lk], s);
+            pending &= ~__ballot(mine);
+        }
     }
-    own = wave_sum(own);
-    if (lane == 0 && own) atomicAdd(&events[i], own);
+    __syncthreads();
+    if (t < kEventTx && tx0 + t < count && totals[t]) atomicAdd(&events[tx0 + t], totals[t]);
 }
 // joint_prefix / joint_accept (exchange.cuh:263-289).  One wave per part
 // pa over its moves (the list is sorted by part): gain and net prefixes by
@@ -2113,9 +2275,10 @@ public:
     template <class Body>
     void tiers(const Body& body, const std::int32_t* list, std::int64_t n, const unsigned long long* count,
                std::int32_t* out, unsigned long long* out_count, unsigned long long* acc, int scratch,
-               int slot = kTier, bool zeroed = false) {
+               int slot = kTier, bool zeroed = false, int entries_slot = -1) {
         run_tiers(c_, body, list, n, count, medium_.get(), heavy_.get(), ctr_.get() + slot, out, out_count, acc,
-                  scratch, a_.n, g_.row_len_bound, zeroed);
+                  scratch, a_.n, g_.row_len_bound, zeroed,
+                  entries_slot >= 0 && c_.profiling() ? ctr_.get() + entries_slot : nullptr);
     }
 
     // One LP round over the frontier; returns the number of movers and the cut delta.
@@ -2139,8 +2302,10 @@ public:
                 tiers(LpProposeBody<unsigned long long>{{a_}}, frontier_.get(), 0, ctr_.get() + kFrontier, cand_.get(),
                       ctr_.get() + kCand, ctr_.get() + kRowSum, a_.k, kTier, true);
         });
-        tiers(AfterBody{{a_}}, cand_.get(), 0, ctr_.get() + kCand, movers_.get(), ctr_.get() + kMovers, nullptr, 0,
-              kTierB, true);
+        c_.timed("after", 0.0, [&] {
+            tiers(AfterBody{{a_}}, cand_.get(), 0, ctr_.get() + kCand, movers_.get(), ctr_.get() + kMovers, nullptr, 0,
+                  kTierB, true, kEntAfter);
+        });
         return apply(true, false, delta, true);
     }
 
@@ -2150,11 +2315,14 @@ public:
     std::int64_t apply(bool lp, bool samples, std::int64_t& delta, bool zeroed = false) {
         const hipStream_t s = c_.stream();
         if (!zeroed) fill<unsigned long long>(c_, ctr_.get() + kDelta, 1, 0ULL);
-        tiers(UpdateBody{{a_}, samples}, movers_.get(), 0, ctr_.get() + kMovers, nullptr, nullptr,
-              ctr_.get() + kDelta, 0, kTierC, zeroed);
-        hipLaunchKernelGGL(k_commit, dim3(grid_for(a_.n, 256, 1024)), dim3(256), 0, s, a_, movers_.get(),
-                           ctr_.get() + kMovers, lp);
-        GP_LAUNCH_CHECK();
+        if (!zeroed && c_.profiling()) fill<unsigned long long>(c_, ctr_.get() + kEntUpdate, 2, 0ULL);
+        c_.timed("update", 0.0, [&] {
+            tiers(UpdateBody{{a_}, samples}, movers_.get(), 0, ctr_.get() + kMovers, nullptr, nullptr,
+                  ctr_.get() + kDelta, 0, kTierC, zeroed, kEntUpdate);
+            hipLaunchKernelGGL(k_commit, dim3(grid_for(a_.n, 256, 1024)), dim3(256), 0, s, a_, movers_.get(),
+                               ctr_.get() + kMovers, lp);
+            GP_LAUNCH_CHECK();
+        });
         if (samples) samples_clear_ = false;
         std::vector<unsigned long long> h(static_cast<std::size_t>(kCtrCount + a_.k));
         // The read also clears the counters (not the part loads behind them).
@@ -2162,8 +2330,22 @@ public:
         ctr_clear_ = true;
         for (int p = 0; p < a_.k; ++p) loads_host_[static_cast<std::size_t>(p)] = h[static_cast<std::size_t>(kCtrCount + p)];
         delta = (std::int64_t)h[kDelta];
-        if (lp && c_.profiling())
-            c_.add_bytes("lp_propose", 28.0 * (double)h[kFrontier] + (8.0 + (a_.ew ? 4.0 : 0.0)) * (double)h[kRowSum]);
+        if (c_.profiling()) {
+            // Algorithmic bytes per row and per entry of each sweep (weighted
+            // graphs add 4 B of edge weight per entry): LP proposal 28 + 8;
+            // afterburner 16 + 20 (neighbour, target, gain, label); update and
+            // commit 24 + 16 (neighbour, label, target, flag); repair proposal
+            // 24 + 8 (offsets, label, weight, bucket and target out; neighbour
+            // and its label per entry).
+            const double w = a_.ew ? 4.0 : 0.0;
+            if (lp) {
+                c_.add_bytes("lp_propose", 28.0 * (double)h[kFrontier] + (8.0 + w) * (double)h[kRowSum]);
+                c_.add_bytes("after", 16.0 * (double)h[kEntAfter + 1] + (20.0 + w) * (double)h[kEntAfter]);
+            }
+            c_.add_bytes("update", 24.0 * (double)h[kEntUpdate + 1] + (16.0 + w) * (double)h[kEntUpdate]);
+            if (samples)
+                c_.add_bytes("repair_propose", 24.0 * (double)h[kEntRepair + 1] + (8.0 + w) * (double)h[kEntRepair]);
+        }
         return (std::int64_t)h[kMovers];
     }
 
@@ -2173,7 +2355,19 @@ public:
             cut -= (std::uint64_t)delta;
         } else
             cut += (std::uint64_t)(-delta);
+        if (o_.strict_verify) verify(cut);
         return true;
+    }
+
+    // Strict mode (incremental.cuh:846-853): after every committed batch the
+    // incremental cut must equal a full CSR recount of the labels and the
+    // part loads a full recount of the vertex weights.
+    void verify(std::uint64_t cut) {
+        if (cut_of(c_, g_, a_.labels) != cut) throw std::runtime_error("incremental cut differs from CSR scan");
+        std::vector<std::uint64_t> loads(static_cast<std::size_t>(a_.k));
+        loads_of(c_, g_, a_.labels, a_.k, loads.data());
+        if (loads != loads_host_) throw std::runtime_error("incremental part weights differ from a recount");
+        ++st_.strict_checks;
     }
 
     // Receiver reservation over the flagged donors L (vertex order).
@@ -2316,12 +2510,14 @@ public:
             hipLaunchKernelGGL(k_repair_donors, dim3(grid_for(n, 256 * kScanRounds, 2048)), dim3(256), 0, s, a_, cand_.get(),
                                ctr_.get() + kDonors);
             GP_LAUNCH_CHECK();
-            if (narrow_bins())
-                tiers(RepairProposeBody<unsigned>{{a_}}, cand_.get(), 0, ctr_.get() + kDonors, nullptr, nullptr, nullptr,
-                      a_.k, kTier, true);
-            else
-                tiers(RepairProposeBody<unsigned long long>{{a_}}, cand_.get(), 0, ctr_.get() + kDonors, nullptr,
-                      nullptr, nullptr, a_.k, kTier, true);
+            c_.timed("repair_propose", 0.0, [&] {
+                if (narrow_bins())
+                    tiers(RepairProposeBody<unsigned>{{a_}}, cand_.get(), 0, ctr_.get() + kDonors, nullptr, nullptr,
+                          nullptr, a_.k, kTier, true, kEntRepair);
+                else
+                    tiers(RepairProposeBody<unsigned long long>{{a_}}, cand_.get(), 0, ctr_.get() + kDonors, nullptr,
+                          nullptr, nullptr, a_.k, kTier, true, kEntRepair);
+            });
             hipLaunchKernelGGL(k_bucket_scan, dim3(k), dim3(kScanThreads), 0, s, a_, prefix.get(), bstar.get(),
                                excess.get());
             GP_LAUNCH_CHECK();
@@ -2381,7 +2577,8 @@ public:
             hipLaunchKernelGGL(k_joint_seeds, dim3(grid_for(n)), dim3(256), 0, s, a_, A.get(), partner.get(),
                                seeds.get(), sw, pass);
             GP_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_joint_grow, dim3(ceil_div(slots, 4)), dim3(256), 0, s, a_, A.get(), slots, sw,
+            hipLaunchKernelGGL(k_joint_grow, dim3(ceil_div(slots, kGrowWaves)), dim3(kGrowWaves * kWave), 0, s, a_,
+                               A.get(), slots, sw,
                                seeds.get(), partner.get(), grown.get(), archive.get());
             GP_LAUNCH_CHECK();
             // Archive: non-empty, sorted by net order, unique.
@@ -2413,7 +2610,8 @@ public:
             GP_LAUNCH_CHECK();
             DBuf<unsigned long long> events = c_.buf<unsigned long long>(static_cast<std::size_t>(active));
             fill<unsigned long long>(c_, events.get(), active, 0ULL);
-            hipLaunchKernelGGL(k_joint_events, dim3(ceil_div(active, 4)), dim3(256), 0, s, a_, moves.get(), active,
+            hipLaunchKernelGGL(k_joint_events, dim3(ceil_div(active, kEventTx)), dim3(kEventThreads), 0, s, a_,
+                               moves.get(), active,
                                owners.get(), events.get());
             hipLaunchKernelGGL(k_joint_prefix, dim3(ceil_div(k, 4)), dim3(256), 0, s, a_, moves.get(), active,
                                events.get(), stops.get());
@@ -2547,6 +2745,11 @@ LevelStats refine_level_impl(Context& c, const DGraph& g, Label* labels, std::ve
     const int k = o.parts;
     LevelRefiner R(c, g, labels, loads, o, cap, st);
     std::uint64_t cut = has_known_cut ? known_cut : cut_of(c, g, labels);
+    if (o.strict_verify && has_known_cut) {
+        // incremental.cuh:777-778: the carried cut must equal a recount.
+        if (cut_of(c, g, labels) != known_cut) throw std::runtime_error("known projection cut differs from CSR scan");
+        ++st.strict_checks;
+    }
     st.initial_cut = cut;
     if (R.max_load() > cap) {
         R.reset_samples();
@@ -2603,6 +2806,10 @@ LevelStats refine_level_impl(Context& c, const DGraph& g, Label* labels, std::ve
     for (auto x : loads) mx = std::max(mx, x);
     if (cut != best_cut || cut > feasible_start || mx > cap)
         throw std::runtime_error("incremental refinement postcondition failed");
+    if (o.strict_verify) {
+        // The restored checkpoint: its cut and loads against recounts.
+        R.verify(cut);
+    }
     return st;
 }
 
@@ -2645,14 +2852,24 @@ UncoarsenResult uncoarsen(Context& c, const Hierarchy& h, const std::int32_t* co
             GP_LAUNCH_CHECK();
         }
         labels = std::move(fl);
-        std::uint64_t maxload = 0;
-        for (auto x : loads) maxload = std::max(maxload, x);
-        const bool refine = lvl == 0 || maxload > cap || (double)fg.n >= o.growth_ratio * (double)last_refined;
         LevelStats ls;
         ls.n = fg.n;
         ls.m = fg.m;
+        if (o.strict_verify) {
+            // refine.cu:641-659: projection preserves the cut and the part loads.
+            std::vector<std::uint64_t> projected(static_cast<std::size_t>(k));
+            loads_of(c, fg, labels.get(), k, projected.data());
+            if (cut_of(c, fg, labels.get()) != last_cut || projected != loads)
+                throw std::runtime_error("GPU projection failed cut or part-weight preservation");
+            ++ls.strict_checks;
+        }
+        std::uint64_t maxload = 0;
+        for (auto x : loads) maxload = std::max(maxload, x);
+        const bool refine = lvl == 0 || maxload > cap || (double)fg.n >= o.growth_ratio * (double)last_refined;
         if (refine) {
+            const std::uint64_t projection_checks = ls.strict_checks;
             ls = refine_level_impl(c, fg, labels.get(), loads, o, cap, o.max_rounds, true, last_cut);
+            ls.strict_checks += projection_checks;
             if (!ls.initial_repair_failed) last_refined = fg.n;
         } else {
             ls.initial_cut = ls.final_cut = last_cut;

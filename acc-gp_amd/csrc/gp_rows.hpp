@@ -189,6 +189,9 @@ struct TierLists {
     std::int32_t* out;          // rows whose body asked for an append
     unsigned long long* out_count;
     unsigned long long* acc;    // sum of body accumulators (modular)
+    // Null, or two words that receive the entries and the number of the rows
+    // handled (algorithmic-byte accounting of the profiled runs).
+    unsigned long long* entries;
 };
 
 // Light tier: list (or the range [0, n) when list is null) of count rows,
@@ -215,7 +218,7 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const st
     __syncthreads();
     const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
     const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;
-    std::int64_t acc = 0;
+    std::int64_t acc = 0, rows_entries = 0, rows_done = 0;
     for (std::int64_t base = wave * G; base < count; base += waves * G) {
         const std::int64_t idx = base + gw;
         std::int32_t v = 0;
@@ -238,6 +241,10 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const st
         const int wmax = (int)wave_max((std::uint64_t)(idx < count && !defer ? deg : 0));
         if (idx < count && !defer) r = body(GroupTeam<W>{lg, wmax, part}, v, scratch);
         if (lg == 0) acc += r.acc;
+        if (lg == 0 && idx < count && !defer) {
+            rows_entries += deg;
+            ++rows_done;
+        }
         defer_stage.push(lg == 0 && defer && !heavy, v);
         out_stage.push(lg == 0 && r.append, v);
     }
@@ -246,6 +253,14 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const st
     if (t.acc) {
         acc = wave_sum(acc);
         if (lane == 0 && acc) atomicAdd(t.acc, (unsigned long long)acc);
+    }
+    if (t.entries) {
+        rows_entries = wave_sum(rows_entries);
+        rows_done = wave_sum(rows_done);
+        if (lane == 0 && rows_done) {
+            atomicAdd(t.entries, (unsigned long long)rows_entries);
+            atomicAdd(t.entries + 1, (unsigned long long)rows_done);
+        }
     }
 }
 
@@ -264,7 +279,7 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLis
     Stage out_stage{stage_buf[threadIdx.x / kWave], t.out, t.out_count, 0};
     const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
     const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;
-    std::int64_t acc = 0;
+    std::int64_t acc = 0, rows_entries = 0, rows_done = 0;
     for (std::int64_t base = wave * G; base < count; base += waves * G) {
         const std::int64_t idx = base + gw;
         std::int32_t v = 0;
@@ -274,6 +289,10 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLis
         if (idx < count) {
             v = t.medium[idx];
             r = body(GroupTeam<W>{lg, 1 << 30, part}, v, scratch);
+            if (t.entries && lg == 0) {
+                rows_entries += body.degree(v);
+                ++rows_done;
+            }
         }
         if (lg == 0) acc += r.acc;
         out_stage.push(lg == 0 && r.append, v);
@@ -282,6 +301,14 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLis
     if (t.acc) {
         acc = wave_sum(acc);
         if (lane == 0 && acc) atomicAdd(t.acc, (unsigned long long)acc);
+    }
+    if (t.entries) {
+        rows_entries = wave_sum(rows_entries);
+        rows_done = wave_sum(rows_done);
+        if (lane == 0 && rows_done) {
+            atomicAdd(t.entries, (unsigned long long)rows_entries);
+            atomicAdd(t.entries + 1, (unsigned long long)rows_done);
+        }
     }
 }
 
@@ -299,17 +326,25 @@ __global__ void __launch_bounds__(kHeavyThreads) k_tier_heavy(Body body, TierLis
     if ((std::int64_t)blockIdx.x >= count) return;
     const unsigned long long* part = stage_parts(body, smem);
     unsigned long long* scratch = smem + part_words_of(body);
-    std::int64_t acc = 0;
+    std::int64_t acc = 0, rows_entries = 0, rows_done = 0;
     for (std::int64_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
         const std::int32_t v = t.heavy[idx];
         RowOut r = body(BlockTeam{red, part}, v, scratch);
         if (threadIdx.x == 0) {
             acc += r.acc;
             if (t.out && r.append) t.out[atomicAdd(t.out_count, 1ULL)] = v;
+            if (t.entries) {
+                rows_entries += body.degree(v);
+                ++rows_done;
+            }
         }
         __syncthreads();
     }
     if (t.acc && threadIdx.x == 0 && acc) atomicAdd(t.acc, (unsigned long long)acc);
+    if (t.entries && threadIdx.x == 0 && rows_done) {
+        atomicAdd(t.entries, (unsigned long long)rows_entries);
+        atomicAdd(t.entries + 1, (unsigned long long)rows_done);
+    }
 }
 
 // Light launches stop at 1024 blocks (four per CU, all resident): each block
@@ -329,12 +364,12 @@ template <class Body>
 void run_tiers(Context& c, const Body& body, const std::int32_t* list, std::int64_t n, const unsigned long long* count_ptr,
                std::int32_t* medium, std::int32_t* heavy, unsigned long long* tier_counts, std::int32_t* out,
                unsigned long long* out_count, unsigned long long* acc, int scratch, std::int64_t max_rows,
-               std::uint64_t row_bound = ~0ULL, bool counts_zeroed = false) {
+               std::uint64_t row_bound = ~0ULL, bool counts_zeroed = false, unsigned long long* entries = nullptr) {
     const hipStream_t s = c.stream();
     const bool heavy_rows = row_bound > static_cast<std::uint64_t>(kTierMedium);
     const bool medium_rows = row_bound > static_cast<std::uint64_t>(kTierLight);
     if (!counts_zeroed) fill<unsigned long long>(c, tier_counts, 2, 0ULL);
-    TierLists t{medium, tier_counts, heavy, tier_counts + 1, out, out_count, acc};
+    TierLists t{medium, tier_counts, heavy, tier_counts + 1, out, out_count, acc, entries};
     const int lscratch = light_scratch_words<Body>(scratch);
     int lanes = c.light_lanes();
     while (lanes < 16 && (kTierThreads / lanes) * lscratch * sizeof(unsigned long long) > 48 * 1024) lanes *= 2;

@@ -29,6 +29,7 @@ class RefineOptions(ctypes.Structure):
     _fields_ = [
         ("parts", ctypes.c_int), ("imbalance", ctypes.c_double), ("max_rounds", ctypes.c_int),
         ("stale_rounds", ctypes.c_int), ("growth_ratio", ctypes.c_double), ("exchange_rounds", ctypes.c_int),
+        ("strict_verify", ctypes.c_int),
     ]
 
 
@@ -38,7 +39,7 @@ class LevelStats(ctypes.Structure):
         ("initial_repair_failed", ctypes.c_int), ("rounds", ctypes.c_int),
         ("initial_cut", ctypes.c_uint64), ("final_cut", ctypes.c_uint64),
         ("lp_moves", ctypes.c_uint64), ("repair_moves", ctypes.c_uint64), ("group_moves", ctypes.c_uint64),
-        ("repair_iterations", ctypes.c_int),
+        ("repair_iterations", ctypes.c_int), ("strict_checks", ctypes.c_uint64),
     ]
 
 
@@ -65,9 +66,12 @@ class Request(ctypes.Structure):
 # Every symbol declared in include/gpart_mi355x.h.
 EXPORTS = [
     "gp_last_error", "gp_context_create", "gp_context_destroy", "gp_context_reserved_bytes", "gp_context_set_overlap",
-    "gp_graph_create", "gp_graph_info", "gp_graph_download", "gp_graph_destroy", "gp_capacity",
+    "gp_graph_create", "gp_graph_info", "gp_graph_download", "gp_graph_metadata", "gp_graph_destroy", "gp_capacity",
     "gp_default_coarsen_options", "gp_coarsen", "gp_hierarchy_levels", "gp_hierarchy_level",
-    "gp_hierarchy_map_download", "gp_hierarchy_level_info", "gp_hierarchy_destroy",
+    "gp_hierarchy_map_download", "gp_hierarchy_level_info", "gp_hierarchy_assemble", "gp_graph_edge_weight",
+    "gp_hierarchy_level_capacity", "gp_hierarchy_stop_reason",
+    "gp_hierarchy_destroy", "gp_workspace_create", "gp_workspace_destroy", "gp_aggregate_level", "gp_aggregate_info",
+    "gp_aggregate_map_download", "gp_aggregate_vertex_weights_download", "gp_aggregate_destroy", "gp_contract",
     "gp_initial_partition_host", "gp_initial_partition", "gp_repair_capacity_host", "gp_initial_partition_backend",
     "gp_default_refine_options", "gp_uncoarsen",
     "gp_result_labels", "gp_result_cut", "gp_result_loads", "gp_result_levels", "gp_result_level_stats",
@@ -97,6 +101,7 @@ def lib():
         L.gp_graph_info.argtypes = [vp, vp, vp]
         L.gp_graph_download.argtypes = [vp, vp, vp, vp, vp, vp]
         L.gp_graph_destroy.argtypes = [vp]
+        L.gp_graph_metadata.argtypes = [vp, vp, vp, vp, vp]
         L.gp_capacity.argtypes = [u64, i32, dbl, vp]
         L.gp_default_coarsen_options.argtypes = [i32, u32, vp]
         L.gp_coarsen.argtypes = [vp, vp, vp, pp]
@@ -105,7 +110,20 @@ def lib():
         L.gp_hierarchy_level.restype = vp
         L.gp_hierarchy_map_download.argtypes = [vp, vp, i32, vp]
         L.gp_hierarchy_level_info.argtypes = [vp, i32, vp, vp, vp]
+        L.gp_hierarchy_level_capacity.argtypes = [vp, i32, vp]
+        L.gp_hierarchy_assemble.argtypes = [vp, i32, vp, vp, pp]
+        L.gp_graph_edge_weight.argtypes = [vp, vp, vp]
+        L.gp_hierarchy_stop_reason.argtypes = [vp]
+        L.gp_hierarchy_stop_reason.restype = ctypes.c_char_p
         L.gp_hierarchy_destroy.argtypes = [vp]
+        L.gp_workspace_create.argtypes = [pp]
+        L.gp_workspace_destroy.argtypes = [vp]
+        L.gp_aggregate_level.argtypes = [vp, vp, vp, i32, u32, i32, i32, i32, i32, i32, pp]
+        L.gp_aggregate_info.argtypes = [vp, vp, vp, vp]
+        L.gp_aggregate_map_download.argtypes = [vp, vp, vp]
+        L.gp_aggregate_vertex_weights_download.argtypes = [vp, vp, vp]
+        L.gp_aggregate_destroy.argtypes = [vp]
+        L.gp_contract.argtypes = [vp, vp, vp, vp, pp]
         L.gp_initial_partition_host.argtypes = [i64, vp, vp, vp, vp, i32, dbl, u32, i32, vp]
         L.gp_initial_partition.argtypes = [vp, vp, i32, dbl, u32, i32, vp]
         L.gp_repair_capacity_host.argtypes = [i64, vp, vp, vp, vp, i32, u64, vp, vp]

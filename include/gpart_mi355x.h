@@ -55,6 +55,7 @@ typedef struct gp_refine_options {        /* include/refine.hpp:13-26 */
     int stale_rounds;
     double growth_ratio;
     int exchange_rounds;
+    int strict_verify;                    /* recount cut and loads after every batch and projection */
 } gp_refine_options;
 
 typedef struct gp_level_stats {           /* RefineLevelResult, include/refine.hpp:60-68 */
@@ -65,6 +66,7 @@ typedef struct gp_level_stats {           /* RefineLevelResult, include/refine.h
     uint64_t initial_cut, final_cut;
     uint64_t lp_moves, repair_moves, group_moves;
     int repair_iterations;
+    uint64_t strict_checks;               /* recounts that matched under strict_verify */
 } gp_level_stats;
 
 typedef struct gp_partition_stats {       /* partition_timing line, app/partition_main.cu:163-177 */
@@ -104,6 +106,11 @@ int gp_graph_info(const gp_graph* g, int64_t* n, int64_t* m);
 /* copy_device_weighted: any output pointer may be NULL; ew expands unit weights. */
 int gp_graph_download(gp_context* ctx, const gp_graph* g, int32_t* offsets, int32_t* neighbors,
                       uint32_t* edge_weights, uint32_t* vertex_weights);
+/* DeviceWeightedGraph metadata (include/graph.hpp:34-57): degree bound,
+ * heavy rows, incident-weight bound (unknown: UINT64_MAX / 1 until the
+ * graph has been coarsened or aggregated) and implicit unit edge weights. */
+int gp_graph_metadata(const gp_graph* g, uint64_t* max_degree, int* has_heavy, uint64_t* incident_bound,
+                      int* implicit_unit_weights);
 void gp_graph_destroy(gp_graph* g);
 
 /* partition_capacity (include/partition_capacity.hpp:14-25). */
@@ -122,7 +129,47 @@ int gp_hierarchy_map_download(gp_context* ctx, const gp_hierarchy* h, int level,
 /* Level metadata carried by contraction: max degree bound, heavy rows, incident weight bound. */
 int gp_hierarchy_level_info(const gp_hierarchy* h, int level, uint64_t* max_degree, int* has_heavy,
                             uint64_t* incident_bound);
+/* A hierarchy from levels built elsewhere (the host-hierarchy overloads of
+ * refine_hierarchy_device / uncoarsen, include/refine.hpp:90-94 and
+ * include/uncoarsen.hpp:11-14): level_graphs[0..levels) made by
+ * gp_graph_create or gp_contract, finest first, and host maps[i] of
+ * level i's length with ids in [0, n_{i+1}).  On success the hierarchy owns
+ * every level graph (their handles are no longer the caller's; level 0 is
+ * released with the hierarchy); on failure nothing is taken.  Level
+ * metadata is recomputed exactly; the cluster caps are unknown (UINT64_MAX). */
+int gp_hierarchy_assemble(gp_context* ctx, int levels, gp_graph* const* level_graphs, const int32_t* const* maps,
+                          gp_hierarchy** out);
+/* Undirected edge weight of a device graph (DeviceUncoarsenResult::total_edge_weight,
+ * src/refine.cu:704-712); GP_ERR_RUNTIME when the directed sum is odd. */
+int gp_graph_edge_weight(gp_context* ctx, const gp_graph* g, uint64_t* total);
+/* Cluster weight cap of the aggregation that produced level+1 (the cap
+ * checked by validate_coarsening_step, src/coarsen.cu:1692-1758), level < levels-1. */
+int gp_hierarchy_level_capacity(const gp_hierarchy* h, int level, uint64_t* capacity);
+/* DeviceHierarchy::stop_reason: "capacity_floor", "insufficient_contraction" or "max_levels". */
+const char* gp_hierarchy_stop_reason(const gp_hierarchy* h);
 void gp_hierarchy_destroy(gp_hierarchy* h);
+
+/* One level at a time: aggregate() and contract() with an SclpWorkspace
+ * (include/coarsen.hpp:55-104).  The workspace carries the coarsening state
+ * from a graph to the graph gp_contract makes from it, so aggregating level 0,
+ * contracting, aggregating the result, ... reproduces gp_coarsen's levels
+ * when the caller passes the level loop's per-level seed, factor and
+ * seed-matching decision; a graph the workspace did not produce starts from
+ * its level-0 metadata. */
+typedef struct gp_workspace gp_workspace;
+typedef struct gp_aggregate gp_aggregate; /* DeviceAggregateResult: map + coarse vertex weights */
+int gp_workspace_create(gp_workspace** out);
+void gp_workspace_destroy(gp_workspace* ws);
+int gp_aggregate_level(gp_context* ctx, gp_workspace* ws, gp_graph* g, int parts, uint32_t seed, int level,
+                       int cluster_size_factor, int pack_isolated, int seed_matching, int fast_path,
+                       gp_aggregate** out);
+int gp_aggregate_info(const gp_aggregate* a, int64_t* coarse_vertices, uint64_t* maximum_weight, uint64_t* capacity);
+int gp_aggregate_map_download(gp_context* ctx, const gp_aggregate* a, int32_t* map);
+int gp_aggregate_vertex_weights_download(gp_context* ctx, const gp_aggregate* a, uint32_t* vertex_weights);
+void gp_aggregate_destroy(gp_aggregate* a);
+/* contract(fine, std::move(aggregate), workspace): the coarse graph; the
+ * aggregate's vertex weights move into it (its map stays readable). */
+int gp_contract(gp_context* ctx, gp_workspace* ws, const gp_graph* fine, gp_aggregate* a, gp_graph** out);
 
 /* ---- initial partition (include/initial_partition.hpp:6-8) ------------ */
 /* initial_partition_metis(graph, parts, eps, seed, require_feasible) on a host

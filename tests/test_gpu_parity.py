@@ -169,3 +169,34 @@ def test_edge_cut_of_device_labels(gp, og, ctx):
         finally:
             h.close()
             g.close()
+
+
+def test_strict_verify_recounts_and_changes_nothing(gp, ctx, cases):
+    # strict_verify (the CLI's --verify; include/refine.hpp:17) recounts the
+    # cut and the loads after every projection and every committed batch of
+    # LP, repair and exchange moves (refine.cu:641-659, incremental.cuh:846-853)
+    # and must leave the result bit-identical.
+    picked = {"grid256_k8", "knn3000_k16_weighted", "hub12000_k8_skewed_weights", "rmat16_k64_heavy_rows"}
+    for name, off, nbr, ew, vw, k in cases:
+        if name not in picked:
+            continue
+        _say("case %s: strict verification" % name)
+        g = gp.Graph(ctx, off, nbr, ew, vw)
+        h = gp.Hierarchy(ctx, g, gp.coarsen_options(k))
+        try:
+            coff, cnbr, cew, cvw = h.graph_at(h.levels - 1)
+            init = gp.initial_partition_host(coff, cnbr, cew, cvw, k)
+            lab, cut, loads, st = h.uncoarsen(init, gp.refine_options(k))
+            slab, scut, sloads, sst = h.uncoarsen(init, gp.refine_options(k, strict_verify=1))
+        finally:
+            h.close()
+            g.close()
+        assert np.array_equal(lab, slab) and cut == scut and np.array_equal(loads, sloads), name
+        assert all(s["strict_checks"] == 0 for s in st), name
+        for lvl, s in enumerate(sst):
+            moved = s["lp_moves"] + s["repair_moves"] + s["group_moves"]
+            if lvl > 0:
+                assert s["strict_checks"] >= 1, (name, lvl, s)  # the projection check
+            if s["refined"] and not s["initial_repair_failed"]:
+                assert s["strict_checks"] >= 1 + (1 if moved else 0), (name, lvl, s)
+        _say("case %s: ok, %d recounts" % (name, sum(s["strict_checks"] for s in sst)))

@@ -10,8 +10,15 @@ line.
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
+
+# The CPU baseline's OpenMP threads are pinned to cores; set before the
+# oracle library (and its OpenMP runtime) is loaded.
+os.environ.setdefault("OMP_PLACES", "cores")
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 import numpy as np
 
@@ -29,16 +36,29 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--parts", type=int, default=64)
-    ap.add_argument("--cpu-scale", type=int, default=19, help="R-MAT scale of the bounded CPU-oracle sample")
+    ap.add_argument("--cpu-scale", type=int, default=22, help="R-MAT scale of the CPU baseline run")
+    ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-graphs", type=int, default=256, help="graphs in the batched mix (0: skip)")
     ap.add_argument("--batch-workers", type=int, default=6)
     ap.add_argument("--serve-requests", type=int, default=128, help="requests of the serving leg (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 1 GiB device-to-device copies (counter calibration) and exit")
+    ap.add_argument("--trace-mix", action="store_true",
+                    help="only run the batched mix through the plain per-graph loop (one host thread, for kernel "
+                         "traces of the mixed workload), then exit")
+    ap.add_argument("--quality-anchor", action="store_true",
+                    help="only partition the 4000x2000 grid at k=4..128 and compare the cuts and times with the "
+                         "reference's published run of the same graph, then exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.quality_anchor:
+        quality_anchor()
+        return
+    if args.trace_mix:
+        trace_mix(args.batch_graphs)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; multi-GPU sharding is a design note, not a run.",
@@ -76,7 +96,7 @@ def main():
         print("bench: step %d %.3fs (algorithm %.3fs) cut=%d" % (i, per_step[-1], st["algorithm_total_seconds"],
                                                                    st["final_cut"]), file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
-    prof = {name: ctx.profile_query(name) for name in ("lp_propose", "contract_scatter")}
+    prof = {name: ctx.profile_query(name) for name, _, _ in ROOFLINE_CLASSES}
     ctx.profile(False)
 
     # Verification of one result: exact recount and strict capacity.
@@ -119,12 +139,10 @@ def main():
             "last_step_phases": phases,
         },
     }
-    # Dominant kernel class: the LP proposal sweep (three tier launches per
-    # sweep, timed as one unit with HIP events on the library's stream).
-    # The contraction scatter is reported beside it.
-    for key, name, kernels in (("roofline", "lp_propose", "k_tier_{light,medium,heavy}<LpProposeBody>"),
-                               ("roofline_contraction", "contract_scatter",
-                                "k_tier_{light,medium,heavy}<ScatterBody>")):
+    # Rooflines of the row-sweep classes, each timed as one unit (its tier
+    # launches) with HIP events on the library's stream; "roofline" is the
+    # LP proposal sweep, the class the headline fraction is quoted on.
+    for name, key, kernels in ROOFLINE_CLASSES:
         ms, byts, launches = prof[name]
         if not launches:
             continue
@@ -140,10 +158,38 @@ def main():
                                          0.7 * out["batched"]["graphs"] / out["batched"]["seconds"])
     if not args.no_cpu:
         from oracle import oracle as og
-        out["cpu_baseline"] = cpu_baseline(gp, og, args.cpu_scale, args.parts)
+        out["cpu_baseline"] = cpu_baseline(gp, og, args.cpu_scale, args.parts, args.cpu_repeats)
     g.close()
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+ROOFLINE_CLASSES = (
+    ("lp_propose", "roofline", "k_tier_{light,medium,heavy}<LpProposeBody>"),
+    ("contract_scatter", "roofline_contraction", "k_tier_{light,medium,heavy}<ScatterBody>"),
+    ("after", "roofline_afterburner", "k_tier_{light,medium,heavy}<AfterBody>"),
+    ("update", "roofline_update", "k_tier_{light,medium,heavy}<UpdateBody> + k_commit"),
+    ("repair_propose", "roofline_repair", "k_tier_{light,medium,heavy}<RepairProposeBody>"),
+)
+
+
+def trace_mix(count):
+    """The mixed workload through the plain per-graph loop on one host
+    thread: the form the kernel traces of the mix are taken from."""
+    from tests import workloads
+    from tests.conftest import load_package
+    gp = load_package()
+    mix = workloads.batch_mix(gp, count)
+    ctx = gp.Context(0)
+    t0 = time.perf_counter()
+    for i, (name, off, nbr, k) in enumerate(mix):
+        g = gp.Graph(ctx, off, nbr)
+        g.partition(k)
+        g.close()
+        if i % 32 == 31:
+            print("trace-mix: %d graphs, %.1fs" % (i + 1, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    ctx.close()
+    print(json.dumps({"trace_mix_graphs": len(mix), "seconds": time.perf_counter() - t0}), flush=True)
 
 
 def batched_leg(gp, ctx, count, workers):
@@ -233,6 +279,51 @@ def serving_leg(gp, ctx, mix, done_items, workers, count, rate, seed=6):
             "warm_vs_cold_cut_geomean": float(np.exp(np.mean(np.log(ratios)))) if ratios else None}
 
 
+# The reference's own run of its partitioner on the 4000x2000 grid
+# (15,994,000 edges): k -> (cut, seconds), the "ours" columns of
+# results/manual/regret-20261017/comparison.csv:44-49 (median of 3 repeats on
+# an RTX 5090), and Jet's cut from the same rows.
+GRID_ANCHOR = {4: (8620, 0.068675156, 8228), 8: (14582, 0.071301289, 13798), 16: (24194, 0.090550177, 23186),
+               32: (36898, 0.088336934, 34214), 64: (55214, 0.09714246, 51845), 128: (77453, 0.12128948, 73225)}
+
+
+def quality_anchor(repeats=3):
+    """Cut quality against the reference on the one published input that can
+    be regenerated here: the 4000x2000 4-neighbour grid.  Vertex numbering
+    (row-major here) and the initial partitioner (METIS 5.2.1 there, the
+    in-repo bisection here) differ, so the ratio is a sanity anchor."""
+    from tests.conftest import load_package
+    gp = load_package()
+    off, nbr = gp.generate_grid(4000, 2000)
+    edges = len(nbr) // 2
+    ctx = gp.Context(0)
+    g = gp.Graph(ctx, off, nbr)
+    rows = {}
+    for k, (ref_cut, ref_s, jet_cut) in sorted(GRID_ANCHOR.items()):
+        g.partition(k, 1.03, 0, download=False)
+        secs = []
+        for _ in range(repeats):
+            _, st = g.partition(k, 1.03, 0, download=False)
+            secs.append(st["algorithm_total_seconds"])
+        labels, st = g.partition(k, 1.03, 0, download=True)
+        src = np.repeat(np.arange(len(off) - 1, dtype=np.int64), np.diff(off))
+        recount = int(np.count_nonzero(labels[src] != labels[nbr])) // 2
+        cap = gp.capacity(len(off) - 1, k, 1.03)
+        if recount != st["final_cut"] or np.bincount(labels, minlength=k).max() > cap:
+            raise RuntimeError("quality anchor: result failed verification")
+        rows[k] = {"cut": int(st["final_cut"]), "reference_cut": ref_cut, "cut_ratio_vs_reference": st["final_cut"] / ref_cut,
+                   "jet_cut": jet_cut, "seconds_median": float(np.median(secs)), "reference_seconds": ref_s,
+                   "speedup_vs_reference": ref_s / float(np.median(secs)), "levels": st["levels"]}
+        print("anchor: grid 4000x2000 k=%d cut=%d (reference %d, ratio %.4f) %.1f ms (reference %.1f ms)" % (
+            k, st["final_cut"], ref_cut, st["final_cut"] / ref_cut, 1e3 * np.median(secs), 1e3 * ref_s),
+              file=sys.stderr, flush=True)
+    g.close()
+    ctx.close()
+    geo = float(np.exp(np.mean([np.log(r["cut_ratio_vs_reference"]) for r in rows.values()])))
+    print(json.dumps({"anchor": "grid-4000x2000", "undirected_edges": edges, "imbalance": 1.03, "seed": 0,
+                      "cut_geomean_ratio_vs_reference": geo, "parts": rows}), flush=True)
+
+
 def copy_calibration():
     """Three plain 1 GiB device copies: what FETCH_SIZE / WRITE_SIZE read
     for a known byte count scales the counters of the partitioner."""
@@ -258,19 +349,69 @@ def traffic_of(name):
     return None
 
 
-def cpu_baseline(gp, og, scale, parts):
-    """Single-threaded CPU oracle on a bounded R-MAT sample (same schedule)."""
-    off, nbr = gp.generate_rmat(scale, 16, 0.57, 0.19, 0.19, seed=1)
+def cpu_partition(gp, og, off, nbr, parts):
+    """The CPU path: the oracle's coarsening (OpenMP), the host initial
+    partition of the coarsest graph, the oracle's uncoarsening; seconds in
+    the algorithm_total_seconds scope (graph in memory -> labels in memory)."""
     t = time.perf_counter()
     oh = og.Hierarchy(off, nbr, None, None, og.coarsen_opts(parts))
     coff, cnbr, cew, cvw = oh.graph(oh.levels - 1)
     init = gp.initial_partition_host(coff, cnbr, cew, cvw, parts)
-    oh.uncoarsen(init, og.refine_opts(parts))
-    secs = time.perf_counter() - t
-    edges = len(nbr) // 2
-    return {"value": edges / secs, "unit": "edges/s", "cores": 1, "kind": "port",
-            "sample": "R-MAT scale %d ef16 (%d undirected edges) k=%d, oracle coarsen + host initial partition + "
-                      "oracle uncoarsen, %.1fs" % (scale, edges, parts, secs)}
+    labels, cut, loads, _ = oh.uncoarsen(init, og.refine_opts(parts))
+    return time.perf_counter() - t, labels, cut
+
+
+def verify_with_checker(gp, off, nbr, labels, parts, cut):
+    """gpart_verify_csr on the written CSR and partition: exact cut and loads
+    within the integer capacity."""
+    tool = os.path.join(ROOT, "acc-gp_amd", "bin", "gpart_verify_csr")
+    with tempfile.TemporaryDirectory() as d:
+        ip, ix, pp = os.path.join(d, "g.indptr"), os.path.join(d, "g.indices"), os.path.join(d, "part.bin")
+        gp.save_csr(ip, ix, off, nbr)
+        np.asarray(labels, "<i4").tofile(pp)
+        res = subprocess.run([tool, ip, ix, str(parts), pp, "binary", "--symmetric"], capture_output=True, text=True)
+    got = json.loads(res.stdout)
+    ok = res.returncode == 0 and got["cut"] == cut and max(got["loads"]) <= got["capacity"]
+    if not ok:
+        raise RuntimeError("cpu baseline: gpart_verify_csr rejected the result: %s" % res.stdout)
+    return got
+
+
+def cpu_baseline(gp, og, scale, parts, repeats):
+    """The repo's CPU path on the box's host cores: the OpenMP oracle
+    (the same schedule as the device) with the host initial partitioner, on
+    the 256x256 grid at k=8 and on the bench's R-MAT configuration; median
+    of `repeats` runs (one when a run exceeds 20 s), every result checked by
+    gpart_verify_csr."""
+    threads = og.get_threads()
+    out = {"unit": "edges/s", "cores": threads, "kind": "port", "nproc": os.cpu_count(),
+           "omp_places": os.environ.get("OMP_PLACES"), "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
+    runs = {}
+    for name, (off, nbr), k in (("grid256_k8", gp.generate_grid(256, 256), 8),
+                               ("rmat%d_k%d" % (scale, parts), gp.generate_rmat(scale, 16, 0.57, 0.19, 0.19, seed=1),
+                                parts)):
+        secs, cut = [], None
+        for r in range(repeats):
+            s, labels, cut = cpu_partition(gp, og, off, nbr, k)
+            secs.append(s)
+            if r == 0:
+                verify_with_checker(gp, off, nbr, labels, k, cut)
+            if s > 20.0:
+                break
+        edges = len(nbr) // 2
+        med = float(np.median(secs))
+        runs[name] = {"undirected_edges": edges, "parts": k, "seconds_median": med, "repeats": len(secs),
+                      "edges_per_s": edges / med, "cut": int(cut), "verified": True}
+        print("bench: cpu baseline %s %.2fs (%d threads) cut=%d" % (name, med, threads, cut), file=sys.stderr,
+              flush=True)
+    main_run = runs["rmat%d_k%d" % (scale, parts)]
+    out["value"] = main_run["edges_per_s"]
+    out["sample"] = ("R-MAT scale %d ef16 (%d undirected edges) k=%d, median of %d: OpenMP oracle coarsen + host "
+                     "initial partition + oracle uncoarsen on %d threads, checked by gpart_verify_csr; also the "
+                     "256x256 grid at k=8" % (scale, main_run["undirected_edges"], parts, main_run["repeats"],
+                                              threads))
+    out["runs"] = runs
+    return out
 
 
 if __name__ == "__main__":

@@ -1,8 +1,13 @@
 /*
  * gp_oracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY) for the multilevel
  * k-way partitioner hot path: coarsening, contraction, projection and
- * refinement.  Plain single-threaded C; every function cites the reference
- * file:line it restates (paths relative to the Acc-GP source tree).
+ * refinement.  Plain C; every function cites the reference file:line it
+ * restates (paths relative to the Acc-GP source tree).  Built with -fopenmp,
+ * the loops whose iterations are independent (per-vertex proposals, seed
+ * choices, coarse-row merges, LP proposals, afterburner, cut and affinity
+ * sums) run on all threads: each iteration writes only its own outputs, the
+ * order-carrying steps (compactions, bucket counts, commits) stay serial, and
+ * sums are integer, so results do not depend on the thread count.
  *
  * Canonical orders (the reference's atomic arrival order is replaced by vertex
  * id order; the GPU library follows the same rule):
@@ -29,6 +34,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define OG_INVALID 0x7fffffff
 #define OG_KMAX 128
@@ -56,6 +64,24 @@ static void* og_alloc(size_t count, size_t size) {
 }
 
 /* src/coarsen.cu:173-179, src/coarsening/seed_matching.cuh:7-10, src/refine.cu:50-56 */
+#ifdef _OPENMP
+static int og_threads(void) { return omp_get_max_threads(); }
+static int og_thread(void) { return omp_get_thread_num(); }
+#else
+static int og_threads(void) { return 1; }
+static int og_thread(void) { return 0; }
+#endif
+
+void og_set_threads(int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+}
+
+int og_get_threads(void) { return og_threads(); }
+
 static inline uint32_t mix32(uint32_t x) {
     x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU;
     return x ^ (x >> 16);
@@ -212,7 +238,8 @@ typedef struct {
     uint64_t* regret;
     uint32_t* ties;
     uint64_t cap;
-    og_cw_pair* scratch;
+    og_cw_pair* scratch;     /* scratch_stride entries per thread */
+    int64_t scratch_stride;
 } og_agg;
 
 /* src/coarsening/aggregation.cuh:79-106 */
@@ -231,10 +258,13 @@ static void og_form_proposals(og_agg* A, uint32_t seed, int level, int round,
     const uint32_t role_salt = seed ^ ((uint32_t)level * 0x9e3779b9U) ^ ((uint32_t)round * 0x85ebca6bU) ^ 0x27d4eb2dU;
     const uint32_t tie_salt = seed ^ ((uint32_t)level * 0xc2b2ae35U) ^ ((uint32_t)round * 0x165667b1U) ^ 0xd3a2646cU;
     const int use_second = filter_roles && need_regret;
+#pragma omp parallel for schedule(static)
     for (int64_t v = 0; v < n; ++v)
         A->roles[v] = A->clusters[v] == v && A->cw[v] == (uint64_t)g->vw[v] &&
                       (!filter_roles || mix32((uint32_t)v ^ role_salt) < threshold);
+#pragma omp parallel for schedule(dynamic, 256)
     for (int64_t v = 0; v < n; ++v) {
+        og_cw_pair* scratch = A->scratch + (size_t)og_thread() * (size_t)A->scratch_stride;
         A->proposals[v] = OG_INVALID; A->gains[v] = 0; A->ties[v] = 0;
         const int32_t src = A->clusters[v];
         if (!A->roles[src]) continue;
@@ -242,16 +272,16 @@ static void og_form_proposals(og_agg* A, uint32_t seed, int level, int round,
         for (int64_t e = g->off[v]; e < g->off[v + 1]; ++e) {
             const int32_t u = g->nbr[e];
             if (u == v) continue;
-            A->scratch[cnt].c = A->clusters[u];
-            A->scratch[cnt].w = ew_at(g->ew, e);
+            scratch[cnt].c = A->clusters[u];
+            scratch[cnt].w = ew_at(g->ew, e);
             ++cnt;
         }
-        qsort(A->scratch, (size_t)cnt, sizeof(og_cw_pair), cmp_cw_pair);
+        qsort(scratch, (size_t)cnt, sizeof(og_cw_pair), cmp_cw_pair);
         uint64_t current = 0, best = 0, tie = 0, second = 0;
         for (int i = 0; i < cnt;) {
-            const int32_t c = A->scratch[i].c;
+            const int32_t c = scratch[i].c;
             uint64_t val = 0;
-            while (i < cnt && A->scratch[i].c == c) val += A->scratch[i++].w;
+            while (i < cnt && scratch[i].c == c) val += scratch[i++].w;
             if (c == src) { current = val; continue; }
             if (filter_roles) {
                 const uint64_t base = A->cw[c];
@@ -291,6 +321,7 @@ static int64_t og_seed_match(og_agg* A, uint32_t salt) {
     const int64_t n = A->n;
     const uint64_t cap = A->cap;
     int32_t* choices = og_alloc((size_t)n, sizeof(int32_t));
+#pragma omp parallel for schedule(dynamic, 256)
     for (int64_t v = 0; v < n; ++v) {
         choices[v] = -1;
         const int64_t deg = g->off[v + 1] - g->off[v];
@@ -444,7 +475,8 @@ static int og_aggregate(const og_graph* g, og_cstate* st, int parts, uint32_t se
         int64_t d = g->off[v + 1] - g->off[v];
         if (d > maxdeg_row) maxdeg_row = d;
     }
-    A.scratch = og_alloc((size_t)maxdeg_row + 1, sizeof(og_cw_pair));
+    A.scratch_stride = maxdeg_row + 1;
+    A.scratch = og_alloc((size_t)A.scratch_stride * (size_t)og_threads(), sizeof(og_cw_pair));
 
     int64_t initial_count = n;
     if (seed_matching) initial_count = og_seed_match(&A, seed ^ ((uint32_t)level * 0x9e3779b9U));
@@ -563,6 +595,8 @@ static int og_contract_csr(int64_t n, const int32_t* off, const int32_t* nbr, co
     }
     int32_t* coff = og_alloc((size_t)nc + 1, sizeof(int32_t));
     int64_t total = 0;
+    int overflow = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total) reduction(| : overflow)
     for (int64_t c = 0; c < nc; ++c) {
         qsort(rec + raw[c], (size_t)(raw[c + 1] - raw[c]), sizeof(og_tw), cmp_tw);
         int64_t out = raw[c];
@@ -571,12 +605,13 @@ static int og_contract_csr(int64_t n, const int32_t* off, const int32_t* nbr, co
             uint64_t sum = 0;
             while (i < raw[c + 1] && rec[i].t == t) sum += rec[i++].w;
             if (t == INT32_MAX) continue;
-            if (sum > UINT32_MAX) { free(raw); free(rec); free(cur); free(coff); return og_fail("coarse edge weight exceeds uint32"); }
+            if (sum > UINT32_MAX) overflow = 1;
             rec[out].t = t; rec[out].w = (uint32_t)sum; ++out;
         }
         cur[c] = out - raw[c];
         total += cur[c];
     }
+    if (overflow) { free(raw); free(rec); free(cur); free(coff); return og_fail("coarse edge weight exceeds uint32"); }
     if (total > INT32_MAX) return og_fail("coarse graph exceeds int32 offsets");
     int32_t* cn = og_alloc((size_t)total, sizeof(int32_t));
     uint32_t* cwt = og_alloc((size_t)total, sizeof(uint32_t));
@@ -730,6 +765,7 @@ int og_contract_graph(int64_t n, int64_t m, const int32_t* off, const int32_t* n
 /* ============================================================ refinement */
 uint64_t og_cut(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew, const int32_t* labels) {
     uint64_t d = 0;                                                        /* refine.cu:63-98 */
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : d)
     for (int64_t v = 0; v < n; ++v)
         for (int64_t e = off[v]; e < off[v + 1]; ++e)
             if (labels[v] != labels[nbr[e]]) d += ew_at(ew, e);
@@ -749,7 +785,7 @@ typedef struct {
     int64_t* gains;
     int32_t* locks;
     int32_t* flags;
-    uint64_t* conn;      /* k scratch */
+    uint64_t* conn;      /* k scratch per thread */
     double* prices;      /* NULL when disabled */
     og_sample* samples;
     double pow15[48];    /* 1.5^j, j=-23..24, index j+23 */
@@ -789,6 +825,8 @@ static uint64_t og_max_load(const og_ref* R) {
 static int64_t og_apply(og_ref* R, const int32_t* movers, int64_t count, int lp, int with_samples) {
     const og_graph* g = R->g;
     uint64_t delta = 0;
+    const int sampled = with_samples && R->samples;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : delta) if (!sampled)
     for (int64_t i = 0; i < count; ++i) {
         const int32_t v = movers[i];
         const int32_t s = R->labels[v], t = R->targets[v];
@@ -805,7 +843,7 @@ static int64_t og_apply(og_ref* R, const int32_t* movers, int64_t count, int lp,
             }
         }
         delta += gain;
-        if (with_samples && R->samples) {
+        if (sampled) {
             R->samples[s].loss += -(int64_t)gain;
             R->samples[s].weight += g->vw[v];
         }
@@ -841,17 +879,21 @@ static int64_t og_lp_propose(og_ref* R, double filter, int32_t* movers) {
     const int64_t n = R->n;
     const int k = R->k;
     int64_t ncand = 0;
+    /* Per-vertex choices in parallel (flags marks the candidates), then the
+     * candidate list in vertex order. */
+#pragma omp parallel for schedule(dynamic, 256)
     for (int64_t v = 0; v < n; ++v) {
+        uint64_t* conn = R->conn + (size_t)og_thread() * (size_t)k;
         R->targets[v] = -1; R->gains[v] = INT64_MIN; R->flags[v] = 0;
         if (R->locks[v]) { R->locks[v] = 0; continue; }
         const int s = R->labels[v];
-        const uint64_t total = og_affinity(R, v, R->conn);
-        const uint64_t current = R->conn[s];
+        const uint64_t total = og_affinity(R, v, conn);
+        const uint64_t current = conn[s];
         if (current == total) continue;
         const uint64_t w = g->vw[v];
         uint64_t best = 0; int target = -1; double best_score = -1.7976931348623157e308;
         for (int p = 0; p < k; ++p) {
-            const uint64_t c = R->conn[p];
+            const uint64_t c = conn[p];
             if (p == s || !c) continue;
             if (R->prices) {
                 const double score = (double)c - og_repair_cost(R, p, w);
@@ -861,9 +903,11 @@ static int64_t og_lp_propose(og_ref* R, double filter, int32_t* movers) {
         if (target >= 0 && (best >= current || current - best < (uint64_t)(filter * (double)current))) {
             R->targets[v] = target;
             R->gains[v] = best >= current ? (int64_t)(best - current) : -(int64_t)(current - best);
-            movers[ncand++] = (int32_t)v;
+            R->flags[v] = 1;
         }
     }
+    for (int64_t v = 0; v < n; ++v)
+        if (R->flags[v]) { R->flags[v] = 0; movers[ncand++] = (int32_t)v; }
     return ncand;
 }
 
@@ -876,6 +920,7 @@ static int64_t og_lp_propose(og_ref* R, double filter, int32_t* movers) {
 static int64_t og_afterburner(og_ref* R, int32_t* movers, int64_t ncand) {
     const og_graph* g = R->g;
     int64_t count = 0;
+#pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < ncand; ++i) {
         const int32_t v = movers[i];
         const int s = R->labels[v], t = R->targets[v];
@@ -1015,14 +1060,16 @@ static int64_t og_repair_select(og_ref* R, og_repair_ws* W) {
     memcpy(W->loads0, R->loads, (size_t)k * sizeof(uint64_t));
     memset(W->counts, 0, (size_t)k * OG_BALANCE_WIDTH * sizeof(uint64_t));
     for (int p = 0; p < k; ++p) { W->slack[p] = W->loads0[p] < cap ? cap - W->loads0[p] : 0; W->requests[p] = 0; }
+#pragma omp parallel for schedule(dynamic, 256)
     for (int64_t v = 0; v < n; ++v) {
+        uint64_t* conn = R->conn + (size_t)og_thread() * (size_t)k;
         W->bucket[v] = -1; R->targets[v] = -1; R->flags[v] = 0;
         const int s = R->labels[v];
         if (!(W->loads0[s] > cap)) continue;
         const uint64_t w = g->vw[v];
-        const uint64_t total = og_affinity(R, v, R->conn);
+        const uint64_t total = og_affinity(R, v, conn);
         uint64_t best_conn, current;
-        const int best = og_repair_destination(R, v, s, w, R->conn, total, &best_conn, &current);
+        const int best = og_repair_destination(R, v, s, w, conn, total, &best_conn, &current);
         if (best < 0) continue;
         const double gain = best_conn >= current ? (double)(best_conn - current) : -(double)(current - best_conn);
         const uint64_t excess = W->loads0[s] - cap;
@@ -1031,8 +1078,9 @@ static int64_t og_repair_select(og_ref* R, og_repair_ws* W) {
         R->targets[v] = best;
         W->bucket[v] = s * OG_BALANCE_WIDTH + og_gain_bucket(R, gpw) * OG_BALANCE_SECTIONS +
                        (int)(mix32((uint32_t)v) % OG_BALANCE_SECTIONS);
-        W->counts[W->bucket[v]] += w;
     }
+    for (int64_t v = 0; v < n; ++v)
+        if (W->bucket[v] >= 0) W->counts[W->bucket[v]] += g->vw[v];
     for (int p = 0; p < k; ++p) {
         uint64_t sum = 0;
         for (int b = 0; b < OG_BALANCE_WIDTH; ++b) {
@@ -1333,6 +1381,7 @@ static void og_joint_combine(og_ref* R, int count, const og_move* archive, int i
 }
 
 static void og_dense_affinity(og_ref* R, uint64_t* A) {
+#pragma omp parallel for schedule(dynamic, 256)
     for (int64_t v = 0; v < R->n; ++v) og_affinity(R, v, A + (size_t)v * R->k);
 }
 
@@ -1508,7 +1557,7 @@ static int og_refine_level(const og_graph* g, int32_t* labels, uint64_t* loads, 
     R.gains = og_alloc((size_t)n, sizeof(int64_t));
     R.locks = og_alloc((size_t)n, sizeof(int32_t));
     R.flags = og_alloc((size_t)n, sizeof(int32_t));
-    R.conn = og_alloc((size_t)k, sizeof(uint64_t));
+    R.conn = og_alloc((size_t)k * (size_t)og_threads(), sizeof(uint64_t));
     og_pow15_table(R.pow15);
     uint64_t total_mass = 0;
     for (int p = 0; p < k; ++p) total_mass += loads[p];
@@ -1700,7 +1749,8 @@ static void og_agg_view(og_agg* A, const og_graph* g, uint64_t cap, int32_t* clu
     A->ties = og_alloc((size_t)g->n, sizeof(uint32_t));
     int64_t maxdeg = 0;
     for (int64_t v = 0; v < g->n; ++v) if (g->off[v + 1] - g->off[v] > maxdeg) maxdeg = g->off[v + 1] - g->off[v];
-    A->scratch = og_alloc((size_t)maxdeg + 1, sizeof(og_cw_pair));
+    A->scratch_stride = maxdeg + 1;
+    A->scratch = og_alloc((size_t)A->scratch_stride * (size_t)og_threads(), sizeof(og_cw_pair));
 }
 
 static void og_agg_view_free(og_agg* A) {
@@ -1766,7 +1816,7 @@ static void og_ref_view(og_ref* R, const og_graph* g, int k, uint64_t cap, int32
     R->gains = og_alloc((size_t)g->n, sizeof(int64_t));
     R->locks = og_alloc((size_t)g->n, sizeof(int32_t));
     R->flags = og_alloc((size_t)g->n, sizeof(int32_t));
-    R->conn = og_alloc((size_t)k, sizeof(uint64_t));
+    R->conn = og_alloc((size_t)k * (size_t)og_threads(), sizeof(uint64_t));
     og_pow15_table(R->pow15);
     if (prices) {
         R->prices = og_alloc((size_t)k, sizeof(double));

@@ -59,6 +59,11 @@ typedef struct og_level_stats {
     int repair_iterations;
 } og_level_stats;
 
+/* Threads of the OpenMP loops (0 or less: unchanged); results do not
+ * depend on it.  og_get_threads: the current maximum. */
+void og_set_threads(int threads);
+int og_get_threads(void);
+
 /* Production option defaults (app/partition_main.cu:52-87). */
 void og_default_coarsen_opts(int parts, uint32_t seed, og_coarsen_opts* out);
 void og_default_refine_opts(int parts, double imbalance, og_refine_opts* out);

@@ -68,6 +68,8 @@ def lib():
         L.og_contract_graph.argtypes = [ctypes.c_int64, ctypes.c_int64, vp, vp, vp, vp, ctypes.c_int64,
                                         vp, vp, vp, vp]
         L.og_last_error.restype = ctypes.c_char_p
+        L.og_set_threads.argtypes = [ctypes.c_int]
+        L.og_get_threads.restype = ctypes.c_int
         L.og_default_coarsen_opts.argtypes = [ctypes.c_int, ctypes.c_uint32, vp]
         L.og_default_refine_opts.argtypes = [ctypes.c_int, ctypes.c_double, vp]
         L.og_admission_batch.argtypes = [ctypes.c_int64, vp, vp, vp, vp, vp, vp, ctypes.c_uint64, vp]
@@ -96,6 +98,15 @@ def refine_opts(parts, imbalance=1.03, **overrides):
     for key, value in overrides.items():
         setattr(o, key, value)
     return o
+
+
+def set_threads(threads):
+    """Threads of the oracle's OpenMP loops (results do not depend on it)."""
+    lib().og_set_threads(int(threads))
+
+
+def get_threads():
+    return int(lib().og_get_threads())
 
 
 def capacity(total, parts, imbalance=1.03):

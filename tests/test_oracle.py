@@ -249,3 +249,31 @@ def test_schedule_stops_at_cutoff(gp, og):
         assert all(s > cutoff for s in sizes[:-1]), (k, sizes)
         assert all(b <= 0.90 * a for a, b in zip(sizes, sizes[1:])), (k, sizes)
         assert h.levels <= 25
+
+
+def test_oracle_threads_do_not_change_results(gp, og):
+    # The OpenMP loops of the oracle write per-index outputs and integer
+    # sums: one thread and all threads give the same hierarchy, labels,
+    # loads and statistics (weighted and unit graphs, a hub row, k up to 64).
+    cases = [(gp.generate_rmat(13, 16, seed=5) + (None, None), 64), (gp.generate_knn3d(6000, 8, seed=7), 16),
+             (graphs.hub_ring(9000, 4000, seed=3), 8)]
+    full = og.get_threads()
+    try:
+        for (off, nbr, ew, vw), k in cases:
+            runs = []
+            for threads in (1, max(full, 4)):
+                og.set_threads(threads)
+                h = og.Hierarchy(off, nbr, ew, vw, og.coarsen_opts(k))
+                levels = [h.graph(lvl) for lvl in range(h.levels)]
+                maps = [h.map(lvl) for lvl in range(h.levels - 1)]
+                coff, cnbr, cew, cvw = levels[-1]
+                init = (np.arange(len(coff) - 1) % k).astype(np.int32)
+                lab, cut, loads, stats = h.uncoarsen(init, og.refine_opts(k))
+                runs.append((levels, maps, lab, cut, loads, stats))
+            (l1, m1, a1, c1, d1, s1), (l2, m2, a2, c2, d2, s2) = runs
+            assert len(l1) == len(l2) and c1 == c2 and s1 == s2
+            assert all(all(np.array_equal(x, y) for x, y in zip(p, q)) for p, q in zip(l1, l2))
+            assert all(np.array_equal(x, y) for x, y in zip(m1, m2))
+            assert np.array_equal(a1, a2) and np.array_equal(d1, d2)
+    finally:
+        og.set_threads(full)

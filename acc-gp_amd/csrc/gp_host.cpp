@@ -8,7 +8,9 @@
 #include <fstream>
 #include <limits>
 #include <stdexcept>
+#include <queue>
 #include <thread>
+#include <tuple>
 
 namespace gp {
 namespace {
@@ -110,6 +112,12 @@ HostGraph build_csr(std::int64_t n, std::vector<std::vector<std::uint64_t>>& par
 }
 
 // ------------------------------------------------------------ partitioner
+// In-repo initial partitioner of the coarsest graph (the role METIS 5 has in
+// the reference, src/initial_partition_metis.cu:86-183; it is not METIS):
+// recursive bisection, each bisection the best of eight breadth-first
+// growths from hashed start vertices refined by Fiduccia-Mattheyses passes,
+// and the whole recursion repeated from eight seeds keeping the labels of
+// least capacity excess, then least cut.  Deterministic for a given seed.
 struct Bisector {
     std::int64_t n;
     const std::int32_t* off;
@@ -140,37 +148,85 @@ struct Bisector {
 
     static std::uint64_t dev(std::uint64_t a, std::uint64_t b) { return a > b ? a - b : b - a; }
 
+    // Fiduccia-Mattheyses passes over the bisection of the subset S: each
+    // pass moves, one at a time, the unlocked vertex of largest gain (cut
+    // reduction; lowest id on ties) whose move keeps the side weight within
+    // tol of the target or brings it closer, locks it and updates its
+    // neighbours' gains; the pass then keeps its best prefix (balanced
+    // before unbalanced, then smaller cut, then smaller deviation) and undoes
+    // the rest.  Passes repeat while they improve, at most eight.
     void refine(const std::vector<std::int32_t>& S, std::vector<std::uint8_t>& side, std::uint64_t& wa,
                 std::uint64_t target, std::uint64_t tol) {
         const std::int32_t m = static_cast<std::int32_t>(S.size());
         std::vector<std::int64_t> gain(static_cast<std::size_t>(m));
-        std::vector<std::int32_t> order;
+        std::vector<std::uint32_t> stamp(static_cast<std::size_t>(m));
+        std::vector<std::uint8_t> locked(static_cast<std::size_t>(m));
+        std::vector<std::int32_t> moves;
+        struct Entry {
+            std::int64_t gain;
+            std::int32_t id;  // vertex id (ties: lower first)
+            std::int32_t i;
+            std::uint32_t stamp;
+            bool operator<(const Entry& o) const { return gain != o.gain ? gain < o.gain : id > o.id; }
+        };
+        std::int64_t cut = static_cast<std::int64_t>(subset_cut(S, side));
         for (int pass = 0; pass < 8; ++pass) {
-            order.clear();
+            std::priority_queue<Entry> heap;
             for (std::int32_t i = 0; i < m; ++i) {
                 std::uint64_t same, other;
                 side_conn(S, side, i, same, other);
                 gain[static_cast<std::size_t>(i)] = (std::int64_t)other - (std::int64_t)same;
-                if (gain[static_cast<std::size_t>(i)] > 0) order.push_back(i);
+                stamp[static_cast<std::size_t>(i)] = 0;
+                locked[static_cast<std::size_t>(i)] = 0;
+                heap.push(Entry{gain[static_cast<std::size_t>(i)], S[static_cast<std::size_t>(i)], i, 0});
             }
-            std::sort(order.begin(), order.end(), [&](std::int32_t x, std::int32_t y) {
-                if (gain[static_cast<std::size_t>(x)] != gain[static_cast<std::size_t>(y)])
-                    return gain[static_cast<std::size_t>(x)] > gain[static_cast<std::size_t>(y)];
-                return S[static_cast<std::size_t>(x)] < S[static_cast<std::size_t>(y)];
-            });
-            int moved = 0;
-            for (std::int32_t i : order) {
-                std::uint64_t same, other;
-                side_conn(S, side, i, same, other);
-                if ((std::int64_t)other - (std::int64_t)same <= 0) continue;
-                const std::uint64_t wv = w(S[static_cast<std::size_t>(i)]);
-                const std::uint64_t nwa = side[static_cast<std::size_t>(i)] == 0 ? wa - wv : wa + wv;
-                if (dev(nwa, target) > tol && dev(nwa, target) > dev(wa, target)) continue;
-                side[static_cast<std::size_t>(i)] ^= 1;
-                wa = nwa;
-                ++moved;
+            moves.clear();
+            const auto key = [&](std::uint64_t x, std::int64_t c) {
+                return std::make_tuple(dev(x, target) > tol ? 1 : 0, c, dev(x, target));
+            };
+            auto best = key(wa, cut);
+            std::size_t best_len = 0;
+            std::uint64_t cur_wa = wa;
+            std::int64_t cur_cut = cut;
+            while (!heap.empty()) {
+                const Entry top = heap.top();
+                heap.pop();
+                const std::size_t ti = static_cast<std::size_t>(top.i);
+                if (locked[ti] || top.stamp != stamp[ti]) continue;
+                const std::uint64_t wv = w(S[ti]);
+                const std::uint64_t nwa = side[ti] == 0 ? cur_wa - wv : cur_wa + wv;
+                if (dev(nwa, target) > tol && dev(nwa, target) >= dev(cur_wa, target)) continue;
+                locked[ti] = 1;
+                side[ti] ^= 1;
+                cur_wa = nwa;
+                cur_cut -= gain[ti];
+                moves.push_back(top.i);
+                const std::int32_t v = S[ti];
+                for (std::int64_t x = off[v]; x < off[v + 1]; ++x) {
+                    const std::int32_t j = pos[static_cast<std::size_t>(nbr[x])];
+                    if (j < 0 || j == top.i || locked[static_cast<std::size_t>(j)]) continue;
+                    const std::size_t tj = static_cast<std::size_t>(j);
+                    // v left j's side: the edge now counts against staying;
+                    // v joined j's side: the edge now counts for staying.
+                    gain[tj] += side[tj] == side[ti] ? -2 * (std::int64_t)e(x) : 2 * (std::int64_t)e(x);
+                    heap.push(Entry{gain[tj], S[tj], j, ++stamp[tj]});
+                }
+                const auto now = key(cur_wa, cur_cut);
+                if (now < best) {
+                    best = now;
+                    best_len = moves.size();
+                }
             }
-            if (!moved) break;
+            for (std::size_t q = moves.size(); q > best_len; --q) {
+                const std::size_t ti = static_cast<std::size_t>(moves[q - 1]);
+                const std::uint64_t wv = w(S[ti]);
+                side[ti] ^= 1;
+                cur_wa = side[ti] == 0 ? cur_wa + wv : cur_wa - wv;
+            }
+            const bool improved = best_len > 0;
+            wa = cur_wa;
+            cut = std::get<1>(best);
+            if (!improved) break;
         }
     }
 
@@ -206,7 +262,7 @@ struct Bisector {
         std::vector<std::uint8_t> visited;
         std::vector<std::int32_t> queue;
         std::uint64_t best_cut = std::numeric_limits<std::uint64_t>::max(), best_dev = best_cut;
-        const int trials = std::min<std::int32_t>(4, m);
+        const int trials = std::min<std::int32_t>(8, m);
         for (int t = 0; t < trials; ++t) {
             const std::uint32_t h = hmix32(seed ^ ((std::uint32_t)p0 * 0x9e3779b9U) ^ ((std::uint32_t)kk * 0x85ebca6bU) ^
                                            ((std::uint32_t)t * 0xc2b2ae35U));
@@ -263,12 +319,37 @@ struct Bisector {
 void initial_partition_host(std::int64_t n, const std::int32_t* off, const std::int32_t* nbr, const std::uint32_t* ew,
                             const std::uint32_t* vw, int parts, double imbalance, std::uint32_t seed,
                             std::int32_t* labels) {
-    (void)imbalance;
     if (n <= 0 || parts < 2) throw std::invalid_argument("invalid initial partition input");
-    Bisector b{n, off, nbr, ew, vw, seed, labels, std::vector<std::int32_t>(static_cast<std::size_t>(n), -1)};
-    std::vector<std::int32_t> all(static_cast<std::size_t>(n));
-    for (std::int64_t v = 0; v < n; ++v) all[static_cast<std::size_t>(v)] = static_cast<std::int32_t>(v);
-    b.bisect(all, 0, parts);
+    // Whole recursive bisections from eight seeds; the labels kept are the
+    // ones of smallest (load above the integer capacity, cut), first seed on
+    // ties.
+    std::uint64_t total = 0;
+    for (std::int64_t v = 0; v < n; ++v) total += vw ? vw[v] : 1u;
+    const long double raw = (long double)total * (imbalance >= 1.0 ? imbalance : 1.0) / parts;
+    const std::uint64_t cap = (std::uint64_t)std::floor(raw);
+    std::vector<std::int32_t> trial(static_cast<std::size_t>(n));
+    std::uint64_t best_over = std::numeric_limits<std::uint64_t>::max(), best_cut = best_over;
+    for (int t = 0; t < 8; ++t) {
+        const std::uint32_t s = t == 0 ? seed : hmix32(seed ^ (0x68e31da4U * (std::uint32_t)t));
+        Bisector b{n, off, nbr, ew, vw, s, trial.data(), std::vector<std::int32_t>(static_cast<std::size_t>(n), -1)};
+        std::vector<std::int32_t> all(static_cast<std::size_t>(n));
+        for (std::int64_t v = 0; v < n; ++v) all[static_cast<std::size_t>(v)] = static_cast<std::int32_t>(v);
+        b.bisect(all, 0, parts);
+        std::vector<std::uint64_t> loads(static_cast<std::size_t>(parts), 0);
+        std::uint64_t directed = 0;
+        for (std::int64_t v = 0; v < n; ++v) {
+            loads[static_cast<std::size_t>(trial[static_cast<std::size_t>(v)])] += vw ? vw[v] : 1u;
+            for (std::int64_t x = off[v]; x < off[v + 1]; ++x)
+                if (trial[static_cast<std::size_t>(nbr[x])] != trial[static_cast<std::size_t>(v)]) directed += ew ? ew[x] : 1u;
+        }
+        std::uint64_t over = 0;
+        for (auto l : loads) over += l > cap ? l - cap : 0;
+        if (over < best_over || (over == best_over && directed / 2 < best_cut)) {
+            best_over = over;
+            best_cut = directed / 2;
+            std::copy(trial.begin(), trial.end(), labels);
+        }
+    }
 }
 
 // ------------------------------------------------------------- generators

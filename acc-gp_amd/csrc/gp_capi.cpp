@@ -5,6 +5,7 @@
 #include "gp_host.hpp"
 #include "gp_metrics.hpp"
 #include "gp_refine.hpp"
+#include "gp_update.hpp"
 
 #include <atomic>
 #include <chrono>
@@ -276,6 +277,17 @@ int gp_graph_metadata(const gp_graph* g, uint64_t* max_degree, int* has_heavy, u
     if (incident_bound) *incident_bound = d.incident_bound;
     if (implicit_unit_weights) *implicit_unit_weights = d.unit_edges();
     return GP_OK;
+}
+
+int gp_graph_apply_delta(gp_context* ctx, gp_graph* g, int64_t count, const int32_t* u, const int32_t* v,
+                         const uint32_t* weights, const uint8_t* insert, int64_t* m) {
+    if (!ctx || !g || !g->view) return fail(GP_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (!g->owned) throw std::invalid_argument("only graphs made by gp_graph_create or gp_contract can change");
+        ctx->ctx.activate();
+        const std::int64_t m2 = gp::apply_delta(ctx->ctx, *g->view, count, u, v, weights, insert);
+        if (m) *m = m2;
+    });
 }
 
 void gp_graph_destroy(gp_graph* g) { delete g; }

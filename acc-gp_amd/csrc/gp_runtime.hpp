@@ -142,6 +142,8 @@ public:
     }
     T* get() const { return p_; }
     std::size_t size() const { return n_; }
+    // The arena the array came from (null when empty).
+    Arena* arena() const { return a_; }
     explicit operator bool() const { return p_ != nullptr; }
 
 private:

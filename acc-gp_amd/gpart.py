@@ -58,7 +58,10 @@ class Request(ctypes.Structure):
         ("n", ctypes.c_int64), ("m", ctypes.c_int64), ("offsets", ctypes.c_void_p), ("neighbors", ctypes.c_void_p),
         ("edge_weights", ctypes.c_void_p), ("vertex_weights", ctypes.c_void_p), ("parts", ctypes.c_int),
         ("imbalance", ctypes.c_double), ("seed", ctypes.c_uint32), ("warm_labels", ctypes.c_void_p),
-        ("labels", ctypes.c_void_p), ("stats", PartitionStats), ("status", ctypes.c_int),
+        ("labels", ctypes.c_void_p), ("graph_id", ctypes.c_uint64), ("delta_count", ctypes.c_int64),
+        ("delta_u", ctypes.c_void_p), ("delta_v", ctypes.c_void_p), ("delta_w", ctypes.c_void_p),
+        ("delta_insert", ctypes.c_void_p), ("warm_resident", ctypes.c_int),
+        ("stats", PartitionStats), ("status", ctypes.c_int),
         ("queue_seconds", ctypes.c_double), ("device_seconds", ctypes.c_double), ("error", ctypes.c_char * 256),
     ]
 
@@ -78,7 +81,8 @@ EXPORTS = [
     "gp_result_destroy", "gp_result_device_labels", "gp_edge_cut", "gp_edge_cut_noalloc", "gp_refine_partition", "gp_partition", "gp_generate_grid", "gp_generate_rmat",
     "gp_generate_rgg2d", "gp_generate_knn3d", "gp_free_host", "gp_load_csr", "gp_save_csr",
     "gp_profile_enable", "gp_profile_query", "gp_profile_reset", "gp_server_create", "gp_server_submit",
-    "gp_server_wait", "gp_server_destroy", "gp_partition_batch",
+    "gp_server_wait", "gp_server_destroy", "gp_partition_batch", "gp_server_graph_add", "gp_server_graph_download",
+    "gp_server_graph_release", "gp_graph_apply_delta",
 ]
 
 
@@ -158,6 +162,10 @@ def lib():
         L.gp_server_wait.argtypes = [vp, vp]
         L.gp_server_destroy.argtypes = [vp]
         L.gp_partition_batch.argtypes = [i32, vp, i64, i32, u64]
+        L.gp_server_graph_add.argtypes = [vp, i64, i64, vp, vp, vp, vp, vp]
+        L.gp_server_graph_download.argtypes = [vp, u64, vp, vp, vp, vp, vp, vp]
+        L.gp_server_graph_release.argtypes = [vp, u64]
+        L.gp_graph_apply_delta.argtypes = [vp, vp, i64, vp, vp, vp, vp, vp]
         _LIB = L
     return _LIB
 
@@ -464,6 +472,42 @@ class BatchItem:
         req.warm_labels, req.labels = _p(self.warm), _p(self.labels)
 
 
+class ResidentItem:
+    """A request on a server's resident graph: optional edge changes
+    (arrays u, v, insert flags, weights), then a cold partition at `parts`,
+    or with warm=True a refinement from the graph's current partition."""
+
+    def __init__(self, graph_id, n, parts, imbalance=1.03, seed=0, delta=None, warm=False):
+        self.graph_id, self.parts, self.imbalance, self.seed, self.warm = graph_id, parts, imbalance, seed, warm
+        self.labels = np.zeros(n, np.int32)
+        self.delta = None
+        if delta is not None:
+            u, v, ins, w = delta
+            self.delta = (np.ascontiguousarray(u, np.int32), np.ascontiguousarray(v, np.int32),
+                          np.ascontiguousarray(ins, np.uint8), None if w is None else np.ascontiguousarray(w, np.uint32))
+
+    def fill(self, req):
+        req.graph_id = self.graph_id
+        req.parts, req.imbalance, req.seed = self.parts, self.imbalance, self.seed
+        req.labels = _p(self.labels)
+        req.warm_resident = int(self.warm)
+        if self.delta is not None:
+            u, v, ins, w = self.delta
+            req.delta_count = len(u)
+            req.delta_u, req.delta_v, req.delta_insert, req.delta_w = _p(u), _p(v), _p(ins), _p(w)
+
+
+def apply_delta(ctx, g_handle, u, v, insert, w=None):
+    """gp_graph_apply_delta on a device graph; returns the new entry count."""
+    u = np.ascontiguousarray(u, np.int32)
+    v = np.ascontiguousarray(v, np.int32)
+    insert = np.ascontiguousarray(insert, np.uint8)
+    w = None if w is None else np.ascontiguousarray(w, np.uint32)
+    m = ctypes.c_int64()
+    check(lib().gp_graph_apply_delta(ctx.h, g_handle, len(u), _p(u), _p(v), _p(w), _p(insert), ctypes.byref(m)))
+    return m.value
+
+
 def _stats_dict(req):
     return dict((f, getattr(req.stats, f)) for f, _ in PartitionStats._fields_)
 
@@ -496,6 +540,32 @@ class Server:
         check(lib().gp_server_submit(self.h, ctypes.byref(req), ctypes.byref(t)))
         self._live[t.value] = (req, item)
         return t.value
+
+    def add_graph(self, off, nbr, ew=None, vw=None):
+        """Keeps a graph resident in HBM; returns its id."""
+        off = np.ascontiguousarray(off, np.int32)
+        nbr = np.ascontiguousarray(nbr, np.int32)
+        ew = None if ew is None else np.ascontiguousarray(ew, np.uint32)
+        vw = None if vw is None else np.ascontiguousarray(vw, np.uint32)
+        gid = ctypes.c_uint64()
+        check(lib().gp_server_graph_add(self.h, len(off) - 1, len(nbr), _p(off), _p(nbr), _p(ew), _p(vw),
+                                        ctypes.byref(gid)))
+        return gid.value
+
+    def graph(self, graph_id):
+        """(offsets, neighbors, edge weights, current labels) of a resident graph."""
+        n, m = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().gp_server_graph_download(self.h, graph_id, ctypes.byref(n), ctypes.byref(m), None, None, None,
+                                             None))
+        off = np.zeros(n.value + 1, np.int32)
+        nbr = np.zeros(max(m.value, 1), np.int32)
+        ew = np.zeros(max(m.value, 1), np.uint32)
+        lab = np.full(n.value, -1, np.int32)
+        check(lib().gp_server_graph_download(self.h, graph_id, None, None, _p(off), _p(nbr), _p(ew), _p(lab)))
+        return off, nbr[:m.value], ew[:m.value], lab
+
+    def release_graph(self, graph_id):
+        check(lib().gp_server_graph_release(self.h, graph_id))
 
     def wait(self, ticket):
         req, item = self._live.pop(ticket)

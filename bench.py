@@ -220,6 +220,8 @@ def batched_leg(gp, ctx, count, workers):
     t0 = time.perf_counter()
     stats = gp.partition_batch(items, workers=workers)
     batch_s = time.perf_counter() - t0
+    for it, st in zip(items, stats):
+        it.final_cut = st["final_cut"]
     same = all(np.array_equal(it.labels, lab) for it, lab in zip(items, plain_labels))
     cut_ratio = [(st["final_cut"] + 1) / (c + 1) for st, c in zip(stats, plain_cuts)]
     lat = [st["device_seconds"] for st in stats]
@@ -239,19 +241,32 @@ def batched_leg(gp, ctx, count, workers):
 def serving_leg(gp, ctx, mix, done_items, workers, count, rate, seed=6):
     """Poisson arrivals at `rate` requests/s (70% of the batched capacity):
     cold requests from the mix with k in [2, 128], and 10% repartitions of
-    resident graphs with 1% edge insertions and deletions, warm-started from
-    their labels.  Latency = queueing + upload + partition + download."""
+    resident graphs (kept in HBM by the server with their current
+    partition) after a 1% edge delta applied on the device, warm-started
+    from that partition.  Latency = queueing + (upload) + partition +
+    download.  Quality: each repartition's cut against a cold partition of
+    the same updated graph and against the cold cut of the graph before its
+    deltas."""
     from tests import workloads
-    reqs = []
-    for kind, j, off, nbr, k in workloads.serving_requests(gp, mix, count, seed):
-        warm = done_items[j].labels if kind == "warm" else None
-        reqs.append((kind, gp.BatchItem(off, nbr, k, warm=warm), (off, nbr, k)))
-    arrivals = workloads.poisson_arrivals(count, rate, seed)
+    reqs = workloads.serving_requests(gp, mix, count, seed)
     srv = gp.Server(workers=workers)
     try:
+        resident = {}
+        for kind, j, _, _ in reqs:  # set-up before the stream: resident graphs and their partitions
+            if kind == "warm" and j not in resident:
+                _, off, nbr, k0 = mix[j]
+                resident[j] = srv.add_graph(off, nbr)
+                srv.wait(srv.submit(gp.ResidentItem(resident[j], len(off) - 1, k0)))
+        items = []
+        for kind, j, payload, k in reqs:
+            if kind == "warm":
+                items.append(gp.ResidentItem(resident[j], len(mix[j][1]) - 1, k, delta=payload + (None,), warm=True))
+            else:
+                items.append(gp.BatchItem(payload[0], payload[1], k))
+        arrivals = workloads.poisson_arrivals(count, rate, seed)
         tickets = []
         t0 = time.perf_counter()
-        for (kind, item, _), at in zip(reqs, arrivals):
+        for item, at in zip(items, arrivals):
             delay = t0 + at - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
@@ -261,67 +276,28 @@ def serving_leg(gp, ctx, mix, done_items, workers, count, rate, seed=6):
     finally:
         srv.close()
     lat = [st["queue_seconds"] + st["device_seconds"] for st in stats]
-    edges = sum(len(item.nbr) // 2 for _, item, _ in reqs)
-    ratios = []
-    for (kind, item, extra), st in zip(reqs, stats):
-        if kind != "warm":
+    edges = 0
+    host = {j: (mix[j][1], mix[j][2], None) for j in resident}
+    vs_cold, vs_original = [], []
+    for (kind, j, payload, k), st in zip(reqs, stats):
+        if kind == "cold":
+            edges += len(payload[1]) // 2
             continue
-        off2, nbr2, k0 = extra
-        g = gp.Graph(ctx, off2, nbr2)
-        _, cold = g.partition(k0)
+        off, nbr, ew = workloads.apply_delta(*host[j], *payload)
+        host[j] = (off, nbr, ew)
+        edges += len(nbr) // 2
+        g = gp.Graph(ctx, off, nbr, ew)
+        _, cold = g.partition(k)
         g.close()
-        ratios.append((st["final_cut"] + 1) / (cold["final_cut"] + 1))
+        vs_cold.append((st["final_cut"] + 1) / (cold["final_cut"] + 1))
+        vs_original.append((st["final_cut"] + 1) / (done_items[j].final_cut + 1))
     print("bench: served %d requests in %.2fs, p99 %.1f ms" % (count, wall, 1000 * np.percentile(lat, 99)),
           file=sys.stderr, flush=True)
+    geo = lambda xs: float(np.exp(np.mean(np.log(xs)))) if xs else None
     return {"requests": count, "rate_per_s": rate, "workers": workers, "edges_per_s": edges / wall,
             "p50_ms": 1000.0 * float(np.percentile(lat, 50)), "p99_ms": 1000.0 * float(np.percentile(lat, 99)),
-            "repartitions": len(ratios),
-            "warm_vs_cold_cut_geomean": float(np.exp(np.mean(np.log(ratios)))) if ratios else None}
-
-
-# The reference's own run of its partitioner on the 4000x2000 grid
-# (15,994,000 edges): k -> (cut, seconds), the "ours" columns of
-# results/manual/regret-20261017/comparison.csv:44-49 (median of 3 repeats on
-# an RTX 5090), and Jet's cut from the same rows.
-GRID_ANCHOR = {4: (8620, 0.068675156, 8228), 8: (14582, 0.071301289, 13798), 16: (24194, 0.090550177, 23186),
-               32: (36898, 0.088336934, 34214), 64: (55214, 0.09714246, 51845), 128: (77453, 0.12128948, 73225)}
-
-
-def quality_anchor(repeats=3):
-    """Cut quality against the reference on the one published input that can
-    be regenerated here: the 4000x2000 4-neighbour grid.  Vertex numbering
-    (row-major here) and the initial partitioner (METIS 5.2.1 there, the
-    in-repo bisection here) differ, so the ratio is a sanity anchor."""
-    from tests.conftest import load_package
-    gp = load_package()
-    off, nbr = gp.generate_grid(4000, 2000)
-    edges = len(nbr) // 2
-    ctx = gp.Context(0)
-    g = gp.Graph(ctx, off, nbr)
-    rows = {}
-    for k, (ref_cut, ref_s, jet_cut) in sorted(GRID_ANCHOR.items()):
-        g.partition(k, 1.03, 0, download=False)
-        secs = []
-        for _ in range(repeats):
-            _, st = g.partition(k, 1.03, 0, download=False)
-            secs.append(st["algorithm_total_seconds"])
-        labels, st = g.partition(k, 1.03, 0, download=True)
-        src = np.repeat(np.arange(len(off) - 1, dtype=np.int64), np.diff(off))
-        recount = int(np.count_nonzero(labels[src] != labels[nbr])) // 2
-        cap = gp.capacity(len(off) - 1, k, 1.03)
-        if recount != st["final_cut"] or np.bincount(labels, minlength=k).max() > cap:
-            raise RuntimeError("quality anchor: result failed verification")
-        rows[k] = {"cut": int(st["final_cut"]), "reference_cut": ref_cut, "cut_ratio_vs_reference": st["final_cut"] / ref_cut,
-                   "jet_cut": jet_cut, "seconds_median": float(np.median(secs)), "reference_seconds": ref_s,
-                   "speedup_vs_reference": ref_s / float(np.median(secs)), "levels": st["levels"]}
-        print("anchor: grid 4000x2000 k=%d cut=%d (reference %d, ratio %.4f) %.1f ms (reference %.1f ms)" % (
-            k, st["final_cut"], ref_cut, st["final_cut"] / ref_cut, 1e3 * np.median(secs), 1e3 * ref_s),
-              file=sys.stderr, flush=True)
-    g.close()
-    ctx.close()
-    geo = float(np.exp(np.mean([np.log(r["cut_ratio_vs_reference"]) for r in rows.values()])))
-    print(json.dumps({"anchor": "grid-4000x2000", "undirected_edges": edges, "imbalance": 1.03, "seed": 0,
-                      "cut_geomean_ratio_vs_reference": geo, "parts": rows}), flush=True)
+            "repartitions": len(vs_cold), "resident_graphs": len(resident),
+            "warm_vs_cold_cut_geomean": geo(vs_cold), "warm_vs_original_cold_cut_geomean": geo(vs_original)}
 
 
 def copy_calibration():

@@ -111,6 +111,17 @@ int gp_graph_download(gp_context* ctx, const gp_graph* g, int32_t* offsets, int3
  * graph has been coarsened or aggregated) and implicit unit edge weights. */
 int gp_graph_metadata(const gp_graph* g, uint64_t* max_degree, int* has_heavy, uint64_t* incident_bound,
                       int* implicit_unit_weights);
+/* Changes a device graph in place by count undirected edge changes (u[i],
+ * v[i]): insert[i] != 0 inserts the edge with weights[i] (weights NULL: 1),
+ * else deletes it.  No counterpart in the reference (its graphs are read
+ * once); the request server's resident graphs use it for repartitioning.
+ * Deletions apply first, then insertions; deleting an absent edge changes
+ * nothing, inserting a present or repeated edge is dropped.  Each row keeps
+ * its surviving entries in their old order followed by its new neighbours in
+ * ascending order.  m (optional) gets the new entry count.  GP_ERR_INVALID
+ * for a vertex out of range, a self loop or a zero weight (nothing changes). */
+int gp_graph_apply_delta(gp_context* ctx, gp_graph* g, int64_t count, const int32_t* u, const int32_t* v,
+                         const uint32_t* weights, const uint8_t* insert, int64_t* m);
 void gp_graph_destroy(gp_graph* g);
 
 /* partition_capacity (include/partition_capacity.hpp:14-25). */
@@ -276,6 +287,17 @@ typedef struct gp_request {
     uint32_t seed;
     const int32_t* warm_labels;       /* NULL: cold partition; else refine from these labels */
     int32_t* labels;                  /* out: n labels */
+    /* Resident graphs (gp_server_graph_add): graph_id != 0 runs the request on
+     * that graph instead of n/m/offsets/...; its delta_count edge changes
+     * (gp_graph_apply_delta) apply first; warm_resident refines from the
+     * graph's current partition (the result of the last request on it). */
+    uint64_t graph_id;
+    int64_t delta_count;
+    const int32_t* delta_u;
+    const int32_t* delta_v;
+    const uint32_t* delta_w;          /* NULL: 1 */
+    const uint8_t* delta_insert;      /* 1 insert, 0 delete */
+    int warm_resident;
     gp_partition_stats stats;         /* out */
     int status;                       /* out: gp_status (-1 while pending) */
     double queue_seconds;             /* out: submission to admission */
@@ -288,6 +310,18 @@ typedef struct gp_ticket gp_ticket;
 
 /* workers: concurrent requests (contexts); hbm_budget_bytes 0 = 64 GiB. */
 int gp_server_create(int device, int workers, uint64_t hbm_budget_bytes, gp_server** out);
+/* Keeps a graph in HBM for the server's requests (uploaded once); graph_id
+ * out.  Requests on one resident graph run one at a time in submission
+ * order. */
+int gp_server_graph_add(gp_server* s, int64_t n, int64_t m, const int32_t* offsets, const int32_t* neighbors,
+                        const uint32_t* edge_weights, const uint32_t* vertex_weights, uint64_t* graph_id);
+/* The resident graph as it stands (any output may be NULL; call with NULL
+ * arrays first for n and m) and its current partition; GP_ERR_INVALID while
+ * requests on it are queued or running. */
+int gp_server_graph_download(gp_server* s, uint64_t graph_id, int64_t* n, int64_t* m, int32_t* offsets,
+                             int32_t* neighbors, uint32_t* edge_weights, int32_t* labels);
+/* Frees a resident graph; GP_ERR_INVALID while requests on it are pending. */
+int gp_server_graph_release(gp_server* s, uint64_t graph_id);
 int gp_server_submit(gp_server* s, gp_request* r, gp_ticket** ticket);
 /* Blocks until the request is done; releases the ticket; returns its status
  * (GP_ERR_INVALID for a ticket this server does not hold). */

@@ -8,6 +8,10 @@ from tests import graphs
 pytestmark = pytest.mark.gpu
 
 
+def _say(msg):
+    print(msg, flush=True)
+
+
 def _mix(gp):
     items = []
     rng = np.random.default_rng(5)
@@ -84,3 +88,40 @@ def test_server_mixed_order_and_warm_start(gp, ctx, og):
         assert og.cut(off2, nbr2, None, it.labels) == st["final_cut"]
     finally:
         srv.close()
+
+
+def test_apply_delta_matches_host(gp, ctx):
+    # gp_graph_apply_delta against its host restatement (tests/workloads.py),
+    # on a unit grid (also with weights that force an explicit weight
+    # array), a weighted kNN graph and an R-MAT graph with hub rows; edge
+    # cases: an absent deletion, a present insertion, a repeated insertion,
+    # a deletion and re-insertion of one edge.
+    from tests import graphs, workloads
+    rng = np.random.default_rng(9)
+    off, nbr = graphs.grid(40, 40)
+    cases = [("grid40", off, nbr, None, None, False), ("grid40_weighted_inserts", off, nbr, None, None, True)]
+    o, nb, ew, vw = gp.generate_knn3d(3000, 8, seed=4)
+    cases.append(("knn3000_weighted", o, nb, ew, vw, True))
+    o, nb = gp.generate_rmat(12, 16, seed=2)
+    cases.append(("rmat12", o, nb, None, None, False))
+    for name, off, nbr, ew, vw, with_w in cases:
+        _say("case %s: edge delta" % name)
+        u, v, ins = workloads.random_delta(off, nbr, 0.02, rng)
+        a = int(nbr[off[5]])
+        u = np.concatenate([u, [5, 5, 11, 11, a, a]]).astype(np.int32)
+        v = np.concatenate([v, [(a + 1) % (len(off) - 1) or 1, a, 17, 17, 5, 5]]).astype(np.int32)
+        ins = np.concatenate([ins, [0, 1, 1, 1, 0, 1]]).astype(np.uint8)
+        w = rng.integers(1, 30, len(u)).astype(np.uint32) if with_w else None
+        want_off, want_nbr, want_ew = workloads.apply_delta(off, nbr, ew, u, v, ins, w)
+        g = gp.Graph(ctx, off, nbr, ew, vw)
+        try:
+            m = gp.apply_delta(ctx, g.h, u, v, ins, w)
+            got_off, got_nbr, got_ew, _ = gp.download_graph(ctx, g.h)
+            assert m == len(want_nbr)
+            assert np.array_equal(got_off, want_off) and np.array_equal(got_nbr, want_nbr), name
+            assert np.array_equal(got_ew, want_ew if want_ew is not None else np.ones(m, np.uint32)), name
+            with pytest.raises(RuntimeError, match="invalid vertex or a self loop"):
+                gp.apply_delta(ctx, g.h, np.array([3], np.int32), np.array([3], np.int32), np.array([1], np.uint8))
+            assert np.array_equal(gp.download_graph(ctx, g.h)[1], want_nbr)
+        finally:
+            g.close()

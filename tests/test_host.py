@@ -309,3 +309,44 @@ def test_initial_partition_strict_is_feasible(gp):
             cap = gp.capacity(int(vw.sum()), k)
             assert np.bincount(lab, weights=vw, minlength=k).max() <= cap, (seed, k)
     assert gp.initial_partition_backend() in ("metis", "bisection")
+
+
+def test_edge_delta_host_semantics():
+    # The host restatement of gp_graph_apply_delta against the edge-set
+    # definition: the new edge set is (E - D) + I, rows keep their surviving
+    # entries in order and then the new neighbours ascending, weights of old
+    # edges stay, inserted ones carry their first weight.
+    from tests import graphs, workloads
+    rng = np.random.default_rng(3)
+    off, nbr, ew = graphs.random_graph(300, 1200, seed=5, max_weight=9)
+    u, v, ins = workloads.random_delta(off, nbr, 0.05, rng)
+    # edge cases: delete an absent edge, insert a present one, insert twice, delete and re-insert
+    present = (0, int(nbr[off[0]]))
+    u = np.concatenate([u, [0, present[0], 7, 7, present[0], present[0]]]).astype(np.int32)
+    v = np.concatenate([v, [299 if 299 not in set(nbr[off[0]:off[1]]) else 298, present[1], 9, 9,
+                            present[1], present[1]]]).astype(np.int32)
+    ins = np.concatenate([ins, [0, 1, 1, 1, 0, 1]]).astype(np.uint8)
+    w = rng.integers(1, 50, len(u)).astype(np.uint32)
+    off2, nbr2, ew2 = workloads.apply_delta(off, nbr, ew, u, v, ins, w)
+    old = {}
+    for s in range(len(off) - 1):
+        for j in range(off[s], off[s + 1]):
+            old[(s, int(nbr[j]))] = int(ew[j])
+    dels = {(int(a), int(b)) for a, b, i in zip(u, v, ins) if not i} | {(int(b), int(a)) for a, b, i in zip(u, v, ins) if not i}
+    want = {e: x for e, x in old.items() if e not in dels}
+    for a, b, i, x in zip(u, v, ins, w):
+        if i:
+            for e in ((int(a), int(b)), (int(b), int(a))):
+                if e not in want:
+                    want[e] = int(x)
+    got = {}
+    for s in range(len(off2) - 1):
+        row = [int(x) for x in nbr2[off2[s]:off2[s + 1]]]
+        assert len(set(row)) == len(row)
+        old_part = [x for x in row if (s, x) in old and (s, x) not in dels]
+        assert row[:len(old_part)] == [int(x) for x in nbr[off[s]:off[s + 1]] if (s, int(x)) not in dels]
+        assert row[len(old_part):] == sorted(row[len(old_part):])
+        for j in range(off2[s], off2[s + 1]):
+            got[(s, int(nbr2[j]))] = int(ew2[j])
+    assert got == want
+    assert got[present] == want[present]  # deleted then re-inserted: the new weight

@@ -50,12 +50,14 @@ def main():
     ap.add_argument("--quality-anchor", action="store_true",
                     help="only partition the 4000x2000 grid at k=4..128 and compare the cuts and times with the "
                          "reference's published run of the same graph, then exit")
+    ap.add_argument("--anchor-parts", default="4,8,16,32,64,128", help="parts of the quality anchor")
+    ap.add_argument("--anchor-repeats", type=int, default=3)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
         return
     if args.quality_anchor:
-        quality_anchor()
+        quality_anchor([int(x) for x in args.anchor_parts.split(",")], args.anchor_repeats)
         return
     if args.trace_mix:
         trace_mix(args.batch_graphs)
@@ -298,6 +300,52 @@ def serving_leg(gp, ctx, mix, done_items, workers, count, rate, seed=6):
             "p50_ms": 1000.0 * float(np.percentile(lat, 50)), "p99_ms": 1000.0 * float(np.percentile(lat, 99)),
             "repartitions": len(vs_cold), "resident_graphs": len(resident),
             "warm_vs_cold_cut_geomean": geo(vs_cold), "warm_vs_original_cold_cut_geomean": geo(vs_original)}
+
+
+# The reference's own run of its partitioner on the 4000x2000 grid
+# (15,994,000 edges): k -> (cut, seconds), the "ours" columns of
+# results/manual/regret-20261017/comparison.csv:44-49 (median of 3 repeats on
+# an RTX 5090), and Jet's cut from the same rows.
+GRID_ANCHOR = {4: (8620, 0.068675156, 8228), 8: (14582, 0.071301289, 13798), 16: (24194, 0.090550177, 23186),
+               32: (36898, 0.088336934, 34214), 64: (55214, 0.09714246, 51845), 128: (77453, 0.12128948, 73225)}
+
+
+def quality_anchor(parts, repeats=3):
+    """Cut quality against the reference on the one published input that can
+    be regenerated here: the 4000x2000 4-neighbour grid.  Vertex numbering
+    (row-major here) and the initial partitioner (METIS 5.2.1 there, the
+    in-repo bisection here) differ, so the ratio is a sanity anchor."""
+    from tests.conftest import load_package
+    gp = load_package()
+    off, nbr = gp.generate_grid(4000, 2000)
+    edges = len(nbr) // 2
+    ctx = gp.Context(0)
+    g = gp.Graph(ctx, off, nbr)
+    rows = {}
+    for k in sorted(parts):
+        ref_cut, ref_s, jet_cut = GRID_ANCHOR[k]
+        g.partition(k, 1.03, 0, download=False)
+        secs = []
+        for _ in range(repeats):
+            _, st = g.partition(k, 1.03, 0, download=False)
+            secs.append(st["algorithm_total_seconds"])
+        labels, st = g.partition(k, 1.03, 0, download=True)
+        src = np.repeat(np.arange(len(off) - 1, dtype=np.int64), np.diff(off))
+        recount = int(np.count_nonzero(labels[src] != labels[nbr])) // 2
+        cap = gp.capacity(len(off) - 1, k, 1.03)
+        if recount != st["final_cut"] or np.bincount(labels, minlength=k).max() > cap:
+            raise RuntimeError("quality anchor: result failed verification")
+        rows[k] = {"cut": int(st["final_cut"]), "reference_cut": ref_cut, "cut_ratio_vs_reference": st["final_cut"] / ref_cut,
+                   "jet_cut": jet_cut, "seconds_median": float(np.median(secs)), "reference_seconds": ref_s,
+                   "speedup_vs_reference": ref_s / float(np.median(secs)), "levels": st["levels"]}
+        print("anchor: grid 4000x2000 k=%d cut=%d (reference %d, ratio %.4f) %.1f ms (reference %.1f ms)" % (
+            k, st["final_cut"], ref_cut, st["final_cut"] / ref_cut, 1e3 * np.median(secs), 1e3 * ref_s),
+              file=sys.stderr, flush=True)
+    g.close()
+    ctx.close()
+    geo = float(np.exp(np.mean([np.log(r["cut_ratio_vs_reference"]) for r in rows.values()])))
+    print(json.dumps({"anchor": "grid-4000x2000", "undirected_edges": edges, "imbalance": 1.03, "seed": 0,
+                      "cut_geomean_ratio_vs_reference": geo, "parts": rows}), flush=True)
 
 
 def copy_calibration():

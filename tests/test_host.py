@@ -350,3 +350,16 @@ def test_edge_delta_host_semantics():
             got[(s, int(nbr2[j]))] = int(ew2[j])
     assert got == want
     assert got[present] == want[present]  # deleted then re-inserted: the new weight
+
+
+def test_bench_modes_are_wired():
+    # Every mode bench.py dispatches to exists (its GPU legs run on the box).
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    for name in ("main", "quality_anchor", "trace_mix", "copy_calibration", "batched_leg", "serving_leg",
+                 "cpu_baseline", "cpu_partition", "verify_with_checker", "traffic_of"):
+        assert callable(getattr(bench, name)), name
+    assert set(bench.GRID_ANCHOR) == {4, 8, 16, 32, 64, 128}
+    assert [c[0] for c in bench.ROOFLINE_CLASSES][0] == "lp_propose"

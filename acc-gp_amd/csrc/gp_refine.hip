@@ -2328,6 +2328,10 @@ public:
         // The read also clears the counters (not the part loads behind them).
         c_.read_clear(h.data(), ctr_.get(), h.size(), kCtrCount);
         ctr_clear_ = true;
+        // With no row above kTierLight entries no medium launch was made; a
+        // row the light launch deferred would have gone unprocessed.
+        if (g_.row_len_bound <= static_cast<std::uint64_t>(kTierLight) && (h[kTier] || h[kTierB] || h[kTierC]))
+            throw std::logic_error("rows longer than the level's row-length bound were deferred and not processed");
         for (int p = 0; p < a_.k; ++p) loads_host_[static_cast<std::size_t>(p)] = h[static_cast<std::size_t>(kCtrCount + p)];
         delta = (std::int64_t)h[kDelta];
         if (c_.profiling()) {

@@ -8,7 +8,6 @@
 #include <fstream>
 #include <limits>
 #include <stdexcept>
-#include <queue>
 #include <thread>
 #include <tuple>
 
@@ -114,10 +113,17 @@ HostGraph build_csr(std::int64_t n, std::vector<std::vector<std::uint64_t>>& par
 // ------------------------------------------------------------ partitioner
 // In-repo initial partitioner of the coarsest graph (the role METIS 5 has in
 // the reference, src/initial_partition_metis.cu:86-183; it is not METIS):
-// recursive bisection, each bisection the best of eight breadth-first
-// growths from hashed start vertices refined by Fiduccia-Mattheyses passes,
-// and the whole recursion repeated from eight seeds keeping the labels of
-// least capacity excess, then least cut.  Deterministic for a given seed.
+// recursive bisection.  Each bisection grows kGrowths breadth-first regions
+// from hashed start vertices, refines the kRefined best of them (cut, then
+// deviation from the target weight) with Fiduccia-Mattheyses passes and
+// keeps the better.  The whole recursion runs from kRecursions seeds on
+// threads of their own and the labels of least capacity excess, then least
+// cut, are kept.  Deterministic for a given seed and independent of timing.
+constexpr int kGrowths = 8;
+constexpr int kRefined = 2;
+constexpr int kRecursions = 8;
+constexpr int kPasses = 8;
+
 struct Bisector {
     std::int64_t n;
     const std::int32_t* off;
@@ -126,120 +132,168 @@ struct Bisector {
     const std::uint32_t* vw;
     std::uint32_t seed;
     std::int32_t* labels;
-    std::vector<std::int32_t> pos;
+    std::vector<std::int32_t> pos;  // subset slot of each vertex, -1 outside
 
     std::uint64_t w(std::int32_t v) const { return vw ? vw[v] : 1u; }
-    std::uint64_t e(std::int64_t x) const { return ew ? ew[x] : 1u; }
+    std::int64_t e(std::int64_t x) const { return ew ? static_cast<std::int64_t>(ew[x]) : 1; }
+    static std::uint64_t dev(std::uint64_t a, std::uint64_t b) { return a > b ? a - b : b - a; }
 
-    // Connection of local vertex i to each side within the subset.
-    void side_conn(const std::vector<std::int32_t>& S, const std::vector<std::uint8_t>& side, std::int32_t i,
-                   std::uint64_t& same, std::uint64_t& other) const {
-        same = other = 0;
-        const std::int32_t v = S[static_cast<std::size_t>(i)];
+    // Indexed binary max-heap of subset slots keyed by (gain, lower vertex id).
+    struct GainHeap {
+        std::vector<std::int32_t> heap, where;  // where[i] = heap slot of i, -1 outside
+        const std::vector<std::int64_t>* gain = nullptr;
+        const std::vector<std::int32_t>* ids = nullptr;
+        bool above(std::int32_t x, std::int32_t y) const {
+            const std::int64_t gx = (*gain)[static_cast<std::size_t>(x)], gy = (*gain)[static_cast<std::size_t>(y)];
+            return gx != gy ? gx > gy : (*ids)[static_cast<std::size_t>(x)] < (*ids)[static_cast<std::size_t>(y)];
+        }
+        void place(std::size_t slot, std::int32_t x) {
+            heap[slot] = x;
+            where[static_cast<std::size_t>(x)] = static_cast<std::int32_t>(slot);
+        }
+        void up(std::size_t slot) {
+            const std::int32_t x = heap[slot];
+            while (slot && above(x, heap[(slot - 1) / 2])) {
+                place(slot, heap[(slot - 1) / 2]);
+                slot = (slot - 1) / 2;
+            }
+            place(slot, x);
+        }
+        void down(std::size_t slot) {
+            const std::int32_t x = heap[slot];
+            for (;;) {
+                std::size_t c = 2 * slot + 1;
+                if (c >= heap.size()) break;
+                if (c + 1 < heap.size() && above(heap[c + 1], heap[c])) ++c;
+                if (!above(heap[c], x)) break;
+                place(slot, heap[c]);
+                slot = c;
+            }
+            place(slot, x);
+        }
+        bool has(std::int32_t x) const { return where[static_cast<std::size_t>(x)] >= 0; }
+        void push(std::int32_t x) {
+            heap.push_back(x);
+            up(heap.size() - 1);
+        }
+        void update(std::int32_t x) {
+            up(static_cast<std::size_t>(where[static_cast<std::size_t>(x)]));
+            down(static_cast<std::size_t>(where[static_cast<std::size_t>(x)]));
+        }
+        std::int32_t pop() {
+            const std::int32_t top = heap[0];
+            where[static_cast<std::size_t>(top)] = -1;
+            const std::int32_t last = heap.back();
+            heap.pop_back();
+            if (!heap.empty()) {
+                place(0, last);
+                down(0);
+            }
+            return top;
+        }
+    };
+
+    // Flip slot i to the other side and bring its neighbours' gains (weight
+    // to the other side minus weight to their own) up to date: a neighbour
+    // that i joined loses 2w, one that i left gains 2w.  Calls touch(j) for
+    // each neighbour slot j.
+    template <typename Touch>
+    void flip(const std::vector<std::int32_t>& S, std::vector<std::uint8_t>& side, std::vector<std::int64_t>& gain,
+              std::int32_t i, Touch touch) {
+        const std::size_t ti = static_cast<std::size_t>(i);
+        side[ti] ^= 1;
+        gain[ti] = -gain[ti];
+        const std::int32_t v = S[ti];
         for (std::int64_t x = off[v]; x < off[v + 1]; ++x) {
-            const std::int32_t u = nbr[x];
-            if (u == v) continue;
-            const std::int32_t j = pos[static_cast<std::size_t>(u)];
-            if (j < 0) continue;
-            if (side[static_cast<std::size_t>(j)] == side[static_cast<std::size_t>(i)]) same += e(x);
-            else other += e(x);
+            const std::int32_t j = pos[static_cast<std::size_t>(nbr[x])];
+            if (j < 0 || j == i) continue;
+            gain[static_cast<std::size_t>(j)] += side[static_cast<std::size_t>(j)] == side[ti] ? -2 * e(x) : 2 * e(x);
+            touch(j);
         }
     }
 
-    static std::uint64_t dev(std::uint64_t a, std::uint64_t b) { return a > b ? a - b : b - a; }
-
-    // Fiduccia-Mattheyses passes over the bisection of the subset S: each
-    // pass moves, one at a time, the unlocked vertex of largest gain (cut
-    // reduction; lowest id on ties) whose move keeps the side weight within
-    // tol of the target or brings it closer, locks it and updates its
-    // neighbours' gains; the pass then keeps its best prefix (balanced
-    // before unbalanced, then smaller cut, then smaller deviation) and undoes
-    // the rest.  Passes repeat while they improve, at most eight.
-    void refine(const std::vector<std::int32_t>& S, std::vector<std::uint8_t>& side, std::uint64_t& wa,
-                std::uint64_t target, std::uint64_t tol) {
+    // Fiduccia-Mattheyses passes over the bisection of the subset S with
+    // cut `cut`: each pass moves, one at a time, the unlocked vertex of
+    // largest gain among the boundary and the neighbours of moved vertices
+    // (lowest id on ties) whose move keeps the side-0 weight within tol of
+    // the target or brings it closer, and locks it.  A pass ends after
+    // min(max(m/100, 15), 100) moves without a new best and keeps its best
+    // prefix (balanced before unbalanced, then smaller cut, then smaller
+    // deviation), flipping the rest back; passes repeat while they improve,
+    // at most kPasses.  Gains carry over between passes.  Returns the cut.
+    std::int64_t refine(const std::vector<std::int32_t>& S, std::vector<std::uint8_t>& side, std::uint64_t& wa,
+                        std::uint64_t target, std::uint64_t tol, std::int64_t cut) {
         const std::int32_t m = static_cast<std::int32_t>(S.size());
-        std::vector<std::int64_t> gain(static_cast<std::size_t>(m));
-        std::vector<std::uint32_t> stamp(static_cast<std::size_t>(m));
+        const int stall = std::min(std::max(m / 100, 15), 100);
+        std::vector<std::int64_t> gain(static_cast<std::size_t>(m), 0), internal(static_cast<std::size_t>(m), 0);
+        for (std::int32_t i = 0; i < m; ++i) {
+            const std::int32_t v = S[static_cast<std::size_t>(i)];
+            for (std::int64_t x = off[v]; x < off[v + 1]; ++x) {
+                const std::int32_t j = pos[static_cast<std::size_t>(nbr[x])];
+                if (j < 0 || j == i) continue;
+                const bool other = side[static_cast<std::size_t>(j)] != side[static_cast<std::size_t>(i)];
+                gain[static_cast<std::size_t>(i)] += other ? e(x) : -e(x);
+                internal[static_cast<std::size_t>(i)] += e(x);
+            }
+        }
         std::vector<std::uint8_t> locked(static_cast<std::size_t>(m));
         std::vector<std::int32_t> moves;
-        struct Entry {
-            std::int64_t gain;
-            std::int32_t id;  // vertex id (ties: lower first)
-            std::int32_t i;
-            std::uint32_t stamp;
-            bool operator<(const Entry& o) const { return gain != o.gain ? gain < o.gain : id > o.id; }
+        GainHeap H;
+        H.gain = &gain;
+        H.ids = &S;
+        H.where.assign(static_cast<std::size_t>(m), -1);
+        const auto key = [&](std::uint64_t x, std::int64_t c) {
+            return std::make_tuple(dev(x, target) > tol ? 1 : 0, c, dev(x, target));
         };
-        std::int64_t cut = static_cast<std::int64_t>(subset_cut(S, side));
-        for (int pass = 0; pass < 8; ++pass) {
-            std::priority_queue<Entry> heap;
+        for (int pass = 0; pass < kPasses; ++pass) {
+            H.heap.clear();
             for (std::int32_t i = 0; i < m; ++i) {
-                std::uint64_t same, other;
-                side_conn(S, side, i, same, other);
-                gain[static_cast<std::size_t>(i)] = (std::int64_t)other - (std::int64_t)same;
-                stamp[static_cast<std::size_t>(i)] = 0;
                 locked[static_cast<std::size_t>(i)] = 0;
-                heap.push(Entry{gain[static_cast<std::size_t>(i)], S[static_cast<std::size_t>(i)], i, 0});
+                H.where[static_cast<std::size_t>(i)] = -1;
+                // on the boundary: some weight to the other side
+                if (gain[static_cast<std::size_t>(i)] > -internal[static_cast<std::size_t>(i)]) H.push(i);
             }
             moves.clear();
-            const auto key = [&](std::uint64_t x, std::int64_t c) {
-                return std::make_tuple(dev(x, target) > tol ? 1 : 0, c, dev(x, target));
-            };
             auto best = key(wa, cut);
             std::size_t best_len = 0;
             std::uint64_t cur_wa = wa;
             std::int64_t cur_cut = cut;
-            while (!heap.empty()) {
-                const Entry top = heap.top();
-                heap.pop();
-                const std::size_t ti = static_cast<std::size_t>(top.i);
-                if (locked[ti] || top.stamp != stamp[ti]) continue;
+            int since_best = 0;
+            while (!H.heap.empty() && since_best <= stall) {
+                const std::int32_t i = H.pop();
+                const std::size_t ti = static_cast<std::size_t>(i);
                 const std::uint64_t wv = w(S[ti]);
                 const std::uint64_t nwa = side[ti] == 0 ? cur_wa - wv : cur_wa + wv;
                 if (dev(nwa, target) > tol && dev(nwa, target) >= dev(cur_wa, target)) continue;
                 locked[ti] = 1;
-                side[ti] ^= 1;
                 cur_wa = nwa;
                 cur_cut -= gain[ti];
-                moves.push_back(top.i);
-                const std::int32_t v = S[ti];
-                for (std::int64_t x = off[v]; x < off[v + 1]; ++x) {
-                    const std::int32_t j = pos[static_cast<std::size_t>(nbr[x])];
-                    if (j < 0 || j == top.i || locked[static_cast<std::size_t>(j)]) continue;
-                    const std::size_t tj = static_cast<std::size_t>(j);
-                    // v left j's side: the edge now counts against staying;
-                    // v joined j's side: the edge now counts for staying.
-                    gain[tj] += side[tj] == side[ti] ? -2 * (std::int64_t)e(x) : 2 * (std::int64_t)e(x);
-                    heap.push(Entry{gain[tj], S[tj], j, ++stamp[tj]});
-                }
+                moves.push_back(i);
+                flip(S, side, gain, i, [&](std::int32_t j) {
+                    if (locked[static_cast<std::size_t>(j)]) return;
+                    if (H.has(j)) H.update(j);
+                    else H.push(j);
+                });
                 const auto now = key(cur_wa, cur_cut);
                 if (now < best) {
                     best = now;
                     best_len = moves.size();
+                    since_best = 0;
+                } else {
+                    ++since_best;
                 }
             }
             for (std::size_t q = moves.size(); q > best_len; --q) {
-                const std::size_t ti = static_cast<std::size_t>(moves[q - 1]);
-                const std::uint64_t wv = w(S[ti]);
-                side[ti] ^= 1;
-                cur_wa = side[ti] == 0 ? cur_wa + wv : cur_wa - wv;
+                const std::int32_t i = moves[q - 1];
+                const std::uint64_t wv = w(S[static_cast<std::size_t>(i)]);
+                cur_wa = side[static_cast<std::size_t>(i)] == 0 ? cur_wa - wv : cur_wa + wv;
+                flip(S, side, gain, i, [](std::int32_t) {});
             }
-            const bool improved = best_len > 0;
             wa = cur_wa;
             cut = std::get<1>(best);
-            if (!improved) break;
+            if (best_len == 0) break;
         }
-    }
-
-    std::uint64_t subset_cut(const std::vector<std::int32_t>& S, const std::vector<std::uint8_t>& side) const {
-        std::uint64_t cut = 0;
-        for (std::size_t i = 0; i < S.size(); ++i) {
-            const std::int32_t v = S[i];
-            for (std::int64_t x = off[v]; x < off[v + 1]; ++x) {
-                const std::int32_t j = pos[static_cast<std::size_t>(nbr[x])];
-                if (j >= 0 && side[static_cast<std::size_t>(j)] != side[i]) cut += e(x);
-            }
-        }
-        return cut / 2;
+        return cut;
     }
 
     void bisect(std::vector<std::int32_t>& S, int p0, int kk) {
@@ -258,15 +312,19 @@ struct Bisector {
         const std::uint64_t tol = std::max<std::uint64_t>(maxw, W / 100);
         const std::int32_t m = static_cast<std::int32_t>(S.size());
         for (std::int32_t i = 0; i < m; ++i) pos[static_cast<std::size_t>(S[static_cast<std::size_t>(i)])] = i;
-        std::vector<std::uint8_t> side, best_side;
+        // Breadth-first growths of side 0 from hashed starts (restarting at
+        // the lowest unvisited slot when a component is exhausted) until it
+        // holds the target weight; the cut is kept up to date as vertices
+        // join side 0.
+        std::vector<std::vector<std::uint8_t>> grown;
+        std::vector<std::tuple<std::int64_t, std::uint64_t, int, std::uint64_t>> rank;  // cut, dev, growth, weight
         std::vector<std::uint8_t> visited;
         std::vector<std::int32_t> queue;
-        std::uint64_t best_cut = std::numeric_limits<std::uint64_t>::max(), best_dev = best_cut;
-        const int trials = std::min<std::int32_t>(8, m);
-        for (int t = 0; t < trials; ++t) {
+        const int growths = std::min<std::int32_t>(kGrowths, m);
+        for (int t = 0; t < growths; ++t) {
             const std::uint32_t h = hmix32(seed ^ ((std::uint32_t)p0 * 0x9e3779b9U) ^ ((std::uint32_t)kk * 0x85ebca6bU) ^
                                            ((std::uint32_t)t * 0xc2b2ae35U));
-            side.assign(static_cast<std::size_t>(m), 1);
+            std::vector<std::uint8_t> side(static_cast<std::size_t>(m), 1);
             visited.assign(static_cast<std::size_t>(m), 0);
             queue.clear();
             std::size_t head = 0;
@@ -275,6 +333,7 @@ struct Bisector {
             queue.push_back(s0);
             visited[static_cast<std::size_t>(s0)] = 1;
             std::uint64_t wa = 0;
+            std::int64_t cut = 0;
             while (wa < target) {
                 if (head == queue.size()) {
                     while (next_unvisited < m && visited[static_cast<std::size_t>(next_unvisited)]) ++next_unvisited;
@@ -288,14 +347,25 @@ struct Bisector {
                 wa += w(v);
                 for (std::int64_t x = off[v]; x < off[v + 1]; ++x) {
                     const std::int32_t j = pos[static_cast<std::size_t>(nbr[x])];
-                    if (j >= 0 && !visited[static_cast<std::size_t>(j)]) {
+                    if (j < 0 || j == i) continue;
+                    cut += side[static_cast<std::size_t>(j)] == 0 ? -e(x) : e(x);
+                    if (!visited[static_cast<std::size_t>(j)]) {
                         visited[static_cast<std::size_t>(j)] = 1;
                         queue.push_back(j);
                     }
                 }
             }
-            refine(S, side, wa, target, tol);
-            const std::uint64_t cut = subset_cut(S, side);
+            rank.emplace_back(cut, dev(wa, target), t, wa);
+            grown.push_back(std::move(side));
+        }
+        std::sort(rank.begin(), rank.end());
+        std::int64_t best_cut = std::numeric_limits<std::int64_t>::max();
+        std::uint64_t best_dev = std::numeric_limits<std::uint64_t>::max();
+        std::vector<std::uint8_t> best_side;
+        for (std::size_t r = 0; r < rank.size() && r < static_cast<std::size_t>(kRefined); ++r) {
+            std::vector<std::uint8_t>& side = grown[static_cast<std::size_t>(std::get<2>(rank[r]))];
+            std::uint64_t wa = std::get<3>(rank[r]);
+            const std::int64_t cut = refine(S, side, wa, target, tol, std::get<0>(rank[r]));
             const std::uint64_t d = dev(wa, target);
             if (cut < best_cut || (cut == best_cut && d < best_dev)) {
                 best_cut = cut;
@@ -320,36 +390,39 @@ void initial_partition_host(std::int64_t n, const std::int32_t* off, const std::
                             const std::uint32_t* vw, int parts, double imbalance, std::uint32_t seed,
                             std::int32_t* labels) {
     if (n <= 0 || parts < 2) throw std::invalid_argument("invalid initial partition input");
-    // Whole recursive bisections from eight seeds; the labels kept are the
-    // ones of smallest (load above the integer capacity, cut), first seed on
-    // ties.
     std::uint64_t total = 0;
     for (std::int64_t v = 0; v < n; ++v) total += vw ? vw[v] : 1u;
     const long double raw = (long double)total * (imbalance >= 1.0 ? imbalance : 1.0) / parts;
     const std::uint64_t cap = (std::uint64_t)std::floor(raw);
-    std::vector<std::int32_t> trial(static_cast<std::size_t>(n));
-    std::uint64_t best_over = std::numeric_limits<std::uint64_t>::max(), best_cut = best_over;
-    for (int t = 0; t < 8; ++t) {
+    std::vector<std::vector<std::int32_t>> trial(static_cast<std::size_t>(kRecursions));
+    std::vector<std::pair<std::uint64_t, std::uint64_t>> score(static_cast<std::size_t>(kRecursions));
+    auto run = [&](int t) {
         const std::uint32_t s = t == 0 ? seed : hmix32(seed ^ (0x68e31da4U * (std::uint32_t)t));
-        Bisector b{n, off, nbr, ew, vw, s, trial.data(), std::vector<std::int32_t>(static_cast<std::size_t>(n), -1)};
+        std::vector<std::int32_t>& lab = trial[static_cast<std::size_t>(t)];
+        lab.assign(static_cast<std::size_t>(n), 0);
+        Bisector b{n, off, nbr, ew, vw, s, lab.data(), std::vector<std::int32_t>(static_cast<std::size_t>(n), -1)};
         std::vector<std::int32_t> all(static_cast<std::size_t>(n));
         for (std::int64_t v = 0; v < n; ++v) all[static_cast<std::size_t>(v)] = static_cast<std::int32_t>(v);
         b.bisect(all, 0, parts);
         std::vector<std::uint64_t> loads(static_cast<std::size_t>(parts), 0);
         std::uint64_t directed = 0;
         for (std::int64_t v = 0; v < n; ++v) {
-            loads[static_cast<std::size_t>(trial[static_cast<std::size_t>(v)])] += vw ? vw[v] : 1u;
+            loads[static_cast<std::size_t>(lab[static_cast<std::size_t>(v)])] += vw ? vw[v] : 1u;
             for (std::int64_t x = off[v]; x < off[v + 1]; ++x)
-                if (trial[static_cast<std::size_t>(nbr[x])] != trial[static_cast<std::size_t>(v)]) directed += ew ? ew[x] : 1u;
+                if (lab[static_cast<std::size_t>(nbr[x])] != lab[static_cast<std::size_t>(v)]) directed += ew ? ew[x] : 1u;
         }
         std::uint64_t over = 0;
         for (auto l : loads) over += l > cap ? l - cap : 0;
-        if (over < best_over || (over == best_over && directed / 2 < best_cut)) {
-            best_over = over;
-            best_cut = directed / 2;
-            std::copy(trial.begin(), trial.end(), labels);
-        }
-    }
+        score[static_cast<std::size_t>(t)] = {over, directed / 2};
+    };
+    std::vector<std::thread> threads;
+    for (int t = 1; t < kRecursions; ++t) threads.emplace_back(run, t);
+    run(0);
+    for (auto& th : threads) th.join();
+    int best = 0;
+    for (int t = 1; t < kRecursions; ++t)
+        if (score[static_cast<std::size_t>(t)] < score[static_cast<std::size_t>(best)]) best = t;
+    std::copy(trial[static_cast<std::size_t>(best)].begin(), trial[static_cast<std::size_t>(best)].end(), labels);
 }
 
 // ------------------------------------------------------------- generators

@@ -20,6 +20,13 @@ struct gp_context {
     gp::Context ctx;
 };
 
+namespace gp {
+std::uint64_t& thread_launches() {
+    thread_local std::uint64_t count = 0;
+    return count;
+}
+}  // namespace gp
+
 namespace {
 std::atomic<std::uint64_t> g_graph_serial{0};
 }
@@ -635,6 +642,7 @@ int gp_partition(gp_context* ctx, gp_graph* g, int parts, double imbalance, uint
         gp::Context& c = ctx->ctx;
         gp::DGraph& d = *g->view;
         gp::capacity_of(static_cast<std::uint64_t>(d.n), parts, imbalance);
+        const std::uint64_t launches0 = gp::thread_launches(), syncs0 = c.syncs();
         const auto t0 = std::chrono::steady_clock::now();
         gp::Hierarchy h = gp::coarsen(c, d, gp::CoarsenOptions::production(parts, seed));
         c.sync();
@@ -664,6 +672,7 @@ int gp_partition(gp_context* ctx, gp_graph* g, int parts, double imbalance, uint
         lab.reset();
         c.sync();
         const double t_total = seconds_since(t0);
+        const std::uint64_t launches = gp::thread_launches() - launches0, syncs = c.syncs() - syncs0;
         if (labels) c.download(labels, r.labels.get(), static_cast<std::size_t>(d.n) * 4);
         if (stats) {
             stats->coarsen_seconds = t_coarsen;
@@ -681,6 +690,8 @@ int gp_partition(gp_context* ctx, gp_graph* g, int parts, double imbalance, uint
             stats->max_part_weight = mx;
             stats->capacity = gp::capacity_of(total, parts, imbalance);
             stats->total_edge_weight = gp::device_edge_weight_sum(c, d) / 2;
+            stats->launches = launches;
+            stats->host_syncs = syncs;
         }
     });
 }

@@ -22,7 +22,14 @@ struct HipError : std::runtime_error {
                                  hipGetErrorString(gp_err__));                         \
     } while (0)
 
-#define GP_LAUNCH_CHECK() GP_HIP(hipGetLastError())
+// Kernel launches made by the calling host thread: every launch site goes
+// through GP_LAUNCH_CHECK, and a partition reads the count before and after.
+std::uint64_t& thread_launches();
+#define GP_LAUNCH_CHECK()              \
+    do {                               \
+        ++::gp::thread_launches();     \
+        GP_HIP(hipGetLastError());     \
+    } while (0)
 
 constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kMaxParts = 128;

@@ -252,7 +252,11 @@ public:
     // instead of sleeping in hipStreamSynchronize: the loop control reads
     // counters back about a thousand times per partition, and each wake-up
     // is GPU idle time.
+    // Host waits on the stream (read-backs included) since creation.
+    std::uint64_t syncs() const { return syncs_; }
+
     void sync() {
+        ++syncs_;
         if (!spin_) {
             GP_HIP(hipStreamSynchronize(stream_));
             return;
@@ -271,6 +275,7 @@ public:
     void read(void* host, const void* dev, std::size_t bytes) {
         if (!bytes) return;
         if (bytes > kPinnedBytes) {
+            ++::gp::thread_launches();
             GP_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream_));
             sync();
             return;
@@ -286,7 +291,7 @@ public:
         else
             hipLaunchKernelGGL(k_copy_out<std::uint8_t>, dim3(1), dim3(256), 0, stream_,
                                static_cast<const std::uint8_t*>(dev), static_cast<std::uint8_t*>(pinned_), bytes);
-        GP_HIP(hipGetLastError());
+        GP_LAUNCH_CHECK();
         sync();
         std::memcpy(host, pinned_, bytes);
     }
@@ -298,7 +303,7 @@ public:
             throw std::invalid_argument("read_clear: region exceeds the read-back buffer");
         hipLaunchKernelGGL(k_copy_clear<unsigned long long>, dim3(1), dim3(256), 0, stream_, dev,
                            static_cast<unsigned long long*>(pinned_), words, nclear);
-        GP_HIP(hipGetLastError());
+        GP_LAUNCH_CHECK();
         sync();
         std::memcpy(host, pinned_, words * sizeof(unsigned long long));
     }
@@ -310,16 +315,19 @@ public:
     }
     void upload(void* dev, const void* host, std::size_t bytes) {
         if (!bytes) return;
+        ++::gp::thread_launches();  // copies count as stream commands
         GP_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, stream_));
         sync();
     }
     void download(void* host, const void* dev, std::size_t bytes) {
         if (!bytes) return;
+        ++::gp::thread_launches();
         GP_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream_));
         sync();
     }
     void copy_d2d(void* dst, const void* src, std::size_t bytes) {
         if (!bytes) return;
+        ++::gp::thread_launches();
         GP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_));
     }
 
@@ -388,6 +396,7 @@ private:
     void* pinned_ = nullptr;
     int light_lanes_ = 4;
     int medium_lanes_ = 64;
+    std::uint64_t syncs_ = 0;
 };
 
 }  // namespace gp

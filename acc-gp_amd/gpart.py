@@ -49,7 +49,7 @@ class PartitionStats(ctypes.Structure):
         ("uncoarsen_seconds", ctypes.c_double), ("algorithm_total_seconds", ctypes.c_double),
         ("levels", ctypes.c_int), ("coarsest_vertices", ctypes.c_int64), ("final_cut", ctypes.c_uint64),
         ("max_part_weight", ctypes.c_uint64), ("capacity", ctypes.c_uint64),
-        ("total_edge_weight", ctypes.c_uint64),
+        ("total_edge_weight", ctypes.c_uint64), ("launches", ctypes.c_uint64), ("host_syncs", ctypes.c_uint64),
     ]
 
 

@@ -94,7 +94,8 @@ def main():
         per_step.append(time.perf_counter() - s0)
         phases = {"coarsen_ms": 1000.0 * st["coarsen_seconds"],
                   "initial_partition_ms": 1000.0 * st["initial_partition_seconds"],
-                  "uncoarsen_ms": 1000.0 * st["uncoarsen_seconds"]}
+                  "uncoarsen_ms": 1000.0 * st["uncoarsen_seconds"], "launches": st["launches"],
+                  "host_syncs": st["host_syncs"]}
         print("bench: step %d %.3fs (algorithm %.3fs) cut=%d" % (i, per_step[-1], st["algorithm_total_seconds"],
                                                                    st["final_cut"]), file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
@@ -156,8 +157,7 @@ def main():
     if args.batch_graphs > 0:
         out["batched"], mix, items = batched_leg(gp, ctx, args.batch_graphs, args.batch_workers)
         if args.serve_requests > 0:
-            out["serving"] = serving_leg(gp, ctx, mix, items, args.batch_workers, args.serve_requests,
-                                         0.7 * out["batched"]["graphs"] / out["batched"]["seconds"])
+            out["serving"] = serving_leg(gp, ctx, mix, items, args.batch_workers, args.serve_requests)
     if not args.no_cpu:
         from oracle import oracle as og
         out["cpu_baseline"] = cpu_baseline(gp, og, args.cpu_scale, args.parts, args.cpu_repeats)
@@ -237,11 +237,16 @@ def batched_leg(gp, ctx, count, workers):
             "cut_geomean_ratio": float(np.exp(np.mean(np.log(cut_ratio)))),
             "p50_request_ms": 1000.0 * float(np.percentile(lat, 50)),
             "p99_request_ms": 1000.0 * float(np.percentile(lat, 99)),
-            "plain_p99_request_ms": 1000.0 * float(np.percentile(plain_lat, 99))}, mix, items
+            "plain_p99_request_ms": 1000.0 * float(np.percentile(plain_lat, 99)),
+            "launches_per_graph": float(np.mean([st["launches"] for st in stats])),
+            "host_syncs_per_graph": float(np.mean([st["host_syncs"] for st in stats])),
+            "initial_partition_ms_per_graph": 1000.0 * float(np.mean([st["initial_partition_seconds"]
+                                                                       for st in stats]))}, mix, items
 
 
-def serving_leg(gp, ctx, mix, done_items, workers, count, rate, seed=6):
-    """Poisson arrivals at `rate` requests/s (70% of the batched capacity):
+def serving_leg(gp, ctx, mix, done_items, workers, count, load=0.7, seed=6):
+    """Poisson arrivals at `load` times the stream's own capacity (its cold
+    requests partitioned back to back by the batched form, in requests/s):
     cold requests from the mix with k in [2, 128], and 10% repartitions of
     resident graphs (kept in HBM by the server with their current
     partition) after a 1% edge delta applied on the device, warm-started
@@ -251,6 +256,11 @@ def serving_leg(gp, ctx, mix, done_items, workers, count, rate, seed=6):
     deltas."""
     from tests import workloads
     reqs = workloads.serving_requests(gp, mix, count, seed)
+    cold = [gp.BatchItem(p[0], p[1], k) for kind, _, p, k in reqs if kind == "cold"]
+    t0 = time.perf_counter()
+    gp.partition_batch(cold, workers=workers)
+    capacity = len(cold) / (time.perf_counter() - t0)
+    rate = load * capacity
     srv = gp.Server(workers=workers)
     try:
         resident = {}
@@ -296,8 +306,12 @@ def serving_leg(gp, ctx, mix, done_items, workers, count, rate, seed=6):
     print("bench: served %d requests in %.2fs, p99 %.1f ms" % (count, wall, 1000 * np.percentile(lat, 99)),
           file=sys.stderr, flush=True)
     geo = lambda xs: float(np.exp(np.mean(np.log(xs)))) if xs else None
-    return {"requests": count, "rate_per_s": rate, "workers": workers, "edges_per_s": edges / wall,
+    mean = lambda key: float(np.mean([st[key] for st in stats]))
+    return {"requests": count, "capacity_per_s": capacity, "rate_per_s": rate, "workers": workers,
+            "edges_per_s": edges / wall,
             "p50_ms": 1000.0 * float(np.percentile(lat, 50)), "p99_ms": 1000.0 * float(np.percentile(lat, 99)),
+            "mean_queue_ms": 1000.0 * mean("queue_seconds"), "mean_device_ms": 1000.0 * mean("device_seconds"),
+            "mean_initial_partition_ms": 1000.0 * mean("initial_partition_seconds"),
             "repartitions": len(vs_cold), "resident_graphs": len(resident),
             "warm_vs_cold_cut_geomean": geo(vs_cold), "warm_vs_original_cold_cut_geomean": geo(vs_original)}
 

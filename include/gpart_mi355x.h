@@ -80,6 +80,8 @@ typedef struct gp_partition_stats {       /* partition_timing line, app/partitio
     uint64_t max_part_weight;
     uint64_t capacity;
     uint64_t total_edge_weight;
+    uint64_t launches;      /* no counterpart: kernel launches and copies issued by the partition */
+    uint64_t host_syncs;    /* no counterpart: host waits on the stream (read-backs included) */
 } gp_partition_stats;
 
 const char* gp_last_error(void);

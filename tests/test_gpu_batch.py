@@ -108,8 +108,10 @@ def test_apply_delta_matches_host(gp, ctx):
         _say("case %s: edge delta" % name)
         u, v, ins = workloads.random_delta(off, nbr, 0.02, rng)
         a = int(nbr[off[5]])
+        row5 = set(nbr[off[5]:off[6]].tolist())
+        b = next(x for x in range(len(off) - 1) if x != 5 and x not in row5)  # not adjacent to 5
         u = np.concatenate([u, [5, 5, 11, 11, a, a]]).astype(np.int32)
-        v = np.concatenate([v, [(a + 1) % (len(off) - 1) or 1, a, 17, 17, 5, 5]]).astype(np.int32)
+        v = np.concatenate([v, [b, a, 17, 17, 5, 5]]).astype(np.int32)
         ins = np.concatenate([ins, [0, 1, 1, 1, 0, 1]]).astype(np.uint8)
         w = rng.integers(1, 30, len(u)).astype(np.uint32) if with_w else None
         want_off, want_nbr, want_ew = workloads.apply_delta(off, nbr, ew, u, v, ins, w)

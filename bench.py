@@ -42,6 +42,7 @@ def main():
     ap.add_argument("--batch-graphs", type=int, default=256, help="graphs in the batched mix (0: skip)")
     ap.add_argument("--batch-workers", type=int, default=6)
     ap.add_argument("--serve-requests", type=int, default=128, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--serve-detail", default="", help="write the serving leg's per-request records to this JSON file")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 1 GiB device-to-device copies (counter calibration) and exit")
     ap.add_argument("--trace-mix", action="store_true",
@@ -157,7 +158,8 @@ def main():
     if args.batch_graphs > 0:
         out["batched"], mix, items = batched_leg(gp, ctx, args.batch_graphs, args.batch_workers)
         if args.serve_requests > 0:
-            out["serving"] = serving_leg(gp, ctx, mix, items, args.batch_workers, args.serve_requests)
+            out["serving"] = serving_leg(gp, ctx, mix, items, args.batch_workers, args.serve_requests,
+                                         detail=args.serve_detail)
     if not args.no_cpu:
         from oracle import oracle as og
         out["cpu_baseline"] = cpu_baseline(gp, og, args.cpu_scale, args.parts, args.cpu_repeats)
@@ -244,7 +246,7 @@ def batched_leg(gp, ctx, count, workers):
                                                                        for st in stats]))}, mix, items
 
 
-def serving_leg(gp, ctx, mix, done_items, workers, count, load=0.7, seed=6):
+def serving_leg(gp, ctx, mix, done_items, workers, count, load=0.7, seed=6, detail=""):
     """Poisson arrivals at `load` times the stream's own capacity (its cold
     requests partitioned back to back by the batched form, in requests/s):
     cold requests from the mix with k in [2, 128], and 10% repartitions of
@@ -288,6 +290,12 @@ def serving_leg(gp, ctx, mix, done_items, workers, count, load=0.7, seed=6):
     finally:
         srv.close()
     lat = [st["queue_seconds"] + st["device_seconds"] for st in stats]
+    if detail:
+        with open(detail, "w") as f:
+            json.dump([{"kind": kind, "graph": mix[j][0], "k": k, "arrival_s": float(at), "queue_s": st["queue_seconds"],
+                        "device_s": st["device_seconds"], "initial_partition_s": st["initial_partition_seconds"],
+                        "launches": st["launches"], "host_syncs": st["host_syncs"]}
+                       for (kind, j, _, k), at, st in zip(reqs, arrivals, stats)], f)
     edges = 0
     host = {j: (mix[j][1], mix[j][2], None) for j in resident}
     vs_cold, vs_original = [], []

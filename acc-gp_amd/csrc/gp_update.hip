@@ -13,9 +13,10 @@
 // Layout on the device: the directed delta records (both directions) sorted
 // by (source, kind, neighbour), with one record range per changed row.  Rows
 // without a record move by an entry-parallel copy (each entry finds its row
-// by a binary search over the old offsets); changed rows are rebuilt by one
-// thread each (few rows, short record lists).  Bound: HBM, one read and one
-// write of the CSR.
+// by a binary search over the old offsets); a changed row is scanned and
+// rebuilt by one wave, each entry looked up by binary search among the row's
+// sorted deletions and insertions (hub rows carry many records).  Bound:
+// HBM, one read and one write of the CSR.
 #include "gp_update.hpp"
 
 #include <algorithm>
@@ -30,25 +31,22 @@ namespace {
 struct DeltaRows {
     const std::int32_t* row;      // changed rows, ascending
     const std::int32_t* first;    // records of row i: [first[i], first[i + 1])
-    const std::int32_t* dst;      // record neighbour
-    const std::uint8_t* insert;   // 0 delete, 1 insert (deletions first within a row)
+    const std::int32_t* mid;      // deletions [first[i], mid[i]), insertions [mid[i], first[i + 1])
+    const std::int32_t* dst;      // record neighbour, ascending within each kind
     const std::uint32_t* weight;  // insert weight
+    std::uint8_t* present;        // per insertion: its edge is in the row after the deletions
     std::int64_t count;           // changed rows
 };
 
-__device__ inline bool deleted_in(const DeltaRows& d, std::int64_t i, std::int32_t u) {
-    for (std::int32_t r = d.first[i]; r < d.first[i + 1] && !d.insert[r]; ++r)
-        if (d.dst[r] == u) return true;
-    return false;
-}
-
-// Whether u survives in row v (present and not deleted).
-__device__ inline bool survives(const std::int32_t* nbr, std::int64_t b, std::int64_t e, const DeltaRows& d,
-                                std::int64_t i, std::int32_t u) {
-    if (deleted_in(d, i, u)) return false;
-    for (std::int64_t x = b; x < e; ++x)
-        if (nbr[x] == u) return true;
-    return false;
+// Position of u in the ascending range [lo, hi) of a, or -1.
+__device__ inline std::int32_t find_sorted(const std::int32_t* a, std::int32_t lo, std::int32_t hi, std::int32_t u) {
+    const std::int32_t end = hi;
+    while (lo < hi) {
+        const std::int32_t mid = lo + ((hi - lo) >> 1);
+        if (a[mid] < u) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < end && a[lo] == u ? lo : -1;
 }
 
 __global__ void k_delta_degree(std::int64_t n, const std::int32_t* off, std::int32_t* deg) {
@@ -57,17 +55,37 @@ __global__ void k_delta_degree(std::int64_t n, const std::int32_t* off, std::int
         deg[v] = off[v + 1] - off[v];
 }
 
-__global__ void k_delta_row_degree(const std::int32_t* off, const std::int32_t* nbr, DeltaRows d, std::int32_t* deg,
-                                   std::uint8_t* changed) {
+// One wave per changed row: the entries that survive the row's deletions
+// (a binary search over the sorted deletions each), and the insertions whose
+// edge is among them (flagged present).  deg[v] receives the survivors.
+__global__ void k_delta_scan_rows(const std::int32_t* off, const std::int32_t* nbr, DeltaRows d, std::int32_t* deg) {
+    const int lane = lane_id();
+    const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;
+    for (std::int64_t i = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave; i < d.count; i += waves) {
+        const std::int32_t v = d.row[i];
+        const std::int64_t b = off[v], e = off[v + 1];
+        const std::int32_t f0 = d.first[i], fm = d.mid[i], f1 = d.first[i + 1];
+        std::int64_t kept = 0;
+        for (std::int64_t x = b + lane; x < e; x += kWave) {
+            const std::int32_t u = nbr[x];
+            if (fm > f0 && find_sorted(d.dst, f0, fm, u) >= 0) continue;
+            ++kept;
+            const std::int32_t r = f1 > fm ? find_sorted(d.dst, fm, f1, u) : -1;
+            if (r >= 0) d.present[r] = 1;
+        }
+        kept = wave_sum(kept);
+        if (lane == 0) deg[v] = (std::int32_t)kept;
+    }
+}
+
+// One thread per changed row: survivors plus the insertions not present.
+__global__ void k_delta_row_degree(DeltaRows d, std::int32_t* deg, std::uint8_t* changed) {
     for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < d.count;
          i += (std::int64_t)gridDim.x * blockDim.x) {
         const std::int32_t v = d.row[i];
-        const std::int64_t b = off[v], e = off[v + 1];
-        std::int64_t kept = 0, added = 0;
-        for (std::int64_t x = b; x < e; ++x) kept += !deleted_in(d, i, nbr[x]);
-        for (std::int32_t r = d.first[i]; r < d.first[i + 1]; ++r)
-            if (d.insert[r] && !survives(nbr, b, e, d, i, d.dst[r])) ++added;
-        deg[v] = (std::int32_t)(kept + added);
+        std::int32_t added = 0;
+        for (std::int32_t r = d.mid[i]; r < d.first[i + 1]; ++r) added += !d.present[r];
+        deg[v] += added;
         changed[v] = 1;
     }
 }
@@ -91,24 +109,44 @@ __global__ void k_delta_copy(std::int64_t n, std::int64_t m, const std::int32_t*
     }
 }
 
+// One wave per changed row: the survivors in their old order (ballot
+// compaction), then the insertions not present, ascending.
 __global__ void k_delta_rows(const std::int32_t* off, const std::int32_t* nbr, const std::uint32_t* ew, DeltaRows d,
                              const std::int32_t* noff, std::int32_t* nnbr, std::uint32_t* new_ew) {
-    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < d.count;
-         i += (std::int64_t)gridDim.x * blockDim.x) {
+    const int lane = lane_id();
+    const unsigned long long below = (1ULL << lane) - 1ULL;
+    const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;
+    for (std::int64_t i = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave; i < d.count; i += waves) {
         const std::int32_t v = d.row[i];
         const std::int64_t b = off[v], e = off[v + 1];
+        const std::int32_t f0 = d.first[i], fm = d.mid[i], f1 = d.first[i + 1];
         std::int64_t y = noff[v];
-        for (std::int64_t x = b; x < e; ++x) {
-            if (deleted_in(d, i, nbr[x])) continue;
-            nnbr[y] = nbr[x];
-            if (new_ew) new_ew[y] = ew ? ew[x] : 1u;
-            ++y;
+        for (std::int64_t x0 = b; x0 < e; x0 += kWave) {
+            const std::int64_t x = x0 + lane;
+            std::int32_t u = 0;
+            bool keep = false;
+            if (x < e) {
+                u = nbr[x];
+                keep = fm == f0 || find_sorted(d.dst, f0, fm, u) < 0;
+            }
+            const unsigned long long mask = __ballot(keep);
+            if (keep) {
+                const std::int64_t at = y + __popcll(mask & below);
+                nnbr[at] = u;
+                if (new_ew) new_ew[at] = ew ? ew[x] : 1u;
+            }
+            y += __popcll(mask);
         }
-        for (std::int32_t r = d.first[i]; r < d.first[i + 1]; ++r) {
-            if (!d.insert[r] || survives(nbr, b, e, d, i, d.dst[r])) continue;
-            nnbr[y] = d.dst[r];
-            if (new_ew) new_ew[y] = d.weight[r];
-            ++y;
+        for (std::int32_t r0 = fm; r0 < f1; r0 += kWave) {
+            const std::int32_t r = r0 + lane;
+            const bool add = r < f1 && !d.present[r];
+            const unsigned long long mask = __ballot(add);
+            if (add) {
+                const std::int64_t at = y + __popcll(mask & below);
+                nnbr[at] = d.dst[r];
+                if (new_ew) new_ew[at] = d.weight[r];
+            }
+            y += __popcll(mask);
         }
     }
 }
@@ -163,26 +201,37 @@ std::int64_t apply_delta(Context& c, DGraph& g, std::int64_t count, const std::i
     first.push_back(static_cast<std::int32_t>(recs.size()));
     const std::int64_t nrows = static_cast<std::int64_t>(rows.size());
     if (!nrows) return g.m;
+    // Within a row the deletions come first: mid[i] is its first insertion.
+    std::vector<std::int32_t> mid(static_cast<std::size_t>(nrows));
+    for (std::int64_t i = 0; i < nrows; ++i) {
+        std::int32_t r = first[static_cast<std::size_t>(i)];
+        while (r < first[static_cast<std::size_t>(i) + 1] && !kind[static_cast<std::size_t>(r)]) ++r;
+        mid[static_cast<std::size_t>(i)] = r;
+    }
 
     Arena* home = g.off.arena();  // the new arrays belong where the graph's do
     const hipStream_t s = c.stream();
-    DBuf<std::int32_t> d_rows(home, rows.size()), d_first(home, first.size()), d_dst(home, dst.size());
-    DBuf<std::uint8_t> d_kind(home, kind.size());
+    DBuf<std::int32_t> d_rows(home, rows.size()), d_first(home, first.size()), d_mid(home, mid.size()),
+        d_dst(home, dst.size());
+    DBuf<std::uint8_t> d_present(home, dst.size());
     DBuf<std::uint32_t> d_weight(home, weight.size());
     c.upload(d_rows.get(), rows.data(), rows.size() * 4);
     c.upload(d_first.get(), first.data(), first.size() * 4);
+    c.upload(d_mid.get(), mid.data(), mid.size() * 4);
     c.upload(d_dst.get(), dst.data(), dst.size() * 4);
-    c.upload(d_kind.get(), kind.data(), kind.size());
     c.upload(d_weight.get(), weight.data(), weight.size() * 4);
-    const DeltaRows d{d_rows.get(), d_first.get(), d_dst.get(), d_kind.get(), d_weight.get(), nrows};
+    fill<std::uint8_t>(c, d_present.get(), static_cast<std::int64_t>(dst.size()), 0);
+    const DeltaRows d{d_rows.get(), d_first.get(), d_mid.get(), d_dst.get(), d_weight.get(), d_present.get(), nrows};
 
     DBuf<std::int32_t> deg(home, static_cast<std::size_t>(g.n) + 1);
     DBuf<std::uint8_t> changed(home, static_cast<std::size_t>(g.n));
     fill<std::uint8_t>(c, changed.get(), g.n, 0);
     hipLaunchKernelGGL(k_delta_degree, dim3(blocks_for(g.n)), dim3(256), 0, s, g.n, g.off.get(), deg.get());
     GP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_delta_row_degree, dim3(blocks_for(nrows)), dim3(256), 0, s, g.off.get(), g.nbr.get(), d,
-                       deg.get(), changed.get());
+    hipLaunchKernelGGL(k_delta_scan_rows, dim3(blocks_for(nrows * kWave)), dim3(256), 0, s, g.off.get(), g.nbr.get(),
+                       d, deg.get());
+    GP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_delta_row_degree, dim3(blocks_for(nrows)), dim3(256), 0, s, d, deg.get(), changed.get());
     GP_LAUNCH_CHECK();
     fill<std::int32_t>(c, deg.get() + g.n, 1, 0);
     DBuf<unsigned long long> total(home, 1);
@@ -199,7 +248,7 @@ std::int64_t apply_delta(Context& c, DGraph& g, std::int64_t count, const std::i
                            g.ew_ptr(), changed.get(), noff.get(), nnbr.get(), weighted ? new_ew.get() : nullptr);
         GP_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_delta_rows, dim3(blocks_for(nrows)), dim3(256), 0, s, g.off.get(), g.nbr.get(), g.ew_ptr(),
+    hipLaunchKernelGGL(k_delta_rows, dim3(blocks_for(nrows * kWave)), dim3(256), 0, s, g.off.get(), g.nbr.get(), g.ew_ptr(),
                        d, noff.get(), nnbr.get(), weighted ? new_ew.get() : nullptr);
     GP_LAUNCH_CHECK();
     c.sync();

@@ -20,6 +20,15 @@ namespace {
 constexpr int kBalanceBuckets = 50;
 constexpr int kBalanceSections = 16;
 constexpr int kBalanceWidth = kBalanceBuckets * kBalanceSections;
+
+// Slot of bucket b = (gain bucket, section) in a part's row of the counts
+// array: section-major, so the sections of one gain bucket (the donors of a
+// part with similar gains, often most of them) sit kBalanceBuckets words
+// apart, in different cache lines, and their atomics spread over memory
+// channels instead of queueing on one line.
+__host__ __device__ constexpr int count_slot(int b) {
+    return (b % kBalanceSections) * kBalanceBuckets + b / kBalanceSections;
+}
 constexpr int kJointSeeds = 128;
 constexpr int kJointSide = 4;
 constexpr int kJointArchive = 4;
@@ -657,10 +666,9 @@ __device__ inline void repair_finish(const RefArgs& a, std::int64_t v, int s, st
     const std::uint64_t excess = load0[s] - a.cap;
     const std::uint64_t relief = w < excess ? w : excess;
     const double gpw = gain / (double)relief;
-    const int b = s * kBalanceWidth + gain_bucket(a, gpw) * kBalanceSections +
-                  (int)(mix32((std::uint32_t)v) % kBalanceSections);
-    a.bucket[v] = b;
-    atomicAdd(&a.counts[b], (unsigned long long)w);
+    const int lb = gain_bucket(a, gpw) * kBalanceSections + (int)(mix32((std::uint32_t)v) % kBalanceSections);
+    a.bucket[v] = s * kBalanceWidth + lb;
+    atomicAdd(&a.counts[s * kBalanceWidth + count_slot(lb)], (unsigned long long)w);
 }
 
 template <class Bin>
@@ -730,7 +738,7 @@ __global__ void __launch_bounds__(kScanThreads) k_bucket_scan(RefArgs a, unsigne
     unsigned long long carry = 0;
     for (int b0 = 0; b0 < kBalanceWidth; b0 += kScanThreads) {
         const int b = b0 + threadIdx.x;
-        const unsigned long long cnt = b < kBalanceWidth ? a.counts[p * kBalanceWidth + b] : 0;
+        const unsigned long long cnt = b < kBalanceWidth ? a.counts[p * kBalanceWidth + count_slot(b)] : 0;
         unsigned long long inc, exc, tot;
         block_scan<PlusOp>(cnt, wsum, inc, exc, tot);
         const unsigned long long sum = carry + exc;

@@ -631,8 +631,32 @@ __device__ inline RepairPick team_reduce(const BlockTeam& t, RepairPick c) {
 // enumerate_repair_destination (rebalance_select.cuh:8-75) on frozen loads,
 // then the gain bucket (incremental.cuh:559-626).
 // load0: the frozen part loads (the block's LDS copy where it has one).
+// Block-level table of repair bucket sums in LDS: kAggSlots keys (u32 count
+// slots, all ones when free) packed two per word, then kAggSlots u64 sums.
+// Linear probing over at most kAggProbes slots; a sum that finds no slot goes
+// to global memory directly.  Sums are order-independent, so the counts are
+// the same with or without the table.
+constexpr int kAggSlots = 512;
+constexpr int kAggProbes = 8;
+constexpr int kAggWords = kAggSlots / 2 + kAggSlots;
+
+__device__ inline bool agg_add(unsigned long long* agg, std::uint32_t key, unsigned long long w) {
+    std::uint32_t* keys = reinterpret_cast<std::uint32_t*>(agg);
+    unsigned long long* sums = agg + kAggSlots / 2;
+    std::uint32_t h = mix32(key) & (kAggSlots - 1);
+    for (int probe = 0; probe < kAggProbes; ++probe) {
+        const std::uint32_t old = atomicCAS(&keys[h], 0xffffffffu, key);
+        if (old == 0xffffffffu || old == key) {
+            atomicAdd(&sums[h], w);
+            return true;
+        }
+        h = (h + 1) & (kAggSlots - 1);
+    }
+    return false;
+}
+
 __device__ inline void repair_finish(const RefArgs& a, std::int64_t v, int s, std::uint64_t w, std::uint64_t current,
-                                     RepairPick best, const unsigned long long* load0) {
+                                     RepairPick best, const unsigned long long* load0, unsigned long long* agg) {
     if (best.p < 0) {
         // Slack-weighted hash routing in part order.
         std::uint64_t tot = 0;
@@ -668,16 +692,26 @@ __device__ inline void repair_finish(const RefArgs& a, std::int64_t v, int s, st
     const double gpw = gain / (double)relief;
     const int lb = gain_bucket(a, gpw) * kBalanceSections + (int)(mix32((std::uint32_t)v) % kBalanceSections);
     a.bucket[v] = s * kBalanceWidth + lb;
-    atomicAdd(&a.counts[s * kBalanceWidth + count_slot(lb)], (unsigned long long)w);
+    const std::uint32_t slot = (std::uint32_t)(s * kBalanceWidth + count_slot(lb));
+    if (!agg || !agg_add(agg, slot, (unsigned long long)w)) atomicAdd(&a.counts[slot], (unsigned long long)w);
 }
 
 template <class Bin>
 struct RepairProposeBody : RowBody {
     using LightBin = Bin;
-    // Frozen part loads in LDS for the destination scan.
-    __host__ __device__ int part_words() const { return a.k; }
+    // Frozen part loads in LDS for the destination scan, then the block's
+    // table of bucket sums: the donors of one part mostly share a few
+    // buckets, so each block adds its sums to the global counts once.
+    __host__ __device__ int part_words() const { return a.k + kAggWords; }
     __device__ void stage(unsigned long long* t) const {
         for (int p = threadIdx.x; p < a.k; p += blockDim.x) t[p] = a.loads0[p];
+        for (int i = threadIdx.x; i < kAggWords; i += blockDim.x) t[a.k + i] = i < kAggSlots / 2 ? ~0ULL : 0ULL;
+    }
+    __device__ void flush(const unsigned long long* t) const {
+        const std::uint32_t* keys = reinterpret_cast<const std::uint32_t*>(t + a.k);
+        const unsigned long long* sums = t + a.k + kAggSlots / 2;
+        for (int i = threadIdx.x; i < kAggSlots; i += blockDim.x)
+            if (keys[i] != 0xffffffffu && sums[i]) atomicAdd(&a.counts[keys[i]], sums[i]);
     }
     template <class Team>
     __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long* hist) const {
@@ -711,7 +745,8 @@ struct RepairProposeBody : RowBody {
                 for (int p = t.rank(); p < a.k; p += Team::size) consider(p, hist[p]);
             best = team_reduce(t, best);
         }
-        if (t.leader()) repair_finish(a, v, s, w, current, best, load0);
+        unsigned long long* agg = t.part ? const_cast<unsigned long long*>(t.part) + a.k : nullptr;
+        if (t.leader()) repair_finish(a, v, s, w, current, best, load0, agg);
         return RowOut{false, 0};
     }
 };

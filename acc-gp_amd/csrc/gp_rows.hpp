@@ -161,6 +161,25 @@ __device__ inline const unsigned long long* stage_parts(const Body& body, unsign
     return nullptr;
 }
 
+// Bodies may also keep block-level sums in their staged table (updated with
+// LDS atomics while rows run) and add them to global memory once per block:
+// flush(part) runs after the block's last row.
+template <class B, class = void>
+struct HasFlush : std::false_type {};
+template <class B>
+struct HasFlush<B, std::void_t<decltype(std::declval<const B&>().flush(std::declval<const unsigned long long*>()))>>
+    : std::true_type {};
+
+template <class Body>
+__device__ inline void flush_parts(const Body& body, const unsigned long long* part) {
+    if constexpr (HasFlush<Body>::value) {
+        if (part) {
+            __syncthreads();
+            body.flush(part);
+        }
+    }
+}
+
 struct RowOut {
     bool append;
     std::int64_t acc;
@@ -262,6 +281,7 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_light(Body body, const st
             atomicAdd(t.entries + 1, (unsigned long long)rows_done);
         }
     }
+    flush_parts(body, part);
 }
 
 // Medium tier: teams of W (32 or 64) lanes, G = 64 / W rows per wave step.
@@ -310,6 +330,7 @@ __global__ void __launch_bounds__(kTierThreads) k_tier_medium(Body body, TierLis
             atomicAdd(t.entries + 1, (unsigned long long)rows_done);
         }
     }
+    flush_parts(body, part);
 }
 
 // More blocks than stay resident: with about one row per block the
@@ -345,6 +366,7 @@ __global__ void __launch_bounds__(kHeavyThreads) k_tier_heavy(Body body, TierLis
         atomicAdd(t.entries, (unsigned long long)rows_entries);
         atomicAdd(t.entries + 1, (unsigned long long)rows_done);
     }
+    flush_parts(body, part);
 }
 
 // Light launches stop at 1024 blocks (four per CU, all resident): each block

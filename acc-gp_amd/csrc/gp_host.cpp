@@ -113,14 +113,21 @@ HostGraph build_csr(std::int64_t n, std::vector<std::vector<std::uint64_t>>& par
 // ------------------------------------------------------------ partitioner
 // In-repo initial partitioner of the coarsest graph (the role METIS 5 has in
 // the reference, src/initial_partition_metis.cu:86-183; it is not METIS):
-// recursive bisection.  Each bisection grows kGrowths breadth-first regions
-// from hashed start vertices, refines the kRefined best of them (cut, then
+// recursive bisection.  Each bisection grows breadth-first regions from
+// hashed start vertices (kGrowths, or kWideGrowths on sparse coarse graphs),
+// refines the best of them (kRefined, or kWideRefined; ranked by cut, then
 // deviation from the target weight) with Fiduccia-Mattheyses passes and
-// keeps the better.  The whole recursion runs from kRecursions seeds on
+// keeps the best.  The whole recursion runs from kRecursions seeds on
 // threads of their own and the labels of least capacity excess, then least
 // cut, are kept.  Deterministic for a given seed and independent of timing.
+// The wider search costs time in proportion to the edges; coarse graphs of
+// at most kWideEntries row entries (meshes and other sparse graphs) take it,
+// dense coarse cores (power-law graphs) keep the narrow one.
 constexpr int kGrowths = 8;
 constexpr int kRefined = 2;
+constexpr int kWideGrowths = 16;
+constexpr int kWideRefined = 4;
+constexpr std::int64_t kWideEntries = 1 << 14;
 constexpr int kRecursions = 8;
 constexpr int kPasses = 8;
 
@@ -133,6 +140,8 @@ struct Bisector {
     std::uint32_t seed;
     std::int32_t* labels;
     std::vector<std::int32_t> pos;  // subset slot of each vertex, -1 outside
+    int growths_per_bisection;
+    int refined_per_bisection;
 
     std::uint64_t w(std::int32_t v) const { return vw ? vw[v] : 1u; }
     std::int64_t e(std::int64_t x) const { return ew ? static_cast<std::int64_t>(ew[x]) : 1; }
@@ -320,7 +329,7 @@ struct Bisector {
         std::vector<std::tuple<std::int64_t, std::uint64_t, int, std::uint64_t>> rank;  // cut, dev, growth, weight
         std::vector<std::uint8_t> visited;
         std::vector<std::int32_t> queue;
-        const int growths = std::min<std::int32_t>(kGrowths, m);
+        const int growths = std::min<std::int32_t>(growths_per_bisection, m);
         for (int t = 0; t < growths; ++t) {
             const std::uint32_t h = hmix32(seed ^ ((std::uint32_t)p0 * 0x9e3779b9U) ^ ((std::uint32_t)kk * 0x85ebca6bU) ^
                                            ((std::uint32_t)t * 0xc2b2ae35U));
@@ -362,7 +371,7 @@ struct Bisector {
         std::int64_t best_cut = std::numeric_limits<std::int64_t>::max();
         std::uint64_t best_dev = std::numeric_limits<std::uint64_t>::max();
         std::vector<std::uint8_t> best_side;
-        for (std::size_t r = 0; r < rank.size() && r < static_cast<std::size_t>(kRefined); ++r) {
+        for (std::size_t r = 0; r < rank.size() && r < static_cast<std::size_t>(refined_per_bisection); ++r) {
             std::vector<std::uint8_t>& side = grown[static_cast<std::size_t>(std::get<2>(rank[r]))];
             std::uint64_t wa = std::get<3>(rank[r]);
             const std::int64_t cut = refine(S, side, wa, target, tol, std::get<0>(rank[r]));
@@ -394,13 +403,15 @@ void initial_partition_host(std::int64_t n, const std::int32_t* off, const std::
     for (std::int64_t v = 0; v < n; ++v) total += vw ? vw[v] : 1u;
     const long double raw = (long double)total * (imbalance >= 1.0 ? imbalance : 1.0) / parts;
     const std::uint64_t cap = (std::uint64_t)std::floor(raw);
+    const bool wide = off[n] <= kWideEntries;
     std::vector<std::vector<std::int32_t>> trial(static_cast<std::size_t>(kRecursions));
     std::vector<std::pair<std::uint64_t, std::uint64_t>> score(static_cast<std::size_t>(kRecursions));
     auto run = [&](int t) {
         const std::uint32_t s = t == 0 ? seed : hmix32(seed ^ (0x68e31da4U * (std::uint32_t)t));
         std::vector<std::int32_t>& lab = trial[static_cast<std::size_t>(t)];
         lab.assign(static_cast<std::size_t>(n), 0);
-        Bisector b{n, off, nbr, ew, vw, s, lab.data(), std::vector<std::int32_t>(static_cast<std::size_t>(n), -1)};
+        Bisector b{n,   off, nbr, ew, vw, s, lab.data(), std::vector<std::int32_t>(static_cast<std::size_t>(n), -1),
+                   wide ? kWideGrowths : kGrowths, wide ? kWideRefined : kRefined};
         std::vector<std::int32_t> all(static_cast<std::size_t>(n));
         for (std::int64_t v = 0; v < n; ++v) all[static_cast<std::size_t>(v)] = static_cast<std::int32_t>(v);
         b.bisect(all, 0, parts);

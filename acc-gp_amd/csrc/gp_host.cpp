@@ -120,14 +120,18 @@ HostGraph build_csr(std::int64_t n, std::vector<std::vector<std::uint64_t>>& par
 // keeps the best.  The whole recursion runs from kRecursions seeds on
 // threads of their own and the labels of least capacity excess, then least
 // cut, are kept.  Deterministic for a given seed and independent of timing.
-// The wider search costs time in proportion to the edges; coarse graphs of
-// at most kWideEntries row entries (meshes and other sparse graphs) take it,
-// dense coarse cores (power-law graphs) keep the narrow one.
+// The wider search pays on sparse coarse graphs (meshes: a bisection's cut
+// depends on where its regions start) and is cheap there; coarse graphs of
+// at most kWideEntries row entries and average degree at most kWideDegree
+// take it.  Dense coarse cores (power-law graphs) keep the narrow search:
+// on R-MAT scale 22 the wide one moved the cut by 0.01 % and, through a
+// different start, added two refinement rounds at the finest level.
 constexpr int kGrowths = 8;
 constexpr int kRefined = 2;
 constexpr int kWideGrowths = 16;
 constexpr int kWideRefined = 4;
 constexpr std::int64_t kWideEntries = 1 << 14;
+constexpr std::int64_t kWideDegree = 16;
 constexpr int kRecursions = 8;
 constexpr int kPasses = 8;
 
@@ -403,7 +407,7 @@ void initial_partition_host(std::int64_t n, const std::int32_t* off, const std::
     for (std::int64_t v = 0; v < n; ++v) total += vw ? vw[v] : 1u;
     const long double raw = (long double)total * (imbalance >= 1.0 ? imbalance : 1.0) / parts;
     const std::uint64_t cap = (std::uint64_t)std::floor(raw);
-    const bool wide = off[n] <= kWideEntries;
+    const bool wide = off[n] <= kWideEntries && off[n] <= kWideDegree * n;
     std::vector<std::vector<std::int32_t>> trial(static_cast<std::size_t>(kRecursions));
     std::vector<std::pair<std::uint64_t, std::uint64_t>> score(static_cast<std::size_t>(kRecursions));
     auto run = [&](int t) {

@@ -299,6 +299,8 @@ int gp_graph_apply_delta(gp_context* ctx, gp_graph* g, int64_t count, const int3
 
 void gp_graph_destroy(gp_graph* g) { delete g; }
 
+int gp_debug_below_0999(uint64_t cut, uint64_t best) { return gp::debug_below_0999(cut, best) ? 1 : 0; }
+
 int gp_capacity(uint64_t total, int parts, double imbalance, uint64_t* out) {
     if (!out) return fail(GP_ERR_INVALID, "null output");
     return guarded([&] { *out = gp::capacity_of(total, parts, imbalance); });

@@ -520,8 +520,12 @@ __global__ void k_set_lane(std::uint32_t* state, std::int64_t n, int lane, std::
 // Frontier list from member flags (order irrelevant: per-vertex decisions).
 // Each wave reads kScanRounds x 64 consecutive flags before appending them.
 constexpr int kScanRounds = 8;
+// gate: null, or a device word; when it is zero the launch does nothing (a
+// round of a device-controlled loop that has ended).
 template <typename F>
-__global__ void k_compact_flags(std::int64_t n, F flags, std::int32_t* out, unsigned long long* count) {
+__global__ void k_compact_flags(std::int64_t n, F flags, std::int32_t* out, unsigned long long* count,
+                                const unsigned long long* gate) {
+    if (gate && !*gate) return;
     __shared__ std::int32_t sb[kTierThreads / kWave][kStage];
     Stage st{sb[threadIdx.x / kWave], out, count, 0};
     const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
@@ -1303,6 +1307,195 @@ __global__ void k_publish_prices(RepairSample* samples, double* prices, int k) {
         const long long loss = (long long)x.loss;
         prices[p] = loss > 0 ? (double)loss / (double)x.weight : 0.0;
     }
+}
+
+// ------------------------------------------------- device LP loop control
+// The LP round loop of one level decided on the device (the rules of the
+// host loop, incremental.cuh:911-961, which lp_graph.cuh:57-92 records as a
+// GPU-controlled loop), so the host can issue rounds ahead instead of reading
+// counters back after each.  Its words sit behind the part loads in the
+// counter block; the best loads (k words) follow them.
+enum LoopWord {
+    kRun = 0,      // 1 while rounds run; a round issued after the stop does nothing
+    kPhase,        // rounds run
+    kStale,
+    kEmpty,
+    kCut,
+    kBestCut,
+    kLpMoves,
+    kReason,       // LoopReason of the stop
+    kImproved,     // the last round set a new best cut: checkpoint its labels
+    kAccFrontier,  // profiled runs: rows and entries of the three sweeps, summed
+    kAccRowSum,
+    kAccAfterE,
+    kAccAfterR,
+    kAccUpdateE,
+    kAccUpdateR,
+    kLoopWords
+};
+enum LoopReason { kRunning = 0, kEnded = 1, kNeedRepair = 2, kCutUnderflow = 3, kDeferred = 4 };
+// Words of the loop state mirrored into the host mailbox after each round.
+constexpr int kMailWords = kImproved + 1;
+
+struct LoopParams {
+    int k;
+    int rounds;
+    int stale_rounds;
+    int graph_mode;  // stale rule in integers (graph loop) or long double
+    int light_only;  // no medium launch: a deferred row is an error
+    int profiling;
+    std::uint64_t cap;
+    std::uint64_t m0999;  // 64-bit significand of the long double 0.999, scaled by 2^64
+};
+
+// (long double)cut < 0.999L * (long double)best exactly as x87 extended
+// precision evaluates it (64-bit significands, round to nearest even), in
+// integers: 0.999L = m0999 / 2^64, so the product is best * m0999 / 2^64
+// rounded to 64 significant bits.
+__host__ __device__ inline bool below_0999(std::uint64_t cut, std::uint64_t best, std::uint64_t m0999) {
+    using u128 = unsigned __int128;
+    u128 p = (u128)best * m0999;
+    const std::uint64_t hi = (std::uint64_t)(p >> 64);
+    const int len = hi ? 128 - __builtin_clzll(hi) : 0;
+    if (len > 64) {
+        const int s = len - 64;
+        const u128 q = p >> s;
+        const u128 rem = p - (q << s);
+        const u128 half = (u128)1 << (s - 1);
+        const u128 r = (rem > half || (rem == half && (q & 1))) ? q + 1 : q;
+        p = r << s;
+    }
+    return ((u128)cut << 64) < p;
+}
+
+// 0.999L as m / 2^64 (0.999L lies in [1/2, 1), so m has 64 bits).
+inline std::uint64_t significand_0999() { return static_cast<std::uint64_t>(std::ldexp(0.999L, 64)); }
+
+// Loop state at the start of a run of device-controlled rounds; clears the
+// counter block; copy_best: the best loads take the current loads.
+__global__ void k_lp_arm(unsigned long long* ctr, int k, unsigned long long run, unsigned long long phase,
+                         unsigned long long stale, unsigned long long empty, unsigned long long cut,
+                         unsigned long long best, unsigned long long lp_moves, int copy_best) {
+    unsigned long long* loads = ctr + kCtrCount;
+    unsigned long long* L = loads + k;
+    for (int i = threadIdx.x; i < kCtrCount; i += blockDim.x) ctr[i] = 0;
+    if (copy_best)
+        for (int p = threadIdx.x; p < k; p += blockDim.x) L[kLoopWords + p] = loads[p];
+    if (threadIdx.x == 0) {
+        L[kRun] = run;
+        L[kPhase] = phase;
+        L[kStale] = stale;
+        L[kEmpty] = empty;
+        L[kCut] = cut;
+        L[kBestCut] = best;
+        L[kLpMoves] = lp_moves;
+        L[kReason] = kRunning;
+        L[kImproved] = 0;
+        for (int i = kAccFrontier; i < kLoopWords; ++i) L[i] = 0;
+    }
+}
+
+// End of one LP round: the movers, cut delta and part loads of the round
+// decide whether the loop goes on; the counter block is cleared for the next
+// round and the state is mirrored to the host mailbox.
+__global__ void __launch_bounds__(256) k_lp_decide(unsigned long long* ctr, LoopParams P,
+                                                   unsigned long long* mailbox) {
+    unsigned long long* loads = ctr + kCtrCount;
+    unsigned long long* L = loads + P.k;
+    __shared__ int s_run, s_improved, s_stop;
+    if (threadIdx.x == 0) {
+        s_run = L[kRun] != 0;
+        s_improved = 0;
+        s_stop = 0;
+        L[kImproved] = 0;
+        if (s_run) {
+            const std::uint64_t phase = L[kPhase] + 1;
+            std::uint64_t stale = L[kStale], empty = L[kEmpty], cut = L[kCut], best = L[kBestCut];
+            const std::uint64_t movers = ctr[kMovers];
+            const std::int64_t delta = (std::int64_t)ctr[kDelta];
+            int reason = kRunning;
+            if (P.profiling) {
+                L[kAccFrontier] += ctr[kFrontier];
+                L[kAccRowSum] += ctr[kRowSum];
+                L[kAccAfterE] += ctr[kEntAfter];
+                L[kAccAfterR] += ctr[kEntAfter + 1];
+                L[kAccUpdateE] += ctr[kEntUpdate];
+                L[kAccUpdateR] += ctr[kEntUpdate + 1];
+            }
+            if (P.light_only && (ctr[kTier] || ctr[kTierB] || ctr[kTierC])) {
+                reason = kDeferred;
+            } else if (!movers) {
+                if (++empty >= 2)
+                    reason = kEnded;
+                else
+                    ++stale;
+            } else {
+                empty = 0;
+                if (delta >= 0 && (std::uint64_t)delta > cut) {
+                    reason = kCutUnderflow;
+                } else {
+                    cut = delta >= 0 ? cut - (std::uint64_t)delta : cut + (std::uint64_t)(-delta);
+                    L[kLpMoves] += movers;
+                    std::uint64_t largest = 0;
+                    for (int p = 0; p < P.k; ++p) largest = loads[p] > largest ? loads[p] : largest;
+                    if (largest > P.cap) {
+                        reason = kNeedRepair;  // the host repairs and finishes the round
+                    } else {
+                        ++stale;
+                        if (cut < best) {
+                            if (P.graph_mode ? best - cut > best / 1000 : below_0999(cut, best, P.m0999)) stale = 0;
+                            best = cut;
+                            s_improved = 1;
+                            L[kImproved] = 1;
+                        }
+                    }
+                }
+            }
+            if (reason == kRunning && !(phase < (std::uint64_t)P.rounds && stale < (std::uint64_t)P.stale_rounds))
+                reason = kEnded;
+            L[kPhase] = phase;
+            L[kStale] = stale;
+            L[kEmpty] = empty;
+            L[kCut] = cut;
+            L[kBestCut] = best;
+            L[kReason] = reason;
+            if (reason != kRunning) {
+                L[kRun] = 0;
+                s_stop = 1;
+            }
+            // Running: the first kMailWords loop words to the mailbox;
+            // stopped: every loop word, then (below) the loads and best loads.
+            if (mailbox)
+                for (int i = 0; i < (s_stop ? (int)kLoopWords : kMailWords); ++i) mailbox[i] = L[i];
+        }
+    }
+    __syncthreads();
+    if (!s_run) return;
+    for (int p = threadIdx.x; p < P.k; p += blockDim.x) {
+        const unsigned long long load = loads[p];
+        const unsigned long long best = s_improved ? load : L[kLoopWords + p];
+        if (s_improved) L[kLoopWords + p] = load;
+        if (mailbox && s_stop) {
+            mailbox[kLoopWords + p] = load;
+            mailbox[kLoopWords + P.k + p] = best;
+        }
+    }
+    for (int i = threadIdx.x; i < kCtrCount; i += blockDim.x) ctr[i] = 0;
+}
+
+// Checkpoint of the labels of a round that set a new best cut.
+// vec: both arrays are 16-byte aligned (copied in 16-byte words).
+__global__ void k_lp_keep(const unsigned long long* flag, const Label* labels, Label* best, std::int64_t n, bool vec) {
+    if (!*flag) return;
+    const std::int64_t words = vec ? n / 16 : 0;
+    const uint4* src = reinterpret_cast<const uint4*>(labels);
+    uint4* dst = reinterpret_cast<uint4*>(best);
+    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < words;
+         i += (std::int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+    for (std::int64_t v = words * 16 + blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x)
+        best[v] = labels[v];
 }
 
 // Donors that were not admitted carry no target (rebalance_graph.cuh:155).
@@ -2250,7 +2443,8 @@ public:
         medium_ = c.buf<std::int32_t>(n);
         heavy_ = c.buf<std::int32_t>(n);
         // Counters and part loads share one allocation: one read returns both.
-        ctr_ = c.buf<unsigned long long>(static_cast<std::size_t>(kCtrCount + k));
+        // Counters, part loads, then the device loop words and best loads.
+        ctr_ = c.buf<unsigned long long>(static_cast<std::size_t>(kCtrCount + k + kLoopWords + k));
         loads0_ = c.buf<unsigned long long>(static_cast<std::size_t>(k));
         bucket_ = c.buf<std::int32_t>(n);
         counts_ = c.buf<unsigned long long>(static_cast<std::size_t>(k) * kBalanceWidth);
@@ -2332,11 +2526,36 @@ public:
         ctr_clear_ = false;
         hipLaunchKernelGGL(k_compact_flags<StateLane<std::uint8_t>>, dim3(grid_for(n, 256 * kScanRounds, 2048)), dim3(256), 0, s, n,
                            a_.member,
-                           frontier_.get(), ctr_.get() + kFrontier);
+                           frontier_.get(), ctr_.get() + kFrontier, nullptr);
         GP_LAUNCH_CHECK();
         // Algorithmic bytes per scanned row: offsets 8, label and weight 8,
         // target and gain 12, and per entry neighbour 4, neighbour label 4
         // and edge weight 4 when weighted; added once the row sums are read.
+        propose_sweeps();
+        return apply(true, false, delta, true);
+    }
+
+    // The launches of one LP round of a device-controlled loop: the frontier
+    // compaction does nothing once the loop word kRun is zero, so every later
+    // launch of the round finds empty lists; k_lp_decide ends the round.
+    void round_issue(const LoopParams& P, Label* best_labels, bool vec) {
+        const hipStream_t s = c_.stream();
+        const std::int64_t n = a_.n;
+        unsigned long long* L = ctr_.get() + kCtrCount + a_.k;
+        hipLaunchKernelGGL(k_compact_flags<StateLane<std::uint8_t>>, dim3(grid_for(n, 256 * kScanRounds, 2048)),
+                           dim3(256), 0, s, n, a_.member, frontier_.get(), ctr_.get() + kFrontier, L + kRun);
+        GP_LAUNCH_CHECK();
+        propose_sweeps();
+        apply_launch(true, false, true);
+        hipLaunchKernelGGL(k_lp_decide, dim3(1), dim3(256), 0, s, ctr_.get(), P, c_.mailbox());
+        GP_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_lp_keep, dim3(grid_for(ceil_div(n, 16), 256, 1024)), dim3(256), 0, s, L + kImproved,
+                           a_.labels, best_labels, n, vec);
+        GP_LAUNCH_CHECK();
+    }
+
+    // The LP proposal and afterburner sweeps of a round (counter block clear).
+    void propose_sweeps() {
         c_.timed("lp_propose", 0.0, [&] {
             if (narrow_bins())
                 tiers(LpProposeBody<unsigned>{{a_}}, frontier_.get(), 0, ctr_.get() + kFrontier, cand_.get(),
@@ -2349,13 +2568,108 @@ public:
             tiers(AfterBody{{a_}}, cand_.get(), 0, ctr_.get() + kCand, movers_.get(), ctr_.get() + kMovers, nullptr, 0,
                   kTierB, true, kEntAfter);
         });
-        return apply(true, false, delta, true);
+    }
+
+    // The LP round loop of the level with its control on the device
+    // (k_lp_decide applies the host loop's rules after each round).  The
+    // host issues each round before it has seen the previous one end
+    // (ahead) or right after, watching only the mailbox; it takes over when
+    // a round leaves a part over capacity, for the repair and the rest of
+    // that round, and then re-arms the loop.  Same rounds, moves and
+    // checkpoints as the host loop.
+    void lp_loop(int rounds, bool graph_mode, std::uint64_t& cut, std::uint64_t& best_cut, Label* best_labels,
+                 std::vector<std::uint64_t>& best_loads) {
+        const hipStream_t s = c_.stream();
+        const int k = a_.k;
+        const bool vec = reinterpret_cast<std::uintptr_t>(a_.labels) % 16 == 0 &&
+                         reinterpret_cast<std::uintptr_t>(best_labels) % 16 == 0;
+        const LoopParams P{k,
+                           rounds,
+                           o_.stale_rounds,
+                           graph_mode ? 1 : 0,
+                           g_.row_len_bound <= static_cast<std::uint64_t>(kTierLight) ? 1 : 0,
+                           c_.profiling() ? 1 : 0,
+                           a_.cap,
+                           significand_0999()};
+        const int ahead = c_.loop_ahead();
+        volatile unsigned long long* mail = c_.mailbox();
+        std::uint64_t phase = 0, stale = 0, empty = 0, lp_moves = 0;
+        bool copy_best = true;
+        while (phase < static_cast<std::uint64_t>(rounds) && stale < static_cast<std::uint64_t>(o_.stale_rounds)) {
+            reset_samples();
+            const std::uint64_t phase0 = phase;
+            hipLaunchKernelGGL(k_lp_arm, dim3(1), dim3(256), 0, s, ctr_.get(), k, 1ULL, phase, stale, empty, cut,
+                               best_cut, lp_moves, copy_best ? 1 : 0);
+            GP_LAUNCH_CHECK();
+            mail[kRun] = 1;  // the stream is idle: no round of an earlier run is in flight
+            std::vector<std::size_t> marks;
+            int issued = 0;
+            auto issue = [&] {
+                marks.push_back(c_.pending_mark());
+                round_issue(P, best_labels, vec);
+                c_.mark(issued++);
+            };
+            issue();
+            for (;;) {
+                if (ahead) issue();
+                c_.wait_mark(issued - 1 - ahead);
+                if (!mail[kRun]) break;
+                if (!ahead) issue();
+            }
+            // Issuing ahead, the stop may come from the round still in
+            // flight: wait for the stream before reading the full state.
+            if (ahead) c_.sync();
+            std::vector<unsigned long long> h(static_cast<std::size_t>(kLoopWords + 2 * k));
+            for (std::size_t i = 0; i < h.size(); ++i) h[i] = mail[i];
+            ctr_clear_ = true;
+            phase = h[kPhase];
+            stale = h[kStale];
+            empty = h[kEmpty];
+            cut = h[kCut];
+            best_cut = h[kBestCut];
+            lp_moves = h[kLpMoves];
+            for (int p = 0; p < k; ++p) {
+                loads_host_[static_cast<std::size_t>(p)] = h[static_cast<std::size_t>(kLoopWords + p)];
+                best_loads[static_cast<std::size_t>(p)] = h[static_cast<std::size_t>(kLoopWords + k + p)];
+            }
+            // Rounds issued after the stop did nothing: drop their intervals.
+            const std::size_t ran = static_cast<std::size_t>(phase - phase0);
+            if (ran < marks.size()) c_.drop_pending(marks[ran]);
+            if (c_.profiling()) {
+                const double w = a_.ew ? 4.0 : 0.0;
+                c_.add_bytes("lp_propose", 28.0 * (double)h[kAccFrontier] + (8.0 + w) * (double)h[kAccRowSum]);
+                c_.add_bytes("after", 16.0 * (double)h[kAccAfterR] + (20.0 + w) * (double)h[kAccAfterE]);
+                c_.add_bytes("update", 24.0 * (double)h[kAccUpdateR] + (16.0 + w) * (double)h[kAccUpdateE]);
+            }
+            const int reason = static_cast<int>(h[kReason]);
+            if (reason == kCutUnderflow) throw std::runtime_error("incremental cut underflow");
+            if (reason == kDeferred)
+                throw std::logic_error("rows longer than the level's row-length bound were deferred and not processed");
+            if (reason != kNeedRepair) break;
+            // The rest of the round as the host loop runs it.
+            if (!rebalance(cut)) break;
+            publish_prices();
+            ++stale;
+            copy_best = false;
+            if (cut < best_cut) {
+                if (graph_mode) {
+                    if (best_cut - cut > best_cut / 1000) stale = 0;
+                } else if ((long double)cut < 0.999L * (long double)best_cut)
+                    stale = 0;
+                best_cut = cut;
+                c_.copy_d2d(best_labels, a_.labels, static_cast<std::size_t>(a_.n) * sizeof(Label));
+                best_loads = loads_host_;
+                copy_best = true;
+            }
+        }
+        st_.rounds = static_cast<int>(phase);
+        st_.lp_moves += lp_moves;
     }
 
     // Update + commit of the current mover list.  zeroed: the delta and
     // third-sweep tier counters are still zero from the counter block's
     // clear at the start of the round or repair iteration.
-    std::int64_t apply(bool lp, bool samples, std::int64_t& delta, bool zeroed = false) {
+    void apply_launch(bool lp, bool samples, bool zeroed) {
         const hipStream_t s = c_.stream();
         if (!zeroed) fill<unsigned long long>(c_, ctr_.get() + kDelta, 1, 0ULL);
         if (!zeroed && c_.profiling()) fill<unsigned long long>(c_, ctr_.get() + kEntUpdate, 2, 0ULL);
@@ -2367,6 +2681,10 @@ public:
             GP_LAUNCH_CHECK();
         });
         if (samples) samples_clear_ = false;
+    }
+
+    std::int64_t apply(bool lp, bool samples, std::int64_t& delta, bool zeroed = false) {
+        apply_launch(lp, samples, zeroed);
         std::vector<unsigned long long> h(static_cast<std::size_t>(kCtrCount + a_.k));
         // The read also clears the counters (not the part loads behind them).
         c_.read_clear(h.data(), ctr_.get(), h.size(), kCtrCount);
@@ -2668,7 +2986,7 @@ public:
             fill<unsigned long long>(c_, ctr_.get() + kMovers, 1, 0ULL);
             hipLaunchKernelGGL(k_compact_flags<StateLane<std::uint8_t>>, dim3(grid_for(n, 256 * kScanRounds, 2048)), dim3(256), 0, s,
                                n, a_.flags,
-                               movers_.get(), ctr_.get() + kMovers);
+                               movers_.get(), ctr_.get() + kMovers, nullptr);
             GP_LAUNCH_CHECK();
             const unsigned long long cnt = c_.read_scalar(ctr_.get() + kMovers);
             if (!cnt) {
@@ -2813,7 +3131,11 @@ LevelStats refine_level_impl(Context& c, const DGraph& g, Label* labels, std::ve
     c.copy_d2d(best_labels.get(), labels, static_cast<std::size_t>(g.n) * sizeof(Label));
     const bool graph_mode = g.n >= o.graph_loop_threshold;
     int stale = 0, empty = 0;
-    for (int phase = 0; phase < rounds && stale < o.stale_rounds; ++phase) {
+    // Strict mode recounts on the host after every batch: host loop.
+    const bool device_loop = c.device_loop() && !o.strict_verify;
+    if (device_loop) R.lp_loop(rounds, graph_mode, cut, best_cut, best_labels.get(), best_loads);
+    const int host_rounds = device_loop ? 0 : rounds;
+    for (int phase = 0; phase < host_rounds && stale < o.stale_rounds; ++phase) {
         st.rounds = phase + 1;
         R.reset_samples();
         std::int64_t delta = 0;
@@ -2861,6 +3183,8 @@ LevelStats refine_level_impl(Context& c, const DGraph& g, Label* labels, std::ve
 }
 
 }  // namespace
+
+bool debug_below_0999(std::uint64_t cut, std::uint64_t best) { return below_0999(cut, best, significand_0999()); }
 
 LevelStats refine_level(Context& c, const DGraph& g, std::int32_t* labels, std::vector<std::uint64_t>& loads,
                         const RefineOptions& o, std::uint64_t cap, int rounds, bool has_known_cut,

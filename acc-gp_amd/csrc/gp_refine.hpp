@@ -28,4 +28,8 @@ LevelStats refine_level(Context& c, const DGraph& g, std::int32_t* labels, std::
 // Coarsest refinement, then project and refine level by level.
 UncoarsenResult uncoarsen(Context& c, const Hierarchy& h, const std::int32_t* coarsest_labels, const RefineOptions& o);
 
+// cut < 0.999 * best as x87 long double evaluates it, in integers (the
+// device loop control's form of the stale rule of small levels).
+bool debug_below_0999(std::uint64_t cut, std::uint64_t best);
+
 }  // namespace gp

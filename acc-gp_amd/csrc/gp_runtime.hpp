@@ -189,15 +189,23 @@ public:
         // Created here, not on first wait, so that a wait (and every read-back
         // through the pinned buffer) creates no runtime objects.
         GP_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+        GP_HIP(hipHostMalloc(&mailbox_, kMailboxBytes, hipHostMallocCoherent));
+        std::memset(mailbox_, 0, kMailboxBytes);
+        for (auto& e : marks_) GP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         if (const char* e = std::getenv("GP_LIGHT_LANES")) set_light_lanes(std::atoi(e));
         if (const char* e = std::getenv("GP_MEDIUM_LANES")) set_medium_lanes(std::atoi(e));
         if (const char* e = std::getenv("GP_TIER_OVERLAP")) overlap_ = std::atoi(e) != 0;
         if (const char* e = std::getenv("GP_SPIN_SYNC")) spin_ = std::atoi(e) != 0;
+        if (const char* e = std::getenv("GP_DEVICE_LOOP")) device_loop_ = std::atoi(e) != 0;
+        if (const char* e = std::getenv("GP_LOOP_AHEAD")) loop_ahead_ = std::atoi(e) != 0 ? 1 : 0;
     }
     ~Context() {
         delete arena_;
         if (pinned_) (void)hipHostFree(pinned_);
         if (done_) (void)hipEventDestroy(done_);
+        if (mailbox_) (void)hipHostFree(mailbox_);
+        for (auto e : marks_)
+            if (e) (void)hipEventDestroy(e);
         if (fork_) (void)hipEventDestroy(fork_);
         if (join_) (void)hipEventDestroy(join_);
         if (aux_) (void)hipStreamDestroy(aux_);
@@ -244,6 +252,11 @@ public:
         GP_HIP(hipEventRecord(join_, aux_));
         GP_HIP(hipStreamWaitEvent(stream_, join_, 0));
     }
+    // Refinement loops decided on the device (GP_DEVICE_LOOP=0: on the host,
+    // one counter read-back per round), and whether the host issues a round
+    // before it has seen the previous one end (GP_LOOP_AHEAD=0: after).
+    bool device_loop() const { return device_loop_; }
+    int loop_ahead() const { return loop_ahead_; }
     int device() const { return device_; }
     void activate() const { GP_HIP(hipSetDevice(device_)); }
 
@@ -266,6 +279,28 @@ public:
             const hipError_t e = hipEventQuery(done_);
             if (e == hipSuccess) return;
             if (e != hipErrorNotReady) GP_HIP(e);
+        }
+    }
+
+    // Words of coherent host memory that device-side loop control kernels
+    // write their state to, so the host can follow a loop without a copy.
+    unsigned long long* mailbox() const { return static_cast<unsigned long long*>(mailbox_); }
+    // Stream positions: mark(i) records ring event i at the stream tail;
+    // wait_mark(i) spins until the stream has passed it (counted as a host
+    // wait).  Two events, so a loop can run one step ahead of the host.
+    static constexpr int kMarks = 2;
+    void mark(int i) { GP_HIP(hipEventRecord(marks_[i % kMarks], stream_)); }
+    void wait_mark(int i) {
+        ++syncs_;
+        hipEvent_t e = marks_[i % kMarks];
+        if (!spin_) {
+            GP_HIP(hipEventSynchronize(e));
+            return;
+        }
+        for (;;) {
+            const hipError_t r = hipEventQuery(e);
+            if (r == hipSuccess) return;
+            if (r != hipErrorNotReady) GP_HIP(r);
         }
     }
 
@@ -351,6 +386,17 @@ public:
         GP_HIP(hipEventRecord(b, stream_));
         pending_.push_back(Pending{name, bytes, a, b});
     }
+    // Position in the list of timed intervals not yet resolved, and removal
+    // of the intervals recorded after such a position (work that a
+    // device-side loop skipped: its launches returned at once).
+    std::size_t pending_mark() const { return pending_.size(); }
+    void drop_pending(std::size_t from) {
+        while (pending_.size() > from) {
+            (void)hipEventDestroy(pending_.back().a);
+            (void)hipEventDestroy(pending_.back().b);
+            pending_.pop_back();
+        }
+    }
     // Algorithmic bytes known only after the work ran (device counters).
     void add_bytes(const char* name, double bytes) {
         if (profiling_) profile_[name].bytes += bytes;
@@ -383,6 +429,7 @@ private:
         hipEvent_t a, b;
     };
     static constexpr std::size_t kPinnedBytes = 1 << 16;
+    static constexpr std::size_t kMailboxBytes = 4096;
     bool profiling_ = false;
     std::vector<Pending> pending_;
     std::map<std::string, ProfileEntry> profile_;
@@ -392,8 +439,12 @@ private:
     hipEvent_t fork_ = nullptr, join_ = nullptr, done_ = nullptr;
     bool overlap_ = true;
     bool spin_ = true;
+    bool device_loop_ = true;
+    int loop_ahead_ = 0;
     Arena* arena_ = nullptr;
     void* pinned_ = nullptr;
+    void* mailbox_ = nullptr;
+    hipEvent_t marks_[kMarks] = {nullptr, nullptr};
     int light_lanes_ = 4;
     int medium_lanes_ = 64;
     std::uint64_t syncs_ = 0;

@@ -82,7 +82,7 @@ EXPORTS = [
     "gp_generate_rgg2d", "gp_generate_knn3d", "gp_free_host", "gp_load_csr", "gp_save_csr",
     "gp_profile_enable", "gp_profile_query", "gp_profile_reset", "gp_server_create", "gp_server_submit",
     "gp_server_wait", "gp_server_destroy", "gp_partition_batch", "gp_server_graph_add", "gp_server_graph_download",
-    "gp_server_graph_release", "gp_graph_apply_delta",
+    "gp_server_graph_release", "gp_graph_apply_delta", "gp_debug_below_0999",
 ]
 
 
@@ -166,6 +166,7 @@ def lib():
         L.gp_server_graph_download.argtypes = [vp, u64, vp, vp, vp, vp, vp, vp]
         L.gp_server_graph_release.argtypes = [vp, u64]
         L.gp_graph_apply_delta.argtypes = [vp, vp, i64, vp, vp, vp, vp, vp]
+        L.gp_debug_below_0999.argtypes = [u64, u64]
         _LIB = L
     return _LIB
 
@@ -177,6 +178,12 @@ def _p(a):
 def check(rc):
     if rc != GP_OK:
         raise RuntimeError("gpart error %d: %s" % (rc, lib().gp_last_error().decode()))
+
+
+def below_0999(cut, best):
+    """The device loop control's integer form of the small-level stale rule
+    cut < 0.999 * best (long double)."""
+    return bool(lib().gp_debug_below_0999(cut, best))
 
 
 def capacity(total, parts, imbalance=1.03):

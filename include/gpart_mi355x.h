@@ -341,6 +341,11 @@ int gp_profile_enable(gp_context* ctx, int enable);
 int gp_profile_query(gp_context* ctx, const char* name, double* ms, double* bytes, int64_t* launches);
 int gp_profile_reset(gp_context* ctx);
 
+/* The stale-round rule of small levels (incremental.cuh:911-961: a new best
+ * cut resets the stale count when cut < 0.999 * best in long double) as the
+ * device-side loop control evaluates it, in integers; 1 or 0.  For tests. */
+int gp_debug_below_0999(uint64_t cut, uint64_t best);
+
 #ifdef __cplusplus
 }
 #endif

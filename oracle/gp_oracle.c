@@ -1770,6 +1770,8 @@ int64_t og_t_seed_match(int64_t n, const int32_t* off, const int32_t* nbr, const
 }
 
 uint32_t og_t_mover_threshold(int64_t clusters, int64_t n) { return og_mover_threshold(clusters, n); }
+/* The small-level stale rule of the LP loop above (incremental.cuh:911-961). */
+int og_t_below_0999(uint64_t cut, uint64_t best) { return (long double)cut < 0.999L * (long double)best; }
 
 void og_t_proposals(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew, const uint32_t* vw,
                     const int32_t* clusters, const uint64_t* cw, uint64_t cap, uint32_t seed, int level, int round,

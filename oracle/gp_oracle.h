@@ -119,6 +119,7 @@ int64_t og_t_seed_match(int64_t n, const int32_t* off, const int32_t* nbr, const
                         uint64_t cap, uint32_t salt, int32_t* roots);
 /* Mover threshold of a growth round with `clusters` clusters over n vertices. */
 uint32_t og_t_mover_threshold(int64_t clusters, int64_t n);
+int og_t_below_0999(uint64_t cut, uint64_t best);
 /* Roles and exact top-two proposals of one growth round (filter_roles = 1) or
  * the unrestricted favourites of two-hop pairing (filter_roles = 0). */
 void og_t_proposals(int64_t n, const int32_t* off, const int32_t* nbr, const uint32_t* ew, const uint32_t* vw,

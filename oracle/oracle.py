@@ -232,6 +232,8 @@ def _bind_stages():
     L.og_t_seed_match.argtypes = [i64, vp, vp, vp, vp, u64, u32, vp]
     L.og_t_mover_threshold.restype = u32
     L.og_t_mover_threshold.argtypes = [i64, i64]
+    L.og_t_below_0999.restype = ci
+    L.og_t_below_0999.argtypes = [u64, u64]
     L.og_t_proposals.argtypes = [i64, vp, vp, vp, vp, vp, vp, u64, u32, ci, ci, ci, u32, ci, vp, vp, vp, vp, vp]
     L.og_t_pack_isolated.argtypes = [i64, vp, vp, u64, vp, vp]
     L.og_t_two_hop.argtypes = [i64, vp, vp, vp, vp, vp, vp, u64, u32, ci, i64]
@@ -262,6 +264,11 @@ def seed_match(off, nbr, ew, vw, cap, salt):
     roots = np.zeros(n, np.int32)
     count = lib().og_t_seed_match(n, _p(off), _p(nbr), _p(ew), _p(vw), cap, salt, _p(roots))
     return roots, int(count)
+
+
+def below_0999(cut, best):
+    _bind_stages()
+    return bool(lib().og_t_below_0999(cut, best))
 
 
 def mover_threshold(clusters, n):

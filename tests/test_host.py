@@ -363,3 +363,30 @@ def test_bench_modes_are_wired():
         assert callable(getattr(bench, name)), name
     assert set(bench.GRID_ANCHOR) == {4, 8, 16, 32, 64, 128}
     assert [c[0] for c in bench.ROOFLINE_CLASSES][0] == "lp_propose"
+
+
+def test_device_stale_rule_equals_long_double(gp, og):
+    """The device loop control decides the small-level stale rule
+    cut < 0.999 * best (incremental.cuh:911-961, long double on the host) in
+    integers; it must agree with the x87 evaluation everywhere, including
+    where the rounded product meets an integer."""
+    rng = np.random.default_rng(11)
+    cases = []
+    for bits in range(1, 64):
+        for _ in range(40):
+            best = int(rng.integers(1 << (bits - 1), 1 << bits, dtype=np.uint64))
+            base = 999 * best // 1000
+            cases += [(max(base + d, 0), best) for d in (-2, -1, 0, 1, 2)]
+    for best in list(range(1000, 200001, 1000)) + [10 ** 12, 10 ** 15, 10 ** 18, (1 << 63) + 1000, (1 << 64) - 1]:
+        base = 999 * best // 1000
+        cases += [(max(base + d, 0), best) for d in (-1, 0, 1)]
+    cases += [(0, 0), (0, 1), (1, 1), ((1 << 64) - 1, (1 << 64) - 1), (0, (1 << 64) - 1)]
+    rational_differs = 0
+    for cut, best in cases:
+        want = og.below_0999(cut, best)
+        assert gp.below_0999(cut, best) == want, (cut, best)
+        rational_differs += want != (1000 * cut < 999 * best)
+    # The long double constant is not 999/1000: the exact rational rule
+    # decides some of these cases differently, so the integer form above is
+    # not the rational one.
+    assert rational_differs > 0

@@ -1153,10 +1153,19 @@ __global__ void __launch_bounds__(kTailThreads) k_reserve_tail(RefArgs a, Reserv
     __shared__ std::uint8_t order[kMaxParts];
     __shared__ unsigned long long wt[kWaves][kMaxParts];
     __shared__ std::int32_t sb[kWaves][kStage];
+    // Positions in r.live of the donors still choosing, ascending: donors
+    // that were admitted or found no part are dropped after each attempt
+    // (they would choose nothing again), so later attempts visit only the
+    // waiting ones.  Ranks are weight prefixes over choosing donors in
+    // list order, which dropping non-choosers leaves unchanged.
+    __shared__ std::uint16_t pos[kTailDonors];
+    __shared__ int wcount[kWaves];
     __shared__ int active;
     const int t = threadIdx.x, w = t / kWave;
     Stage st{sb[w], movers, mcount, 0};
     for (int i = t; i < kWaves * kMaxParts; i += kTailThreads) wt[i / kMaxParts][i % kMaxParts] = 0;
+    int nl = static_cast<int>(r.nlive);
+    for (int i = t; i < nl; i += kTailThreads) pos[i] = static_cast<std::uint16_t>(i);
     for (int attempt = attempt0; attempt < a.k; ++attempt) {
         for (int p = t; p < a.k; p += kTailThreads) {
             room[p] = r.requests[p] < r.slack[p] ? r.slack[p] - r.requests[p] : 0;
@@ -1170,12 +1179,12 @@ __global__ void __launch_bounds__(kTailThreads) k_reserve_tail(RefArgs a, Reserv
             order[rank] = static_cast<std::uint8_t>(p);
         }
         __syncthreads();
-        for (std::int64_t base = 0; base < r.nlive; base += kTailThreads) {
-            const std::int64_t i = base + t;
+        for (int base = 0; base < nl; base += kTailThreads) {
+            const int i = base + t;
             int key = -1;
             DonorInfo d{-1, 0, 0, 0};
-            if (i < r.nlive) {
-                d = r.live[i];
+            if (i < nl) {
+                d = r.live[pos[i]];
                 const std::int64_t e = d.e;
                 if (!r.done[e] && attempt == 0) {
                     key = a.targets[d.v];
@@ -1238,6 +1247,34 @@ __global__ void __launch_bounds__(kTailThreads) k_reserve_tail(RefArgs a, Reserv
         for (int p = t; p < a.k; p += kTailThreads) r.requests[p] += run[p];
         __syncthreads();
         if (act == 0) break;
+        // Stable compaction of the waiting donors: thread t keeps its
+        // kPer consecutive positions in registers, a block scan of the
+        // counts gives where they go.
+        constexpr int kPer = kTailDonors / kTailThreads;
+        std::uint16_t q[kPer];
+        bool waiting[kPer];
+        int keep = 0;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int i = t * kPer + j;
+            q[j] = i < nl ? pos[i] : 0;
+            waiting[j] = i < nl && !r.done[r.live[q[j]].e];
+            keep += waiting[j];
+        }
+        const int inc = wave_inclusive_scan<PlusOp>(keep);
+        if (lane_id() == kWave - 1) wcount[w] = inc;
+        __syncthreads();  // every position has been read
+        int at = inc - keep, total = 0;
+        for (int j = 0; j < kWaves; ++j) {
+            if (j < w) at += wcount[j];
+            total += wcount[j];
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+            if (waiting[j]) pos[at++] = q[j];
+        nl = total;
+        __syncthreads();
+        if (nl == 0) break;
     }
     st.flush();
 }
@@ -1403,7 +1440,16 @@ __global__ void __launch_bounds__(256) k_lp_decide(unsigned long long* ctr, Loop
     unsigned long long* loads = ctr + kCtrCount;
     unsigned long long* L = loads + P.k;
     __shared__ int s_run, s_improved, s_stop;
+    __shared__ unsigned long long s_words[kLoopWords];
+    __shared__ unsigned long long s_max[256 / kWave];
+    // Largest part load, by every thread.
+    unsigned long long mx = 0;
+    for (int p = threadIdx.x; p < P.k; p += blockDim.x) mx = loads[p] > mx ? loads[p] : mx;
+    mx = wave_max(mx);
+    if (lane_id() == 0) s_max[threadIdx.x / kWave] = mx;
+    __syncthreads();
     if (threadIdx.x == 0) {
+        for (unsigned i = 1; i < blockDim.x / kWave; ++i) mx = s_max[i] > mx ? s_max[i] : mx;
         s_run = L[kRun] != 0;
         s_improved = 0;
         s_stop = 0;
@@ -1436,9 +1482,7 @@ __global__ void __launch_bounds__(256) k_lp_decide(unsigned long long* ctr, Loop
                 } else {
                     cut = delta >= 0 ? cut - (std::uint64_t)delta : cut + (std::uint64_t)(-delta);
                     L[kLpMoves] += movers;
-                    std::uint64_t largest = 0;
-                    for (int p = 0; p < P.k; ++p) largest = loads[p] > largest ? loads[p] : largest;
-                    if (largest > P.cap) {
+                    if (mx > P.cap) {
                         reason = kNeedRepair;  // the host repairs and finishes the round
                     } else {
                         ++stale;
@@ -1463,14 +1507,15 @@ __global__ void __launch_bounds__(256) k_lp_decide(unsigned long long* ctr, Loop
                 L[kRun] = 0;
                 s_stop = 1;
             }
-            // Running: the first kMailWords loop words to the mailbox;
-            // stopped: every loop word, then (below) the loads and best loads.
-            if (mailbox)
-                for (int i = 0; i < (s_stop ? (int)kLoopWords : kMailWords); ++i) mailbox[i] = L[i];
+            for (int i = 0; i < kLoopWords; ++i) s_words[i] = L[i];
         }
     }
     __syncthreads();
     if (!s_run) return;
+    // Running: the first kMailWords loop words to the mailbox; stopped:
+    // every loop word, then the loads and the best loads.
+    if (mailbox)
+        for (int i = threadIdx.x; i < (s_stop ? (int)kLoopWords : kMailWords); i += blockDim.x) mailbox[i] = s_words[i];
     for (int p = threadIdx.x; p < P.k; p += blockDim.x) {
         const unsigned long long load = loads[p];
         const unsigned long long best = s_improved ? load : L[kLoopWords + p];

@@ -18,6 +18,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -40,9 +41,14 @@ std::uint64_t footprint_of(const gp_request* r) {
     return 3 * graph + 48 * n + 24 * m + (64ULL << 20);
 }
 
+// CSR entries per concurrency slot of a request (GP_SHARE_ENTRIES; 0: every
+// request takes one slot).
+constexpr std::int64_t kShareEntries = 0;
+
 struct Job {
     gp_request* req;
     std::uint64_t bytes;
+    int share = 1;  // concurrency slots the request takes while it runs
     std::chrono::steady_clock::time_point submitted;
     bool done = false;
 };
@@ -73,6 +79,19 @@ struct gp_server {
     std::unordered_set<std::shared_ptr<Job>*> tickets;
     std::uint64_t in_flight = 0;
     int running = 0;
+    // Concurrency slots (one per worker) and the CSR entries that make a
+    // request take one more: a large request shares the GPU with fewer
+    // others, so its latency stays near its latency alone, while small
+    // requests, bound by their launch chains, still overlap fully.
+    int slots = 1;
+    int used = 0;
+    std::int64_t share_entries = 0;
+
+    int share_of(std::int64_t m) const {
+        if (share_entries <= 0) return 1;
+        const std::int64_t k = 1 + m / share_entries;
+        return static_cast<int>(std::min<std::int64_t>(k, slots));
+    }
     bool stopping = false;
     std::string init_error;
     std::atomic<int> init_failures{0};
@@ -101,6 +120,7 @@ struct gp_server {
                 job->req->queue_seconds =
                     std::chrono::duration<double>(std::chrono::steady_clock::now() - job->submitted).count();
                 in_flight += job->bytes;
+                used += job->share;
                 ++running;
                 if (job->req->graph_id) {
                     res = graphs.at(job->req->graph_id).get();
@@ -114,6 +134,7 @@ struct gp_server {
             {
                 std::lock_guard<std::mutex> g(mu);
                 in_flight -= job->bytes;
+                used -= job->share;
                 --running;
                 job->done = true;
                 if (res) {
@@ -139,7 +160,8 @@ struct gp_server {
                 seen.insert(id);
                 if (blocked) continue;
             }
-            return running == 0 || in_flight + queue[i]->bytes <= budget ? static_cast<long>(i) : -1;
+            const bool fits = in_flight + queue[i]->bytes <= budget && used + queue[i]->share <= slots;
+            return running == 0 || fits ? static_cast<long>(i) : -1;
         }
         return -1;
     }
@@ -219,6 +241,9 @@ int gp_server_create(int device, int workers, uint64_t hbm_budget_bytes, gp_serv
     auto s = std::make_unique<gp_server>();
     s->device = device;
     s->budget = hbm_budget_bytes ? hbm_budget_bytes : (64ULL << 30);
+    s->slots = workers;
+    s->share_entries = kShareEntries;
+    if (const char* e = std::getenv("GP_SHARE_ENTRIES")) s->share_entries = std::atoll(e);
     for (int i = 0; i < workers; ++i) s->workers.emplace_back([p = s.get()] { p->worker_main(); });
     *out = s.release();
     return GP_OK;
@@ -309,9 +334,11 @@ int gp_server_submit(gp_server* s, gp_request* r, gp_ticket** ticket) {
             shape.m = m;
             shape.edge_weights = nullptr;
             job->bytes = footprint_of(&shape);
+            job->share = s->share_of(m);
             ++it->second->pending;
         } else {
             job->bytes = footprint_of(r);
+            job->share = s->share_of(r->m);
         }
         s->queue.push_back(job);
         s->tickets.insert(handle);

@@ -1435,8 +1435,12 @@ __global__ void k_lp_arm(unsigned long long* ctr, int k, unsigned long long run,
 // End of one LP round: the movers, cut delta and part loads of the round
 // decide whether the loop goes on; the counter block is cleared for the next
 // round and the state is mirrored to the host mailbox.
+// keep: non-null on small levels, where this block also checkpoints the
+// labels of an improving round (n bytes from labels to keep) instead of a
+// separate k_lp_keep launch.
 __global__ void __launch_bounds__(256) k_lp_decide(unsigned long long* ctr, LoopParams P,
-                                                   unsigned long long* mailbox) {
+                                                   unsigned long long* mailbox, const Label* labels, Label* keep,
+                                                   std::int64_t n) {
     unsigned long long* loads = ctr + kCtrCount;
     unsigned long long* L = loads + P.k;
     __shared__ int s_run, s_improved, s_stop;
@@ -1526,9 +1530,12 @@ __global__ void __launch_bounds__(256) k_lp_decide(unsigned long long* ctr, Loop
         }
     }
     for (int i = threadIdx.x; i < kCtrCount; i += blockDim.x) ctr[i] = 0;
+    if (keep && s_improved)
+        for (std::int64_t v = threadIdx.x; v < n; v += blockDim.x) keep[v] = labels[v];
 }
 
 // Checkpoint of the labels of a round that set a new best cut.
+constexpr std::int64_t kKeepInline = 16384;
 // vec: both arrays are 16-byte aligned (copied in 16-byte words).
 __global__ void k_lp_keep(const unsigned long long* flag, const Label* labels, Label* best, std::int64_t n, bool vec) {
     if (!*flag) return;
@@ -2592,11 +2599,17 @@ public:
         GP_LAUNCH_CHECK();
         propose_sweeps();
         apply_launch(true, false, true);
-        hipLaunchKernelGGL(k_lp_decide, dim3(1), dim3(256), 0, s, ctr_.get(), P, c_.mailbox());
+        // Levels of at most kKeepInline vertices checkpoint inside the
+        // decision kernel; larger ones in a launch of their own.
+        const bool inline_keep = n <= kKeepInline;
+        hipLaunchKernelGGL(k_lp_decide, dim3(1), dim3(256), 0, s, ctr_.get(), P, c_.mailbox(), a_.labels,
+                           inline_keep ? best_labels : nullptr, n);
         GP_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_lp_keep, dim3(grid_for(ceil_div(n, 16), 256, 1024)), dim3(256), 0, s, L + kImproved,
-                           a_.labels, best_labels, n, vec);
-        GP_LAUNCH_CHECK();
+        if (!inline_keep) {
+            hipLaunchKernelGGL(k_lp_keep, dim3(grid_for(ceil_div(n, 16), 256, 1024)), dim3(256), 0, s, L + kImproved,
+                               a_.labels, best_labels, n, vec);
+            GP_LAUNCH_CHECK();
+        }
     }
 
     // The LP proposal and afterburner sweeps of a round (counter block clear).

@@ -120,9 +120,9 @@ struct RefArgs {
 
 // Part-connectivity table of one vertex in team-private LDS; returns the
 // incident weight excluding self loops (src/refinement/affinity.cuh:9-66).
-template <class Team>
-__device__ inline std::uint64_t team_histogram(const Team& t, const RefArgs& a, std::int64_t v,
-                                               unsigned long long* hist) {
+// Bin: the table's word (32 bits when no row of the level weighs 2^32).
+template <class Team, class Bin = unsigned long long>
+__device__ inline std::uint64_t team_histogram(const Team& t, const RefArgs& a, std::int64_t v, Bin* hist) {
     t.sync();  // earlier readers of the table are done
     for (int p = t.rank(); p < a.k; p += Team::size) hist[p] = 0;
     t.sync();
@@ -139,7 +139,7 @@ __device__ inline std::uint64_t team_histogram(const Team& t, const RefArgs& a, 
         },
         [&](std::int64_t, const Rec& r) {
             if (r.label < 0) return;
-            atomicAdd(&hist[r.label], (unsigned long long)r.w);
+            atomicAdd(&hist[r.label], (Bin)r.w);
             total += r.w;
         });
     t.sync();
@@ -151,10 +151,10 @@ __device__ inline std::uint64_t team_histogram(const Team& t, const RefArgs& a, 
 // serialise the whole block on those bins); the wave tables are then summed
 // into hist[0, k).  The block's scratch holds 1 + waves tables.
 constexpr int kHeavyTables = 1 + BlockTeam::size / kWave;
-__device__ inline std::uint64_t team_histogram(const BlockTeam& t, const RefArgs& a, std::int64_t v,
-                                               unsigned long long* hist) {
+template <class Bin = unsigned long long>
+__device__ inline std::uint64_t team_histogram(const BlockTeam& t, const RefArgs& a, std::int64_t v, Bin* hist) {
     constexpr int kWaves = BlockTeam::size / kWave;
-    unsigned long long* mine = hist + (threadIdx.x / kWave + 1) * a.k;
+    Bin* mine = hist + (threadIdx.x / kWave + 1) * a.k;
     t.sync();  // earlier readers of the table are done
     for (int i = t.rank(); i < kHeavyTables * a.k; i += BlockTeam::size) hist[i] = 0;
     t.sync();
@@ -171,12 +171,12 @@ __device__ inline std::uint64_t team_histogram(const BlockTeam& t, const RefArgs
         },
         [&](std::int64_t, const Rec& r) {
             if (r.label < 0) return;
-            atomicAdd(&mine[r.label], (unsigned long long)r.w);
+            atomicAdd(&mine[r.label], (Bin)r.w);
             total += r.w;
         });
     t.sync();
     for (int p = t.rank(); p < a.k; p += BlockTeam::size) {
-        unsigned long long sum = 0;
+        Bin sum = 0;
         for (int w = 1; w <= kWaves; ++w) sum += hist[w * a.k + p];
         hist[p] = sum;
     }
@@ -308,8 +308,9 @@ struct RowBody {
 };
 
 // LP proposal (incremental.cuh:166-198, 229-296).  Bin is the bin type of
-// the short-row tables: 32 bits when no row of the level weighs 2^32 or
-// more (half the LDS per team, twice the resident teams), else 64.
+// the part tables: 32 bits when no row of the level weighs 2^32 or more
+// (short rows: half the LDS per team, twice the resident teams; longer rows:
+// one bank per LDS atomic instead of two), else 64.
 template <class Bin>
 struct LpProposeBody : RowBody {
     using LightBin = Bin;
@@ -363,10 +364,11 @@ struct LpProposeBody : RowBody {
             best = team_reduce(t, best, priced);
             light_clear(t, lh, r);
         } else {
-            total = team_histogram(t, a, v, hist);
-            current = hist[s];
+            Bin* bh = reinterpret_cast<Bin*>(hist);
+            total = team_histogram(t, a, v, bh);
+            current = bh[s];
             if (current != total)
-                for (int p = t.rank(); p < a.k; p += Team::size) consider(p, hist[p]);
+                for (int p = t.rank(); p < a.k; p += Team::size) consider(p, bh[p]);
             best = team_reduce(t, best, priced);
         }
         if (current == total) return RowOut{false, degree(v)};
@@ -743,10 +745,11 @@ struct RepairProposeBody : RowBody {
             best = team_reduce(t, best);
             light_clear(t, lh, r);
         } else {
-            total = team_histogram(t, a, v, hist);
-            current = hist[s];
+            Bin* bh = reinterpret_cast<Bin*>(hist);
+            total = team_histogram(t, a, v, bh);
+            current = bh[s];
             if (current != total)
-                for (int p = t.rank(); p < a.k; p += Team::size) consider(p, hist[p]);
+                for (int p = t.rank(); p < a.k; p += Team::size) consider(p, bh[p]);
             best = team_reduce(t, best);
         }
         unsigned long long* agg = t.part ? const_cast<unsigned long long*>(t.part) + a.k : nullptr;

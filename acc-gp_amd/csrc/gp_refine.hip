@@ -107,6 +107,10 @@ struct RefArgs {
     std::uint32_t* state;  // packed lanes below
     StateLane<std::int8_t> targets;  // -1: none
     std::int64_t* gains;
+    // When every row of the level weighs less than 2^31, the gains are kept
+    // in 32 bits here instead (gains is null): the afterburner gathers half
+    // the bytes from an array half the size.
+    std::int32_t* gains32;
     StateLane<std::uint8_t> locks;
     StateLane<std::uint8_t> flags;
     StateLane<std::uint8_t> member;
@@ -302,6 +306,15 @@ __device__ inline Cand team_reduce(const BlockTeam& t, Cand c, bool priced) {
     return r;
 }
 
+// Stores a vertex's proposal gain in the level's gain array (32 or 64 bits;
+// the "none" value INT64_MIN is stored as INT32_MIN in 32 bits).
+__device__ inline void set_gain(const RefArgs& a, std::int64_t v, std::int64_t g) {
+    if (a.gains32)
+        a.gains32[v] = g == INT64_MIN ? INT32_MIN : (std::int32_t)g;
+    else
+        a.gains[v] = g;
+}
+
 struct RowBody {
     RefArgs a;
     __device__ std::int64_t degree(std::int32_t v) const { return a.off[v + 1] - a.off[v]; }
@@ -335,7 +348,7 @@ struct LpProposeBody : RowBody {
         t.sync();
         if (t.leader()) {
             a.targets[v] = -1;
-            a.gains[v] = INT64_MIN;
+            set_gain(a, v, INT64_MIN);
             a.flags[v] = 0;
             a.locks[v] = 0;
         }
@@ -376,18 +389,28 @@ struct LpProposeBody : RowBody {
                                             current - best.conn < (std::uint64_t)(a.filter * (double)current));
         if (accept && t.leader()) {
             a.targets[v] = best.p;
-            a.gains[v] = best.conn >= current ? (std::int64_t)(best.conn - current) : -(std::int64_t)(current - best.conn);
+            set_gain(a, v, best.conn >= current ? (std::int64_t)(best.conn - current) : -(std::int64_t)(current - best.conn));
         }
         return RowOut{accept, degree(v)};
     }
 };
 
-// Afterburner (incremental.cuh:361-438).
+// Afterburner (incremental.cuh:361-438).  G: the level's gain word
+// (std::int32_t: gains32, else gains).
+template <class G>
 struct AfterBody : RowBody {
+    __device__ std::int64_t gain(std::int64_t u) const {
+        if constexpr (sizeof(G) == 4) {
+            const std::int32_t g = a.gains32[u];
+            return g == INT32_MIN ? INT64_MIN : (std::int64_t)g;
+        } else {
+            return a.gains[u];
+        }
+    }
     template <class Team>
     __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long*) const {
         const int s = a.labels[v], tv = a.targets[v];
-        const std::int64_t gv = a.gains[v];
+        const std::int64_t gv = gain(v);
         std::int64_t corr = 0;
         // Neighbour state packed in one word: target (signed byte, -1 none)
         // in bits 0-7, label in bits 8-15.
@@ -404,7 +427,7 @@ struct AfterBody : RowBody {
                 // The gain is gathered beside the target rather than after
                 // it: one round trip per batch instead of two.
                 const int tu = a.targets[uu];
-                return Rec{u, (tu & 0xff) | ((int)a.labels[uu] << 8), a.gains[uu]};
+                return Rec{u, (tu & 0xff) | ((int)a.labels[uu] << 8), gain(uu)};
             },
             [&](std::int64_t x, const Rec& r) {
                 const int tu = (int)(std::int8_t)(r.tl & 0xff), lu = r.tl >> 8;
@@ -482,7 +505,7 @@ struct BoundaryBody : RowBody {
         if (t.leader()) {
             a.member[v] = any != 0;
             a.targets[v] = -1;
-            a.gains[v] = INT64_MIN;
+            set_gain(a, v, INT64_MIN);
             a.flags[v] = 0;
             a.locks[v] = 0;
         }
@@ -809,12 +832,14 @@ __global__ void k_repair_flag(RefArgs a, const std::int32_t* bstar, std::uint8_t
 // order, as (weight, part) pairs in slots [0, nnz) of the vertex's k slots.
 // One wave per vertex (donor rows can be long); ballots give the slot of
 // each nonzero part.
+// Bin: the LDS table's word (32 bits when no row of the level weighs 2^32).
+template <class Bin>
 __global__ void k_conn_rows(RefArgs a, const std::int32_t* list, std::int64_t count, const std::uint8_t* skip,
                             unsigned long long* vals, std::uint8_t* parts, std::uint8_t* nnz) {
     extern __shared__ unsigned long long smem[];
     const int lane = lane_id(), gi = threadIdx.x / kWave;
     const unsigned long long lt = (1ULL << lane) - 1ULL;
-    unsigned long long* hist = smem + gi * a.k;
+    Bin* hist = reinterpret_cast<Bin*>(smem) + gi * a.k;
     const std::int64_t gpb = blockDim.x / kWave;
     for (std::int64_t idx = blockIdx.x * gpb + gi; idx < count; idx += (std::int64_t)gridDim.x * gpb) {
         if (skip && skip[idx]) continue;
@@ -2491,7 +2516,11 @@ public:
         a_.cap = cap;
         a_.labels = labels;
         state_ = c.buf<std::uint32_t>(n);
-        gains_ = c.buf<std::int64_t>(n);
+        // Gains of at most the row weight: 32 bits below 2^31.
+        if (g.incident_bound < (1ULL << 31))
+            gains32_ = c.buf<std::int32_t>(n);
+        else
+            gains_ = c.buf<std::int64_t>(n);
         frontier_ = c.buf<std::int32_t>(n);
         cand_ = c.buf<std::int32_t>(n);
         movers_ = c.buf<std::int32_t>(n);
@@ -2510,6 +2539,7 @@ public:
         a_.member = StateLane<std::uint8_t>{lanes + kStateMember};
         a_.locks = StateLane<std::uint8_t>{lanes + kStateLock};
         a_.gains = gains_.get();
+        a_.gains32 = gains32_.get();
         a_.ctr = ctr_.get();
         a_.loads = ctr_.get() + kCtrCount;
         a_.loads0 = loads0_.get();
@@ -2626,8 +2656,12 @@ public:
                       ctr_.get() + kCand, ctr_.get() + kRowSum, a_.k, kTier, true);
         });
         c_.timed("after", 0.0, [&] {
-            tiers(AfterBody{{a_}}, cand_.get(), 0, ctr_.get() + kCand, movers_.get(), ctr_.get() + kMovers, nullptr, 0,
-                  kTierB, true, kEntAfter);
+            if (a_.gains32)
+                tiers(AfterBody<std::int32_t>{{a_}}, cand_.get(), 0, ctr_.get() + kCand, movers_.get(),
+                      ctr_.get() + kMovers, nullptr, 0, kTierB, true, kEntAfter);
+            else
+                tiers(AfterBody<std::int64_t>{{a_}}, cand_.get(), 0, ctr_.get() + kCand, movers_.get(),
+                      ctr_.get() + kMovers, nullptr, 0, kTierB, true, kEntAfter);
         });
     }
 
@@ -2699,7 +2733,8 @@ public:
             if (c_.profiling()) {
                 const double w = a_.ew ? 4.0 : 0.0;
                 c_.add_bytes("lp_propose", 28.0 * (double)h[kAccFrontier] + (8.0 + w) * (double)h[kAccRowSum]);
-                c_.add_bytes("after", 16.0 * (double)h[kAccAfterR] + (20.0 + w) * (double)h[kAccAfterE]);
+                const double g = a_.gains32 ? 4.0 : 0.0;  // 32-bit gains: 4 B less per row and entry
+                c_.add_bytes("after", (16.0 - g) * (double)h[kAccAfterR] + (20.0 - g + w) * (double)h[kAccAfterE]);
                 c_.add_bytes("update", 24.0 * (double)h[kAccUpdateR] + (16.0 + w) * (double)h[kAccUpdateE]);
             }
             const int reason = static_cast<int>(h[kReason]);
@@ -2759,14 +2794,16 @@ public:
         if (c_.profiling()) {
             // Algorithmic bytes per row and per entry of each sweep (weighted
             // graphs add 4 B of edge weight per entry): LP proposal 28 + 8;
-            // afterburner 16 + 20 (neighbour, target, gain, label); update and
+            // afterburner 16 + 20 (neighbour, target, gain, label; 4 B less
+            // each with 32-bit gains); update and
             // commit 24 + 16 (neighbour, label, target, flag); repair proposal
             // 24 + 8 (offsets, label, weight, bucket and target out; neighbour
             // and its label per entry).
             const double w = a_.ew ? 4.0 : 0.0;
             if (lp) {
                 c_.add_bytes("lp_propose", 28.0 * (double)h[kFrontier] + (8.0 + w) * (double)h[kRowSum]);
-                c_.add_bytes("after", 16.0 * (double)h[kEntAfter + 1] + (20.0 + w) * (double)h[kEntAfter]);
+                const double g = a_.gains32 ? 4.0 : 0.0;
+                c_.add_bytes("after", (16.0 - g) * (double)h[kEntAfter + 1] + (20.0 - g + w) * (double)h[kEntAfter]);
             }
             c_.add_bytes("update", 24.0 * (double)h[kEntUpdate + 1] + (16.0 + w) * (double)h[kEntUpdate]);
             if (samples)
@@ -2794,6 +2831,19 @@ public:
         loads_of(c_, g_, a_.labels, a_.k, loads.data());
         if (loads != loads_host_) throw std::runtime_error("incremental part weights differ from a recount");
         ++st_.strict_checks;
+    }
+
+    // Sparse connection rows of the listed donors (k_conn_rows).
+    void conn_rows(const std::int32_t* E, std::int64_t ne, const std::uint8_t* skip, unsigned long long* vals,
+                   std::uint8_t* parts, std::uint8_t* nnz) {
+        const std::size_t lds = (256 / kWave) * static_cast<std::size_t>(a_.k) * sizeof(unsigned long long);
+        if (narrow_bins())
+            hipLaunchKernelGGL(k_conn_rows<unsigned>, dim3(grid_for(ne, 256 / kWave, 2048)), dim3(256), lds,
+                               c_.stream(), a_, E, ne, skip, vals, parts, nnz);
+        else
+            hipLaunchKernelGGL(k_conn_rows<unsigned long long>, dim3(grid_for(ne, 256 / kWave, 2048)), dim3(256),
+                               lds, c_.stream(), a_, E, ne, skip, vals, parts, nnz);
+        GP_LAUNCH_CHECK();
     }
 
     // Receiver reservation over the flagged donors L (vertex order).
@@ -2849,9 +2899,7 @@ public:
         if (ne <= kTailDonors) {
             // Every donor's connection row first (the rows of donors admitted
             // in attempt 0 go unused), then all attempts in one launch.
-            hipLaunchKernelGGL(k_conn_rows, dim3(grid_for(ne, 256 / kWave, 2048)), dim3(256),
-                               (256 / kWave) * k * sizeof(unsigned long long), s, a_, E.get(), ne, nullptr, conn.get(),
-                               conn_part.get(), conn_nnz.get());
+            conn_rows(E.get(), ne, nullptr, conn.get(), conn_part.get(), conn_nnz.get());
             GP_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_reserve_tail, dim3(1), dim3(kTailThreads), 0, s, a_, r, 0, movers_.get(),
                                ctr_.get() + kMovers);
@@ -2865,9 +2913,7 @@ public:
             fill<unsigned long long>(c_, active.get(), kAttemptGroup, 0ULL);
             for (int j = 0; j < group; ++j, ++attempt) {
                 if (attempt == 1) {
-                    hipLaunchKernelGGL(k_conn_rows, dim3(grid_for(ne, 256 / kWave, 2048)), dim3(256),
-                                       (256 / kWave) * k * sizeof(unsigned long long), s, a_, E.get(), ne, done.get(),
-                                       conn.get(), conn_part.get(), conn_nnz.get());
+                    conn_rows(E.get(), ne, done.get(), conn.get(), conn_part.get(), conn_nnz.get());
                     GP_LAUNCH_CHECK();
                 }
                 // Blocks stride over the donors: the per-block room table and
@@ -3148,6 +3194,7 @@ public:
     DBuf<std::uint32_t> state_;
     DBuf<std::int32_t> frontier_, cand_, movers_, medium_, heavy_, bucket_;
     DBuf<std::int64_t> gains_;
+    DBuf<std::int32_t> gains32_;
     DBuf<unsigned long long> ctr_, loads0_, counts_;
     DBuf<double> prices_;
     DBuf<RepairSample> samples_;

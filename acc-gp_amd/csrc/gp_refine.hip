@@ -493,15 +493,28 @@ struct BoundaryBody : RowBody {
     template <class Team>
     __device__ RowOut operator()(const Team& t, std::int32_t v, unsigned long long*) const {
         const int s = a.labels[v];
+        // Batches of kBatch entries per lane, with a team-wide test after
+        // each: most rows of a cut graph meet another part in their first
+        // batch and stop there.  The loop bound is the same for the whole
+        // team, so every lane takes part in each test.
+        const std::int64_t b = a.off[v], e = a.off[v + 1];
         int any = 0;
-        row_batches(
-            t, a.off[v], a.off[v + 1],
-            [&](std::int64_t x) {
-                const std::int32_t u = a.nbr[(std::uint32_t)x];
-                return u != v && a.labels[(std::uint32_t)u] != s;
-            },
-            [&](std::int64_t, bool other) { any |= other; });
-        any = t.sum(any);
+        for (std::int64_t x0 = b; x0 < e && !any; x0 += kBatch * Team::size) {
+            bool other[kBatch];
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) {
+                const std::int64_t x = x0 + t.rank() + j * Team::size;
+                other[j] = false;
+                if (x < e) {
+                    const std::int32_t u = a.nbr[(std::uint32_t)x];
+                    other[j] = u != v && a.labels[(std::uint32_t)u] != s;
+                }
+            }
+            int mine = 0;
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) mine |= other[j];
+            any = t.sum(mine);
+        }
         if (t.leader()) {
             a.member[v] = any != 0;
             a.targets[v] = -1;

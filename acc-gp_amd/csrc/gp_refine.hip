@@ -841,39 +841,84 @@ __global__ void k_repair_flag(RefArgs a, const std::int32_t* bstar, std::uint8_t
     }
 }
 
-// Sparse connection rows: the parts a listed vertex connects to, in part
-// order, as (weight, part) pairs in slots [0, nnz) of the vertex's k slots.
-// One wave per vertex (donor rows can be long); ballots give the slot of
-// each nonzero part.
-// Bin: the LDS table's word (32 bits when no row of the level weighs 2^32).
+// Position of the calling lane among the team's lanes with pred set, and
+// their number (every lane of the team calls it).
+template <int W>
+__device__ inline int team_ballot_prefix(const GroupTeam<W>& t, bool pred, int& total) {
+    const unsigned long long m = __ballot(pred);
+    const unsigned long long bits = W == kWave ? m : (m >> (lane_id() & ~(W - 1))) & ((1ULL << W) - 1ULL);
+    total = __popcll(bits);
+    return __popcll(bits & ((1ULL << t.lg) - 1ULL));
+}
+__device__ inline int team_ballot_prefix(const BlockTeam& t, bool pred, int& total) {
+    constexpr int kWaves = BlockTeam::size / kWave;
+    const unsigned long long m = __ballot(pred);
+    const int w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane_id() == 0) t.red[w] = (unsigned long long)__popcll(m);
+    __syncthreads();
+    int before = 0;
+    total = 0;
+    for (int i = 0; i < kWaves; ++i) {
+        if (i < w) before += (int)t.red[i];
+        total += (int)t.red[i];
+    }
+    __syncthreads();
+    return before + __popcll(m & ((1ULL << lane_id()) - 1ULL));
+}
+
+// Sparse connection rows: the parts a listed donor connects to, in part
+// order, as (weight, part) pairs in slots [0, nnz) of its k slots, by the
+// row tiers over the positions [0, ne) of the donor list E (the tier
+// kernels hand the position over as the row id): short rows take 4-lane
+// teams, hub rows whole blocks; team ballots give each nonzero part's slot.
 template <class Bin>
-__global__ void k_conn_rows(RefArgs a, const std::int32_t* list, std::int64_t count, const std::uint8_t* skip,
-                            unsigned long long* vals, std::uint8_t* parts, std::uint8_t* nnz) {
-    extern __shared__ unsigned long long smem[];
-    const int lane = lane_id(), gi = threadIdx.x / kWave;
-    const unsigned long long lt = (1ULL << lane) - 1ULL;
-    Bin* hist = reinterpret_cast<Bin*>(smem) + gi * a.k;
-    const std::int64_t gpb = blockDim.x / kWave;
-    for (std::int64_t idx = blockIdx.x * gpb + gi; idx < count; idx += (std::int64_t)gridDim.x * gpb) {
-        if (skip && skip[idx]) continue;
-        const std::int64_t v = list[idx];
-        team_histogram(GroupTeam<kWave>{lane}, a, v, hist);
+struct ConnBody : RowBody {
+    using LightBin = Bin;
+    const std::int32_t* E;
+    const std::uint8_t* skip;  // null, or nonzero for donors whose row is not needed
+    unsigned long long* vals;
+    std::uint8_t* parts;
+    std::uint8_t* nnz;
+    __device__ std::int64_t degree(std::int32_t i) const {
+        const std::int32_t v = E[i];
+        return a.off[v + 1] - a.off[v];
+    }
+    template <class Team>
+    __device__ void emit(const Team& t, std::int64_t i, const Bin* bh) const {
         int base = 0;
-        for (int p0 = 0; p0 < a.k; p0 += kWave) {
-            const int p = p0 + lane;
-            const unsigned long long c = p < a.k ? hist[p] : 0;
-            const unsigned long long mask = __ballot(c != 0);
+        for (int p0 = 0; p0 < a.k; p0 += Team::size) {
+            const int p = p0 + t.rank();
+            const unsigned long long c = p < a.k ? (unsigned long long)bh[p] : 0ULL;
+            int cnt = 0;
+            const int pos = team_ballot_prefix(t, c != 0, cnt);
             if (c) {
-                const std::int64_t j = idx * a.k + base + __popcll(mask & lt);
+                const std::int64_t j = i * a.k + base + pos;
                 vals[j] = c;
                 parts[j] = static_cast<std::uint8_t>(p);
             }
-            base += __popcll(mask);
+            base += cnt;
         }
-        if (lane == 0) nnz[idx] = static_cast<std::uint8_t>(base);
-        wave_sync();
+        if (t.leader()) nnz[i] = static_cast<std::uint8_t>(base);
     }
-}
+    template <class Team>
+    __device__ RowOut operator()(const Team& t, std::int32_t i, unsigned long long* hist) const {
+        if (skip && skip[i]) return RowOut{false, 0};
+        const std::int32_t v = E[i];
+        Bin* bh = reinterpret_cast<Bin*>(hist);
+        if constexpr (IsLight<Team>::value) {
+            LightRow<Team::size> r;
+            light_histogram(t, a, v, bh, r);
+            emit(t, i, bh);
+            light_clear(t, bh, r);
+        } else {
+            team_histogram(t, a, v, bh);
+            emit(t, i, bh);
+        }
+        return RowOut{false, 0};
+    }
+};
+
 
 // Receiver reservation (rebalance_graph.cuh:86-156) in canonical order:
 // requesters of one part are ranked by vertex id on top of the part's
@@ -2846,17 +2891,16 @@ public:
         ++st_.strict_checks;
     }
 
-    // Sparse connection rows of the listed donors (k_conn_rows).
+    // Sparse connection rows of the listed donors (ConnBody by the row
+    // tiers over the positions [0, ne) of E).
     void conn_rows(const std::int32_t* E, std::int64_t ne, const std::uint8_t* skip, unsigned long long* vals,
                    std::uint8_t* parts, std::uint8_t* nnz) {
-        const std::size_t lds = (256 / kWave) * static_cast<std::size_t>(a_.k) * sizeof(unsigned long long);
         if (narrow_bins())
-            hipLaunchKernelGGL(k_conn_rows<unsigned>, dim3(grid_for(ne, 256 / kWave, 2048)), dim3(256), lds,
-                               c_.stream(), a_, E, ne, skip, vals, parts, nnz);
+            tiers(ConnBody<unsigned>{{a_}, E, skip, vals, parts, nnz}, nullptr, ne, nullptr, nullptr, nullptr, nullptr,
+                  a_.k);
         else
-            hipLaunchKernelGGL(k_conn_rows<unsigned long long>, dim3(grid_for(ne, 256 / kWave, 2048)), dim3(256),
-                               lds, c_.stream(), a_, E, ne, skip, vals, parts, nnz);
-        GP_LAUNCH_CHECK();
+            tiers(ConnBody<unsigned long long>{{a_}, E, skip, vals, parts, nnz}, nullptr, ne, nullptr, nullptr, nullptr,
+                  nullptr, a_.k);
     }
 
     // Receiver reservation over the flagged donors L (vertex order).

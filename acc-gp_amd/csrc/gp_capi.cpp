@@ -125,6 +125,7 @@ gp::RefineOptions to_options(const gp_refine_options& o) {
     r.growth_ratio = o.growth_ratio;
     r.exchange_rounds = o.exchange_rounds;
     r.strict_verify = o.strict_verify != 0;
+    r.device_loop = o.gpu_lp_loop != 0;
     return r;
 }
 
@@ -553,6 +554,7 @@ void gp_default_refine_options(int parts, double imbalance, gp_refine_options* o
     out->growth_ratio = o.growth_ratio;
     out->exchange_rounds = o.exchange_rounds;
     out->strict_verify = 0;
+    out->gpu_lp_loop = o.device_loop ? 1 : 0;
 }
 
 int gp_uncoarsen(gp_context* ctx, const gp_hierarchy* h, const int32_t* coarsest_labels,

@@ -66,6 +66,9 @@ struct RefineOptions {
     // Recount the cut and the loads after every committed batch and every
     // projection (include/refine.hpp:17, the CLI's --verify).
     bool strict_verify = false;
+    // LP round loop decided on the device (incremental.cuh:911-961 as
+    // lp_graph.cuh records it); false: on the host, one read-back per round.
+    bool device_loop = true;
 };
 
 // Coarsening state carried across levels (SclpWorkspace::Impl, src/coarsen.cu:299-396).

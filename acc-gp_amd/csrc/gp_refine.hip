@@ -3297,7 +3297,7 @@ LevelStats refine_level_impl(Context& c, const DGraph& g, Label* labels, std::ve
     const bool graph_mode = g.n >= o.graph_loop_threshold;
     int stale = 0, empty = 0;
     // Strict mode recounts on the host after every batch: host loop.
-    const bool device_loop = c.device_loop() && !o.strict_verify;
+    const bool device_loop = o.device_loop && c.device_loop() && !o.strict_verify;
     if (device_loop) R.lp_loop(rounds, graph_mode, cut, best_cut, best_labels.get(), best_loads);
     const int host_rounds = device_loop ? 0 : rounds;
     for (int phase = 0; phase < host_rounds && stale < o.stale_rounds; ++phase) {

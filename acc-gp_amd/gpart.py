@@ -29,7 +29,7 @@ class RefineOptions(ctypes.Structure):
     _fields_ = [
         ("parts", ctypes.c_int), ("imbalance", ctypes.c_double), ("max_rounds", ctypes.c_int),
         ("stale_rounds", ctypes.c_int), ("growth_ratio", ctypes.c_double), ("exchange_rounds", ctypes.c_int),
-        ("strict_verify", ctypes.c_int),
+        ("strict_verify", ctypes.c_int), ("gpu_lp_loop", ctypes.c_int),
     ]
 
 

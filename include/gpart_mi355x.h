@@ -56,6 +56,8 @@ typedef struct gp_refine_options {        /* include/refine.hpp:13-26 */
     double growth_ratio;
     int exchange_rounds;
     int strict_verify;                    /* recount cut and loads after every batch and projection */
+    int gpu_lp_loop;                      /* 1: LP round loop decided on the device (default); 0: on the host
+                                             after a counter read-back per round (include/refine.hpp:23) */
 } gp_refine_options;
 
 typedef struct gp_level_stats {           /* RefineLevelResult, include/refine.hpp:60-68 */

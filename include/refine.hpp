@@ -4,10 +4,13 @@
 // libgpart_mi355x.
 //
 // Options the reference keeps that select among its own implementations
-// (affinity_layout, gpu_lp_loop, repair_enumeration, sparse_cut_bound) have
-// one implementation here and are accepted and ignored; they do not change
-// results in the reference either.  strict_verify recounts the cut and the
-// loads after every projection and committed batch on the device.
+// (affinity_layout, repair_enumeration, sparse_cut_bound) have one
+// implementation here and are accepted and ignored; they do not change
+// results in the reference either.  gpu_lp_loop selects, as in the
+// reference, whether the LP round loop is decided on the device (true) or
+// by the host after a read-back per round (false); both give the same
+// partition.  strict_verify recounts the cut and the loads after every
+// projection and committed batch on the device (with the host loop).
 #pragma once
 
 #include "graph.hpp"
@@ -110,6 +113,7 @@ inline gp_refine_options to_c(const RefineOptions& o) {
     c.growth_ratio = o.refinement_growth_ratio;
     c.exchange_rounds = o.exchange_rounds;
     c.strict_verify = o.strict_verify ? 1 : 0;
+    c.gpu_lp_loop = o.gpu_lp_loop ? 1 : 0;
     return c;
 }
 

@@ -1,7 +1,8 @@
 """LP loop control: the round loop decided on the device (the default), the
 same with each round issued before the previous one is seen to end
 (GP_LOOP_AHEAD=1), and the loop decided on the host after a counter
-read-back per round (GP_DEVICE_LOOP=0) give the same partitions -- labels,
+read-back per round (the reference's option gpu_lp_loop = false, or
+GP_DEVICE_LOOP=0 for a whole context) give the same partitions -- labels,
 cut, loads and every level's statistics -- on graphs whose levels run the
 small-level (long double) and graph-mode stale rules, repairs in the middle
 of the loop, and priced proposals.  Each mode is a context created with its
@@ -15,7 +16,9 @@ from tests import graphs
 
 pytestmark = pytest.mark.gpu
 
-MODES = (("device", {}), ("ahead", {"GP_LOOP_AHEAD": "1"}), ("host", {"GP_DEVICE_LOOP": "0"}))
+# (name, environment of the context, refinement option overrides)
+MODES = (("device", {}, {}), ("ahead", {"GP_LOOP_AHEAD": "1"}, {}), ("host", {"GP_DEVICE_LOOP": "0"}, {}),
+         ("option", {}, {"gpu_lp_loop": 0}))
 KEYS = ("n", "refined", "initial_repair_failed", "rounds", "initial_cut", "final_cut", "lp_moves", "repair_moves",
         "group_moves", "repair_iterations")
 
@@ -47,17 +50,17 @@ def _cases(gp):
 
 
 def test_device_loop_equals_host_loop(gp):
-    ctxs = [(name, _context(gp, env)) for name, env in MODES]
+    ctxs = [(name, _context(gp, env), ropts) for name, env, ropts in MODES]
     try:
         for case, off, nbr, ew, vw, k in _cases(gp):
             print("case %s: n=%d k=%d" % (case, len(off) - 1, k), flush=True)
             results = []
-            for mode, c in ctxs:
+            for mode, c, ropts in ctxs:
                 g = gp.Graph(c, off, nbr, ew, vw)
                 h = gp.Hierarchy(c, g, gp.coarsen_options(k))
                 coff, cnbr, cew, cvw = h.graph_at(h.levels - 1)
                 init = gp.initial_partition_host(coff, cnbr, cew, cvw, k)
-                lab, cut, loads, st = h.uncoarsen(init, gp.refine_options(k))
+                lab, cut, loads, st = h.uncoarsen(init, gp.refine_options(k, **ropts))
                 results.append((mode, lab, cut, loads, st))
                 h.close()
                 g.close()
@@ -72,5 +75,5 @@ def test_device_loop_equals_host_loop(gp):
                     for key in KEYS:
                         assert a[key] == b[key], (case, mode, key, a, b)
     finally:
-        for _, c in ctxs:
+        for _, c, _ in ctxs:
             c.close()

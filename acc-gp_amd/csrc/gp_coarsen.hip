@@ -1566,27 +1566,33 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
         AffinityArgs a{g.off.get(), g.nbr.get(), g.ew_ptr(), clusters.get(), cw.get(), roles.get(), cap, tie_salt,
                        filter_roles, filter_roles && need_regret, proposals.get(), gains.get(), regret.get(),
                        ties.get()};
+        // The hub rows' records and the other classes touch disjoint
+        // vertices: with hub records (and tier overlap on) the other classes
+        // run on the side stream while the records are sorted here.
+        const bool side = hub_count > 0 && c.overlap();
+        if (side) c.fork();
+        const hipStream_t as = side ? c.aux() : s;
         auto group = [&](int cls, const std::int32_t* list, std::int64_t count) {
             if (!count) return;
             switch (cls) {
                 case 0:
-                    hipLaunchKernelGGL((k_affinity_group<16, 1>), dim3(grid_for(count, 16)), dim3(256), 0, s, a, list,
+                    hipLaunchKernelGGL((k_affinity_group<16, 1>), dim3(grid_for(count, 16)), dim3(256), 0, as, a, list,
                                        count);
                     break;
                 case 1:
-                    hipLaunchKernelGGL((k_affinity_group<64, 1>), dim3(grid_for(count, 4)), dim3(256), 0, s, a, list,
+                    hipLaunchKernelGGL((k_affinity_group<64, 1>), dim3(grid_for(count, 4)), dim3(256), 0, as, a, list,
                                        count);
                     break;
                 case 2:
-                    hipLaunchKernelGGL((k_affinity_group<64, 2>), dim3(grid_for(count, 4)), dim3(256), 0, s, a, list,
+                    hipLaunchKernelGGL((k_affinity_group<64, 2>), dim3(grid_for(count, 4)), dim3(256), 0, as, a, list,
                                        count);
                     break;
                 case 3:
-                    hipLaunchKernelGGL((k_affinity_group<64, 4>), dim3(grid_for(count, 4)), dim3(256), 0, s, a, list,
+                    hipLaunchKernelGGL((k_affinity_group<64, 4>), dim3(grid_for(count, 4)), dim3(256), 0, as, a, list,
                                        count);
                     break;
                 default:
-                    hipLaunchKernelGGL((k_affinity_group<64, 8>), dim3(grid_for(count, 4)), dim3(256), 0, s, a, list,
+                    hipLaunchKernelGGL((k_affinity_group<64, 8>), dim3(grid_for(count, 4)), dim3(256), 0, as, a, list,
                                        count);
                     break;
             }
@@ -1598,16 +1604,16 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
             // brings eight waves.
             if (st.max_row_weight <= 0xffffffffULL)
                 hipLaunchKernelGGL((k_affinity_block_hash<BlockHashSmem, kBlockDegree, kBlockHashThreads>),
-                                   dim3(grid_for(count, 1, 4096)), dim3(kBlockHashThreads), 0, s, a, list, count);
+                                   dim3(grid_for(count, 1, 4096)), dim3(kBlockHashThreads), 0, as, a, list, count);
             else
-                hipLaunchKernelGGL(k_affinity_block, dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, s, a,
+                hipLaunchKernelGGL(k_affinity_block, dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, as, a,
                                    list, count);
             GP_LAUNCH_CHECK();
         };
         launch_classes(c, lists.get(), n, cc.c, group, block);
         if (big_count) {
             hipLaunchKernelGGL((k_affinity_block_hash<BigHashSmem, kBigRowDegree, 1024>),
-                               dim3(grid_for(big_count, 1, 4096)), dim3(1024), 0, s, a, big_hubs.get(), big_count);
+                               dim3(grid_for(big_count, 1, 4096)), dim3(1024), 0, as, a, big_hubs.get(), big_count);
             GP_LAUNCH_CHECK();
         }
         if (hub_count) {
@@ -1638,6 +1644,7 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
                                acc.get());
             GP_LAUNCH_CHECK();
         }
+        if (side) c.join();
     };
 
     auto count_clusters = [&]() {
@@ -1898,30 +1905,37 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
     DBuf<unsigned long long> overflow = c.buf<unsigned long long>(1);
     fill<unsigned long long>(c, overflow.get(), 1, 0ULL);
     RowArgs r{raw_off.get(), rkey.get(), rval.get(), ucount.get(), overflow.get()};
+    // Rows of the short and block classes and the long rows are disjoint:
+    // with long rows present (and tier overlap on) the short classes run on
+    // the side stream while the long rows' record sort runs here, and the
+    // stream waits for them before the unique counts are scanned.
+    const bool side = cc.c[6] > 0 && c.overlap();
+    if (side) c.fork();
+    const hipStream_t rs = side ? c.aux() : s;
     auto group = [&](int cls, const std::int32_t* list, std::int64_t count) {
         if (!count) return;
         switch (cls) {
             case 0:
-                hipLaunchKernelGGL((k_rows_group<16, 1>), dim3(grid_for(count, 16)), dim3(256), 0, s, r, list, count);
+                hipLaunchKernelGGL((k_rows_group<16, 1>), dim3(grid_for(count, 16)), dim3(256), 0, rs, r, list, count);
                 break;
             case 1:
-                hipLaunchKernelGGL((k_rows_group<64, 1>), dim3(grid_for(count, 4)), dim3(256), 0, s, r, list, count);
+                hipLaunchKernelGGL((k_rows_group<64, 1>), dim3(grid_for(count, 4)), dim3(256), 0, rs, r, list, count);
                 break;
             case 2:
-                hipLaunchKernelGGL((k_rows_group<64, 2>), dim3(grid_for(count, 4)), dim3(256), 0, s, r, list, count);
+                hipLaunchKernelGGL((k_rows_group<64, 2>), dim3(grid_for(count, 4)), dim3(256), 0, rs, r, list, count);
                 break;
             case 3:
-                hipLaunchKernelGGL((k_rows_group<64, 4>), dim3(grid_for(count, 4)), dim3(256), 0, s, r, list, count);
+                hipLaunchKernelGGL((k_rows_group<64, 4>), dim3(grid_for(count, 4)), dim3(256), 0, rs, r, list, count);
                 break;
             default:
-                hipLaunchKernelGGL((k_rows_group<64, 8>), dim3(grid_for(count, 4)), dim3(256), 0, s, r, list, count);
+                hipLaunchKernelGGL((k_rows_group<64, 8>), dim3(grid_for(count, 4)), dim3(256), 0, rs, r, list, count);
                 break;
         }
         GP_LAUNCH_CHECK();
     };
     auto block = [&](const std::int32_t* list, std::int64_t count) {
         if (!count) return;
-        hipLaunchKernelGGL(k_rows_block, dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, s, r, list, count,
+        hipLaunchKernelGGL(k_rows_block, dim3(grid_for(count, 1, 4096)), dim3(kBlockSortThreads), 0, rs, r, list, count,
                            std::max(1, bit_width_u64(static_cast<std::uint64_t>(nc))));
         GP_LAUNCH_CHECK();
     };
@@ -1961,6 +1975,7 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
             GP_LAUNCH_CHECK();
         }
     }
+    if (side) c.join();
     if (c.read_scalar(overflow.get())) throw std::runtime_error("coarse edge weight exceeds uint32");
     // Final offsets and copy.
     DBuf<std::int64_t> coff64 = c.buf<std::int64_t>(static_cast<std::size_t>(nc + 1));

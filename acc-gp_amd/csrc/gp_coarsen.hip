@@ -1484,6 +1484,19 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
     DBuf<std::uint64_t> regret = c.buf<std::uint64_t>(n);
     DBuf<std::uint32_t> ties = c.buf<std::uint32_t>(n);
     DBuf<unsigned long long> scalar = c.buf<unsigned long long>(4);
+    // scalar[0] accumulates one count or maximum at a time; reading it back
+    // also clears it, so only its first use needs a fill.
+    bool scalar_zero = false;
+    auto scalar_ready = [&] {
+        if (!scalar_zero) fill<unsigned long long>(c, scalar.get(), 1, 0ULL);
+        scalar_zero = false;
+    };
+    auto scalar_take = [&]() -> unsigned long long {
+        unsigned long long v = 0;
+        c.read_clear(&v, scalar.get(), 1, 1);
+        scalar_zero = true;
+        return v;
+    };
     const int vb = grid_for(n);
 
     std::int64_t cluster_count = n;
@@ -1494,11 +1507,11 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
                            g.vw.get(), cap, salt, choices.get());
         GP_LAUNCH_CHECK();
         fill<unsigned long long>(c, cw.get(), n, 0ULL);
-        fill<unsigned long long>(c, scalar.get(), 1, 0ULL);
+        scalar_ready();
         hipLaunchKernelGGL(k_match_roots, dim3(vb), dim3(256), 0, s, n, choices.get(), g.vw.get(), clusters.get(),
                            cw.get(), scalar.get());
         GP_LAUNCH_CHECK();
-        cluster_count = (std::int64_t)c.read_scalar(scalar.get());
+        cluster_count = (std::int64_t)scalar_take();
     } else {
         hipLaunchKernelGGL(k_init_singletons, dim3(vb), dim3(256), 0, s, n, g.vw.get(), clusters.get(), cw.get());
         GP_LAUNCH_CHECK();
@@ -1648,10 +1661,10 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
     };
 
     auto count_clusters = [&]() {
-        fill<unsigned long long>(c, scalar.get(), 1, 0ULL);
+        scalar_ready();
         hipLaunchKernelGGL(k_count_nonzero, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, n, cw.get(), scalar.get());
         GP_LAUNCH_CHECK();
-        return (std::int64_t)c.read_scalar(scalar.get());
+        return (std::int64_t)scalar_take();
     };
 
     const int score_bits = std::max(1, std::min(64, bit_width_u64(st.max_row_weight)));
@@ -1661,15 +1674,15 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
     // select: vertices to order (null: every valid proposal).
     auto admission_order = [&](const std::uint64_t* score, DBuf<std::uint64_t>& keys, DBuf<std::uint32_t>& vals,
                                const std::uint8_t* select) -> std::int64_t {
-        DBuf<std::uint8_t> flags = c.buf<std::uint8_t>(n);
+        DBuf<std::uint8_t> flags;
         DBuf<std::int32_t> order = c.buf<std::int32_t>(n);
         DBuf<std::int64_t> cnt = c.buf<std::int64_t>(1);
-        if (select)
-            c.copy_d2d(flags.get(), select, static_cast<std::size_t>(n));
-        else
+        if (!select) {
+            flags = c.buf<std::uint8_t>(n);
             hipLaunchKernelGGL(k_valid_flags, dim3(vb), dim3(256), 0, s, n, proposals.get(), flags.get());
-        GP_LAUNCH_CHECK();
-        stable_select(c, flags.get(), n, order.get(), cnt.get());
+            GP_LAUNCH_CHECK();
+        }
+        stable_select(c, select ? select : flags.get(), n, order.get(), cnt.get());
         const std::int64_t count = c.read_scalar(cnt.get());
         if (!count) return 0;
         keys = c.buf<std::uint64_t>(static_cast<std::size_t>(count));
@@ -1830,11 +1843,11 @@ AggregateResult aggregate(Context& c, const DGraph& g, CoarsenState& st, int par
                            map.get(), sums.get());
         GP_LAUNCH_CHECK();
         coarse_vw = c.buf<std::uint32_t>(static_cast<std::size_t>(nc));
-        fill<unsigned long long>(c, scalar.get(), 1, 0ULL);
+        scalar_ready();
         hipLaunchKernelGGL(k_narrow_weights, dim3(grid_for(nc)), dim3(256), 0, s, nc, sums.get(), coarse_vw.get(),
                            scalar.get());
         GP_LAUNCH_CHECK();
-        res.max_weight = c.read_scalar(scalar.get());
+        res.max_weight = scalar_take();
         res.coarse_n = nc;
         if (res.max_weight > 0xffffffffULL) throw std::runtime_error("coarse vertex weight exceeds uint32");
         if (res.max_weight > cap) throw std::runtime_error("coarsening exceeded cluster capacity");
@@ -1857,15 +1870,22 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
     hipLaunchKernelGGL(k_contract_bounds, dim3(grid_for(n)), dim3(256), 0, s, n, map, fine.off.get(), st.row_bound.get(),
                        next_rb.get(), rawc.get(), plain_bounds);
     GP_LAUNCH_CHECK();
-    DBuf<unsigned long long> red = c.buf<unsigned long long>(2);
-    fill<unsigned long long>(c, red.get(), 2, 0ULL);
+    // The level's small counters, cleared by one fill: the two maxima, the
+    // tier counters of the scatter and of the copy, the class counts and
+    // the weight-overflow flag.
+    constexpr int kSmallRed = 0, kSmallTierScatter = 2, kSmallTierCopy = 4, kSmallClasses = 6, kSmallOverflow = 14,
+                  kSmallWords = 15;
+    static_assert(sizeof(ClassCounts) == 8 * sizeof(unsigned long long), "class counts are eight words");
+    DBuf<unsigned long long> small = c.buf<unsigned long long>(kSmallWords);
+    fill<unsigned long long>(c, small.get(), kSmallWords, 0ULL);
+    unsigned long long* red = small.get() + kSmallRed;
     hipLaunchKernelGGL((k_reduce_max<unsigned long long>), dim3(grid_for(nc, 256, 1024)), dim3(256), 0, s, next_rb.get(), nc,
-                       red.get());
+                       red);
     hipLaunchKernelGGL((k_reduce_max<unsigned long long>), dim3(grid_for(nc, 256, 1024)), dim3(256), 0, s, rawc.get(), nc,
-                       red.get() + 1);
+                       red + 1);
     GP_LAUNCH_CHECK();
     unsigned long long rb[2];
-    c.read(rb, red.get(), sizeof rb);
+    c.read(rb, red, sizeof rb);
     const bool count_raw = st.positive_ew && n >= 65536 && fine.max_degree <= 512;
     const std::uint64_t crb = rb[0];
     const std::uint64_t craw = count_raw ? rb[1] : ~0ULL;
@@ -1881,7 +1901,6 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
     const std::int64_t m = fine.m;
     DBuf<std::int32_t> tier_a = c.buf<std::int32_t>(static_cast<std::size_t>(std::max(n, nc)));
     DBuf<std::int32_t> tier_b = c.buf<std::int32_t>(static_cast<std::size_t>(std::max(n, nc)));
-    DBuf<unsigned long long> tier_n = c.buf<unsigned long long>(2);
     DBuf<std::int32_t> rkey = c.buf<std::int32_t>(static_cast<std::size_t>(std::max<std::int64_t>(m, 1)));
     DBuf<std::uint32_t> rval = c.buf<std::uint32_t>(static_cast<std::size_t>(std::max<std::int64_t>(m, 1)));
     if (m > 0) {
@@ -1890,21 +1909,19 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
         const double bytes = 8.0 * (double)n + 4.0 + (double)m * (8.0 + (fine.ew ? 4.0 : 0.0) + 8.0);
         const ScatterBody body{fine.off.get(), fine.nbr.get(), fine.ew_ptr(), map, cursor.get(), rkey.get(), rval.get()};
         c.timed("contract_scatter", bytes,
-                [&] { run_tiers(c, body, nullptr, n, nullptr, tier_a.get(), tier_b.get(), tier_n.get(), nullptr, nullptr,
-                                nullptr, 0, n, fine.row_len_bound); });
+                [&] { run_tiers(c, body, nullptr, n, nullptr, tier_a.get(), tier_b.get(), small.get() + kSmallTierScatter,
+                                nullptr, nullptr, nullptr, 0, n, fine.row_len_bound, true); });
     }
     cursor.reset();
     // Row classes by raw length.
     DBuf<std::int32_t> lists = c.buf<std::int32_t>(static_cast<std::size_t>(7 * nc));
-    DBuf<ClassCounts> ccount = c.buf<ClassCounts>(1);
-    fill<unsigned long long>(c, reinterpret_cast<unsigned long long*>(ccount.get()), 8, 0ULL);
-    classify(c, nc, raw_off.get(), lists.get(), nc, ccount.get());
-    ClassCounts cc = c.read_scalar(ccount.get());
+    ClassCounts* ccount = reinterpret_cast<ClassCounts*>(small.get() + kSmallClasses);
+    classify(c, nc, raw_off.get(), lists.get(), nc, ccount);
+    ClassCounts cc = c.read_scalar(ccount);
     DBuf<unsigned long long> ucount = c.buf<unsigned long long>(static_cast<std::size_t>(nc));
     fill<unsigned long long>(c, ucount.get(), nc, 0ULL);
-    DBuf<unsigned long long> overflow = c.buf<unsigned long long>(1);
-    fill<unsigned long long>(c, overflow.get(), 1, 0ULL);
-    RowArgs r{raw_off.get(), rkey.get(), rval.get(), ucount.get(), overflow.get()};
+    unsigned long long* overflow = small.get() + kSmallOverflow;
+    RowArgs r{raw_off.get(), rkey.get(), rval.get(), ucount.get(), overflow};
     // Rows of the short and block classes and the long rows are disjoint:
     // with long rows present (and tier overlap on) the short classes run on
     // the side stream while the long rows' record sort runs here, and the
@@ -1976,7 +1993,7 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
         }
     }
     if (side) c.join();
-    if (c.read_scalar(overflow.get())) throw std::runtime_error("coarse edge weight exceeds uint32");
+    if (c.read_scalar(overflow)) throw std::runtime_error("coarse edge weight exceeds uint32");
     // Final offsets and copy.
     DBuf<std::int64_t> coff64 = c.buf<std::int64_t>(static_cast<std::size_t>(nc + 1));
     DBuf<std::int64_t> mtot = c.buf<std::int64_t>(1);
@@ -1991,8 +2008,8 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
     const CopyBody copy{raw_off.get(), rkey.get(), rval.get(), coff64.get(), out->off.get(), out->nbr.get(),
                         out->ew.get(), nc};
     // Unique coarse rows are no longer than the raw ones (max rb[1]).
-    run_tiers(c, copy, nullptr, nc, nullptr, tier_a.get(), tier_b.get(), tier_n.get(), nullptr, nullptr, nullptr, 0, nc,
-              rb[1]);
+    run_tiers(c, copy, nullptr, nc, nullptr, tier_a.get(), tier_b.get(), small.get() + kSmallTierCopy, nullptr, nullptr,
+              nullptr, 0, nc, rb[1], true);
     out->vw = std::move(coarse_vw);
     out->incident_bound = crb;
     if (st.positive_ew) {

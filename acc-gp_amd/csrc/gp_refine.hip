@@ -2614,8 +2614,9 @@ public:
             a_.prices = prices_.get();
             a_.samples = samples_.get();
         }
-        // Boundary vertices seed the frontier; per-level resets.
-        tiers(BoundaryBody{{a_}}, nullptr, n, nullptr, nullptr, nullptr, nullptr, 0);
+        // Boundary vertices seed the frontier; per-level resets (its tier
+        // counters were cleared with the counter block above).
+        tiers(BoundaryBody{{a_}}, nullptr, n, nullptr, nullptr, nullptr, nullptr, 0, kTier, true);
         // filter constant (incremental.cuh:897-899)
         std::uint64_t max_edge = 1;
         if (g.ew) {
@@ -2895,12 +2896,14 @@ public:
     // tiers over the positions [0, ne) of E).
     void conn_rows(const std::int32_t* E, std::int64_t ne, const std::uint8_t* skip, unsigned long long* vals,
                    std::uint8_t* parts, std::uint8_t* nnz) {
+        // Within a repair iteration the second sweep's tier counters are
+        // still zero from the iteration's counter clear (k_repair_begin).
         if (narrow_bins())
             tiers(ConnBody<unsigned>{{a_}, E, skip, vals, parts, nnz}, nullptr, ne, nullptr, nullptr, nullptr, nullptr,
-                  a_.k);
+                  a_.k, kTierB, true);
         else
             tiers(ConnBody<unsigned long long>{{a_}, E, skip, vals, parts, nnz}, nullptr, ne, nullptr, nullptr, nullptr,
-                  nullptr, a_.k);
+                  nullptr, a_.k, kTierB, true);
     }
 
     // Receiver reservation over the flagged donors L (vertex order).
@@ -3147,7 +3150,9 @@ public:
             set_lane(c_, kStateFlag, 0);
             hipLaunchKernelGGL(k_joint_accept, dim3(ceil_div(active, 256)), dim3(256), 0, s, a_, moves.get(), active,
                                stops.get());
-            fill<unsigned long long>(c_, ctr_.get() + kMovers, 1, 0ULL);
+            // Clears the mover count and every counter the update and commit
+            // of this batch use (delta, its tier lists, profiled entries).
+            fill<unsigned long long>(c_, ctr_.get() + kMovers, kCtrCount - kMovers, 0ULL);
             hipLaunchKernelGGL(k_compact_flags<StateLane<std::uint8_t>>, dim3(grid_for(n, 256 * kScanRounds, 2048)), dim3(256), 0, s,
                                n, a_.flags,
                                movers_.get(), ctr_.get() + kMovers, nullptr);
@@ -3159,7 +3164,7 @@ public:
             }
             empty = 0;
             std::int64_t delta = 0;
-            apply(false, false, delta);
+            apply(false, false, delta, true);
             if (delta <= 0) throw std::runtime_error("non-improving boundary group batch");
             update_cut(cut, delta);
             st_.group_moves += cnt;

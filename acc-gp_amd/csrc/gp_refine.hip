@@ -548,8 +548,8 @@ __global__ void k_commit(RefArgs a, const std::int32_t* list, const unsigned lon
         if (delta[p]) atomicAdd(&a.loads[p], delta[p]);
 }
 
-__global__ void k_set_lane(std::uint32_t* state, std::int64_t n, int lane, std::uint8_t value) {
-    const std::uint32_t keep = ~(0xffu << (8 * lane)), put = (std::uint32_t)value << (8 * lane);
+// Sets byte lanes of every state word: state = (state & keep) | put.
+__global__ void k_set_lanes(std::uint32_t* state, std::int64_t n, std::uint32_t keep, std::uint32_t put) {
     for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
          v += (std::int64_t)gridDim.x * blockDim.x)
         state[v] = (state[v] & keep) | put;
@@ -1365,20 +1365,6 @@ __global__ void __launch_bounds__(kTailThreads) k_reserve_tail(RefArgs a, Reserv
     st.flush();
 }
 
-__global__ void k_gather_i32(const std::int32_t* src, const std::int32_t* idx, std::int64_t count, std::int32_t* dst) {
-    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
-         j += (std::int64_t)gridDim.x * blockDim.x)
-        dst[j] = src[idx[j]];
-}
-
-__global__ void k_donor_info(RefArgs a, const std::int32_t* E, std::int64_t count, DonorInfo* out) {
-    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
-         j += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int32_t v = E[j];
-        out[j] = DonorInfo{static_cast<std::int32_t>(j), v, a.vw[v], (std::int32_t)a.labels[v]};
-    }
-}
-
 __global__ void k_live_flags(const DonorInfo* live, std::int64_t count, const std::uint8_t* done, std::uint8_t* flags) {
     for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < count;
          j += (std::int64_t)gridDim.x * blockDim.x)
@@ -1391,14 +1377,20 @@ __global__ void k_gather_info(const DonorInfo* src, const std::int32_t* idx, std
         dst[j] = src[idx[j]];
 }
 
-// Reservation set-up in one launch: clears the per-donor done flags and
-// tried masks, and sets each part's slack and request total.
-__global__ void k_reserve_begin(RefArgs a, std::int64_t ne, std::uint8_t* done, unsigned long long* tried,
+// Set-up of a reservation in one launch: the eligible donors E[j] =
+// L[idx[j]] in vertex order with their packed records, cleared per-donor
+// state (done, tried bits) and each part's room and request total.
+__global__ void k_reserve_setup(RefArgs a, const std::int32_t* L, const std::int32_t* idx, std::int64_t ne,
+                                std::int32_t* E, DonorInfo* live, std::uint8_t* done, unsigned long long* tried,
                                 unsigned long long* slack, unsigned long long* requests) {
-    for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < 2 * ne;
-         i += (std::int64_t)gridDim.x * blockDim.x) {
-        tried[i] = 0;
-        if (i < ne) done[i] = 0;
+    for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < ne;
+         j += (std::int64_t)gridDim.x * blockDim.x) {
+        const std::int32_t v = L[idx[j]];
+        E[j] = v;
+        live[j] = DonorInfo{static_cast<std::int32_t>(j), v, a.vw[v], (std::int32_t)a.labels[v]};
+        done[j] = 0;
+        tried[2 * j] = 0;
+        tried[2 * j + 1] = 0;
     }
     if (blockIdx.x == 0)
         for (int p = threadIdx.x; p < a.k; p += blockDim.x) {
@@ -1700,12 +1692,16 @@ __global__ void k_quotient(RefArgs a, const unsigned long long* A, unsigned long
 // score, then smallest part) is found by a group of W lanes over
 // q = sub, sub + W, ...
 constexpr int kMatchThreads = 1024;
+// The block also resets the pass's seed slots (seed_words of ~0) for
+// k_joint_seeds, which runs next.
 __global__ void __launch_bounds__(kMatchThreads) k_joint_match(int k, const unsigned long long* Q,
-                                                               std::int32_t* partner, std::int32_t* visits) {
+                                                               std::int32_t* partner, std::int32_t* visits,
+                                                               unsigned long long* seeds, int seed_words) {
     __shared__ double score_of[kMaxParts * kMaxParts];
     __shared__ int choice[kMaxParts];
     __shared__ int mate[kMaxParts];
     __shared__ int progress;
+    for (int x = threadIdx.x; x < seed_words; x += blockDim.x) seeds[x] = ~0ULL;
     for (int x = threadIdx.x; x < k * k; x += blockDim.x) {
         const int p = x / k, q = x % k;
         double v = 0.0;
@@ -2262,9 +2258,28 @@ __global__ void k_joint_resolve(JMove* m, int count, const std::int32_t* owners)
             break;
         }
 }
-__global__ void k_joint_mark(RefArgs a, const JMove* m, int count, std::int32_t* owners) {
+// idx[0, m) then -1 up to N (the power-of-two length a bitonic sort takes).
+__global__ void k_pad_index(const std::int32_t* idx, int m, std::int32_t* out, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) out[i] = i < m ? idx[i] : -1;
+}
+
+// Clears the owner of every vertex and its moving flag before the claimed
+// transactions are marked and accepted (nothing in between reads the flag).
+__global__ void k_exchange_reset(std::int64_t n, std::int32_t* owners, std::uint32_t* state) {
+    const std::uint32_t keep = ~(0xffu << (8 * kStateFlag));
+    for (std::int64_t v = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (std::int64_t)gridDim.x * blockDim.x) {
+        owners[v] = INT_MAX;
+        state[v] &= keep;
+    }
+}
+
+// Also clears each transaction's event total for k_joint_events.
+__global__ void k_joint_mark(RefArgs a, const JMove* m, int count, std::int32_t* owners, unsigned long long* events) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
+    events[i] = 0;
     for (int j = 0; j < m[i].size; ++j) {
         const std::int32_t v = m[i].v[j];
         owners[v] = i;
@@ -2435,8 +2450,12 @@ __global__ void k_joint_prefix(RefArgs a, const JMove* m, int count, const unsig
     }
     if (lane == 0) stops[pa] = stop;
 }
-__global__ void k_joint_accept(RefArgs a, const JMove* m, int count, const std::int32_t* stops) {
+// Block 0 also clears the counter words [first, last) for the compaction,
+// update and commit of the accepted moves.
+__global__ void k_joint_accept(RefArgs a, const JMove* m, int count, const std::int32_t* stops, int first, int last) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0)
+        for (int x = first + (int)threadIdx.x; x < last; x += blockDim.x) a.ctr[x] = 0;
     if (i < count && m[i].size && i <= stops[m[i].a])
         for (int j = 0; j < m[i].size; ++j) a.flags[m[i].v[j]] = 1;
 }
@@ -2640,10 +2659,13 @@ public:
 
     unsigned long long* loads_dev() const { return a_.loads; }
 
-    // Sets one byte lane of every vertex's state word.
-    void set_lane(Context& c, int lane, std::uint8_t value) {
-        hipLaunchKernelGGL(k_set_lane, dim3(grid_for(a_.n, 256, 2048)), dim3(256), 0, c.stream(), state_.get(), a_.n,
-                           lane, value);
+    // Clears the moving flag and the target (-1) of every vertex, the state
+    // the exchange starts from.
+    void clear_moves(Context& c) {
+        const std::uint32_t keep = ~((0xffu << (8 * kStateFlag)) | (0xffu << (8 * kStateTarget)));
+        const std::uint32_t put = 0xffu << (8 * kStateTarget);
+        hipLaunchKernelGGL(k_set_lanes, dim3(grid_for(a_.n, 256, 2048)), dim3(256), 0, c.stream(), state_.get(), a_.n,
+                           keep, put);
         GP_LAUNCH_CHECK();
     }
 
@@ -2929,8 +2951,6 @@ public:
         const std::int64_t ne = c_.read_scalar(cnt.get());
         if (ne <= 0) return;
         DBuf<std::int32_t> E = c_.buf<std::int32_t>(static_cast<std::size_t>(ne));
-        hipLaunchKernelGGL(k_gather_i32, dim3(grid_for(ne)), dim3(256), 0, s, L, idx.get(), ne, E.get());
-        GP_LAUNCH_CHECK();
         // Connection rows are read from the second attempt on, and only for
         // donors still waiting; they are built once attempt 0 is done.
         DBuf<unsigned long long> conn = c_.buf<unsigned long long>(static_cast<std::size_t>(ne) * k);
@@ -2940,11 +2960,9 @@ public:
         DBuf<unsigned long long> slack = c_.buf<unsigned long long>(static_cast<std::size_t>(k));
         DBuf<std::uint8_t> done = c_.buf<std::uint8_t>(static_cast<std::size_t>(ne));
         DBuf<unsigned long long> tried = c_.buf<unsigned long long>(static_cast<std::size_t>(ne) * 2);
-        hipLaunchKernelGGL(k_reserve_begin, dim3(grid_for(2 * ne)), dim3(256), 0, s, a_, ne, done.get(), tried.get(),
-                           slack.get(), requests.get());
-        GP_LAUNCH_CHECK();
         DBuf<DonorInfo> live = c_.buf<DonorInfo>(static_cast<std::size_t>(ne));
-        hipLaunchKernelGGL(k_donor_info, dim3(grid_for(ne)), dim3(256), 0, s, a_, E.get(), ne, live.get());
+        hipLaunchKernelGGL(k_reserve_setup, dim3(grid_for(ne)), dim3(256), 0, s, a_, L, idx.get(), ne, E.get(),
+                           live.get(), done.get(), tried.get(), slack.get(), requests.get());
         GP_LAUNCH_CHECK();
         ReserveState r{E.get(),     ne,         conn.get(), conn_part.get(), conn_nnz.get(), requests.get(),
                        slack.get(), done.get(), tried.get(), live.get(),     ne};
@@ -3103,9 +3121,9 @@ public:
         GP_LAUNCH_CHECK();
         int empty = 0;
         for (int pass = 0; pass < passes; ++pass) {
-            hipLaunchKernelGGL(k_joint_match, dim3(1), dim3(kMatchThreads), 0, s, k, Q.get(), partner.get(), visits.get());
+            hipLaunchKernelGGL(k_joint_match, dim3(1), dim3(kMatchThreads), 0, s, k, Q.get(), partner.get(), visits.get(),
+                               seeds.get(), k * sw);
             GP_LAUNCH_CHECK();
-            fill<unsigned long long>(c_, seeds.get(), (std::int64_t)k * sw, ~0ULL);
             hipLaunchKernelGGL(k_joint_seeds, dim3(grid_for(n)), dim3(256), 0, s, a_, A.get(), partner.get(),
                                seeds.get(), sw, pass);
             GP_LAUNCH_CHECK();
@@ -3136,23 +3154,21 @@ public:
                                owners.get());
             hipLaunchKernelGGL(k_joint_resolve, dim3(ceil_div(active, 256)), dim3(256), 0, s, moves.get(), active,
                                owners.get());
-            fill<std::int32_t>(c_, owners.get(), n, INT_MAX);
-            hipLaunchKernelGGL(k_joint_mark, dim3(ceil_div(active, 256)), dim3(256), 0, s, a_, moves.get(), active,
-                               owners.get());
-            GP_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_exchange_reset, dim3(grid_for(n)), dim3(256), 0, s, n, owners.get(), state_.get());
             DBuf<unsigned long long> events = c_.buf<unsigned long long>(static_cast<std::size_t>(active));
-            fill<unsigned long long>(c_, events.get(), active, 0ULL);
+            hipLaunchKernelGGL(k_joint_mark, dim3(ceil_div(active, 256)), dim3(256), 0, s, a_, moves.get(), active,
+                               owners.get(), events.get());
+            GP_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_joint_events, dim3(ceil_div(active, kEventTx)), dim3(kEventThreads), 0, s, a_,
                                moves.get(), active,
                                owners.get(), events.get());
             hipLaunchKernelGGL(k_joint_prefix, dim3(ceil_div(k, 4)), dim3(256), 0, s, a_, moves.get(), active,
                                events.get(), stops.get());
-            set_lane(c_, kStateFlag, 0);
+            // The accept also clears the mover count and every counter the
+            // update and commit of this batch use (delta, its tier lists,
+            // profiled entries).
             hipLaunchKernelGGL(k_joint_accept, dim3(ceil_div(active, 256)), dim3(256), 0, s, a_, moves.get(), active,
-                               stops.get());
-            // Clears the mover count and every counter the update and commit
-            // of this batch use (delta, its tier lists, profiled entries).
-            fill<unsigned long long>(c_, ctr_.get() + kMovers, kCtrCount - kMovers, 0ULL);
+                               stops.get(), (int)kMovers, (int)kCtrCount);
             hipLaunchKernelGGL(k_compact_flags<StateLane<std::uint8_t>>, dim3(grid_for(n, 256 * kScanRounds, 2048)), dim3(256), 0, s,
                                n, a_.flags,
                                movers_.get(), ctr_.get() + kMovers, nullptr);
@@ -3192,8 +3208,8 @@ public:
         int N = 1;
         while (N < m) N <<= 1;
         DBuf<std::int32_t> sidx = c_.buf<std::int32_t>(static_cast<std::size_t>(N));
-        fill<std::int32_t>(c_, sidx.get(), N, -1);
-        c_.copy_d2d(sidx.get(), idx.get(), static_cast<std::size_t>(m) * sizeof(std::int32_t));
+        hipLaunchKernelGGL(k_pad_index, dim3(ceil_div(N, 256)), dim3(256), 0, s, idx.get(), m, sidx.get(), N);
+        GP_LAUNCH_CHECK();
         if (N <= 1024) {
             hipLaunchKernelGGL((k_bitonic_local<Cmp>), dim3(1), dim3(1024), 0, s, sidx.get(), N, src, cmp);
             GP_LAUNCH_CHECK();
@@ -3336,8 +3352,7 @@ LevelStats refine_level_impl(Context& c, const DGraph& g, Label* labels, std::ve
     c.upload(R.loads_dev(), loads.data(), static_cast<std::size_t>(k) * sizeof(std::uint64_t));
     cut = best_cut;
     if (rounds > 0 && o.exchange_rounds > 0 && g.n <= o.exchange_max_n) {
-        R.set_lane(c, kStateFlag, 0);
-        R.set_lane(c, kStateTarget, 0xff);
+        R.clear_moves(c);
         R.exchange(o.exchange_rounds, cut, best_cut);
     }
     st.final_cut = cut;

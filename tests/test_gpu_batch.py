@@ -127,3 +127,31 @@ def test_apply_delta_matches_host(gp, ctx):
             assert np.array_equal(gp.download_graph(ctx, g.h)[1], want_nbr)
         finally:
             g.close()
+
+
+def test_request_shares_equal_alone(gp, ctx):
+    """With request shares (GP_SHARE_ENTRIES, read when the server starts) a
+    request of m CSR entries takes 1 + m / E of the workers' slots: large
+    requests run beside fewer others.  Only the schedule changes, so every
+    result equals the same graph partitioned alone."""
+    import os
+    mix = _mix(gp)
+    alone = []
+    for name, off, nbr, ew, vw, k in mix:
+        g = gp.Graph(ctx, off, nbr, ew, vw)
+        lab, st = g.partition(k)
+        g.close()
+        alone.append(lab)
+    saved = os.environ.get("GP_SHARE_ENTRIES")
+    os.environ["GP_SHARE_ENTRIES"] = "20000"
+    try:
+        items = [gp.BatchItem(off, nbr, k, ew=ew, vw=vw) for _, off, nbr, ew, vw, k in mix]
+        gp.partition_batch(items, workers=4)
+    finally:
+        if saved is None:
+            os.environ.pop("GP_SHARE_ENTRIES", None)
+        else:
+            os.environ["GP_SHARE_ENTRIES"] = saved
+    assert max(len(nbr) for _, _, nbr, _, _, _ in mix) > 3 * 20000  # some requests take several slots
+    for (name, *_), it, lab in zip(mix, items, alone):
+        assert np.array_equal(it.labels, lab), name

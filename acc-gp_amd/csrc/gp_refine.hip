@@ -1071,6 +1071,99 @@ __global__ void __launch_bounds__(kTierThreads) k_elig_keys(RefArgs a, const std
     tile_key_sums(keys, nl, a.k, nb, f, acc, S);
 }
 
+// Donor selection of a repair iteration on a level of at most
+// kSelectSmallMax vertices, in one block: the flagged donors (bucket at or
+// below the part's boundary bucket) in vertex order, then their eligibility
+// (below the boundary bucket, or inside it while the weighted rank among
+// earlier donors of that bucket stays under the excess), compacted into E
+// with their number in *ne.  One pass in index order gives the same ranks
+// as the tiled form (per-tile sums, a scan over tiles, in-tile ranks), which
+// larger levels use.
+constexpr int kDonorThreads = 1024;
+constexpr int kDonorWaves = kDonorThreads / kWave;
+
+__global__ void __launch_bounds__(kDonorThreads) k_donors_small(RefArgs a, const std::int32_t* bstar,
+                                                               const unsigned long long* prefix,
+                                                               const unsigned long long* excess, std::int32_t* E,
+                                                               std::int64_t* ne) {
+    __shared__ std::int32_t L[kSelectSmallMax];
+    __shared__ unsigned long long wt[kDonorWaves][kMaxParts];
+    __shared__ unsigned long long run[kMaxParts];
+    __shared__ int wsum[kDonorWaves];
+    __shared__ int base;
+    const int t = threadIdx.x, w = t / kWave, lane = lane_id();
+    const unsigned long long lt = (1ULL << lane) - 1ULL;
+    for (int c = t; c < kMaxParts; c += kDonorThreads) {
+        run[c] = 0;
+        for (int j = 0; j < kDonorWaves; ++j) wt[j][c] = 0;
+    }
+    if (t == 0) base = 0;
+    __syncthreads();
+    // Stable compaction of `mine` over one round: writes dst[base + rank].
+    auto compact = [&](bool mine, std::int32_t value, std::int32_t* dst) {
+        const unsigned long long m = __ballot(mine);
+        if (lane == 0) wsum[w] = __popcll(m);
+        __syncthreads();
+        int pre = base, total = 0;
+        for (int j = 0; j < kDonorWaves; ++j) {
+            pre += j < w ? wsum[j] : 0;
+            total += wsum[j];
+        }
+        if (mine) dst[pre + __popcll(m & lt)] = value;
+        __syncthreads();
+        if (t == 0) base += total;
+        __syncthreads();
+    };
+    for (std::int64_t r = 0; r < a.n; r += kDonorThreads) {
+        const std::int64_t v = r + t;
+        bool f = false;
+        if (v < a.n) {
+            const int b = a.bucket[v];
+            f = b >= 0 && (b % kBalanceWidth) <= bstar[b / kBalanceWidth];
+        }
+        compact(f, static_cast<std::int32_t>(v), L);
+    }
+    const int nl = base;
+    __syncthreads();
+    if (t == 0) base = 0;
+    __syncthreads();
+    for (int r = 0; r < nl; r += kDonorThreads) {
+        const int i = r + t;
+        int key = -1, b = 0;
+        bool el = false;
+        std::int32_t v = 0;
+        std::uint64_t wgt = 0;
+        if (i < nl) {
+            v = L[i];
+            b = a.bucket[v];
+            const int s = b / kBalanceWidth, lb = b % kBalanceWidth;
+            if (lb == bstar[s]) {
+                key = s;
+                wgt = a.vw[v];
+            }
+            el = lb < bstar[s];
+        }
+        const std::uint64_t ex = wave_keyed_exclusive(key, wgt, wt[w]);
+        __syncthreads();
+        if (key >= 0) {
+            std::uint64_t rank = run[key] + ex;
+            for (int j = 0; j < w; ++j) rank += wt[j][key];
+            el = prefix[b] + rank < excess[key];
+        }
+        __syncthreads();
+        for (int c = t; c < kMaxParts; c += kDonorThreads) {
+            unsigned long long add = 0;
+            for (int j = 0; j < kDonorWaves; ++j) {
+                add += wt[j][c];
+                wt[j][c] = 0;
+            }
+            run[c] += add;
+        }
+        compact(el, v, E);
+    }
+    if (t == 0) *ne = base;
+}
+
 // A live donor as the attempts read it: index into the eligible list, the
 // vertex, its weight and label (one 16-byte load per donor).
 struct DonorInfo {
@@ -1378,15 +1471,16 @@ __global__ void k_gather_info(const DonorInfo* src, const std::int32_t* idx, std
 }
 
 // Set-up of a reservation in one launch: the eligible donors E[j] =
-// L[idx[j]] in vertex order with their packed records, cleared per-donor
+// L[idx[j]] in vertex order (E already holds them when idx is null; then
+// L is E) with their packed records, cleared per-donor
 // state (done, tried bits) and each part's room and request total.
 __global__ void k_reserve_setup(RefArgs a, const std::int32_t* L, const std::int32_t* idx, std::int64_t ne,
                                 std::int32_t* E, DonorInfo* live, std::uint8_t* done, unsigned long long* tried,
                                 unsigned long long* slack, unsigned long long* requests) {
     for (std::int64_t j = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; j < ne;
          j += (std::int64_t)gridDim.x * blockDim.x) {
-        const std::int32_t v = L[idx[j]];
-        E[j] = v;
+        const std::int32_t v = idx ? L[idx[j]] : L[j];
+        if (idx) E[j] = v;
         live[j] = DonorInfo{static_cast<std::int32_t>(j), v, a.vw[v], (std::int32_t)a.labels[v]};
         done[j] = 0;
         tried[2 * j] = 0;
@@ -2951,6 +3045,17 @@ public:
         const std::int64_t ne = c_.read_scalar(cnt.get());
         if (ne <= 0) return;
         DBuf<std::int32_t> E = c_.buf<std::int32_t>(static_cast<std::size_t>(ne));
+        reserve_donors(E.get(), L, idx.get(), ne);
+    }
+
+    // Reservation over the ne eligible donors E[j] = L[idx[j]] (E = L when
+    // idx is null: the list is already the eligible donors).
+    void reserve_donors(std::int32_t* E, const std::int32_t* L, const std::int32_t* idx, std::int64_t ne) {
+        const hipStream_t s = c_.stream();
+        const int k = a_.k;
+        DBuf<std::int32_t> keys = c_.buf<std::int32_t>(static_cast<std::size_t>(ne));
+        DBuf<unsigned long long> S =
+            c_.buf<unsigned long long>(static_cast<std::size_t>(ceil_div(ne, kKeyTile)) * k);
         // Connection rows are read from the second attempt on, and only for
         // donors still waiting; they are built once attempt 0 is done.
         DBuf<unsigned long long> conn = c_.buf<unsigned long long>(static_cast<std::size_t>(ne) * k);
@@ -2961,10 +3066,10 @@ public:
         DBuf<std::uint8_t> done = c_.buf<std::uint8_t>(static_cast<std::size_t>(ne));
         DBuf<unsigned long long> tried = c_.buf<unsigned long long>(static_cast<std::size_t>(ne) * 2);
         DBuf<DonorInfo> live = c_.buf<DonorInfo>(static_cast<std::size_t>(ne));
-        hipLaunchKernelGGL(k_reserve_setup, dim3(grid_for(ne)), dim3(256), 0, s, a_, L, idx.get(), ne, E.get(),
-                           live.get(), done.get(), tried.get(), slack.get(), requests.get());
+        hipLaunchKernelGGL(k_reserve_setup, dim3(grid_for(ne)), dim3(256), 0, s, a_, L, idx, ne, E, live.get(),
+                           done.get(), tried.get(), slack.get(), requests.get());
         GP_LAUNCH_CHECK();
-        ReserveState r{E.get(),     ne,         conn.get(), conn_part.get(), conn_nnz.get(), requests.get(),
+        ReserveState r{E,           ne,         conn.get(), conn_part.get(), conn_nnz.get(), requests.get(),
                        slack.get(), done.get(), tried.get(), live.get(),     ne};
         // Attempts run in groups between host checks: an attempt without a
         // chooser changes nothing (no tried bits, no requests, no admissions),
@@ -2977,7 +3082,7 @@ public:
         if (ne <= kTailDonors) {
             // Every donor's connection row first (the rows of donors admitted
             // in attempt 0 go unused), then all attempts in one launch.
-            conn_rows(E.get(), ne, nullptr, conn.get(), conn_part.get(), conn_nnz.get());
+            conn_rows(E, ne, nullptr, conn.get(), conn_part.get(), conn_nnz.get());
             GP_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_reserve_tail, dim3(1), dim3(kTailThreads), 0, s, a_, r, 0, movers_.get(),
                                ctr_.get() + kMovers);
@@ -2991,7 +3096,7 @@ public:
             fill<unsigned long long>(c_, active.get(), kAttemptGroup, 0ULL);
             for (int j = 0; j < group; ++j, ++attempt) {
                 if (attempt == 1) {
-                    conn_rows(E.get(), ne, done.get(), conn.get(), conn_part.get(), conn_nnz.get());
+                    conn_rows(E, ne, done.get(), conn.get(), conn_part.get(), conn_nnz.get());
                     GP_LAUNCH_CHECK();
                 }
                 // Blocks stride over the donors: the per-block room table and
@@ -3071,11 +3176,20 @@ public:
             hipLaunchKernelGGL(k_bucket_scan, dim3(k), dim3(kScanThreads), 0, s, a_, prefix.get(), bstar.get(),
                                excess.get());
             GP_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_repair_flag, dim3(grid_for(n)), dim3(256), 0, s, a_, bstar.get(), f.get());
-            GP_LAUNCH_CHECK();
-            stable_select(c_, f.get(), n, list.get(), lcount.get());
-            const std::int64_t nl = c_.read_scalar(lcount.get());
-            if (nl > 0) reserve(list.get(), nl, prefix.get(), bstar.get(), excess.get());
+            if (n <= kSelectSmallMax) {
+                // Flagging, eligibility and both compactions in one block.
+                hipLaunchKernelGGL(k_donors_small, dim3(1), dim3(kDonorThreads), 0, s, a_, bstar.get(), prefix.get(),
+                                   excess.get(), list.get(), lcount.get());
+                GP_LAUNCH_CHECK();
+                const std::int64_t ne = c_.read_scalar(lcount.get());
+                if (ne > 0) reserve_donors(list.get(), list.get(), nullptr, ne);
+            } else {
+                hipLaunchKernelGGL(k_repair_flag, dim3(grid_for(n)), dim3(256), 0, s, a_, bstar.get(), f.get());
+                GP_LAUNCH_CHECK();
+                stable_select(c_, f.get(), n, list.get(), lcount.get());
+                const std::int64_t nl = c_.read_scalar(lcount.get());
+                if (nl > 0) reserve(list.get(), nl, prefix.get(), bstar.get(), excess.get());
+            }
             hipLaunchKernelGGL(k_repair_clear, dim3(grid_for(n)), dim3(256), 0, s, a_, cand_.get(),
                                ctr_.get() + kDonors);
             GP_LAUNCH_CHECK();

@@ -600,8 +600,20 @@ __device__ inline int gain_bucket(const RefArgs& a, double gain) {
 }
 
 // Donor flags, resets (flags, bucket) on frozen loads.
-__global__ void k_repair_donors(RefArgs a, std::int32_t* donors, unsigned long long* count) {
+// With counts set it also starts the repair iteration: the bucket weights
+// cleared and loads0 = loads, as k_repair_begin does but for the counter
+// block, which the caller knows to be zero.  The donor test then reads the
+// part loads themselves, the values loads0 is given.
+__global__ void k_repair_donors(RefArgs a, std::int32_t* donors, unsigned long long* count,
+                                unsigned long long* loads0, unsigned long long* counts, std::int64_t ncounts) {
     __shared__ std::int32_t sb[kTierThreads / kWave][kStage];
+    if (counts)
+        for (std::int64_t i = blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x; i < ncounts;
+             i += (std::int64_t)gridDim.x * blockDim.x) {
+            counts[i] = 0;
+            if (i < a.k) loads0[i] = a.loads[i];
+        }
+    const unsigned long long* frozen = counts ? a.loads : a.loads0;
     Stage st{sb[threadIdx.x / kWave], donors, count, 0};
     const std::int64_t wave = (blockIdx.x * (std::int64_t)blockDim.x + threadIdx.x) / kWave;
     const std::int64_t waves = (std::int64_t)gridDim.x * blockDim.x / kWave;
@@ -619,7 +631,7 @@ __global__ void k_repair_donors(RefArgs a, std::int32_t* donors, unsigned long l
             if (lab[r] >= 0) {
                 a.flags[v] = 0;
                 a.bucket[v] = -1;
-                d = a.loads0[lab[r]] > a.cap;
+                d = frozen[lab[r]] > a.cap;
             }
             st.push(d, (std::int32_t)v);
         }
@@ -3145,7 +3157,6 @@ public:
     // Capacity repair loop (rebalance_graph.cuh:160-280); returns feasibility.
     bool rebalance(std::uint64_t& cut) {
         const hipStream_t s = c_.stream();
-        ctr_clear_ = false;
         const std::int64_t n = a_.n;
         const int k = a_.k;
         int iterations = 0;
@@ -3157,13 +3168,24 @@ public:
         DBuf<std::uint8_t> f = c_.buf<std::uint8_t>(n);
         DBuf<std::int32_t> list = c_.buf<std::int32_t>(n);
         DBuf<std::int64_t> lcount = c_.buf<std::int64_t>(1);
+        const std::int64_t ncounts = (std::int64_t)k * kBalanceWidth;
         do {
             static_assert(kCtrCount <= kBalanceWidth, "the counter block clears inside the bucket clear");
-            hipLaunchKernelGGL(k_repair_begin, dim3(grid_for((std::int64_t)k * kBalanceWidth)), dim3(256), 0, s, a_,
-                               loads0_.get(), counts_.get(), (std::int64_t)k * kBalanceWidth, ctr_.get(), (int)kCtrCount);
-            GP_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_repair_donors, dim3(grid_for(n, 256 * kScanRounds, 2048)), dim3(256), 0, s, a_, cand_.get(),
-                               ctr_.get() + kDonors);
+            // The counter block is zero after a read-back that cleared it
+            // (every iteration's last); then one launch starts the iteration.
+            const bool zeroed = ctr_clear_;
+            ctr_clear_ = false;
+            const int donor_grid = grid_for(n, 256 * kScanRounds, 2048);
+            if (zeroed) {
+                hipLaunchKernelGGL(k_repair_donors, dim3(std::max(donor_grid, grid_for(ncounts))), dim3(256), 0, s, a_,
+                                   cand_.get(), ctr_.get() + kDonors, loads0_.get(), counts_.get(), ncounts);
+            } else {
+                hipLaunchKernelGGL(k_repair_begin, dim3(grid_for(ncounts)), dim3(256), 0, s, a_, loads0_.get(),
+                                   counts_.get(), ncounts, ctr_.get(), (int)kCtrCount);
+                GP_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_repair_donors, dim3(donor_grid), dim3(256), 0, s, a_, cand_.get(),
+                                   ctr_.get() + kDonors, nullptr, nullptr, 0);
+            }
             GP_LAUNCH_CHECK();
             c_.timed("repair_propose", 0.0, [&] {
                 if (narrow_bins())

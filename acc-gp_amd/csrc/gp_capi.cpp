@@ -302,6 +302,14 @@ void gp_graph_destroy(gp_graph* g) { delete g; }
 
 int gp_debug_below_0999(uint64_t cut, uint64_t best) { return gp::debug_below_0999(cut, best) ? 1 : 0; }
 
+int gp_debug_radix_sort(gp_context* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit, int end_bit) {
+    return guarded([&] {
+        if (!ctx || (n > 0 && !keys)) throw std::invalid_argument("null argument");
+        ctx->ctx.activate();
+        gp::debug_radix_sort(ctx->ctx, keys, vals, n, begin_bit, end_bit);
+    });
+}
+
 int gp_capacity(uint64_t total, int parts, double imbalance, uint64_t* out) {
     if (!out) return fail(GP_ERR_INVALID, "null output");
     return guarded([&] { *out = gp::capacity_of(total, parts, imbalance); });

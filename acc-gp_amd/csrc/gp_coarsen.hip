@@ -2085,4 +2085,18 @@ Hierarchy coarsen(Context& c, DGraph& input, const CoarsenOptions& o) {
     return h;
 }
 
+void debug_radix_sort(Context& c, std::uint64_t* keys, std::uint32_t* vals, std::int64_t n, int begin_bit,
+                      int end_bit) {
+    if (n < 0 || begin_bit < 0 || end_bit > 64 || begin_bit > end_bit) throw std::invalid_argument("bad sort range");
+    if (n == 0) return;
+    DBuf<std::uint64_t> k = c.buf<std::uint64_t>(static_cast<std::size_t>(n));
+    DBuf<std::uint32_t> v;
+    if (vals) v = c.buf<std::uint32_t>(static_cast<std::size_t>(n));
+    c.upload(k.get(), keys, static_cast<std::size_t>(n) * sizeof(std::uint64_t));
+    if (vals) c.upload(v.get(), vals, static_cast<std::size_t>(n) * sizeof(std::uint32_t));
+    radix_sort_pairs(c, k, v, n, begin_bit, end_bit);
+    c.download(keys, k.get(), static_cast<std::size_t>(n) * sizeof(std::uint64_t));
+    if (vals) c.download(vals, v.get(), static_cast<std::size_t>(n) * sizeof(std::uint32_t));
+}
+
 }  // namespace gp

@@ -27,4 +27,11 @@ std::unique_ptr<DGraph> contract(Context& c, const DGraph& fine, CoarsenState& s
 // Full level loop; the input graph stays owned by the caller (level 0).
 Hierarchy coarsen(Context& c, DGraph& input, const CoarsenOptions& o);
 
+// The device radix sort of the admission and contraction orders on host
+// arrays (stable, by bits [begin_bit, end_bit) of the keys, taken in whole
+// 8-bit digits from begin_bit, so bits past end_bit in the last digit must
+// be zero; vals may be null).  For tests.
+void debug_radix_sort(Context& c, std::uint64_t* keys, std::uint32_t* vals, std::int64_t n, int begin_bit,
+                      int end_bit);
+
 }  // namespace gp

@@ -374,21 +374,27 @@ static __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
 // registers and LDS across all digit passes (same ranking as the scatter
 // pass), so a short sort costs one launch instead of three per digit.  A
 // pass whose digit is equal for every key is skipped (stable identity).
+// The block has Waves waves of kSortTileRounds x 64 keys each, the fewest
+// of 4, 8 and 16 that hold the input: the ranking rounds, the per-wave
+// count scan and the barriers of every pass scale with the waves.
 constexpr int kSortTileThreads = 1024;
 constexpr int kSortTileRounds = 8;
 constexpr int kSortTileMax = kSortTileThreads * kSortTileRounds;
 
-static __global__ __launch_bounds__(kSortTileThreads) void k_radix_sort_tile(std::uint64_t* keys, std::uint32_t* vals,
-                                                                             int n, int begin_bit, int end_bit) {
-    constexpr int kWaves = kSortTileThreads / kWave;
+template <int Waves>
+static __global__ __launch_bounds__(Waves* kWave) void k_radix_sort_tile(std::uint64_t* keys, std::uint32_t* vals, int n,
+                                                                   int begin_bit, int end_bit) {
+    constexpr int kWaves = Waves;
+    constexpr int kThreads = Waves * kWave;
     constexpr int kPerWave = kSortTileRounds * kWave;
     constexpr int kScanWaves = kRadixBins / kWave;
+    static_assert(kThreads >= kRadixBins, "one thread per digit in the count scan");
     __shared__ std::uint32_t wcount[kWaves][kRadixBins];
     __shared__ std::uint32_t start[kRadixBins];
     __shared__ std::uint32_t wsum[kScanWaves];
     __shared__ int same;
-    __shared__ std::uint64_t skeys[kSortTileMax];
-    __shared__ std::uint32_t svals[kSortTileMax];
+    __shared__ std::uint64_t skeys[kThreads * kSortTileRounds];
+    __shared__ std::uint32_t svals[kThreads * kSortTileRounds];
     const int t = threadIdx.x, w = t / kWave, lane = t % kWave;
     const int wbase = w * kPerWave;
     const unsigned long long lt = (1ULL << lane) - 1ULL;
@@ -406,10 +412,12 @@ static __global__ __launch_bounds__(kSortTileThreads) void k_radix_sort_tile(std
     // after reading it in the previous pass, so it needs no reset.
     if (t == 0) same = -1;
     for (int shift = begin_bit; shift < end_bit; shift += 8) {
-        for (int i = t; i < kWaves * kRadixBins; i += kSortTileThreads) wcount[i / kRadixBins][i % kRadixBins] = 0;
+        for (int i = t; i < kWaves * kRadixBins; i += kThreads) wcount[i / kRadixBins][i % kRadixBins] = 0;
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < kSortTileRounds; ++r) {
+            // A round past the input has no key in this wave (uniform).
+            if (wbase + r * kWave >= n) break;
             const bool valid = wbase + r * kWave + lane < n;
             const std::uint32_t d = static_cast<std::uint32_t>((key[r] >> shift) & 0xff);
             unsigned long long peers = __ballot(valid);
@@ -483,8 +491,16 @@ inline void radix_sort_pairs(Context& c, DBuf<std::uint64_t>& keys, DBuf<std::ui
     if (n <= 1 || end_bit <= begin_bit) return;
     if (n > 0xffffffffLL) throw std::invalid_argument("radix sort size exceeds 32-bit offsets");
     if (n <= kSortTileMax) {
-        hipLaunchKernelGGL(k_radix_sort_tile, dim3(1), dim3(kSortTileThreads), 0, c.stream(), keys.get(), vals.get(),
-                           static_cast<int>(n), begin_bit, end_bit);
+        constexpr int kPerWave = kSortTileRounds * kWave;
+        if (n <= 4 * kPerWave)
+            hipLaunchKernelGGL(k_radix_sort_tile<4>, dim3(1), dim3(4 * kWave), 0, c.stream(), keys.get(), vals.get(),
+                               static_cast<int>(n), begin_bit, end_bit);
+        else if (n <= 8 * kPerWave)
+            hipLaunchKernelGGL(k_radix_sort_tile<8>, dim3(1), dim3(8 * kWave), 0, c.stream(), keys.get(), vals.get(),
+                               static_cast<int>(n), begin_bit, end_bit);
+        else
+            hipLaunchKernelGGL(k_radix_sort_tile<16>, dim3(1), dim3(16 * kWave), 0, c.stream(), keys.get(),
+                               vals.get(), static_cast<int>(n), begin_bit, end_bit);
         GP_LAUNCH_CHECK();
         return;
     }

@@ -83,6 +83,7 @@ EXPORTS = [
     "gp_profile_enable", "gp_profile_query", "gp_profile_reset", "gp_server_create", "gp_server_submit",
     "gp_server_wait", "gp_server_destroy", "gp_partition_batch", "gp_server_graph_add", "gp_server_graph_download",
     "gp_server_graph_release", "gp_graph_apply_delta", "gp_debug_below_0999",
+    "gp_debug_radix_sort",
 ]
 
 
@@ -167,6 +168,7 @@ def lib():
         L.gp_server_graph_release.argtypes = [vp, u64]
         L.gp_graph_apply_delta.argtypes = [vp, vp, i64, vp, vp, vp, vp, vp]
         L.gp_debug_below_0999.argtypes = [u64, u64]
+        L.gp_debug_radix_sort.argtypes = [vp, vp, vp, i64, i32, i32]
         _LIB = L
     return _LIB
 
@@ -184,6 +186,18 @@ def below_0999(cut, best):
     """The device loop control's integer form of the small-level stale rule
     cut < 0.999 * best (long double)."""
     return bool(lib().gp_debug_below_0999(cut, best))
+
+
+def radix_sort(ctx, keys, vals=None, begin_bit=0, end_bit=64):
+    """The device radix sort behind the admission and contraction orders:
+    stable by bits [begin_bit, end_bit) of the uint64 keys, uint32 vals
+    carried along.  The passes take whole 8-bit digits from begin_bit, so
+    key bits past end_bit in the last digit must be zero.  Returns sorted
+    copies."""
+    k = np.ascontiguousarray(keys, dtype=np.uint64).copy()
+    v = None if vals is None else np.ascontiguousarray(vals, dtype=np.uint32).copy()
+    check(lib().gp_debug_radix_sort(ctx.h, _p(k), _p(v), len(k), begin_bit, end_bit))
+    return k, v
 
 
 def capacity(total, parts, imbalance=1.03):

@@ -348,6 +348,15 @@ int gp_profile_reset(gp_context* ctx);
  * device-side loop control evaluates it, in integers; 1 or 0.  For tests. */
 int gp_debug_below_0999(uint64_t cut, uint64_t best);
 
+/* The device radix sort behind the admission order (capacity_admission.hpp:14-33:
+ * proposals ordered by score, target and vertex) and the contraction's record
+ * order, on host arrays: stable by bits [begin_bit, end_bit) of keys, vals
+ * (may be NULL) carried along, sorted in place.  The passes take whole 8-bit
+ * digits from begin_bit, so key bits from end_bit up to the end of the last
+ * digit must be zero (the callers' keys are exactly end_bit wide).  For
+ * tests. */
+int gp_debug_radix_sort(gp_context* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit, int end_bit);
+
 #ifdef __cplusplus
 }
 #endif
